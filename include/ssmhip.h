@@ -1,0 +1,173 @@
+/*
+ * ssmhip.h -- C-ABI of the MI355X (gfx950) filtering library.
+ *
+ * A drop-in beneath GeneralisedFilters' unchanged `filter(model, algo, observations)`:
+ * a host (a Julia package extension calling `ccall`, or the Python mirror shipped with
+ * this library) keeps the model definitions and the `filter`/`step` driver, reads the
+ * model parameters through the unchanged `calc_params` protocol, and hands plain
+ * pointers and sizes to these entry points.  No torch or Julia types cross the boundary.
+ *
+ * Conventions
+ *   - Every entry point returns int32_t status: SSM_OK (0) or a negative SSM_ERR_*;
+ *     ssm_last_error(ctx) returns the message of the last failure on that context.
+ *   - Indices are 0-based int32 (the reference is 1-based Int64; a Julia shim adds 1).
+ *   - Matrices are column-major FP64 (Julia/LAPACK layout): M[i + j*rows].
+ *   - Host arrays are read (or written) only during the call; device state lives behind
+ *     opaque handles until the matching *_destroy.  A context is not thread-safe; use one
+ *     per host thread (the serving loop owns its own).
+ *   - "dev" pointers are device (HBM) addresses from ssm_dev_alloc or any HIP allocation
+ *     on the context's device.
+ *
+ * Reference paths below are relative to the TuringLang/SSMProblems.jl checkout.
+ */
+#ifndef SSMHIP_H
+#define SSMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (Julia exceptions they map to, in parentheses) ----------------- */
+#define SSM_OK 0
+#define SSM_ERR_ARG -1        /* ArgumentError / BoundsError                           */
+#define SSM_ERR_NOT_PD -2     /* PosDefException from PDMat/cholesky                  */
+#define SSM_ERR_OOM -3        /* OutOfMemoryError                                      */
+#define SSM_ERR_ADMISSION -4  /* request exceeds the serving HBM budget               */
+#define SSM_ERR_HIP -5        /* HIP runtime failure                                   */
+#define SSM_ERR_STATE -6      /* call out of order (e.g. step before initialise)      */
+#define SSM_ERR_NO_DEVICE -7  /* no HIP device: the library has no CPU fallback       */
+
+/* ---- resampler kinds (GF/src/resamplers.jl:204-240, GF/ext/CUDAExt.jl:31-78) ------ */
+#define SSM_SYSTEMATIC 0        /* resamplers.jl:216-238, CPU definition (default)     */
+#define SSM_SYSTEMATIC_MURRAY 1 /* CUDAExt.jl:41-58, GPU definition (Murray Code 8)   */
+#define SSM_STRATIFIED 2        /* CUDAExt.jl:60-78 (Murray Code 7)                    */
+#define SSM_MULTINOMIAL 3       /* CUDAExt.jl:31-39 (inverse CDF, Murray Code 5)       */
+
+/* ---- arithmetic types ------------------------------------------------------------- */
+#define SSM_F32 0
+#define SSM_F64 1
+
+typedef struct ssm_ctx ssm_ctx;
+typedef struct ssm_bf ssm_bf;
+typedef struct ssm_rbpf ssm_rbpf;
+typedef struct ssm_server ssm_server;
+
+/*
+ * Homogeneous linear-Gaussian state-space model, the parameters returned by
+ * calc_initial / calc_params (GF/src/models/linear_gaussian.jl:35-63, :96-131).
+ * Dx state dim, Dy observation dim; all column-major FP64.
+ */
+typedef struct {
+  int32_t Dx, Dy;
+  const double* mu0;    /* Dx      */
+  const double* Sigma0; /* Dx x Dx */
+  const double* A;      /* Dx x Dx */
+  const double* b;      /* Dx      */
+  const double* Q;      /* Dx x Dx */
+  const double* H;      /* Dy x Dx */
+  const double* c;      /* Dy      */
+  const double* R;      /* Dy x Dy */
+} ssm_lg_model;
+
+/* ---- context ---------------------------------------------------------------------- */
+int32_t ssm_version(void);
+/* Creates a context on HIP device `device` with its own stream.  hbm_budget_bytes = 0
+ * means no budget (the serving loop uses it for admission). */
+int32_t ssm_ctx_create(int32_t device, uint64_t hbm_budget_bytes, ssm_ctx** out);
+int32_t ssm_ctx_destroy(ssm_ctx* ctx);
+const char* ssm_last_error(const ssm_ctx* ctx);
+int32_t ssm_ctx_synchronize(ssm_ctx* ctx);
+/* Device name and compute-unit count, for reports. */
+int32_t ssm_ctx_device_info(ssm_ctx* ctx, char* name, int32_t name_len, int32_t* cu_count);
+
+/* Device buffers for hosts that stage their own data (Julia ROCArray users need none). */
+int32_t ssm_dev_alloc(ssm_ctx* ctx, uint64_t bytes, void** out);
+int32_t ssm_dev_free(ssm_ctx* ctx, void* p);
+int32_t ssm_memcpy_h2d(ssm_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
+int32_t ssm_memcpy_d2h(ssm_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+
+/* ---- resampling (GF/src/resamplers.jl:208-238, GF/ext/CUDAExt.jl:31-78) ----------- */
+/*
+ * sample_ancestors(rng, resampler, weights) for device weights: replaces the CuVector
+ * methods at CUDAExt.jl:31-78 and the CPU methods at resamplers.jl:208-238.
+ *   d_w      n weights (FP64 or FP32 per w_dtype), need not be normalised for the
+ *            Murray/stratified/multinomial forms (they divide by the scan total, as the
+ *            reference does); the CPU systematic form assumes sum(w) = 1 as the reference.
+ *   d_u      uniforms in [0,1): 1 for systematic, n for stratified/multinomial; NULL draws
+ *            them in-kernel from Philox4x32-10 keyed by (seed, step) -- the shim turns the
+ *            Julia rng into `seed` by drawing one UInt64.
+ *   d_ancestors  n int32 outputs, 0-based.  Systematic/stratified outputs are
+ *            nondecreasing.  The FP64 prefix sum is computed on the device.
+ */
+int32_t ssm_sample_ancestors(ssm_ctx* ctx, int32_t kind, int32_t n, const void* d_w,
+                             int32_t w_dtype, const double* d_u, uint64_t seed, uint32_t step,
+                             int32_t* d_ancestors);
+/* Same, with host buffers (copied in and out on the context stream). */
+int32_t ssm_sample_ancestors_host(ssm_ctx* ctx, int32_t kind, int32_t n, const double* w,
+                                  const double* u, int32_t* ancestors);
+/* CUDAExt.jl:88-117 offspring_to_ancestors: cumulative offspring counts -> ancestors. */
+int32_t ssm_offspring_to_ancestors(ssm_ctx* ctx, int32_t n, const int32_t* d_offspring,
+                                   int32_t* d_ancestors);
+/* CUDAExt.jl:119-142 ancestors_to_offspring: ancestors -> per-particle offspring counts. */
+int32_t ssm_ancestors_to_offspring(ssm_ctx* ctx, int32_t n, const int32_t* d_ancestors,
+                                   int32_t* d_offspring);
+
+/* ---- bootstrap particle filter on an LG model ------------------------------------- */
+/*
+ * Device-resident BootstrapFilter state (GF/src/algorithms/particles.jl:36-245 with
+ * containers.jl:125-173): N particles, state SoA planes x[d][i] in `precision`, log-weights,
+ * ancestors and the FP64 scalars of marginalise!.  `seed` keys the in-kernel Philox noise.
+ */
+int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_t N, int32_t precision,
+                      uint64_t seed, ssm_bf** out);
+int32_t ssm_bf_destroy(ssm_bf* bf);
+/* calc_params(dyn, t) / calc_params(obs, t) for time-varying models: replaces the
+ * parameters used by subsequent steps (particles.jl:117-145 call them per step). */
+int32_t ssm_bf_set_model(ssm_bf* bf, const ssm_lg_model* model);
+/* initialise(rng, prior, BF) (particles.jl:36-50): x_i = mu0 + chol(Sigma0).L z_i.
+ * z: N x Dx host normals (row i = particle i) or NULL for in-kernel Philox noise. */
+int32_t ssm_bf_initialise(ssm_bf* bf, const double* z);
+/*
+ * One `step` (particles.jl:147-162): maybe_resample (ESS rule, resamplers.jl:156-183),
+ * predict (particles.jl:52-74), update + marginalise! (particles.jl:76-90,
+ * containers.jl:160-173).  t is the reference's 1-based iteration.  u (1 or N host
+ * uniforms) and z (N x Dx host normals) override the in-kernel Philox draws (parity mode).
+ * Outputs ll_inc and whether this step resampled.
+ */
+int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double threshold,
+                    int32_t resampler, const double* u, const double* z, double* ll_inc,
+                    int32_t* did_resample);
+/*
+ * filter(model, BF(N; threshold, resampler), ys) from the current state for steps
+ * t0 .. t0+T-1 (GeneralisedFilters.jl:58-84): ys is T x Dy (row t = observation t).
+ * All steps run device-side from captured HIP graphs; the resample decision is made on
+ * the device.  ll_total receives the summed increments; ll_steps (T, optional) each one.
+ */
+int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
+                   int32_t resampler, double* ll_total, double* ll_steps);
+/*
+ * Materialises the state for callbacks / checkpoints (callbacks.jl:53-110):
+ * x (N x Dx, row i = particle i), normalised logw (N), ancestors of the last step (N),
+ * and w (N) the weights the last resample consumed.  Any pointer may be NULL.
+ */
+int32_t ssm_bf_export(ssm_bf* bf, double* x, double* logw, int32_t* ancestors, double* w);
+/* Restores a state (x: N x Dx, logw: N normalised log-weights). */
+int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw);
+/* Device time of the last ssm_bf_run, from HIP events around it (ms). */
+int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
+/*
+ * Same steps as ssm_bf_run, launched one kernel sequence at a time with HIP events around
+ * the resampler kernels and the fused step kernel, for per-kernel timing.  ms_resample /
+ * ms_step receive the mean duration per step (ms); ll_total as in ssm_bf_run.
+ */
+int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
+                       int32_t resampler, double* ms_resample, double* ms_step,
+                       double* ll_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSMHIP_H */

@@ -1,0 +1,408 @@
+"""FP64 CPU oracle for the GeneralisedFilters filtering hot path -- TEST INFRASTRUCTURE.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may
+import this module, and only as the checker (or the timed CPU baseline), never as the thing
+measured or shipped.  The product library (``libssmhip.so``) does not link or call it.
+
+It wraps ``liboracle.so`` (built from ``oracle.c`` by ``oracle/Makefile``) and adds a numpy
+restatement of the closed-form joint Gaussian used by the reference's Kalman tests.
+
+Parity pinning of each oracle function (what checks it beyond retyping its formula):
+
+* ``philox4x32_10``: the Random123 known-answer vectors (Salmon et al. SC'11).
+* ``systematic`` (resamplers.jl:216-238), ``systematic_murray`` (CUDAExt.jl:49-58),
+  ``stratified`` (CUDAExt.jl:68-78), ``multinomial`` (CUDAExt.jl:31-39): brute force in
+  exact rational arithmetic on small inputs (tests/golden/gen_resample_fixtures.py), the
+  reference's known-answer vectors for offspring<->ancestors (test/components/resamplers.jl:
+  76-92), closed special cases (uniform weights -> identity, one-hot -> constant) and the
+  reference's statistical test (resampled mean ~ weighted mean, resamplers.jl:5-34).
+* ``kf_filter`` / ``kf_update`` (kalman.jl:42-79, kalman_gradient.jl:12-29): the closed-form
+  joint Gaussian (``joint_gaussian_loglik``, restating GFTest/models/linear_gaussian.jl:171-215,
+  as test/algorithms/kalman.jl:5-43 does) and the MVN conditional for T=1.
+* ``bf_step`` (particles.jl:36-162, containers.jl:140-173): special cases (H = 0 makes every
+  particle's weight equal, so ll_t is the exact N(y; c, R) density; Q -> 0 makes propagation
+  deterministic; uniform weights give ESS = N and an identity systematic resample; the baseline
+  after a resample is log N) and the reference's statistical test against the exact KF
+  log-likelihood (test/algorithms/particles.jl:5-27).
+* ``rbpf_step`` (rbpf.jl:16-84): RBPF log-likelihood against the KF of the joint model built
+  like GFTest's dummy LG model (test/algorithms/rbpf.jl:5-34), and the degenerate outer-noise-0
+  case where every particle is identical and the RBPF equals a KF with R_t = exp(x_t) R0.
+* ``logsumexp``: parity unpinned beyond closed cases (LogExpFunctions' exact rounding is
+  third-party; only tolerances are claimed).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+SYSTEMATIC, SYSTEMATIC_MURRAY, STRATIFIED, MULTINOMIAL = 0, 1, 2, 3
+STREAM_PROPAGATE, STREAM_RESAMPLE, STREAM_INIT, STREAM_OUTER = 0, 1, 2, 3
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+
+
+class LGModel(C.Structure):
+    _fields_ = [("Dx", C.c_int32), ("Dy", C.c_int32)] + [
+        (n, _dp) for n in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")
+    ]
+
+
+class RBModel(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("om0", "os0", "oa", "ob", "oq")] + [
+        ("D", C.c_int32), ("Dy", C.c_int32), ("rs", C.c_double)] + [
+        (n, _dp) for n in ("mu0", "Sigma0", "A", "b", "C", "Q", "H", "c", "R0")]
+
+
+def _declare(L):
+    L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+    L.orc_philox_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int,
+                                     C.c_int, _dp]
+    L.orc_philox_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_int64, _dp]
+    L.orc_resample.argtypes = [C.c_int, C.c_int64, _dp, _dp, _ip]
+    L.orc_resample.restype = C.c_int
+    L.orc_offspring_to_ancestors.argtypes = [C.c_int64, _ip, _ip]
+    L.orc_ancestors_to_offspring.argtypes = [C.c_int64, _ip, _ip]
+    L.orc_logsumexp.argtypes = [C.c_int64, _dp]
+    L.orc_logsumexp.restype = C.c_double
+    L.orc_bf_initialise.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp]
+    L.orc_bf_initialise.restype = C.c_int
+    L.orc_bf_step.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp, C.POINTER(C.c_int),
+                              C.c_double, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
+                              C.POINTER(C.c_int), _dp]
+    L.orc_bf_step.restype = C.c_int
+    L.orc_kf_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_kf_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp,
+                                _dp]
+    L.orc_kf_update.restype = C.c_int
+    L.orc_kf_filter.argtypes = [C.POINTER(LGModel), C.c_int, _dp, C.c_double, _dp, _dp, _dp,
+                                _dp]
+    L.orc_kf_filter.restype = C.c_int
+    L.orc_rbpf_initialise.argtypes = [C.POINTER(RBModel), C.c_int64, _dp, _dp, _dp, _dp]
+    L.orc_rbpf_step.argtypes = [C.POINTER(RBModel), C.c_int64, _dp, _dp, _dp, _dp,
+                                C.POINTER(C.c_int), C.c_double, C.c_int, _dp, _dp, _dp, _dp,
+                                _ip, _dp, C.POINTER(C.c_int), _dp]
+    L.orc_rbpf_step.restype = C.c_int
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _colmajor(M):
+    """Column-major FP64 buffer of a 2-D array (flattened Fortran order)."""
+    return np.ascontiguousarray(np.asarray(M, dtype=np.float64).flatten(order="F"))
+
+
+# --------------------------------------------------------------------------------------
+# Philox
+# --------------------------------------------------------------------------------------
+
+def philox4x32_10(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(c.ctypes.data_as(_up), k.ctypes.data_as(_up),
+                            out.ctypes.data_as(_up))
+    return out
+
+
+def philox_normals(seed, stream, t, N, D, f64_layout=False):
+    """N x D standard normals exactly as the device draws them for (seed, stream, t)."""
+    z = np.zeros((N, D), dtype=np.float64)
+    lib().orc_philox_normals(C.c_uint64(seed), stream, t, N, D, int(f64_layout), _d(z))
+    return z
+
+
+def philox_uniforms(seed, t, n):
+    u = np.zeros(n, dtype=np.float64)
+    lib().orc_philox_uniforms(C.c_uint64(seed), t, n, _d(u))
+    return u
+
+
+# --------------------------------------------------------------------------------------
+# Resamplers
+# --------------------------------------------------------------------------------------
+
+def resample(kind, w, u):
+    w = _f64(w)
+    u = _f64(np.atleast_1d(u))
+    n = w.shape[0]
+    anc = np.zeros(n, dtype=np.int32)
+    rc = lib().orc_resample(kind, n, _d(w), _d(u), _i(anc))
+    if rc:
+        raise ValueError(f"orc_resample rc={rc}")
+    return anc
+
+
+def offspring_to_ancestors(O):
+    O = np.ascontiguousarray(O, dtype=np.int32)
+    anc = np.zeros_like(O)
+    lib().orc_offspring_to_ancestors(O.shape[0], _i(O), _i(anc))
+    return anc
+
+
+def ancestors_to_offspring(anc):
+    anc = np.ascontiguousarray(anc, dtype=np.int32)
+    off = np.zeros_like(anc)
+    lib().orc_ancestors_to_offspring(anc.shape[0], _i(anc), _i(off))
+    return off
+
+
+def logsumexp(x):
+    x = _f64(x)
+    return lib().orc_logsumexp(x.shape[0], _d(x))
+
+
+# --------------------------------------------------------------------------------------
+# Linear-Gaussian models
+# --------------------------------------------------------------------------------------
+
+class LG:
+    """Homogeneous LG SSM parameters (linear_gaussian.jl:96-131) as numpy arrays."""
+
+    def __init__(self, mu0, Sigma0, A, b, Q, H, c, R):
+        self.mu0 = _f64(mu0)
+        self.Sigma0 = np.asarray(Sigma0, dtype=np.float64)
+        self.A = np.asarray(A, dtype=np.float64)
+        self.b = _f64(b)
+        self.Q = np.asarray(Q, dtype=np.float64)
+        self.H = np.asarray(H, dtype=np.float64)
+        self.c = _f64(c)
+        self.R = np.asarray(R, dtype=np.float64)
+        self.Dx = self.A.shape[0]
+        self.Dy = self.H.shape[0]
+
+    def cstruct(self):
+        keep = [_colmajor(M) for M in (self.Sigma0, self.A, self.Q, self.H, self.R)]
+        m = LGModel(self.Dx, self.Dy, _d(self.mu0), _d(keep[0]), _d(keep[1]), _d(self.b),
+                    _d(keep[2]), _d(keep[3]), _d(self.c), _d(keep[4]))
+        return m, keep
+
+
+def bf_initialise(model: LG, z):
+    z = _f64(z)
+    N = z.shape[0]
+    x = np.zeros((N, model.Dx))
+    m, keep = model.cstruct()
+    rc = lib().orc_bf_initialise(C.byref(m), N, _d(z), _d(x))
+    if rc:
+        raise ValueError(f"orc_bf_initialise rc={rc}")
+    return x
+
+
+class BFState:
+    """AoS oracle particle state: x (N x D), logw (N), typeless flag."""
+
+    def __init__(self, x, logw=None, typeless=True):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        N = self.x.shape[0]
+        self.logw = np.zeros(N) if logw is None else _f64(logw).copy()
+        self.typeless = typeless
+
+
+def bf_step(model: LG, state: BFState, y, threshold, kind, u, z, w_override=None):
+    """One reference BF step in place; returns (anc, ll_inc, did_resample, w_used)."""
+    N = state.x.shape[0]
+    anc = np.zeros(N, dtype=np.int32)
+    ll = C.c_double()
+    did = C.c_int()
+    typ = C.c_int(int(state.typeless))
+    w_out = np.zeros(N)
+    m, keep = model.cstruct()
+    u = _f64(np.atleast_1d(u))
+    z = _f64(z)
+    y = _f64(np.atleast_1d(y))
+    wo = None if w_override is None else _f64(w_override)
+    rc = lib().orc_bf_step(C.byref(m), N, _d(state.x), _d(state.logw), C.byref(typ),
+                           float(threshold), kind, _d(u), _d(z), _d(y), _d(wo), _i(anc),
+                           C.byref(ll), C.byref(did), _d(w_out))
+    if rc:
+        raise ValueError(f"orc_bf_step rc={rc}")
+    state.typeless = bool(typ.value)
+    return anc, ll.value, bool(did.value), w_out
+
+
+def kf_filter(model: LG, ys, jitter=0.0):
+    ys = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(-1, model.Dy))
+    T = ys.shape[0]
+    mu = np.zeros(model.Dx)
+    Sig = np.zeros(model.Dx * model.Dx)
+    ll = C.c_double()
+    lls = np.zeros(T)
+    m, keep = model.cstruct()
+    rc = lib().orc_kf_filter(C.byref(m), T, _d(ys), float(jitter), _d(mu), _d(Sig),
+                             C.byref(ll), _d(lls))
+    if rc:
+        raise ValueError(f"orc_kf_filter rc={rc}")
+    return mu, Sig.reshape(model.Dx, model.Dx, order="F"), ll.value, lls
+
+
+def kf_step(Dx, Dy, mu, Sigma, A, b, Q, H, c, R, y, jitter=0.0):
+    """One KF predict+update (kalman.jl:42-79); returns (mu, Sigma, ll)."""
+    mu = _f64(mu).copy()
+    S = _colmajor(Sigma)
+    A_, Q_, H_, R_ = (_colmajor(M) for M in (A, Q, H, R))
+    lib().orc_kf_predict(Dx, _d(A_), _d(_f64(b)), _d(Q_), _d(mu), _d(S))
+    ll = C.c_double()
+    rc = lib().orc_kf_update(Dx, Dy, _d(H_), _d(_f64(c)), _d(R_), _d(_f64(y)), float(jitter),
+                             _d(mu), _d(S), C.byref(ll))
+    if rc:
+        raise ValueError(f"orc_kf_update rc={rc}")
+    return mu, S.reshape(Dx, Dx, order="F"), ll.value
+
+
+def simulate_lg(model: LG, T, rng):
+    """Forward simulation (SSMProblems forward_simulation.jl:14-21) with numpy noise."""
+    L0 = np.linalg.cholesky(model.Sigma0)
+    LQ = np.linalg.cholesky(model.Q)
+    LR = np.linalg.cholesky(model.R)
+    x = model.mu0 + L0 @ rng.standard_normal(model.Dx)
+    x0 = x.copy()
+    xs, ys = [], []
+    for _ in range(T):
+        x = model.A @ x + model.b + LQ @ rng.standard_normal(model.Dx)
+        xs.append(x.copy())
+    for t in range(T):
+        ys.append(model.H @ xs[t] + model.c + LR @ rng.standard_normal(model.Dy))
+    return x0, np.array(xs), np.array(ys)
+
+
+def joint_gaussian(model: LG, T):
+    """Joint Gaussian of Z = [X0..XT, Y1..YT]; restates GFTest/models/linear_gaussian.jl:171-215."""
+    Dx, Dy = model.Dx, model.Dy
+    n = Dx + T * (Dx + Dy)
+    P = np.zeros((n, n))
+    mu_e = np.zeros(n)
+    S_e = np.zeros((n, n))
+    mu_e[:Dx] = model.mu0
+    S_e[:Dx, :Dx] = model.Sigma0
+    for t in range(1, T + 1):
+        iA, jA = t * Dx, (t - 1) * Dx
+        P[iA:iA + Dx, jA:jA + Dx] = model.A
+        iH, jH = Dx * (T + 1) + (t - 1) * Dy, Dx * t
+        P[iH:iH + Dy, jH:jH + Dx] = model.H
+        mu_e[iA:iA + Dx] = model.b
+        mu_e[iH:iH + Dy] = model.c
+        S_e[iA:iA + Dx, iA:iA + Dx] = model.Q
+        S_e[iH:iH + Dy, iH:iH + Dy] = model.R
+    IP = np.eye(n) - P
+    mu_Z = np.linalg.solve(IP, mu_e)
+    Sig_Z = np.linalg.solve(IP, np.linalg.solve(IP, S_e).T).T
+    return mu_Z, Sig_Z
+
+
+def mvn_logpdf(y, mu, S):
+    S = 0.5 * (S + S.T)
+    L = np.linalg.cholesky(S)
+    r = np.linalg.solve(L, y - mu)
+    return -0.5 * (len(y) * np.log(2 * np.pi) + 2 * np.log(np.diag(L)).sum() + r @ r)
+
+
+def joint_gaussian_loglik(model: LG, ys):
+    """Exact log p(y_1:T) from the joint Gaussian (the reference KF test's oracle)."""
+    ys = np.asarray(ys, dtype=np.float64).reshape(-1, model.Dy)
+    T = ys.shape[0]
+    mu_Z, Sig_Z = joint_gaussian(model, T)
+    iy = slice(model.Dx * (T + 1), None)
+    return mvn_logpdf(ys.reshape(-1), mu_Z[iy], Sig_Z[iy, iy])
+
+
+# --------------------------------------------------------------------------------------
+# RBPF
+# --------------------------------------------------------------------------------------
+
+class RB:
+    """Hierarchical model: scalar outer AR(1) + D-dim inner LG (see oracle.c orc_rb_model)."""
+
+    def __init__(self, om0, os0, oa, ob, oq, mu0, Sigma0, A, b, Cv, Q, H, c, R0, rs):
+        self.om0, self.os0, self.oa, self.ob, self.oq = map(float, (om0, os0, oa, ob, oq))
+        self.mu0, self.b, self.C, self.c = _f64(mu0), _f64(b), _f64(Cv), _f64(c)
+        self.Sigma0, self.A, self.Q, self.H, self.R0 = (
+            np.asarray(M, dtype=np.float64) for M in (Sigma0, A, Q, H, R0))
+        self.D = self.A.shape[0]
+        self.Dy = self.H.shape[0]
+        self.rs = float(rs)
+
+    def cstruct(self):
+        keep = [_colmajor(M) for M in (self.Sigma0, self.A, self.Q, self.H, self.R0)]
+        m = RBModel(self.om0, self.os0, self.oa, self.ob, self.oq, self.D, self.Dy, self.rs,
+                    _d(self.mu0), _d(keep[0]), _d(keep[1]), _d(self.b), _d(self.C),
+                    _d(keep[2]), _d(keep[3]), _d(self.c), _d(keep[4]))
+        return m, keep
+
+
+class RBState:
+    def __init__(self, xo, mu, Sigma, logw=None, typeless=True):
+        self.xo = _f64(xo).copy()
+        N = self.xo.shape[0]
+        self.mu = np.ascontiguousarray(mu, dtype=np.float64).copy()      # N x D
+        self.Sigma = np.ascontiguousarray(Sigma, dtype=np.float64).copy()  # N x D x D (col-major per particle)
+        self.logw = np.zeros(N) if logw is None else _f64(logw).copy()
+        self.typeless = typeless
+
+
+def rbpf_initialise(model: RB, zo):
+    zo = _f64(zo)
+    N = zo.shape[0]
+    xo = np.zeros(N)
+    mu = np.zeros((N, model.D))
+    Sig = np.zeros((N, model.D, model.D))
+    m, keep = model.cstruct()
+    lib().orc_rbpf_initialise(C.byref(m), N, _d(zo), _d(xo), _d(mu), _d(Sig))
+    return RBState(xo, mu, Sig)
+
+
+def rbpf_step(model: RB, st: RBState, y, threshold, kind, u, zo, w_override=None):
+    N = st.xo.shape[0]
+    anc = np.zeros(N, dtype=np.int32)
+    ll = C.c_double()
+    did = C.c_int()
+    typ = C.c_int(int(st.typeless))
+    w_out = np.zeros(N)
+    m, keep = model.cstruct()
+    u = _f64(np.atleast_1d(u))
+    wo = None if w_override is None else _f64(w_override)
+    rc = lib().orc_rbpf_step(C.byref(m), N, _d(st.xo), _d(st.mu), _d(st.Sigma), _d(st.logw),
+                             C.byref(typ), float(threshold), kind, _d(u), _d(_f64(zo)),
+                             _d(_f64(np.atleast_1d(y))), _d(wo), _i(anc), C.byref(ll),
+                             C.byref(did), _d(w_out))
+    if rc:
+        raise ValueError(f"orc_rbpf_step rc={rc}")
+    st.typeless = bool(typ.value)
+    return anc, ll.value, bool(did.value), w_out

@@ -1,0 +1,1025 @@
+// bf.hip -- device-resident BootstrapFilter on a linear-Gaussian model for gfx950.
+//
+// Reference (paths relative to the reference checkout):
+//   step       GF/src/algorithms/particles.jl:147-162  maybe_resample, then move
+//   predict    particles.jl:52-74, :117-131, :228-245   x' = A x + b + chol(Q).L z
+//   update     particles.jl:76-90, :133-145              logw += log N(y; H x' + c, R)
+//   normalise  GF/src/containers.jl:160-173 (marginalise!), :140-141 (get_weights)
+//   ESS rule   GF/src/resamplers.jl:156-183
+//   filter     GF/src/GeneralisedFilters.jl:58-84
+//
+// Layout in HBM: state planes x[d][i] (SoA, one contiguous plane per component, padded to
+// a multiple of 2048 particles), log-weights l[i], ancestors a[i] (int32), and one
+// BfScalars block.  Stored log-weights are not normalised in place: the normalised value
+// of marginalise! is l[i] - lse, applied when the next step reads them, so a step streams
+// each array exactly once.
+//
+// Per step: (1) when the device-side ESS rule says so, the resampler kernels turn the
+// log-weights into ancestors; (2) k_bf_step gathers x[a[i]] (or reads x[i]), propagates
+// with in-register Philox noise, evaluates the observation log-density, writes x' and the
+// new log-weights, and reduces (max, sum exp, sum exp^2) per block; the last block to
+// finish folds the block partials (FP64, fixed order) into the log-likelihood increment,
+// the normaliser and the ESS that decides the next step's resampling.  ssm_bf_run captures
+// chunks of steps in HIP graphs, so T steps run without host round trips.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace ssm {
+
+constexpr int kMaxD = 16;
+constexpr int kMaxDy = 16;
+constexpr int kStepThreads = 256;
+constexpr int kStepItems = 4;
+constexpr int kStepBlock = kStepThreads * kStepItems;
+constexpr int kPadTo = 2048;  // multiple of the step block and of the resampler tile
+
+// Per-filter model constants, precomputed on the host in FP64 and stored in `T`.
+// Column-major with leading dimension D (or Dy).
+template <typename T>
+struct BfModelDev {
+  T A[kMaxD * kMaxD];
+  T b[kMaxD];
+  T LQ[kMaxD * kMaxD];     // lower Cholesky factor of Q
+  T H[kMaxDy * kMaxD];
+  T c[kMaxDy];
+  T LRi[kMaxDy * kMaxDy];  // inverse of the lower Cholesky factor of R
+  T mu0[kMaxD];
+  T L0[kMaxD * kMaxD];     // lower Cholesky factor of Sigma0
+  T cst;                   // -1/2 (Dy log 2 pi + log det R)
+};
+
+template <typename T>
+struct StepParams {
+  int N, Npad, D, Dy, nblocks;
+  const T* __restrict__ xin;
+  T* __restrict__ xout;
+  T* __restrict__ logw;
+  const int32_t* __restrict__ anc;
+  const BfModelDev<T>* __restrict__ model;
+  const double* __restrict__ ys;
+  BfScalars* sc;
+  double threshold_n;
+  double logN;
+  uint64_t seed;
+  const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
+  double* __restrict__ partials;
+  double* __restrict__ ll_steps;
+  int32_t* __restrict__ rs_steps;
+};
+
+template <typename T>
+__device__ __forceinline__ void ld4(const T* p, T v[4]);
+template <>
+__device__ __forceinline__ void ld4<float>(const float* p, float v[4]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  v[0] = q.x;
+  v[1] = q.y;
+  v[2] = q.z;
+  v[3] = q.w;
+}
+template <>
+__device__ __forceinline__ void ld4<double>(const double* p, double v[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+template <typename T>
+__device__ __forceinline__ void st4(T* p, const T v[4]);
+template <>
+__device__ __forceinline__ void st4<float>(float* p, const float v[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+template <>
+__device__ __forceinline__ void st4<double>(double* p, const double v[4]) {
+  *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+}
+
+// Standard normals z[0..D) of particle i at step t from the given stream.
+template <typename T, int MD>
+__device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t stream, uint32_t t,
+                                             uint32_t i, int D, T z[MD]);
+template <>
+__device__ __forceinline__ void draw_normals<float, 1>(uint64_t seed, uint32_t stream,
+                                                       uint32_t t, uint32_t i, int D,
+                                                       float z[1]) {
+  const U4 w = philox4x32_10(i, t, stream, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+  float z1;
+  box_muller_f32(w.x, w.y, z[0], z1);
+}
+template <typename T, int MD>
+__device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t stream, uint32_t t,
+                                             uint32_t i, int D, T z[MD]) {
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int j = 0; j < (MD + 3) / 4; ++j) {
+      if (4 * j < D) {
+        float q[4];
+        normals4_f32(seed, stream, t, i, (uint32_t)j, q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * j + e < MD) z[4 * j + e] = q[e];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < (MD + 1) / 2; ++j) {
+      if (2 * j < D) {
+        double q[2];
+        normals2_f64(seed, stream, t, i, (uint32_t)j, q);
+        z[2 * j] = q[0];
+        if (2 * j + 1 < MD) z[2 * j + 1] = q[1];
+      }
+    }
+  }
+}
+
+// The fused predict + update + partial-reduction kernel.  DT/DYT > 0 fix the dimensions
+// at compile time; 0 means runtime dimensions up to kMaxD / kMaxDy.
+template <typename T, int DT, int DYT>
+__global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
+  constexpr int MD = DT ? DT : kMaxD;
+  constexpr int MDY = DYT ? DYT : kMaxDy;
+  const int D = DT ? DT : p.D;
+  const int Dy = DYT ? DYT : p.Dy;
+  BfScalars* sc = p.sc;
+  const bool rs = p.threshold_n >= sc->ess;
+  const bool fresh = rs || sc->typeless;  // normalised pre-update log-weights are all 0
+  const T lse = (T)sc->lse;
+  const int t = sc->step;
+  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
+  const BfModelDev<T>& m = *p.model;
+  T y[MDY];
+#pragma unroll
+  for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
+
+  const int i0 = (blockIdx.x * kStepThreads + threadIdx.x) * kStepItems;
+  T x[kStepItems][MD];
+  if (rs) {
+    const int4 a4 = *reinterpret_cast<const int4*>(p.anc + i0);
+    const int av[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) {
+      const int src = (i0 + k < p.N) ? av[k] : i0 + k;
+#pragma unroll
+      for (int d = 0; d < MD; ++d)
+        if (d < D) x[k][d] = p.xin[(size_t)d * p.Npad + src];
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < MD; ++d) {
+      if (d < D) {
+        T v[4];
+        ld4<T>(p.xin + (size_t)d * p.Npad + i0, v);
+#pragma unroll
+        for (int k = 0; k < kStepItems; ++k) x[k][d] = v[k];
+      }
+    }
+  }
+  T nl[kStepItems];
+  if (fresh) {
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) nl[k] = (T)0;
+  } else {
+    ld4<T>(p.logw + i0, nl);
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) nl[k] = nl[k] - lse;
+  }
+
+  T xo[MD][kStepItems];
+  T lnew[kStepItems];
+#pragma unroll
+  for (int k = 0; k < kStepItems; ++k) {
+    const int i = i0 + k;
+    T z[MD];
+    if (p.noise) {
+#pragma unroll
+      for (int d = 0; d < MD; ++d)
+        if (d < D) z[d] = p.noise[(size_t)d * p.Npad + i];
+    } else {
+      draw_normals<T, MD>(p.seed, kStreamPropagate, (uint32_t)t, (uint32_t)i, D, z);
+    }
+    // predict: x' = A x + b + L_Q z
+    T xn[MD];
+#pragma unroll
+    for (int r = 0; r < MD; ++r) {
+      if (r < D) {
+        T s = m.b[r];
+#pragma unroll
+        for (int c = 0; c < MD; ++c)
+          if (c < D) s += m.A[r + c * D] * x[k][c];
+#pragma unroll
+        for (int c = 0; c <= r; ++c) s += m.LQ[r + c * D] * z[c];
+        xn[r] = s;
+        xo[r][k] = s;
+      }
+    }
+    // update: log N(y; H x' + c, R) = cst - |L_R^-1 (y - H x' - c)|^2 / 2
+    T res[MDY];
+#pragma unroll
+    for (int q = 0; q < MDY; ++q) {
+      if (q < Dy) {
+        T s = m.c[q];
+#pragma unroll
+        for (int c = 0; c < MD; ++c)
+          if (c < D) s += m.H[q + c * Dy] * xn[c];
+        res[q] = y[q] - s;
+      }
+    }
+    T quad = 0;
+#pragma unroll
+    for (int q = 0; q < MDY; ++q) {
+      if (q < Dy) {
+        T v = 0;
+#pragma unroll
+        for (int c = 0; c <= q; ++c) v += m.LRi[q + c * Dy] * res[c];
+        quad += v * v;
+      }
+    }
+    lnew[k] = nl[k] + (m.cst - (T)0.5 * quad);
+  }
+#pragma unroll
+  for (int d = 0; d < MD; ++d)
+    if (d < D) st4<T>(p.xout + (size_t)d * p.Npad + i0, xo[d]);
+  st4<T>(p.logw + i0, lnew);
+
+  // Thread partials: acc0 over the pre-update normalised log-weights (predict baseline,
+  // particles.jl:71-73), acc1 over the updated ones (marginalise!, containers.jl:162).
+  LseAcc<T> a0 = lse_empty<T>(), a1 = lse_empty<T>();
+  {
+    T m0 = (T)-INFINITY, m1 = (T)-INFINITY;
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) {
+      if (i0 + k < p.N) {
+        m0 = nl[k] > m0 ? nl[k] : m0;
+        m1 = lnew[k] > m1 ? lnew[k] : m1;
+      }
+    }
+    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) {
+      if (i0 + k < p.N) {
+        if (m0 != (T)-INFINITY) s0 += exp_t(nl[k] - m0);
+        if (m1 != (T)-INFINITY) {
+          const T e = exp_t(lnew[k] - m1);
+          s1 += e;
+          s2 += e * e;
+        }
+      }
+    }
+    a0 = LseAcc<T>{m0, s0, (T)0};
+    a1 = LseAcc<T>{m1, s1, s2};
+  }
+  a0 = wave_lse(a0);
+  a1 = wave_lse(a1);
+  __shared__ LseAcc<double> w0[kStepThreads / kWave], w1[kStepThreads / kWave];
+  __shared__ int is_last;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (lane == 0) {
+    w0[wv] = LseAcc<double>{(double)a0.m, (double)a0.s, 0.0};
+    w1[wv] = LseAcc<double>{(double)a1.m, (double)a1.s, (double)a1.s2};
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    LseAcc<double> b0 = w0[0], b1 = w1[0];
+#pragma unroll
+    for (int k = 1; k < kStepThreads / kWave; ++k) {
+      b0 = lse_combine(b0, w0[k]);
+      b1 = lse_combine(b1, w1[k]);
+    }
+    double* P = p.partials + (size_t)blockIdx.x * 5;
+    P[0] = b0.m;
+    P[1] = b0.s;
+    P[2] = b1.m;
+    P[3] = b1.s;
+    P[4] = b1.s2;
+    __threadfence();
+    is_last = (atomicAdd(&sc->cnt_step, 1u) == (uint32_t)(p.nblocks - 1));
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+
+  // Last block: fold the block partials in FP64 (fixed order) and finish the step.
+  LseAcc<double> c0 = lse_empty<double>(), c1 = lse_empty<double>();
+  for (int b = threadIdx.x; b < p.nblocks; b += kStepThreads) {
+    const double* P = p.partials + (size_t)b * 5;
+    c0 = lse_combine(c0, LseAcc<double>{__builtin_nontemporal_load(P + 0),
+                                        __builtin_nontemporal_load(P + 1), 0.0});
+    c1 = lse_combine(c1, LseAcc<double>{__builtin_nontemporal_load(P + 2),
+                                        __builtin_nontemporal_load(P + 3),
+                                        __builtin_nontemporal_load(P + 4)});
+  }
+  c0 = wave_lse(c0);
+  c1 = wave_lse(c1);
+  __syncthreads();
+  if (lane == 0) {
+    w0[wv] = c0;
+    w1[wv] = c1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    LseAcc<double> b0 = w0[0], b1 = w1[0];
+#pragma unroll
+    for (int k = 1; k < kStepThreads / kWave; ++k) {
+      b0 = lse_combine(b0, w0[k]);
+      b1 = lse_combine(b1, w1[k]);
+    }
+    // baseline: logsumexp of the pre-update log-weights (+ ll_baseline = 0); log N for
+    // freshly resampled (all-zero) or TypelessBaseline weights.
+    const double baseline = fresh ? p.logN : b0.m + log(b0.s);
+    const int idx = t - 1 - sc->ybase;
+    if (b1.m == -INFINITY) {
+      // Every particle has zero weight: the increment is -Inf; weights restart uniform.
+      if (p.ll_steps) p.ll_steps[idx] = -INFINITY;
+      sc->ll_total += -INFINITY;
+      sc->lse = 0.0;
+      sc->mxn = 0.0;
+      sc->s1 = (double)p.N;
+      sc->ess = (double)p.N;
+      sc->typeless = 1;
+    } else {
+      const double lse_after = b1.m + log(b1.s);
+      const double ll = lse_after - baseline;
+      double ess = (b1.s * b1.s) / b1.s2;
+      if (!(ess <= (double)p.N)) ess = (double)p.N;  // ESS <= N exactly; clamp rounding
+      sc->lse = lse_after;
+      sc->mxn = (double)((T)b1.m - (T)lse_after);
+      sc->s1 = b1.s;
+      sc->ess = ess;
+      sc->ll_total += ll;
+      sc->typeless = 0;
+      if (p.ll_steps) p.ll_steps[idx] = ll;
+    }
+    if (p.rs_steps) p.rs_steps[idx] = rs ? 1 : 0;
+    sc->last_resampled = rs ? 1 : 0;
+    sc->step = t + 1;
+    sc->cnt_step = 0u;
+  }
+}
+
+// initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).
+template <typename T, int DT>
+__global__ void __launch_bounds__(kStepThreads) k_bf_init(int N, int Npad, int D,
+                                                          const BfModelDev<T>* model, T* x,
+                                                          T* logw, const T* noise,
+                                                          uint64_t seed) {
+  constexpr int MD = DT ? DT : kMaxD;
+  if (DT) D = DT;
+  const int i = blockIdx.x * kStepThreads + threadIdx.x;
+  if (i >= Npad) return;
+  const BfModelDev<T>& m = *model;
+  T z[MD];
+  if (noise) {
+#pragma unroll
+    for (int d = 0; d < MD; ++d)
+      if (d < D) z[d] = noise[(size_t)d * Npad + i];
+  } else {
+    draw_normals<T, MD>(seed, kStreamInit, 0u, (uint32_t)i, D, z);
+  }
+#pragma unroll
+  for (int r = 0; r < MD; ++r) {
+    if (r < D) {
+      T s = m.mu0[r];
+#pragma unroll
+      for (int c = 0; c <= r; ++c) s += m.L0[r + c * D] * z[c];
+      x[(size_t)r * Npad + i] = (i < N) ? s : (T)0;
+    }
+  }
+  logw[i] = (T)0;
+}
+
+}  // namespace ssm
+
+using namespace ssm;
+
+struct ssm_bf {
+  ssm_ctx* ctx = nullptr;
+  int N = 0, Npad = 0, D = 0, Dy = 0, prec = SSM_F32;
+  uint64_t seed = 0;
+  void* x[2] = {nullptr, nullptr};
+  int cur = 0;
+  void* logw = nullptr;
+  int32_t* anc = nullptr;
+  void* model_dev = nullptr;
+  BfScalars* sc = nullptr;
+  double* partials = nullptr;
+  void* rs_scratch = nullptr;
+  double* w_dbg = nullptr;
+  double* ys = nullptr;
+  size_t ys_cap = 0;
+  double* ll_steps = nullptr;
+  int32_t* rs_steps = nullptr;
+  size_t steps_cap = 0;
+  void* noise = nullptr;
+  double* u = nullptr;
+  bool initialised = false;
+  double ess_uniform = 0.0;
+  std::map<std::tuple<int, int, int, double, int>, hipGraphExec_t> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_T = 0;
+};
+
+namespace {
+
+// Lower Cholesky in place (column-major), FP64.
+bool chol_lower(int n, std::vector<double>& A) {
+  for (int j = 0; j < n; ++j) {
+    double d = A[j + j * n];
+    for (int k = 0; k < j; ++k) d -= A[j + k * n] * A[j + k * n];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    A[j + j * n] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[i + j * n];
+      for (int k = 0; k < j; ++k) s -= A[i + k * n] * A[j + k * n];
+      A[i + j * n] = s / d;
+    }
+    for (int i = 0; i < j; ++i) A[i + j * n] = 0.0;
+  }
+  return true;
+}
+
+// Inverse of a lower-triangular matrix (column-major).
+std::vector<double> tri_inverse(int n, const std::vector<double>& L) {
+  std::vector<double> X(n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    X[j + j * n] = 1.0 / L[j + j * n];
+    for (int i = j + 1; i < n; ++i) {
+      double s = 0.0;
+      for (int k = j; k < i; ++k) s += L[i + k * n] * X[k + j * n];
+      X[i + j * n] = -s / L[i + i * n];
+    }
+  }
+  return X;
+}
+
+template <typename T>
+int upload_model(ssm_bf* bf, const ssm_lg_model* mdl) {
+  ssm_ctx* ctx = bf->ctx;
+  const int D = mdl->Dx, Dy = mdl->Dy;
+  BfModelDev<T> h;
+  std::memset(&h, 0, sizeof h);
+  std::vector<double> LQ(mdl->Q, mdl->Q + D * D), LR(mdl->R, mdl->R + Dy * Dy),
+      L0(mdl->Sigma0, mdl->Sigma0 + D * D);
+  if (!chol_lower(D, LQ)) return set_error(ctx, SSM_ERR_NOT_PD, "Q is not positive definite");
+  if (!chol_lower(Dy, LR)) return set_error(ctx, SSM_ERR_NOT_PD, "R is not positive definite");
+  if (!chol_lower(D, L0))
+    return set_error(ctx, SSM_ERR_NOT_PD, "Sigma0 is not positive definite");
+  const std::vector<double> LRi = tri_inverse(Dy, LR);
+  double logdet = 0.0;
+  for (int q = 0; q < Dy; ++q) logdet += 2.0 * std::log(LR[q + q * Dy]);
+  for (int e = 0; e < D * D; ++e) {
+    h.A[e] = (T)mdl->A[e];
+    h.LQ[e] = (T)LQ[e];
+    h.L0[e] = (T)L0[e];
+  }
+  for (int e = 0; e < D; ++e) {
+    h.b[e] = (T)mdl->b[e];
+    h.mu0[e] = (T)mdl->mu0[e];
+  }
+  for (int e = 0; e < Dy * D; ++e) h.H[e] = (T)mdl->H[e];
+  for (int e = 0; e < Dy; ++e) h.c[e] = (T)mdl->c[e];
+  for (int e = 0; e < Dy * Dy; ++e) h.LRi[e] = (T)LRi[e];
+  h.cst = (T)(-0.5 * ((double)Dy * 1.8378770664093454835606594728112 + logdet));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->model_dev, &h, sizeof h, hipMemcpyHostToDevice,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return SSM_OK;
+}
+
+int check_model(ssm_ctx* ctx, const ssm_lg_model* m) {
+  if (!m || m->Dx < 1 || m->Dx > kMaxD || m->Dy < 1 || m->Dy > kMaxDy)
+    return set_error(ctx, SSM_ERR_ARG, "model dims must satisfy 1<=Dx<=%d, 1<=Dy<=%d", kMaxD,
+                     kMaxDy);
+  if (!m->mu0 || !m->Sigma0 || !m->A || !m->b || !m->Q || !m->H || !m->c || !m->R)
+    return set_error(ctx, SSM_ERR_ARG, "model has a null parameter pointer");
+  return SSM_OK;
+}
+
+template <typename T>
+void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise) {
+  StepParams<T> p;
+  p.N = bf->N;
+  p.Npad = bf->Npad;
+  p.D = bf->D;
+  p.Dy = bf->Dy;
+  p.nblocks = (bf->N + kStepBlock - 1) / kStepBlock;
+  p.xin = (const T*)bf->x[bf->cur];
+  p.xout = (T*)bf->x[bf->cur ^ 1];
+  p.logw = (T*)bf->logw;
+  p.anc = bf->anc;
+  p.model = (const BfModelDev<T>*)bf->model_dev;
+  p.ys = bf->ys;
+  p.sc = bf->sc;
+  p.threshold_n = threshold_n;
+  p.logN = std::log((double)bf->N);
+  p.seed = bf->seed;
+  p.noise = noise;
+  p.partials = bf->partials;
+  p.ll_steps = bf->ll_steps;
+  p.rs_steps = bf->rs_steps;
+  const dim3 g(p.nblocks), b(kStepThreads);
+  if (bf->D == 4 && bf->Dy == 2)
+    hipLaunchKernelGGL((k_bf_step<T, 4, 2>), g, b, 0, s, p);
+  else if (bf->D == 1 && bf->Dy == 1)
+    hipLaunchKernelGGL((k_bf_step<T, 1, 1>), g, b, 0, s, p);
+  else if (bf->D <= 4 && bf->Dy <= 4)
+    hipLaunchKernelGGL((k_bf_step<T, 0, 0>), g, b, 0, s, p);
+  else
+    hipLaunchKernelGGL((k_bf_step<T, 0, 0>), g, b, 0, s, p);
+}
+
+// Enqueues the resampler kernels of one step on stream s.
+int enqueue_resample(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
+                     bool debug_weights) {
+  ResampleArgs a;
+  a.kind = kind;
+  a.n = bf->N;
+  a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
+  a.w = bf->logw;
+  a.u = u_dev;
+  a.seed = bf->seed;
+  a.sc = bf->sc;
+  a.threshold_n = threshold * (double)bf->N;
+  a.anc = bf->anc;
+  a.w_debug64 = debug_weights ? bf->w_dbg : nullptr;
+  resample_bind_scratch(a, bf->rs_scratch, bf->N);
+  return launch_resample(bf->ctx, s, a);
+}
+
+// Enqueues the fused predict/update kernel of one step and flips the state buffers.
+void enqueue_move(ssm_bf* bf, hipStream_t s, double threshold, const void* noise_dev) {
+  const double threshold_n = threshold * (double)bf->N;
+  if (bf->prec == SSM_F32)
+    launch_step_t<float>(bf, s, threshold_n, (const float*)noise_dev);
+  else
+    launch_step_t<double>(bf, s, threshold_n, (const double*)noise_dev);
+  bf->cur ^= 1;
+}
+
+// Enqueues one full step (resample kernels + step kernel) on stream s.
+int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
+                 const void* noise_dev, bool debug_weights) {
+  const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights);
+  if (rc) return rc;
+  enqueue_move(bf, s, threshold, noise_dev);
+  return SSM_OK;
+}
+
+void destroy_graphs(ssm_bf* bf) {
+  for (auto& kv : bf->graphs) (void)hipGraphExecDestroy(kv.second);
+  bf->graphs.clear();
+}
+
+int ensure_steps(ssm_bf* bf, size_t T) {
+  ssm_ctx* ctx = bf->ctx;
+  if (T <= bf->steps_cap) return SSM_OK;
+  destroy_graphs(bf);  // graphs hold the old buffer addresses
+  (void)hipFree(bf->ys);
+  (void)hipFree(bf->ll_steps);
+  (void)hipFree(bf->rs_steps);
+  bf->ys = nullptr;
+  bf->ll_steps = nullptr;
+  bf->rs_steps = nullptr;
+  bf->steps_cap = 0;
+  if (hipMalloc(&bf->ys, sizeof(double) * T * bf->Dy) != hipSuccess ||
+      hipMalloc(&bf->ll_steps, sizeof(double) * T) != hipSuccess ||
+      hipMalloc(&bf->rs_steps, sizeof(int32_t) * T) != hipSuccess)
+    return set_error(ctx, SSM_ERR_OOM, "cannot allocate observation buffers for T=%zu", T);
+  bf->steps_cap = T;
+  return SSM_OK;
+}
+
+double ess_uniform(int N) {
+  // ESS = 1 / sum(w^2) with w = 1/N, accumulated left to right in FP64.
+  const double w = 1.0 / (double)N;
+  double s = 0.0;
+  for (int i = 0; i < N; ++i) s += w * w;
+  return 1.0 / s;
+}
+
+int set_step_base(ssm_bf* bf, int t, int ybase) {
+  const int32_t v[2] = {t, ybase};
+  SSM_HIP_CHECK(bf->ctx, hipMemcpyAsync(&bf->sc->step, v, sizeof v, hipMemcpyHostToDevice,
+                                        bf->ctx->stream));
+  return SSM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_t N,
+                                 int32_t precision, uint64_t seed, ssm_bf** out) {
+  if (!ctx || !out) return SSM_ERR_ARG;
+  *out = nullptr;
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if (N < 1) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_create: N must be >= 1");
+  if (precision != SSM_F32 && precision != SSM_F64)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_create: bad precision %d", precision);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  ssm_bf* bf = new ssm_bf();
+  bf->ctx = ctx;
+  bf->N = N;
+  bf->Npad = (int)(((int64_t)N + kPadTo - 1) / kPadTo * kPadTo);
+  bf->D = model->Dx;
+  bf->Dy = model->Dy;
+  bf->prec = precision;
+  bf->seed = seed;
+  const size_t es = (precision == SSM_F32) ? 4 : 8;
+  const size_t nblocks = ((size_t)N + kStepBlock - 1) / kStepBlock;
+  bool ok = hipMalloc(&bf->x[0], es * bf->Npad * bf->D) == hipSuccess &&
+            hipMalloc(&bf->x[1], es * bf->Npad * bf->D) == hipSuccess &&
+            hipMalloc(&bf->logw, es * bf->Npad) == hipSuccess &&
+            hipMalloc(&bf->anc, sizeof(int32_t) * bf->Npad) == hipSuccess &&
+            hipMalloc(&bf->model_dev, (precision == SSM_F32) ? sizeof(BfModelDev<float>)
+                                                             : sizeof(BfModelDev<double>)) ==
+                hipSuccess &&
+            hipMalloc(&bf->sc, sizeof(BfScalars)) == hipSuccess &&
+            hipMalloc(&bf->partials, sizeof(double) * 5 * nblocks) == hipSuccess &&
+            hipMalloc(&bf->rs_scratch, resample_scratch_bytes(N)) == hipSuccess &&
+            hipMalloc(&bf->w_dbg, sizeof(double) * bf->Npad) == hipSuccess;
+  if (!ok) {
+    ssm_bf_destroy(bf);
+    return set_error(ctx, SSM_ERR_OOM, "ssm_bf_create: allocation failed for N=%d", N);
+  }
+  (void)hipMemset(bf->x[0], 0, es * bf->Npad * bf->D);
+  (void)hipMemset(bf->x[1], 0, es * bf->Npad * bf->D);
+  (void)hipMemset(bf->logw, 0, es * bf->Npad);
+  (void)hipMemset(bf->anc, 0, sizeof(int32_t) * bf->Npad);
+  (void)hipMemset(bf->sc, 0, sizeof(BfScalars));
+  (void)hipMemset(bf->rs_scratch, 0, 256);
+  if (hipEventCreate(&bf->ev0) != hipSuccess || hipEventCreate(&bf->ev1) != hipSuccess) {
+    ssm_bf_destroy(bf);
+    return set_error(ctx, SSM_ERR_HIP, "ssm_bf_create: event creation failed");
+  }
+  rc = (precision == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
+  if (rc) {
+    ssm_bf_destroy(bf);
+    return rc;
+  }
+  rc = ensure_steps(bf, 1);
+  if (rc) {
+    ssm_bf_destroy(bf);
+    return rc;
+  }
+  bf->ess_uniform = ess_uniform(N);
+  *out = bf;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
+  if (!bf) return SSM_ERR_ARG;
+  (void)hipSetDevice(bf->ctx->device);
+  (void)hipStreamSynchronize(bf->ctx->stream);
+  destroy_graphs(bf);
+  void* ptrs[] = {bf->x[0],      bf->x[1],     bf->logw,     bf->anc,   bf->model_dev,
+                  bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
+                  bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (bf->ev0) (void)hipEventDestroy(bf->ev0);
+  if (bf->ev1) (void)hipEventDestroy(bf->ev1);
+  delete bf;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_set_model(ssm_bf* bf, const ssm_lg_model* model) {
+  if (!bf) return SSM_ERR_ARG;
+  int rc = check_model(bf->ctx, model);
+  if (rc) return rc;
+  if (model->Dx != bf->D || model->Dy != bf->Dy)
+    return set_error(bf->ctx, SSM_ERR_ARG, "ssm_bf_set_model: dimensions changed");
+  return (bf->prec == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
+}
+
+template <typename T>
+static int upload_noise(ssm_bf* bf, const double* z) {
+  ssm_ctx* ctx = bf->ctx;
+  if (!bf->noise) {
+    if (hipMalloc(&bf->noise, sizeof(T) * bf->Npad * bf->D) != hipSuccess)
+      return set_error(ctx, SSM_ERR_OOM, "noise buffer");
+  }
+  std::vector<T> h((size_t)bf->Npad * bf->D, (T)0);
+  for (int i = 0; i < bf->N; ++i)
+    for (int d = 0; d < bf->D; ++d) h[(size_t)d * bf->Npad + i] = (T)z[(size_t)i * bf->D + d];
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->noise, h.data(), sizeof(T) * h.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_initialise(ssm_bf* bf, const double* z) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (z) {
+    int rc = (bf->prec == SSM_F32) ? upload_noise<float>(bf, z) : upload_noise<double>(bf, z);
+    if (rc) return rc;
+  }
+  BfScalars h;
+  std::memset(&h, 0, sizeof h);
+  h.lse = 0.0;
+  h.mxn = 0.0;
+  h.s1 = (double)bf->N;
+  h.ess = bf->ess_uniform;
+  h.typeless = 1;
+  h.step = 1;
+  h.ybase = 0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->sc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+  const dim3 g((bf->Npad + kStepThreads - 1) / kStepThreads), b(kStepThreads);
+  bf->cur = 0;
+  if (bf->prec == SSM_F32) {
+    const float* nz = z ? (const float*)bf->noise : nullptr;
+    if (bf->D == 4)
+      hipLaunchKernelGGL((k_bf_init<float, 4>), g, b, 0, ctx->stream, bf->N, bf->Npad, bf->D,
+                         (const BfModelDev<float>*)bf->model_dev, (float*)bf->x[0],
+                         (float*)bf->logw, nz, bf->seed);
+    else
+      hipLaunchKernelGGL((k_bf_init<float, 0>), g, b, 0, ctx->stream, bf->N, bf->Npad, bf->D,
+                         (const BfModelDev<float>*)bf->model_dev, (float*)bf->x[0],
+                         (float*)bf->logw, nz, bf->seed);
+  } else {
+    const double* nz = z ? (const double*)bf->noise : nullptr;
+    hipLaunchKernelGGL((k_bf_init<double, 0>), g, b, 0, ctx->stream, bf->N, bf->Npad, bf->D,
+                       (const BfModelDev<double>*)bf->model_dev, (double*)bf->x[0],
+                       (double*)bf->logw, nz, bf->seed);
+  }
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  bf->initialised = true;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double threshold,
+                               int32_t resampler, const double* u, const double* z,
+                               double* ll_inc, int32_t* did_resample) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_step before initialise");
+  if (!y || t < 1) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_step: bad t or null y");
+  if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_step: unknown resampler %d", resampler);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ys, y, sizeof(double) * bf->Dy, hipMemcpyHostToDevice,
+                                    ctx->stream));
+  rc = set_step_base(bf, t, t - 1);
+  if (rc) return rc;
+  const double* u_dev = nullptr;
+  if (u) {
+    if (!bf->u && hipMalloc(&bf->u, sizeof(double) * bf->Npad) != hipSuccess)
+      return set_error(ctx, SSM_ERR_OOM, "uniform buffer");
+    const size_t nu =
+        (resampler == SSM_SYSTEMATIC || resampler == SSM_SYSTEMATIC_MURRAY) ? 1 : bf->N;
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->u, u, sizeof(double) * nu, hipMemcpyHostToDevice,
+                                      ctx->stream));
+    u_dev = bf->u;
+  }
+  const void* noise_dev = nullptr;
+  if (z) {
+    rc = (bf->prec == SSM_F32) ? upload_noise<float>(bf, z) : upload_noise<double>(bf, z);
+    if (rc) return rc;
+    noise_dev = bf->noise;
+  }
+  rc = enqueue_step(bf, ctx->stream, threshold, resampler, u_dev, noise_dev, true);
+  if (rc) return rc;
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  double ll = 0.0;
+  int32_t did = 0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&ll, bf->ll_steps, sizeof ll, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&did, bf->rs_steps, sizeof did, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ll_inc) *ll_inc = ll;
+  if (did_resample) *did_resample = did;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* ys,
+                              double threshold, int32_t resampler, double* ll_total,
+                              double* ll_steps) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_run before initialise");
+  if (T < 1 || t0 < 1 || !ys) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: bad arguments");
+  if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: unknown resampler %d", resampler);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_steps(bf, (size_t)T);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ys, ys, sizeof(double) * T * bf->Dy,
+                                    hipMemcpyHostToDevice, s));
+  const double zero = 0.0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&bf->sc->ll_total, &zero, sizeof zero,
+                                    hipMemcpyHostToDevice, s));
+  rc = set_step_base(bf, t0, t0 - 1);
+  if (rc) return rc;
+  constexpr int kChunk = 16;  // even: the x ping-pong returns to its start
+  const int nfull = T / kChunk, rem = T % kChunk;
+  hipGraphExec_t exec = nullptr;
+  if (nfull > 0) {
+    const auto key = std::make_tuple(kChunk, bf->cur, resampler, threshold, 0);
+    auto it = bf->graphs.find(key);
+    if (it == bf->graphs.end()) {
+      hipGraph_t g;
+      SSM_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int cur0 = bf->cur;
+      for (int k = 0; k < kChunk && rc == SSM_OK; ++k)
+        rc = enqueue_step(bf, s, threshold, resampler, nullptr, nullptr, false);
+      bf->cur = cur0;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) return rc;
+      if (e != hipSuccess)
+        return set_error(ctx, SSM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess)
+        return set_error(ctx, SSM_ERR_HIP, "graph instantiate failed: %s",
+                         hipGetErrorString(e));
+      bf->graphs[key] = exec;
+    } else {
+      exec = it->second;
+    }
+  }
+  SSM_HIP_CHECK(ctx, hipEventRecord(bf->ev0, s));
+  for (int k = 0; k < nfull; ++k) SSM_HIP_CHECK(ctx, hipGraphLaunch(exec, s));
+  for (int k = 0; k < rem && rc == SSM_OK; ++k)
+    rc = enqueue_step(bf, s, threshold, resampler, nullptr, nullptr, false);
+  if (rc) return rc;
+  SSM_HIP_CHECK(ctx, hipEventRecord(bf->ev1, s));
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  double tot = 0.0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&tot, &bf->sc->ll_total, sizeof tot,
+                                    hipMemcpyDeviceToHost, s));
+  if (ll_steps)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_steps, bf->ll_steps, sizeof(double) * T,
+                                      hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  float ms = 0.f;
+  SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, bf->ev0, bf->ev1));
+  bf->last_ms = ms;
+  bf->last_T = T;
+  if (ll_total) *ll_total = tot;
+  return SSM_OK;
+}
+
+template <typename T>
+static int export_t(ssm_bf* bf, double* x, double* logw, int32_t* anc, double* w) {
+  ssm_ctx* ctx = bf->ctx;
+  hipStream_t s = ctx->stream;
+  BfScalars h;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&h, bf->sc, sizeof h, hipMemcpyDeviceToHost, s));
+  std::vector<T> xb, lb;
+  if (x) {
+    xb.resize((size_t)bf->Npad * bf->D);
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(xb.data(), bf->x[bf->cur], sizeof(T) * xb.size(),
+                                      hipMemcpyDeviceToHost, s));
+  }
+  if (logw) {
+    lb.resize(bf->N);
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(lb.data(), bf->logw, sizeof(T) * bf->N,
+                                      hipMemcpyDeviceToHost, s));
+  }
+  if (anc)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(anc, bf->anc, sizeof(int32_t) * bf->N,
+                                      hipMemcpyDeviceToHost, s));
+  if (w)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(w, bf->w_dbg, sizeof(double) * bf->N,
+                                      hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (x)
+    for (int i = 0; i < bf->N; ++i)
+      for (int d = 0; d < bf->D; ++d)
+        x[(size_t)i * bf->D + d] = (double)xb[(size_t)d * bf->Npad + i];
+  if (logw) {
+    const T lse = (T)h.lse;
+    for (int i = 0; i < bf->N; ++i) logw[i] = h.typeless ? 0.0 : (double)(T)(lb[i] - lse);
+  }
+  if (anc && !h.last_resampled)
+    for (int i = 0; i < bf->N; ++i) anc[i] = i;  // preserve_sample: a_i = i
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_export(ssm_bf* bf, double* x, double* logw, int32_t* ancestors,
+                                 double* w) {
+  if (!bf) return SSM_ERR_ARG;
+  SSM_HIP_CHECK(bf->ctx, hipSetDevice(bf->ctx->device));
+  return (bf->prec == SSM_F32) ? export_t<float>(bf, x, logw, ancestors, w)
+                               : export_t<double>(bf, x, logw, ancestors, w);
+}
+
+template <typename T>
+static int import_t(ssm_bf* bf, const double* x, const double* logw) {
+  ssm_ctx* ctx = bf->ctx;
+  hipStream_t s = ctx->stream;
+  std::vector<T> xb((size_t)bf->Npad * bf->D, (T)0), lb(bf->Npad, (T)0);
+  for (int i = 0; i < bf->N; ++i)
+    for (int d = 0; d < bf->D; ++d) xb[(size_t)d * bf->Npad + i] = (T)x[(size_t)i * bf->D + d];
+  double mx = -INFINITY;
+  for (int i = 0; i < bf->N; ++i) {
+    lb[i] = (T)logw[i];
+    mx = std::max(mx, (double)lb[i]);
+  }
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = 0; i < bf->N; ++i) {
+    const double e = std::exp((double)lb[i] - mx);
+    s1 += e;
+    s2 += e * e;
+  }
+  BfScalars h;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&h, bf->sc, sizeof h, hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  h.lse = 0.0;
+  h.mxn = mx;
+  h.s1 = s1;
+  h.ess = std::min((s1 * s1) / s2, (double)bf->N);
+  h.typeless = 0;
+  h.last_resampled = 0;
+  bf->cur = 0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->x[0], xb.data(), sizeof(T) * xb.size(),
+                                    hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->logw, lb.data(), sizeof(T) * lb.size(),
+                                    hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->sc, &h, sizeof h, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  bf->initialised = true;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw) {
+  if (!bf || !x || !logw) return bf ? set_error(bf->ctx, SSM_ERR_ARG, "null input") : SSM_ERR_ARG;
+  SSM_HIP_CHECK(bf->ctx, hipSetDevice(bf->ctx->device));
+  return (bf->prec == SSM_F32) ? import_t<float>(bf, x, logw) : import_t<double>(bf, x, logw);
+}
+
+extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys,
+                                  double threshold, int32_t resampler, double* ms_resample,
+                                  double* ms_step, double* ll_total) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_profile before initialise");
+  if (T < 1 || t0 < 1 || !ys) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_profile: bad arguments");
+  if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_profile: unknown resampler %d", resampler);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_steps(bf, (size_t)T);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ys, ys, sizeof(double) * T * bf->Dy,
+                                    hipMemcpyHostToDevice, s));
+  const double zero = 0.0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(&bf->sc->ll_total, &zero, sizeof zero,
+                                    hipMemcpyHostToDevice, s));
+  rc = set_step_base(bf, t0, t0 - 1);
+  if (rc) return rc;
+  std::vector<hipEvent_t> ev(3 * (size_t)T, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return set_error(ctx, SSM_ERR_HIP, "event create");
+  for (int k = 0; k < T && rc == SSM_OK; ++k) {
+    (void)hipEventRecord(ev[3 * k], s);
+    rc = enqueue_resample(bf, s, threshold, resampler, nullptr, false);
+    (void)hipEventRecord(ev[3 * k + 1], s);
+    enqueue_move(bf, s, threshold, nullptr);
+    (void)hipEventRecord(ev[3 * k + 2], s);
+  }
+  double tot = 0.0;
+  if (rc == SSM_OK) {
+    (void)hipMemcpyAsync(&tot, &bf->sc->ll_total, sizeof tot, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+  }
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < T && rc == SSM_OK; ++k) {
+    float m1 = 0.f, m2 = 0.f;
+    (void)hipEventElapsedTime(&m1, ev[3 * k], ev[3 * k + 1]);
+    (void)hipEventElapsedTime(&m2, ev[3 * k + 1], ev[3 * k + 2]);
+    a += m1;
+    b += m2;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  if (ms_resample) *ms_resample = a / T;
+  if (ms_step) *ms_step = b / T;
+  if (ll_total) *ll_total = tot;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step) {
+  if (!bf) return SSM_ERR_ARG;
+  if (ms_total) *ms_total = bf->last_ms;
+  if (ms_per_step) *ms_per_step = bf->last_T ? bf->last_ms / bf->last_T : 0.0;
+  return SSM_OK;
+}

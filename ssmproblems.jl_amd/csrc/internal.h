@@ -1,0 +1,97 @@
+// internal.h -- host-side internals shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ssmhip.h"
+
+struct ssm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t hbm_budget = 0;
+  int cu_count = 0;
+  std::string err;
+  // Lazily grown scratch for the standalone resampler entry points.
+  void* rs_scratch = nullptr;
+  size_t rs_scratch_bytes = 0;
+  double* rs_w = nullptr;  // FP64 staging for ssm_sample_ancestors_host
+  double* rs_u = nullptr;
+  int32_t* rs_a = nullptr;
+  size_t rs_host_cap = 0;
+};
+
+namespace ssm {
+
+int set_error(ssm_ctx* ctx, int code, const char* fmt, ...);
+
+#define SSM_HIP_CHECK(ctx, expr)                                                     \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::ssm::set_error((ctx), SSM_ERR_HIP, "%s failed: %s (%s:%d)", #expr,    \
+                              hipGetErrorString(e_), __FILE__, __LINE__);            \
+  } while (0)
+
+// Grows *p to at least `bytes` of device memory (on ctx's device).
+int ensure_device(ssm_ctx* ctx, void** p, size_t* cap, size_t bytes);
+
+// ---- resampler launcher shared by the standalone entry and the particle filters ------
+// Where the resampler reads its weights from.
+enum WeightMode : int {
+  kWeightsF64 = 0,   // explicit FP64 array
+  kWeightsF32 = 1,   // explicit FP32 array
+  kWeightsLogF32 = 2,  // particle-filter log-weights, FP32, normalised via BfScalars
+  kWeightsLogF64 = 3,  // same, FP64
+};
+
+// FP64 scalars of a device-resident particle filter (marginalise! state and the
+// resampling decision), laid out once in device memory per filter.
+struct BfScalars {
+  double lse;        // stored log-weights are l; normalised log-weights are l - lse
+  double mxn;        // max over i of the normalised log-weights
+  double s1;         // sum_i exp(normalised_i - mxn)
+  double ess;        // effective sample size of the normalised weights
+  double ll_total;   // summed log-likelihood increments of the current run
+  double total_w;    // resampler: sum of the weights it consumed
+  int32_t typeless;  // 1 while weights are the lazy TypelessZero of initialise
+  int32_t step;      // next step t (1-based) to execute
+  int32_t ybase;     // observation row of step t is t - 1 - ybase
+  int32_t last_resampled;
+  uint32_t cnt_scan;  // last-block-done counters (reset by the last block)
+  uint32_t cnt_step;
+  uint32_t pad[2];
+};
+
+struct ResampleArgs {
+  int kind = SSM_SYSTEMATIC;
+  int n = 0;
+  int weight_mode = kWeightsF64;
+  const void* w = nullptr;           // weights or log-weights
+  const double* u = nullptr;         // explicit uniforms (device) or null -> Philox
+  uint64_t seed = 0;
+  uint32_t step_host = 0;            // Philox step when `sc` is null
+  BfScalars* sc = nullptr;           // particle-filter scalars, or null (always resample)
+  double threshold_n = 0.0;          // threshold * N (particle-filter decision)
+  int32_t* anc = nullptr;            // output
+  float* w_debug = nullptr;          // optional: weights consumed (FP32 path)
+  double* w_debug64 = nullptr;       // optional: weights consumed (FP64 path)
+  // scratch (device): tile sums, prefixes, counter, W (multinomial)
+  double* tile_sum = nullptr;
+  double* prefix = nullptr;
+  uint32_t* counter = nullptr;       // used when sc == null
+  double* Wbuf = nullptr;
+  double* total = nullptr;           // device scalar: scan total
+};
+
+// Scratch bytes the resampler needs for n weights.
+size_t resample_scratch_bytes(int n);
+// Carves ResampleArgs scratch pointers out of one device block of resample_scratch_bytes(n).
+void resample_bind_scratch(ResampleArgs& a, void* scratch, int n);
+// Enqueues the resampler kernels on `stream`.
+int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a);
+
+}  // namespace ssm

@@ -1,0 +1,17 @@
+"""ssmhip -- MI355X (gfx950) filtering hot path of GeneralisedFilters behind a C-ABI.
+
+The Python modules mirror the reference's host API (models, resamplers, filters) over
+libssmhip.so; the numerics all run in the library's HIP kernels.
+"""
+from ._lib import (SSMError, ArgumentError, PosDefException, AdmissionError, Context,
+                   default_context, load, header_symbols, LIB_PATH)
+from .models import (StateSpaceModel, HomogeneousGaussianPrior,
+                     HomogeneousLinearGaussianLatentDynamics,
+                     HomogeneousLinearGaussianObservationProcess,
+                     create_homogeneous_linear_gaussian_model, PackedLG, HierarchicalSSM)
+from .resamplers import (Systematic, Stratified, Multinomial, ESSResampler, sample_ancestors,
+                         offspring_to_ancestors, ancestors_to_offspring)
+from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, run, filter,
+                      rng_to_seed, last_timing)
+
+__all__ = [n for n in dir() if not n.startswith("_")]
