@@ -53,6 +53,7 @@ typedef struct ssm_ctx ssm_ctx;
 typedef struct ssm_bf ssm_bf;
 typedef struct ssm_rbpf ssm_rbpf;
 typedef struct ssm_server ssm_server;
+typedef struct ssm_kf_batch ssm_kf_batch;
 
 /*
  * Homogeneous linear-Gaussian state-space model, the parameters returned by
@@ -165,6 +166,31 @@ int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
                        int32_t resampler, double* ms_resample, double* ms_step,
                        double* ll_total);
+
+/* ---- Kalman filters, batched (GF/src/algorithms/kalman.jl:16-79) ------------------- */
+/*
+ * B independent linear-Gaussian filters of uneven size (1 <= Dx <= 64, 1 <= Dy <= 32),
+ * each run for T steps from (mu0, Sigma0): filter(model_f, KalmanFilter(), ys_f) for every
+ * f, with the Joseph-form update of kalman_gradient.jl:12-29.  Parameters of filter f are
+ * packed column-major FP64 at params + param_off[f] in the order
+ *     mu0 (Dx), Sigma0 (Dx*Dx), A (Dx*Dx), b (Dx), Q (Dx*Dx), H (Dy*Dx), c (Dy), R (Dy*Dy),
+ * and its observations row by row (T x Dy) at ys + y_off[f].  The data is uploaded once
+ * and stays device-resident; each run streams it through persistent kernels.
+ */
+int32_t ssm_kf_batch_create(ssm_ctx* ctx, int32_t B, const int32_t* Dx, const int32_t* Dy,
+                            const double* params, const int64_t* param_off, int32_t T,
+                            const double* ys, const int64_t* y_off, ssm_kf_batch** out);
+/*
+ * Runs all filters.  ll_out (B): total log-likelihood of each filter (NaN where S lost
+ * positive definiteness; status_out then holds SSM_ERR_NOT_PD and the call returns it).
+ * mu_out / Sigma_out (optional): final filtered means and covariances, packed in filter
+ * order (sum of Dx, sum of Dx*Dx entries; each Sigma column-major).
+ */
+int32_t ssm_kf_batch_run(ssm_kf_batch* kb, double* ll_out, int32_t* status_out, double* mu_out,
+                         double* Sigma_out);
+/* Device time of the last run from HIP events (ms). */
+int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms);
+int32_t ssm_kf_batch_destroy(ssm_kf_batch* kb);
 
 #ifdef __cplusplus
 }

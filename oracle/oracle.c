@@ -561,13 +561,8 @@ int orc_kf_update(int Dx, int Dy, const double* H, const double* c, const double
   gemm(Dy, Dx, Dx, H, 0, Sigma, 0, HS, 0);
   memcpy(S, R, sizeof(double) * nyy);
   gemm(Dy, Dy, Dx, HS, 0, H, 1, S, 1);
-  /* symmetrise S as PDMat(X_A_Xt(...)) would see it */
-  for (int j = 0; j < Dy; ++j)
-    for (int i = j + 1; i < Dy; ++i) {
-      const double v = 0.5 * (S[i + j * Dy] + S[j + i * Dy]);
-      S[i + j * Dy] = v;
-      S[j + i * Dy] = v;
-    }
+  /* X_A_Xt(Sigma, H) is formed as (H L)(H L)' in PDMats, exactly symmetric; only the
+   * lower triangle of S is read by the Cholesky factorisation below. */
   double* L = (double*)malloc(sizeof(double) * nyy);
   memcpy(L, S, sizeof(double) * nyy);
   if (chol_lower(Dy, L)) {
@@ -598,12 +593,9 @@ int orc_kf_update(int Dx, int Dy, const double* H, const double* c, const double
   gemm(Dx, Dx, Dy, KR, 0, K, 1, Sigma, 1);         /* + K R K' */
   if (jitter != 0.0)
     for (int i = 0; i < Dx; ++i) Sigma[i + i * Dx] += jitter;
+  /* PDMat(Symmetric(Sigma_raw)) (kalman.jl:65): Symmetric keeps the upper triangle */
   for (int j = 0; j < Dx; ++j)
-    for (int i = j + 1; i < Dx; ++i) {
-      const double v = 0.5 * (Sigma[i + j * Dx] + Sigma[j + i * Dx]);
-      Sigma[i + j * Dx] = v;
-      Sigma[j + i * Dx] = v;
-    }
+    for (int i = j + 1; i < Dx; ++i) Sigma[i + j * Dx] = Sigma[j + i * Dx];
 done:
   free(zv);
   free(S);

@@ -1,0 +1,574 @@
+// kf.hip -- many independent Kalman filters of uneven size in persistent gfx950 kernels.
+//
+// Reference (paths relative to the reference checkout):
+//   filter      GF/src/GeneralisedFilters.jl:58-84 (initialise, then predict/update per t)
+//   initialise  GF/src/algorithms/kalman.jl:23-26
+//   predict     kalman.jl:42-49          mu = A mu + b,  Sigma = A Sigma A' + Q
+//   update      kalman_gradient.jl:12-29 with kalman.jl:51-66:
+//               z = y - H mu - c;  S = H Sigma H' + R;  K = Sigma H' S^-1;
+//               Sigma = (I - K H) Sigma (I - K H)' + K R K'  (Joseph form), Symmetric(:U);
+//               mu = mu + K z;  ll = logpdf(MvNormal(z, S), 0)
+//
+// Design.  Filters are sorted by cost (Dx^3) and split into three size classes; each class
+// runs one persistent kernel whose workgroups pull filters from an atomic work queue and
+// keep one filter's whole state (Sigma, A, Q, H, R and temporaries) in LDS for all T
+// steps, so HBM carries only the parameters once and the observations.  Dimensions are
+// padded to multiples of 16 and every dense product runs on the matrix cores as
+// v_mfma_f32_16x16x4f32 tiles spread over the workgroup's waves (FP32 in, FP32
+// accumulate).  The innovation covariance S is factorised in FP64 (Cholesky, log-det and
+// the solves for K and S^-1 z) by one wave.
+//   class    Dx      Dy     waves/WG  LDS/WG
+//   small    <= 16   <= 16  1         ~13 KB
+//   mid      <= 32   <= 16  2         ~33 KB
+//   large    <= 64   <= 32  4         ~127 KB
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace ssm {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float opget(const float* M, int ld, bool t, int r, int c) {
+  return t ? M[c * ld + r] : M[r * ld + c];
+}
+
+// C (M x N) = alpha * op(A) op(B) [+ op(A2) op(B2)] [+ Add] [+ I on the first `eye`
+// diagonal entries]; matrices row-major in LDS, all sizes multiples of 16.  Output 16x16
+// tiles are dealt round-robin to the NW waves; each tile accumulates K/4 MFMAs.
+template <int NW>
+__device__ __forceinline__ void gemm16(float* C, int ldc, int M, int N, float alpha,
+                                       const float* A, int lda, bool tA, const float* B,
+                                       int ldb, bool tB, int K, const float* A2, int lda2,
+                                       bool tA2, const float* B2, int ldb2, bool tB2, int K2,
+                                       const float* Add, int ldadd, int eye) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tn = N >> 4, ntiles = (M >> 4) * tn;
+  for (int tile = wave; tile < ntiles; tile += NW) {
+    const int r0 = (tile / tn) << 4, c0 = (tile % tn) << 4;
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const float a = opget(A, lda, tA, r0 + li, k0 + lk);
+      const float b = opget(B, ldb, tB, k0 + lk, c0 + li);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+    if (alpha != 1.f) acc *= alpha;
+    if (A2) {
+      for (int k0 = 0; k0 < K2; k0 += 4) {
+        const float a = opget(A2, lda2, tA2, r0 + li, k0 + lk);
+        const float b = opget(B2, ldb2, tB2, k0 + lk, c0 + li);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + 4 * lk + r, j = c0 + li;
+      float v = acc[r];
+      if (Add) v += Add[i * ldadd + j];
+      if (i == j && i < eye) v += 1.f;
+      C[i * ldc + j] = v;
+    }
+  }
+}
+
+struct KfBatchParams {
+  int T;
+  int count;                      // filters in this class
+  const int32_t* __restrict__ order;  // filter ids of this class, costliest first
+  uint32_t* counter;              // work-queue head (zeroed before launch)
+  const int32_t* __restrict__ Dx;
+  const int32_t* __restrict__ Dy;
+  const double* __restrict__ params;
+  const int64_t* __restrict__ param_off;
+  const double* __restrict__ ys;
+  const int64_t* __restrict__ y_off;
+  double* __restrict__ ll_out;
+  int32_t* __restrict__ status;
+  double* __restrict__ mu_out;     // optional final state (packed, offsets below)
+  double* __restrict__ Sigma_out;
+  const int64_t* __restrict__ mu_off;
+  const int64_t* __restrict__ cov_off;
+};
+
+constexpr double kLog2Pi = 1.8378770664093454835606594728112;
+
+template <int DXP, int DYP>
+struct KfLds {
+  static constexpr int LX = DXP + 1, LY = DYP + 1;
+  static constexpr int kScal = 5 * DXP * LX + 3 * DYP * LX + DYP * LY + DXP * LY + 3 * DXP +
+                               3 * DYP;  // float offset of the scalar block
+  static_assert(kScal % 2 == 0, "scalar block must be 8-byte aligned");
+  static constexpr int kFloats = kScal + 16;
+  static constexpr size_t kBytes = ((size_t)kFloats * 4 + 7) / 8 * 8 + (size_t)DYP * DYP * 8;
+};
+
+template <int DXP, int DYP, int NW>
+__global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
+  using L = KfLds<DXP, DYP>;
+  constexpr int LX = L::LX, LY = L::LY;
+  extern __shared__ __align__(16) float lds[];
+  float* Sig = lds;
+  float* Am = Sig + DXP * LX;
+  float* Qm = Am + DXP * LX;
+  float* T1 = Qm + DXP * LX;
+  float* Mm = T1 + DXP * LX;
+  float* Hm = Mm + DXP * LX;
+  float* Pm = Hm + DYP * LX;
+  float* KT = Pm + DYP * LX;
+  float* Rm = KT + DYP * LX;
+  float* KR = Rm + DYP * LY;
+  float* mu = KR + DXP * LY;
+  float* muh = mu + DXP;
+  float* bv = muh + DXP;
+  float* cv = bv + DXP;
+  float* zv = cv + DYP;
+  float* yv = zv + DYP;
+  float* scal = yv + DYP;  // [0] quad, [1] logdet, [2] ok
+  int* iscal = reinterpret_cast<int*>(scal + 8);
+  double* Sd = reinterpret_cast<double*>(lds + ((L::kFloats + 1) & ~1));
+
+  const int tid = threadIdx.x, nthr = 64 * NW;
+  const int lane = tid & 63, wave = tid >> 6;
+  while (true) {
+    if (tid == 0) iscal[0] = (int)atomicAdd(p.counter, 1u);
+    __syncthreads();
+    const int qi = iscal[0];
+    if (qi >= p.count) break;  // every wave of the workgroup reads the same index
+    const int f = p.order[qi];
+    const int dx = p.Dx[f], dy = p.Dy[f];
+    const int dxp = (dx + 15) & ~15, dyp = (dy + 15) & ~15;
+    // ---- load parameters (column-major FP64) into zero-padded row-major FP32 LDS ----
+    const double* g = p.params + p.param_off[f];
+    const double* g_mu0 = g;
+    const double* g_S0 = g_mu0 + dx;
+    const double* g_A = g_S0 + dx * dx;
+    const double* g_b = g_A + dx * dx;
+    const double* g_Q = g_b + dx;
+    const double* g_H = g_Q + dx * dx;
+    const double* g_c = g_H + dy * dx;
+    const double* g_R = g_c + dy;
+    for (int e = tid; e < dxp * dxp; e += nthr) {
+      const int i = e / dxp, j = e % dxp;
+      const bool in = i < dx && j < dx;
+      Sig[i * LX + j] = in ? (float)g_S0[i + j * dx] : 0.f;
+      Am[i * LX + j] = in ? (float)g_A[i + j * dx] : 0.f;
+      Qm[i * LX + j] = in ? (float)g_Q[i + j * dx] : 0.f;
+    }
+    for (int e = tid; e < dyp * dxp; e += nthr) {
+      const int i = e / dxp, j = e % dxp;
+      Hm[i * LX + j] = (i < dy && j < dx) ? (float)g_H[i + j * dy] : 0.f;
+    }
+    for (int e = tid; e < dyp * dyp; e += nthr) {
+      const int i = e / dyp, j = e % dyp;
+      Rm[i * LY + j] = (i < dy && j < dy) ? (float)g_R[i + j * dy] : 0.f;
+    }
+    for (int e = tid; e < dxp; e += nthr) {
+      mu[e] = e < dx ? (float)g_mu0[e] : 0.f;
+      bv[e] = e < dx ? (float)g_b[e] : 0.f;
+    }
+    for (int e = tid; e < dyp; e += nthr) cv[e] = e < dy ? (float)g_c[e] : 0.f;
+    __syncthreads();
+    double ll = 0.0;
+    int ok = 1;
+    const double* yrow = p.ys + p.y_off[f];
+    for (int t = 0; t < p.T; ++t) {
+      // P1: mu_hat = A mu + b ; T1 = A Sigma
+      for (int i = tid; i < dxp; i += nthr) {
+        float s = bv[i];
+        for (int k = 0; k < dx; ++k) s += Am[i * LX + k] * mu[k];
+        muh[i] = (i < dx) ? s : 0.f;
+      }
+      for (int q = tid; q < dyp; q += nthr) yv[q] = q < dy ? (float)yrow[(size_t)t * dy + q] : 0.f;
+      gemm16<NW>(T1, LX, dxp, dxp, 1.f, Am, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
+                 nullptr, 0, false, 0, nullptr, 0, 0);
+      __syncthreads();
+      // P2: Sigma = T1 A' + Q ; z = y - H mu_hat - c
+      gemm16<NW>(Sig, LX, dxp, dxp, 1.f, T1, LX, false, Am, LX, true, dxp, nullptr, 0, false,
+                 nullptr, 0, false, 0, Qm, LX, 0);
+      for (int q = tid; q < dyp; q += nthr) {
+        float s = cv[q];
+        for (int k = 0; k < dx; ++k) s += Hm[q * LX + k] * muh[k];
+        zv[q] = (q < dy) ? yv[q] - s : 0.f;
+      }
+      __syncthreads();
+      // P3: P = H Sigma (dyp x dxp)
+      gemm16<NW>(Pm, LX, dyp, dxp, 1.f, Hm, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
+                 nullptr, 0, false, 0, nullptr, 0, 0);
+      __syncthreads();
+      // P4: S = P H' + R (into KR as scratch, dyp x dyp)
+      gemm16<NW>(KR, LY, dyp, dyp, 1.f, Pm, LX, false, Hm, LX, true, dxp, nullptr, 0, false,
+                 nullptr, 0, false, 0, Rm, LY, 0);
+      __syncthreads();
+      // P5 (wave 0): FP64 Cholesky of S, log-det, K' = S^-1 P, S^-1 z.
+      if (wave == 0) {
+        for (int e = lane; e < dy * dy; e += 64) {
+          const int i = e / dy, j = e % dy;
+          Sd[i * DYP + j] = (double)KR[i * LY + j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int good = 1;
+        double logdet = 0.0;
+        for (int j = 0; j < dy; ++j) {
+          double d = Sd[j * DYP + j];
+          if (!(d > 0.0)) {
+            good = 0;
+            break;
+          }
+          d = sqrt(d);
+          logdet += 2.0 * log(d);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (lane == 0) Sd[j * DYP + j] = d;
+          for (int i = j + 1 + lane; i < dy; i += 64) Sd[i * DYP + j] /= d;
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          for (int i = j + 1 + lane; i < dy; i += 64) {
+            const double lij = Sd[i * DYP + j];
+            for (int k = j + 1; k <= i; ++k) Sd[i * DYP + k] -= lij * Sd[k * DYP + j];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+        if (good) {
+          // columns 0..dx-1 of P give K' = S^-1 P; column dx is z (forward solve only).
+          for (int col = lane; col <= dx; col += 64) {
+            double x[DYP];
+#pragma unroll
+            for (int q = 0; q < DYP; ++q)
+              x[q] = (q < dy) ? (double)((col < dx) ? Pm[q * LX + col] : zv[q]) : 0.0;
+#pragma unroll
+            for (int q = 0; q < DYP; ++q) {
+              if (q < dy) {
+                double s = x[q];
+#pragma unroll
+                for (int k = 0; k < q; ++k) s -= Sd[q * DYP + k] * x[k];
+                x[q] = s / Sd[q * DYP + q];
+              }
+            }
+            if (col == dx) {
+              double qd = 0.0;
+#pragma unroll
+              for (int q = 0; q < DYP; ++q)
+                if (q < dy) qd += x[q] * x[q];
+              reinterpret_cast<double*>(scal)[2] = qd;  // scal[4..5]
+            } else {
+#pragma unroll
+              for (int q = DYP - 1; q >= 0; --q) {
+                if (q < dy) {
+                  double s = x[q];
+#pragma unroll
+                  for (int k = q + 1; k < DYP; ++k)
+                    if (k < dy) s -= Sd[k * DYP + q] * x[k];
+                  x[q] = s / Sd[q * DYP + q];
+                }
+              }
+#pragma unroll
+              for (int q = 0; q < DYP; ++q)
+                if (q < dyp) KT[q * LX + col] = (q < dy) ? (float)x[q] : 0.f;
+            }
+          }
+          for (int e = lane; e < dyp * (dxp - dx); e += 64) {
+            const int q = e / (dxp - dx), col = dx + e % (dxp - dx);
+            KT[q * LX + col] = 0.f;
+          }
+        }
+        if (lane == 0) {
+          iscal[1] = good;
+          reinterpret_cast<double*>(scal)[3] = logdet;  // scal[6..7]
+        }
+      }
+      __syncthreads();
+      if (!iscal[1]) {
+        ok = 0;
+        break;
+      }
+      if (tid == 0) {
+        const double qd = reinterpret_cast<double*>(scal)[2];
+        const double ld = reinterpret_cast<double*>(scal)[3];
+        ll += -0.5 * ((double)dy * kLog2Pi + ld + qd);
+      }
+      // P6: mu = mu_hat + K z ; M = I - K H ; KR = K R
+      for (int i = tid; i < dxp; i += nthr) {
+        float s = muh[i];
+        for (int q = 0; q < dy; ++q) s += KT[q * LX + i] * zv[q];
+        mu[i] = (i < dx) ? s : 0.f;
+      }
+      gemm16<NW>(Mm, LX, dxp, dxp, -1.f, KT, LX, true, Hm, LX, false, dyp, nullptr, 0, false,
+                 nullptr, 0, false, 0, nullptr, 0, dx);
+      gemm16<NW>(KR, LY, dxp, dyp, 1.f, KT, LX, true, Rm, LY, false, dyp, nullptr, 0, false,
+                 nullptr, 0, false, 0, nullptr, 0, 0);
+      __syncthreads();
+      // P7: T1 = M Sigma
+      gemm16<NW>(T1, LX, dxp, dxp, 1.f, Mm, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
+                 nullptr, 0, false, 0, nullptr, 0, 0);
+      __syncthreads();
+      // P8: Sigma = T1 M' + (K R) K'
+      gemm16<NW>(Sig, LX, dxp, dxp, 1.f, T1, LX, false, Mm, LX, true, dxp, KR, LY, false, KT,
+                 LX, false, dyp, nullptr, 0, 0);
+      __syncthreads();
+      // P9: Symmetric(Sigma) keeps the upper triangle
+      for (int e = tid; e < dxp * dxp; e += nthr) {
+        const int i = e / dxp, j = e % dxp;
+        if (i > j) Sig[i * LX + j] = Sig[j * LX + i];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      p.ll_out[f] = ok ? ll : NAN;
+      p.status[f] = ok ? SSM_OK : SSM_ERR_NOT_PD;
+    }
+    if (p.mu_out && ok) {
+      const int64_t mo = p.mu_off[f], co = p.cov_off[f];
+      for (int i = tid; i < dx; i += nthr) p.mu_out[mo + i] = mu[i];
+      for (int e = tid; e < dx * dx; e += nthr) {
+        const int i = e % dx, j = e / dx;  // column-major output
+        p.Sigma_out[co + e] = Sig[i * LX + j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace ssm
+using namespace ssm;
+
+namespace {
+
+constexpr int kClasses = 3;
+struct ClassCfg {
+  int dx_max, dy_max;
+};
+constexpr ClassCfg kCfg[kClasses] = {{16, 16}, {32, 16}, {64, 32}};
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+template <int DXP, int DYP, int NW>
+int launch_class(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int grid_cap) {
+  const size_t lds = KfLds<DXP, DYP>::kBytes;
+  auto fn = k_kf_batch<DXP, DYP, NW>;
+  SSM_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)fn,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int per_cu = 0;
+  SSM_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * NW, lds));
+  if (per_cu < 1) per_cu = 1;
+  int grid = per_cu * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+  if (grid_cap > 0) grid = std::min(grid, grid_cap);
+  grid = std::max(1, std::min(grid, p.count));
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * NW), lds, s, p);
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  return SSM_OK;
+}
+
+}  // namespace
+
+struct ssm_kf_batch {
+  ssm_ctx* ctx = nullptr;
+  int B = 0, T = 0;
+  int32_t* dDx = nullptr;
+  int32_t* dDy = nullptr;
+  double* dparams = nullptr;
+  int64_t* dpoff = nullptr;
+  double* dys = nullptr;
+  int64_t* dyoff = nullptr;
+  double* dll = nullptr;
+  int32_t* dstatus = nullptr;
+  double* dmu = nullptr;
+  double* dSig = nullptr;
+  int64_t* dmuoff = nullptr;
+  int64_t* dcovoff = nullptr;
+  int32_t* dorder = nullptr;
+  uint32_t* dcounter = nullptr;
+  int count[kClasses] = {0, 0, 0};
+  int start[kClasses] = {0, 0, 0};
+  int64_t mu_len = 0, cov_len = 0;
+  hipStream_t streams[kClasses] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kClasses] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+};
+
+static void kf_free(ssm_kf_batch* kb) {
+  void* ptrs[] = {kb->dDx,    kb->dDy,     kb->dparams, kb->dpoff,  kb->dys,
+                  kb->dyoff,  kb->dll,     kb->dstatus, kb->dmu,    kb->dSig,
+                  kb->dmuoff, kb->dcovoff, kb->dorder,  kb->dcounter};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int c = 0; c < kClasses; ++c) {
+    if (kb->streams[c]) (void)hipStreamDestroy(kb->streams[c]);
+    if (kb->join[c]) (void)hipEventDestroy(kb->join[c]);
+  }
+  if (kb->fork) (void)hipEventDestroy(kb->fork);
+  if (kb->ev0) (void)hipEventDestroy(kb->ev0);
+  if (kb->ev1) (void)hipEventDestroy(kb->ev1);
+}
+
+template <typename X>
+static bool dup(X** d, const X* h, size_t n) {
+  if (hipMalloc(d, sizeof(X) * (n ? n : 1)) != hipSuccess) return false;
+  return n == 0 || hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice) == hipSuccess;
+}
+
+extern "C" int32_t ssm_kf_batch_create(ssm_ctx* ctx, int32_t B, const int32_t* Dx,
+                                       const int32_t* Dy, const double* params,
+                                       const int64_t* param_off, int32_t T, const double* ys,
+                                       const int64_t* y_off, ssm_kf_batch** out) {
+  if (!ctx || !out) return SSM_ERR_ARG;
+  *out = nullptr;
+  if (B < 1 || T < 1 || !Dx || !Dy || !params || !param_off || !ys || !y_off)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_kf_batch_create: bad arguments");
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int64_t plen = 0, ylen = 0;
+  std::vector<int64_t> muoff(B), covoff(B);
+  std::vector<std::vector<int32_t>> cls(kClasses);
+  int64_t mu_len = 0, cov_len = 0;
+  for (int f = 0; f < B; ++f) {
+    const int dx = Dx[f], dy = Dy[f];
+    int c = -1;
+    for (int k = 0; k < kClasses && c < 0; ++k)
+      if (dx >= 1 && dy >= 1 && dx <= kCfg[k].dx_max && dy <= kCfg[k].dy_max) c = k;
+    if (c < 0)
+      return set_error(ctx, SSM_ERR_ARG,
+                       "filter %d: dims Dx=%d Dy=%d outside 1..64 / 1..32", f, dx, dy);
+    if (param_off[f] < 0 || y_off[f] < 0)
+      return set_error(ctx, SSM_ERR_ARG, "filter %d: negative offset", f);
+    const int64_t psz = 3LL * dx * dx + 2LL * dx + (int64_t)dy * dx + dy + (int64_t)dy * dy;
+    plen = std::max(plen, param_off[f] + psz);
+    ylen = std::max(ylen, y_off[f] + (int64_t)T * dy);
+    cls[c].push_back(f);
+    muoff[f] = mu_len;
+    covoff[f] = cov_len;
+    mu_len += dx;
+    cov_len += (int64_t)dx * dx;
+  }
+  // costliest first within each class (largest-processing-time order for the queue)
+  std::vector<int32_t> order;
+  ssm_kf_batch* kb = new ssm_kf_batch();
+  kb->ctx = ctx;
+  kb->B = B;
+  kb->T = T;
+  for (int c = 0; c < kClasses; ++c) {
+    std::stable_sort(cls[c].begin(), cls[c].end(), [&](int a, int b) {
+      const double ca = (double)Dx[a] * Dx[a] * Dx[a] + (double)Dy[a] * Dy[a] * Dy[a];
+      const double cb = (double)Dx[b] * Dx[b] * Dx[b] + (double)Dy[b] * Dy[b] * Dy[b];
+      return ca > cb;
+    });
+    kb->start[c] = (int)order.size();
+    kb->count[c] = (int)cls[c].size();
+    order.insert(order.end(), cls[c].begin(), cls[c].end());
+  }
+  kb->mu_len = mu_len;
+  kb->cov_len = cov_len;
+  bool ok = dup(&kb->dDx, Dx, B) && dup(&kb->dDy, Dy, B) && dup(&kb->dparams, params, plen) &&
+            dup(&kb->dpoff, param_off, B) && dup(&kb->dys, ys, ylen) &&
+            dup(&kb->dyoff, y_off, B) && dup(&kb->dmuoff, muoff.data(), B) &&
+            dup(&kb->dcovoff, covoff.data(), B) && dup(&kb->dorder, order.data(), B) &&
+            hipMalloc(&kb->dll, sizeof(double) * B) == hipSuccess &&
+            hipMalloc(&kb->dstatus, sizeof(int32_t) * B) == hipSuccess &&
+            hipMalloc(&kb->dmu, sizeof(double) * std::max<int64_t>(1, mu_len)) == hipSuccess &&
+            hipMalloc(&kb->dSig, sizeof(double) * std::max<int64_t>(1, cov_len)) == hipSuccess &&
+            hipMalloc(&kb->dcounter, sizeof(uint32_t) * 64) == hipSuccess;
+  for (int c = 0; c < kClasses && ok; ++c)
+    ok = hipStreamCreateWithFlags(&kb->streams[c], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&kb->join[c], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&kb->fork, hipEventDisableTiming) == hipSuccess &&
+       hipEventCreate(&kb->ev0) == hipSuccess && hipEventCreate(&kb->ev1) == hipSuccess;
+  if (!ok) {
+    kf_free(kb);
+    delete kb;
+    return set_error(ctx, SSM_ERR_OOM, "ssm_kf_batch_create: allocation failed (B=%d)", B);
+  }
+  *out = kb;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_batch_run(ssm_kf_batch* kb, double* ll_out, int32_t* status_out,
+                                    double* mu_out, double* Sigma_out) {
+  if (!kb) return SSM_ERR_ARG;
+  ssm_ctx* ctx = kb->ctx;
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const bool want_state = mu_out != nullptr || Sigma_out != nullptr;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, kb->dcounter, 64);
+  SSM_HIP_CHECK(ctx, hipEventRecord(kb->ev0, s));
+  SSM_HIP_CHECK(ctx, hipEventRecord(kb->fork, s));
+  // largest class first: it holds most of the work
+  for (int c = kClasses - 1; c >= 0; --c) {
+    if (kb->count[c] == 0) continue;
+    hipStream_t cs = kb->streams[c];
+    SSM_HIP_CHECK(ctx, hipStreamWaitEvent(cs, kb->fork, 0));
+    KfBatchParams p;
+    p.T = kb->T;
+    p.count = kb->count[c];
+    p.order = kb->dorder + kb->start[c];
+    p.counter = kb->dcounter + 16 * c;
+    p.Dx = kb->dDx;
+    p.Dy = kb->dDy;
+    p.params = kb->dparams;
+    p.param_off = kb->dpoff;
+    p.ys = kb->dys;
+    p.y_off = kb->dyoff;
+    p.ll_out = kb->dll;
+    p.status = kb->dstatus;
+    p.mu_out = want_state ? kb->dmu : nullptr;
+    p.Sigma_out = want_state ? kb->dSig : nullptr;
+    p.mu_off = kb->dmuoff;
+    p.cov_off = kb->dcovoff;
+    int rc;
+    if (c == 0)
+      rc = launch_class<16, 16, 1>(ctx, cs, p, 0);
+    else if (c == 1)
+      rc = launch_class<32, 16, 2>(ctx, cs, p, 0);
+    else
+      rc = launch_class<64, 32, 4>(ctx, cs, p, 0);
+    if (rc) return rc;
+    SSM_HIP_CHECK(ctx, hipEventRecord(kb->join[c], cs));
+    SSM_HIP_CHECK(ctx, hipStreamWaitEvent(s, kb->join[c], 0));
+  }
+  SSM_HIP_CHECK(ctx, hipEventRecord(kb->ev1, s));
+  if (ll_out)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_out, kb->dll, sizeof(double) * kb->B,
+                                      hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> st(kb->B);
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(st.data(), kb->dstatus, sizeof(int32_t) * kb->B,
+                                    hipMemcpyDeviceToHost, s));
+  if (mu_out)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(mu_out, kb->dmu, sizeof(double) * kb->mu_len,
+                                      hipMemcpyDeviceToHost, s));
+  if (Sigma_out)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(Sigma_out, kb->dSig, sizeof(double) * kb->cov_len,
+                                      hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  float ms = 0.f;
+  SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, kb->ev0, kb->ev1));
+  kb->last_ms = ms;
+  if (status_out) std::memcpy(status_out, st.data(), sizeof(int32_t) * kb->B);
+  for (int f = 0; f < kb->B; ++f)
+    if (st[f] != SSM_OK)
+      return set_error(ctx, SSM_ERR_NOT_PD,
+                       "filter %d: innovation covariance S is not positive definite", f);
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms) {
+  if (!kb) return SSM_ERR_ARG;
+  if (ms) *ms = kb->last_ms;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_batch_destroy(ssm_kf_batch* kb) {
+  if (!kb) return SSM_ERR_ARG;
+  (void)hipSetDevice(kb->ctx->device);
+  (void)hipStreamSynchronize(kb->ctx->stream);
+  kf_free(kb);
+  delete kb;
+  return SSM_OK;
+}

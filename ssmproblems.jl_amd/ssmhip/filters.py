@@ -181,4 +181,9 @@ def filter(rng, model: StateSpaceModel, algo, observations, callback=None, ctx=N
             state, inc = step(rng, model, algo, t, state, ys[t - 1], callback=callback)
             ll += inc
         return state, ll
+    from .kalman import KalmanFilter, kalman_filter
+    if isinstance(algo, KalmanFilter):
+        if not model.is_homogeneous:
+            raise TypeError("the device KalmanFilter path needs a homogeneous LG model")
+        return kalman_filter(model, observations, ctx=ctx)
     raise TypeError(f"unsupported filter {type(algo).__name__}")

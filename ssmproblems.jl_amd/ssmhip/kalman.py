@@ -1,0 +1,142 @@
+"""KalmanFilter on the device, single and batched (GF/src/algorithms/kalman.jl:16-79).
+
+``filter(rng, model, KalmanFilter(), ys)`` (through ``ssmhip.filter``) runs one filter;
+``KFBatch`` holds many filters of uneven size resident on the device and runs them all in
+one call, returning each filter's log-likelihood (and optionally its final mean and
+covariance).  Parameters come from the unchanged calc_* protocol of each model.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import default_context
+from .models import StateSpaceModel, lg_params
+
+
+@dataclass
+class KalmanFilter:
+    """KalmanFilter(; jitter=nothing) -- kalman.jl:16-21 (jitter is not supported on the
+    device path; the reference default is nothing)."""
+    jitter: float = None
+
+
+KF = KalmanFilter
+
+
+def pack_lg(mu0, Sigma0, A, b, Q, H, c, R):
+    """One filter's parameters, column-major FP64, in the C-ABI packing order."""
+    A = np.atleast_2d(np.asarray(A, dtype=np.float64))
+    H = np.atleast_2d(np.asarray(H, dtype=np.float64))
+    Dx, Dy = A.shape[0], H.shape[0]
+    f = lambda M, r, c_: np.asarray(M, dtype=np.float64).reshape(r, c_).flatten(order="F")
+    return np.concatenate([np.asarray(mu0, dtype=np.float64).reshape(Dx), f(Sigma0, Dx, Dx),
+                           f(A, Dx, Dx), np.asarray(b, dtype=np.float64).reshape(Dx),
+                           f(Q, Dx, Dx), f(H, Dy, Dx), np.asarray(c, dtype=np.float64).reshape(Dy),
+                           f(R, Dy, Dy)]), Dx, Dy
+
+
+class KFBatch:
+    """B device-resident Kalman filters; params: list of (mu0, Sigma0, A, b, Q, H, c, R),
+    ys: list of (T x Dy_f) observation arrays (same T for all)."""
+
+    def __init__(self, params, ys, ctx=None):
+        self.ctx = ctx or default_context()
+        packs = [pack_lg(*p) for p in params]
+        self.B = len(packs)
+        self.Dx = np.array([p[1] for p in packs], dtype=np.int32)
+        self.Dy = np.array([p[2] for p in packs], dtype=np.int32)
+        sizes = np.array([p[0].size for p in packs], dtype=np.int64)
+        self.param_off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.params = np.ascontiguousarray(np.concatenate([p[0] for p in packs]))
+        ys = [np.asarray(y, dtype=np.float64).reshape(-1, int(d)) for y, d in zip(ys, self.Dy)]
+        self.T = ys[0].shape[0]
+        if any(y.shape[0] != self.T for y in ys):
+            raise ValueError("all filters of a batch share T")
+        ysz = np.array([y.size for y in ys], dtype=np.int64)
+        self.y_off = np.concatenate([[0], np.cumsum(ysz)[:-1]]).astype(np.int64)
+        self.ys = np.ascontiguousarray(np.concatenate([y.reshape(-1) for y in ys]))
+        self._create()
+
+    @classmethod
+    def from_packed(cls, Dx, Dy, params, param_off, ys, y_off, T, ctx=None):
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        self.B = len(Dx)
+        self.Dx = np.ascontiguousarray(Dx, dtype=np.int32)
+        self.Dy = np.ascontiguousarray(Dy, dtype=np.int32)
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        self.param_off = np.ascontiguousarray(param_off, dtype=np.int64)
+        self.ys = np.ascontiguousarray(ys, dtype=np.float64)
+        self.y_off = np.ascontiguousarray(y_off, dtype=np.int64)
+        self.T = int(T)
+        self._create()
+        return self
+
+    def _create(self):
+        p = C.c_void_p()
+        i64 = C.POINTER(C.c_int64)
+        self.ctx.check(self.ctx.lib.ssm_kf_batch_create(
+            self.ctx.ptr, self.B, self.Dx.ctypes.data_as(_lib._ip),
+            self.Dy.ctypes.data_as(_lib._ip), self.params.ctypes.data_as(_lib._dp),
+            self.param_off.ctypes.data_as(i64), self.T, self.ys.ctypes.data_as(_lib._dp),
+            self.y_off.ctypes.data_as(i64), C.byref(p)))
+        self.ptr = p
+
+    def run(self, return_state=False):
+        ll = np.zeros(self.B)
+        st = np.zeros(self.B, dtype=np.int32)
+        mu = Sig = None
+        if return_state:
+            mu = np.zeros(int(self.Dx.sum()))
+            Sig = np.zeros(int((self.Dx.astype(np.int64) ** 2).sum()))
+        self.ctx.check(self.ctx.lib.ssm_kf_batch_run(
+            self.ptr, ll.ctypes.data_as(_lib._dp), st.ctypes.data_as(_lib._ip),
+            mu.ctypes.data_as(_lib._dp) if mu is not None else None,
+            Sig.ctypes.data_as(_lib._dp) if Sig is not None else None))
+        if not return_state:
+            return ll
+        mus, sigs, mo, co = [], [], 0, 0
+        for dx in self.Dx:
+            dx = int(dx)
+            mus.append(mu[mo:mo + dx])
+            sigs.append(Sig[co:co + dx * dx].reshape(dx, dx, order="F"))
+            mo += dx
+            co += dx * dx
+        return ll, mus, sigs
+
+    def last_ms(self):
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.ssm_kf_batch_last_timing(self.ptr, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.ctx.lib.ssm_kf_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class MvNormalState:
+    """The filtered distribution MvNormal(mu, Sigma) returned by filter(..., KF(), ...)."""
+    mu: np.ndarray
+    Sigma: np.ndarray
+
+
+def kalman_filter(model: StateSpaceModel, observations, ctx=None):
+    """filter(model, KalmanFilter(), ys) for a homogeneous LG model on the device."""
+    ys = np.asarray(observations, dtype=np.float64)
+    p = lg_params(model, 1)
+    kb = KFBatch([p], [ys.reshape(ys.shape[0], -1)], ctx=ctx)
+    ll, mus, sigs = kb.run(return_state=True)
+    kb.close()
+    return MvNormalState(mus[0], sigs[0]), float(ll[0])
