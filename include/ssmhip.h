@@ -167,6 +167,44 @@ int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, doub
                        int32_t resampler, double* ms_resample, double* ms_step,
                        double* ll_total);
 
+/* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */
+/*
+ * Hierarchical model (GF/src/models/hierarchical.jl:4-45) with a scalar outer state and a
+ * D-dim linear-Gaussian inner state given the outer path:
+ *   outer  x0 ~ N(om0, os0),  x_t = oa x_{t-1} + ob + N(0, oq)
+ *   inner  z0 ~ N(mu0, Sigma0), z_t = A z_{t-1} + b + C x_{t-1} + N(0, Q)
+ *          (calc_b(dyn; prev_outer) = b + C * prev_outer)
+ *   obs    y_t = H z_t + c + N(0, exp(rs * x_t) R0)   (calc_R(obs; new_outer))
+ * D <= 32, Dy <= 16; matrices column-major FP64 (C is a D-vector).
+ */
+typedef struct {
+  double om0, os0, oa, ob, oq;
+  int32_t D, Dy;
+  double rs;
+  const double *mu0, *Sigma0, *A, *b, *C, *Q, *H, *c, *R0;
+} ssm_rb_model;
+
+/* RBPF(BF(N; threshold, resampler), KalmanFilter()) state: N particles each carrying the
+ * outer state and an inner (mu, Sigma) in FP32 on the device; `seed` keys Philox. */
+int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int32_t N, uint64_t seed,
+                        ssm_rbpf** out);
+int32_t ssm_rbpf_destroy(ssm_rbpf* rb);
+/* initialise (rbpf.jl:16-22); z: N outer normals or NULL for Philox. */
+int32_t ssm_rbpf_initialise(ssm_rbpf* rb, const double* z);
+/* One step (particles.jl:147-162 with rbpf.jl:24-84); u / z override the uniforms and the
+ * outer noise (parity mode).  Returns SSM_ERR_NOT_PD if an inner S fails to factorise. */
+int32_t ssm_rbpf_step(ssm_rbpf* rb, int32_t t, const double* y, double threshold,
+                      int32_t resampler, const double* u, const double* z, double* ll_inc,
+                      int32_t* did_resample);
+/* Steps t0 .. t0+T-1 with device-side resampling decisions; ys is T x Dy row-major. */
+int32_t ssm_rbpf_run(ssm_rbpf* rb, int32_t t0, int32_t T, const double* ys, double threshold,
+                     int32_t resampler, double* ll_total, double* ll_steps);
+/* State export: xo (N), mu (N x D), Sigma (N x D x D, each column-major), normalised logw
+ * (N), ancestors of the last step (N), resample weights of the last step-mode resample. */
+int32_t ssm_rbpf_export(ssm_rbpf* rb, double* xo, double* mu, double* Sigma, double* logw,
+                        int32_t* ancestors, double* w);
+int32_t ssm_rbpf_last_timing(ssm_rbpf* rb, double* ms);
+
 /* ---- Kalman filters, batched (GF/src/algorithms/kalman.jl:16-79) ------------------- */
 /*
  * B independent linear-Gaussian filters of uneven size (1 <= Dx <= 64, 1 <= Dy <= 32),

@@ -60,6 +60,12 @@ class LGModelC(C.Structure):
         (n, _dp) for n in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")]
 
 
+class RBModelC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("om0", "os0", "oa", "ob", "oq")] + [
+        ("D", C.c_int32), ("Dy", C.c_int32), ("rs", C.c_double)] + [
+        (n, _dp) for n in ("mu0", "Sigma0", "A", "b", "C", "Q", "H", "c", "R0")]
+
+
 _lib = None
 
 
@@ -116,6 +122,13 @@ def _declare(L):
         "ssm_kf_batch_run": ([_vp, _dp, _ip, _dp, _dp], i32),
         "ssm_kf_batch_last_timing": ([_vp, _dp], i32),
         "ssm_kf_batch_destroy": ([_vp], i32),
+        "ssm_rbpf_create": ([_vp, C.POINTER(RBModelC), i32, u64, C.POINTER(_vp)], i32),
+        "ssm_rbpf_destroy": ([_vp], i32),
+        "ssm_rbpf_initialise": ([_vp, _dp], i32),
+        "ssm_rbpf_step": ([_vp, i32, _dp, dbl, i32, _dp, _dp, _dp, _ip], i32),
+        "ssm_rbpf_run": ([_vp, i32, i32, _dp, dbl, i32, _dp, _dp], i32),
+        "ssm_rbpf_export": ([_vp, _dp, _dp, _dp, _dp, _ip, _dp], i32),
+        "ssm_rbpf_last_timing": ([_vp, _dp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
