@@ -147,17 +147,23 @@ def main():
     # Per-kernel timing with HIP events (plain launches, same kernels and stream).
     import ctypes as C
     ctx.check(lib.ssm_bf_initialise(st.ptr, None))
-    ms_rs, ms_st, llp = C.c_double(), C.c_double(), C.c_double()
+    ms = (C.c_double * 3)()
+    llp = C.c_double()
     TP = min(args.profile_steps, args.T)
     yp = np.ascontiguousarray(ys[:TP])
     ctx.check(lib.ssm_bf_profile(st.ptr, 1, TP, yp.ctypes.data_as(C.POINTER(C.c_double)),
-                                 args.threshold, kind, C.byref(ms_rs), C.byref(ms_st),
-                                 C.byref(llp)))
-    step_bytes = 40.0 * args.N        # per launch: anc/x/logw streams, see DESIGN.md
-    resample_bytes = 12.0 * args.N
-    achieved = step_bytes / (ms_st.value * 1e-3) / 1e9
-    log(f"step kernel {ms_st.value * 1e3:.2f} us ({achieved:.0f} GB/s); resample kernels "
-        f"{ms_rs.value * 1e3:.2f} us ({resample_bytes / (ms_rs.value * 1e-3) / 1e9:.0f} GB/s)")
+                                 args.threshold, kind, ms, C.byref(llp)))
+    ms_rs, ms_st, ms_fin = ms[0], ms[1], ms[2]
+    # Algorithmic bytes per particle and launch: step kernel 40 B
+    # (ancestor 4 + gathered state 16 + new state 16 + log-weight 4 when resampling;
+    # state 16 + log-weight 4 read and 16 + 4 written otherwise); resampler 8 B
+    # (log-weight read 4, ancestor write 4).
+    step_bytes = 40.0 * args.N
+    resample_bytes = 8.0 * args.N
+    achieved = step_bytes / (ms_st * 1e-3) / 1e9
+    log(f"step kernel {ms_st * 1e3:.2f} us ({achieved:.0f} GB/s); resampler "
+        f"{ms_rs * 1e3:.2f} us ({resample_bytes / (ms_rs * 1e-3) / 1e9:.0f} GB/s); "
+        f"finish {ms_fin * 1e3:.2f} us")
 
     cpu = None
     if not args.no_cpu:
@@ -195,8 +201,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_bf_step<float,4,2>", "bytes_per_launch": step_bytes,
-                     "us_per_launch": ms_st.value * 1e3,
-                     "resample_us_per_step": ms_rs.value * 1e3},
+                     "us_per_launch": ms_st * 1e3,
+                     "resample_us_per_step": ms_rs * 1e3, "finish_us_per_step": ms_fin * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -159,13 +159,12 @@ int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw);
 /* Device time of the last ssm_bf_run, from HIP events around it (ms). */
 int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
 /*
- * Same steps as ssm_bf_run, launched one kernel sequence at a time with HIP events around
- * the resampler kernels and the fused step kernel, for per-kernel timing.  ms_resample /
- * ms_step receive the mean duration per step (ms); ll_total as in ssm_bf_run.
+ * Same steps as ssm_bf_run, launched one kernel at a time with HIP events around each:
+ * ms_kernels[0] resampler, [1] fused predict/update step kernel, [2] finish kernel, each
+ * the mean duration per step (ms); ll_total as in ssm_bf_run.
  */
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
-                       int32_t resampler, double* ms_resample, double* ms_step,
-                       double* ll_total);
+                       int32_t resampler, double* ms_kernels, double* ll_total);
 
 /* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */
 /*

@@ -14,13 +14,15 @@
 // of marginalise! is l[i] - lse, applied when the next step reads them, so a step streams
 // each array exactly once.
 //
-// Per step: (1) when the device-side ESS rule says so, the resampler kernels turn the
+// Per step: (1) when the device-side ESS rule says so, the resampler kernel turns the
 // log-weights into ancestors; (2) k_bf_step gathers x[a[i]] (or reads x[i]), propagates
 // with in-register Philox noise, evaluates the observation log-density, writes x' and the
-// new log-weights, and reduces (max, sum exp, sum exp^2) per block; the last block to
-// finish folds the block partials (FP64, fixed order) into the log-likelihood increment,
-// the normaliser and the ESS that decides the next step's resampling.  ssm_bf_run captures
-// chunks of steps in HIP graphs, so T steps run without host round trips.
+// new log-weights, and leaves one weight partial per tile of 1024 particles; (3) a one-block
+// finish kernel folds the partials (FP64, fixed order) into the log-likelihood increment,
+// the normaliser and the ESS that decides the next step's resampling, and prepares the
+// resampler's tile prefixes (pf_common.h).  Particles are lane-striped within a tile, so
+// every access of a wave is one contiguous run.  ssm_bf_run captures chunks of steps in
+// HIP graphs, so T steps run without host round trips.
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -29,15 +31,16 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "pf_common.h"
 
 namespace ssm {
 
 constexpr int kMaxD = 16;
 constexpr int kMaxDy = 16;
-constexpr int kStepThreads = 256;
-constexpr int kStepItems = 4;
-constexpr int kStepBlock = kStepThreads * kStepItems;
-constexpr int kPadTo = 2048;  // multiple of the step block and of the resampler tile
+constexpr int kStepThreads = kTileThreads;
+constexpr int kStepItems = kTileItems;
+constexpr int kStepBlock = kTile;
+constexpr int kPadTo = kTile;
 
 // Per-filter model constants, precomputed on the host in FP64 and stored in `T`.
 // Column-major with leading dimension D (or Dy).
@@ -69,40 +72,7 @@ struct StepParams {
   uint64_t seed;
   const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
   double* __restrict__ partials;
-  double* __restrict__ ll_steps;
-  int32_t* __restrict__ rs_steps;
 };
-
-template <typename T>
-__device__ __forceinline__ void ld4(const T* p, T v[4]);
-template <>
-__device__ __forceinline__ void ld4<float>(const float* p, float v[4]) {
-  const float4 q = *reinterpret_cast<const float4*>(p);
-  v[0] = q.x;
-  v[1] = q.y;
-  v[2] = q.z;
-  v[3] = q.w;
-}
-template <>
-__device__ __forceinline__ void ld4<double>(const double* p, double v[4]) {
-  const double2 a = *reinterpret_cast<const double2*>(p);
-  const double2 b = *reinterpret_cast<const double2*>(p + 2);
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
-}
-template <typename T>
-__device__ __forceinline__ void st4(T* p, const T v[4]);
-template <>
-__device__ __forceinline__ void st4<float>(float* p, const float v[4]) {
-  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
-}
-template <>
-__device__ __forceinline__ void st4<double>(double* p, const double v[4]) {
-  *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
-  *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
-}
 
 // Standard normals z[0..D) of particle i at step t from the given stream.
 template <typename T, int MD>
@@ -162,44 +132,19 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
 #pragma unroll
   for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
 
-  const int i0 = (blockIdx.x * kStepThreads + threadIdx.x) * kStepItems;
-  T x[kStepItems][MD];
-  if (rs) {
-    const int4 a4 = *reinterpret_cast<const int4*>(p.anc + i0);
-    const int av[4] = {a4.x, a4.y, a4.z, a4.w};
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) {
-      const int src = (i0 + k < p.N) ? av[k] : i0 + k;
-#pragma unroll
-      for (int d = 0; d < MD; ++d)
-        if (d < D) x[k][d] = p.xin[(size_t)d * p.Npad + src];
-    }
-  } else {
-#pragma unroll
-    for (int d = 0; d < MD; ++d) {
-      if (d < D) {
-        T v[4];
-        ld4<T>(p.xin + (size_t)d * p.Npad + i0, v);
-#pragma unroll
-        for (int k = 0; k < kStepItems; ++k) x[k][d] = v[k];
-      }
-    }
-  }
-  T nl[kStepItems];
-  if (fresh) {
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) nl[k] = (T)0;
-  } else {
-    ld4<T>(p.logw + i0, nl);
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) nl[k] = nl[k] - lse;
-  }
-
-  T xo[MD][kStepItems];
-  T lnew[kStepItems];
+  const int tile0 = blockIdx.x * kTile;
+  T nl[kStepItems], lnew[kStepItems];
+  bool valid[kStepItems];
 #pragma unroll
   for (int k = 0; k < kStepItems; ++k) {
-    const int i = i0 + k;
+    const int i = tile0 + k * kStepThreads + threadIdx.x;
+    valid[k] = i < p.N;
+    const int src = (rs && valid[k]) ? p.anc[i] : i;
+    T x[MD];
+#pragma unroll
+    for (int d = 0; d < MD; ++d)
+      if (d < D) x[d] = p.xin[(size_t)d * p.Npad + src];
+    nl[k] = fresh ? (T)0 : p.logw[i] - lse;
     T z[MD];
     if (p.noise) {
 #pragma unroll
@@ -216,11 +161,11 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
         T s = m.b[r];
 #pragma unroll
         for (int c = 0; c < MD; ++c)
-          if (c < D) s += m.A[r + c * D] * x[k][c];
+          if (c < D) s += m.A[r + c * D] * x[c];
 #pragma unroll
         for (int c = 0; c <= r; ++c) s += m.LQ[r + c * D] * z[c];
         xn[r] = s;
-        xo[r][k] = s;
+        p.xout[(size_t)r * p.Npad + i] = s;
       }
     }
     // update: log N(y; H x' + c, R) = cst - |L_R^-1 (y - H x' - c)|^2 / 2
@@ -246,125 +191,9 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
       }
     }
     lnew[k] = nl[k] + (m.cst - (T)0.5 * quad);
+    p.logw[i] = lnew[k];
   }
-#pragma unroll
-  for (int d = 0; d < MD; ++d)
-    if (d < D) st4<T>(p.xout + (size_t)d * p.Npad + i0, xo[d]);
-  st4<T>(p.logw + i0, lnew);
-
-  // Thread partials: acc0 over the pre-update normalised log-weights (predict baseline,
-  // particles.jl:71-73), acc1 over the updated ones (marginalise!, containers.jl:162).
-  LseAcc<T> a0 = lse_empty<T>(), a1 = lse_empty<T>();
-  {
-    T m0 = (T)-INFINITY, m1 = (T)-INFINITY;
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) {
-      if (i0 + k < p.N) {
-        m0 = nl[k] > m0 ? nl[k] : m0;
-        m1 = lnew[k] > m1 ? lnew[k] : m1;
-      }
-    }
-    T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) {
-      if (i0 + k < p.N) {
-        if (m0 != (T)-INFINITY) s0 += exp_t(nl[k] - m0);
-        if (m1 != (T)-INFINITY) {
-          const T e = exp_t(lnew[k] - m1);
-          s1 += e;
-          s2 += e * e;
-        }
-      }
-    }
-    a0 = LseAcc<T>{m0, s0, (T)0};
-    a1 = LseAcc<T>{m1, s1, s2};
-  }
-  a0 = wave_lse(a0);
-  a1 = wave_lse(a1);
-  __shared__ LseAcc<double> w0[kStepThreads / kWave], w1[kStepThreads / kWave];
-  __shared__ int is_last;
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) {
-    w0[wv] = LseAcc<double>{(double)a0.m, (double)a0.s, 0.0};
-    w1[wv] = LseAcc<double>{(double)a1.m, (double)a1.s, (double)a1.s2};
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    LseAcc<double> b0 = w0[0], b1 = w1[0];
-#pragma unroll
-    for (int k = 1; k < kStepThreads / kWave; ++k) {
-      b0 = lse_combine(b0, w0[k]);
-      b1 = lse_combine(b1, w1[k]);
-    }
-    double* P = p.partials + (size_t)blockIdx.x * 5;
-    P[0] = b0.m;
-    P[1] = b0.s;
-    P[2] = b1.m;
-    P[3] = b1.s;
-    P[4] = b1.s2;
-    __threadfence();
-    is_last = (atomicAdd(&sc->cnt_step, 1u) == (uint32_t)(p.nblocks - 1));
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-
-  // Last block: fold the block partials in FP64 (fixed order) and finish the step.
-  LseAcc<double> c0 = lse_empty<double>(), c1 = lse_empty<double>();
-  for (int b = threadIdx.x; b < p.nblocks; b += kStepThreads) {
-    const double* P = p.partials + (size_t)b * 5;
-    c0 = lse_combine(c0, LseAcc<double>{__builtin_nontemporal_load(P + 0),
-                                        __builtin_nontemporal_load(P + 1), 0.0});
-    c1 = lse_combine(c1, LseAcc<double>{__builtin_nontemporal_load(P + 2),
-                                        __builtin_nontemporal_load(P + 3),
-                                        __builtin_nontemporal_load(P + 4)});
-  }
-  c0 = wave_lse(c0);
-  c1 = wave_lse(c1);
-  __syncthreads();
-  if (lane == 0) {
-    w0[wv] = c0;
-    w1[wv] = c1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    LseAcc<double> b0 = w0[0], b1 = w1[0];
-#pragma unroll
-    for (int k = 1; k < kStepThreads / kWave; ++k) {
-      b0 = lse_combine(b0, w0[k]);
-      b1 = lse_combine(b1, w1[k]);
-    }
-    // baseline: logsumexp of the pre-update log-weights (+ ll_baseline = 0); log N for
-    // freshly resampled (all-zero) or TypelessBaseline weights.
-    const double baseline = fresh ? p.logN : b0.m + log(b0.s);
-    const int idx = t - 1 - sc->ybase;
-    if (b1.m == -INFINITY) {
-      // Every particle has zero weight: the increment is -Inf; weights restart uniform.
-      if (p.ll_steps) p.ll_steps[idx] = -INFINITY;
-      sc->ll_total += -INFINITY;
-      sc->lse = 0.0;
-      sc->mxn = 0.0;
-      sc->s1 = (double)p.N;
-      sc->ess = (double)p.N;
-      sc->typeless = 1;
-    } else {
-      const double lse_after = b1.m + log(b1.s);
-      const double ll = lse_after - baseline;
-      double ess = (b1.s * b1.s) / b1.s2;
-      if (!(ess <= (double)p.N)) ess = (double)p.N;  // ESS <= N exactly; clamp rounding
-      sc->lse = lse_after;
-      sc->mxn = (double)((T)b1.m - (T)lse_after);
-      sc->s1 = b1.s;
-      sc->ess = ess;
-      sc->ll_total += ll;
-      sc->typeless = 0;
-      if (p.ll_steps) p.ll_steps[idx] = ll;
-    }
-    if (p.rs_steps) p.rs_steps[idx] = rs ? 1 : 0;
-    sc->last_resampled = rs ? 1 : 0;
-    sc->step = t + 1;
-    sc->cnt_step = 0u;
-  }
+  tile_partial<T>(nl, lnew, valid, fresh, p.partials + (size_t)blockIdx.x * kPartial);
 }
 
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).
@@ -507,6 +336,12 @@ int check_model(ssm_ctx* ctx, const ssm_lg_model* m) {
   return SSM_OK;
 }
 
+TileScratch tile_scratch(ssm_bf* bf) {
+  ResampleArgs a;
+  resample_bind_scratch(a, bf->rs_scratch, bf->N);
+  return TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix};
+}
+
 template <typename T>
 void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise) {
   StepParams<T> p;
@@ -526,16 +361,13 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   p.logN = std::log((double)bf->N);
   p.seed = bf->seed;
   p.noise = noise;
-  p.partials = bf->partials;
-  p.ll_steps = bf->ll_steps;
-  p.rs_steps = bf->rs_steps;
+  const TileScratch ts = tile_scratch(bf);
+  p.partials = ts.partials;
   const dim3 g(p.nblocks), b(kStepThreads);
   if (bf->D == 4 && bf->Dy == 2)
     hipLaunchKernelGGL((k_bf_step<T, 4, 2>), g, b, 0, s, p);
   else if (bf->D == 1 && bf->Dy == 1)
     hipLaunchKernelGGL((k_bf_step<T, 1, 1>), g, b, 0, s, p);
-  else if (bf->D <= 4 && bf->Dy <= 4)
-    hipLaunchKernelGGL((k_bf_step<T, 0, 0>), g, b, 0, s, p);
   else
     hipLaunchKernelGGL((k_bf_step<T, 0, 0>), g, b, 0, s, p);
 }
@@ -568,12 +400,28 @@ void enqueue_move(ssm_bf* bf, hipStream_t s, double threshold, const void* noise
   bf->cur ^= 1;
 }
 
-// Enqueues one full step (resample kernels + step kernel) on stream s.
+// Enqueues the one-block finish kernel (record = 1: a filter step; 0: re-derive the
+// weight normalisers of imported log-weights without recording a step).
+void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
+  const int nb = (bf->N + kTile - 1) / kTile;
+  const double threshold_n = threshold * (double)bf->N;
+  const double logN = std::log((double)bf->N);
+  const TileScratch ts = tile_scratch(bf);
+  if (bf->prec == SSM_F32)
+    hipLaunchKernelGGL(k_pf_finish<float>, dim3(1), dim3(256), 0, s, bf->N, nb, bf->sc, bf->sc,
+                       threshold_n, logN, ts, bf->ll_steps, bf->rs_steps, record);
+  else
+    hipLaunchKernelGGL(k_pf_finish<double>, dim3(1), dim3(256), 0, s, bf->N, nb, bf->sc, bf->sc,
+                       threshold_n, logN, ts, bf->ll_steps, bf->rs_steps, record);
+}
+
+// Enqueues one full step (resample kernel, step kernel, finish kernel) on stream s.
 int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
                  const void* noise_dev, bool debug_weights) {
   const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights);
   if (rc) return rc;
   enqueue_move(bf, s, threshold, noise_dev);
+  enqueue_finish(bf, s, threshold, 1);
   return SSM_OK;
 }
 
@@ -637,7 +485,6 @@ extern "C" int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_
   bf->prec = precision;
   bf->seed = seed;
   const size_t es = (precision == SSM_F32) ? 4 : 8;
-  const size_t nblocks = ((size_t)N + kStepBlock - 1) / kStepBlock;
   bool ok = hipMalloc(&bf->x[0], es * bf->Npad * bf->D) == hipSuccess &&
             hipMalloc(&bf->x[1], es * bf->Npad * bf->D) == hipSuccess &&
             hipMalloc(&bf->logw, es * bf->Npad) == hipSuccess &&
@@ -646,7 +493,6 @@ extern "C" int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_
                                                              : sizeof(BfModelDev<double>)) ==
                 hipSuccess &&
             hipMalloc(&bf->sc, sizeof(BfScalars)) == hipSuccess &&
-            hipMalloc(&bf->partials, sizeof(double) * 5 * nblocks) == hipSuccess &&
             hipMalloc(&bf->rs_scratch, resample_scratch_bytes(N)) == hipSuccess &&
             hipMalloc(&bf->w_dbg, sizeof(double) * bf->Npad) == hipSuccess;
   if (!ok) {
@@ -658,7 +504,7 @@ extern "C" int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_
   (void)hipMemset(bf->logw, 0, es * bf->Npad);
   (void)hipMemset(bf->anc, 0, sizeof(int32_t) * bf->Npad);
   (void)hipMemset(bf->sc, 0, sizeof(BfScalars));
-  (void)hipMemset(bf->rs_scratch, 0, 256);
+  (void)hipMemset(bf->rs_scratch, 0, resample_scratch_bytes(N));
   if (hipEventCreate(&bf->ev0) != hipSuccess || hipEventCreate(&bf->ev1) != hipSuccess) {
     ssm_bf_destroy(bf);
     return set_error(ctx, SSM_ERR_HIP, "ssm_bf_create: event creation failed");
@@ -954,6 +800,13 @@ static int import_t(ssm_bf* bf, const double* x, const double* logw) {
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->logw, lb.data(), sizeof(T) * lb.size(),
                                     hipMemcpyHostToDevice, s));
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->sc, &h, sizeof h, hipMemcpyHostToDevice, s));
+  // tile partials of the imported log-weights, then the normalisers the resampler uses
+  const int nb = (bf->N + kTile - 1) / kTile;
+  const TileScratch ts = tile_scratch(bf);
+  hipLaunchKernelGGL(k_tile_partials<T>, dim3(nb), dim3(kTileThreads), 0, s, bf->N,
+                     (const T*)bf->logw, (const T*)nullptr, bf->sc, -1.0, ts.partials);
+  enqueue_finish(bf, s, 0.0, 0);
+  SSM_HIP_CHECK(ctx, hipGetLastError());
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
   bf->initialised = true;
   return SSM_OK;
@@ -966,8 +819,8 @@ extern "C" int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw
 }
 
 extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys,
-                                  double threshold, int32_t resampler, double* ms_resample,
-                                  double* ms_step, double* ll_total) {
+                                  double threshold, int32_t resampler, double* ms_kernels,
+                                  double* ll_total) {
   if (!bf) return SSM_ERR_ARG;
   ssm_ctx* ctx = bf->ctx;
   if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_profile before initialise");
@@ -985,34 +838,35 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
                                     hipMemcpyHostToDevice, s));
   rc = set_step_base(bf, t0, t0 - 1);
   if (rc) return rc;
-  std::vector<hipEvent_t> ev(3 * (size_t)T, nullptr);
+  std::vector<hipEvent_t> ev(4 * (size_t)T, nullptr);
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) return set_error(ctx, SSM_ERR_HIP, "event create");
   for (int k = 0; k < T && rc == SSM_OK; ++k) {
-    (void)hipEventRecord(ev[3 * k], s);
+    (void)hipEventRecord(ev[4 * k], s);
     rc = enqueue_resample(bf, s, threshold, resampler, nullptr, false);
-    (void)hipEventRecord(ev[3 * k + 1], s);
+    (void)hipEventRecord(ev[4 * k + 1], s);
     enqueue_move(bf, s, threshold, nullptr);
-    (void)hipEventRecord(ev[3 * k + 2], s);
+    (void)hipEventRecord(ev[4 * k + 2], s);
+    enqueue_finish(bf, s, threshold, 1);
+    (void)hipEventRecord(ev[4 * k + 3], s);
   }
   double tot = 0.0;
   if (rc == SSM_OK) {
     (void)hipMemcpyAsync(&tot, &bf->sc->ll_total, sizeof tot, hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
   }
-  double a = 0.0, b = 0.0;
-  for (int k = 0; k < T && rc == SSM_OK; ++k) {
-    float m1 = 0.f, m2 = 0.f;
-    (void)hipEventElapsedTime(&m1, ev[3 * k], ev[3 * k + 1]);
-    (void)hipEventElapsedTime(&m2, ev[3 * k + 1], ev[3 * k + 2]);
-    a += m1;
-    b += m2;
-  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < T && rc == SSM_OK; ++k)
+    for (int j = 0; j < 3; ++j) {
+      float m = 0.f;
+      (void)hipEventElapsedTime(&m, ev[4 * k + j], ev[4 * k + j + 1]);
+      acc[j] += m;
+    }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc) return rc;
   SSM_HIP_CHECK(ctx, hipGetLastError());
-  if (ms_resample) *ms_resample = a / T;
-  if (ms_step) *ms_step = b / T;
+  if (ms_kernels)
+    for (int j = 0; j < 3; ++j) ms_kernels[j] = acc[j] / T;
   if (ll_total) *ll_total = tot;
   return SSM_OK;
 }

@@ -52,8 +52,8 @@ enum WeightMode : int {
 // resampling decision), laid out once in device memory per filter.
 struct BfScalars {
   double lse;        // stored log-weights are l; normalised log-weights are l - lse
-  double mxn;        // max over i of the normalised log-weights
-  double s1;         // sum_i exp(normalised_i - mxn)
+  double mxn;        // max over i of the normalised log-weights (informational)
+  double s1;         // sum_i exp(l_i - max l) (informational)
   double ess;        // effective sample size of the normalised weights
   double ll_total;   // summed log-likelihood increments of the current run
   double total_w;    // resampler: sum of the weights it consumed
@@ -61,9 +61,7 @@ struct BfScalars {
   int32_t step;      // next step t (1-based) to execute
   int32_t ybase;     // observation row of step t is t - 1 - ybase
   int32_t last_resampled;
-  uint32_t cnt_scan;  // last-block-done counters (reset by the last block)
-  uint32_t cnt_step;
-  uint32_t pad[2];
+  uint32_t reserved[4];
 };
 
 struct ResampleArgs {
@@ -77,14 +75,13 @@ struct ResampleArgs {
   BfScalars* sc = nullptr;           // particle-filter scalars, or null (always resample)
   double threshold_n = 0.0;          // threshold * N (particle-filter decision)
   int32_t* anc = nullptr;            // output
-  float* w_debug = nullptr;          // optional: weights consumed (FP32 path)
-  double* w_debug64 = nullptr;       // optional: weights consumed (FP64 path)
-  // scratch (device): tile sums, prefixes, counter, W (multinomial)
-  double* tile_sum = nullptr;
+  double* w_debug64 = nullptr;       // optional: weights consumed
+  // tile scratch (device); particle filters pass their own, filled by their finish kernel
+  double* partials = nullptr;
+  double* blk_m = nullptr;
+  double* blk_c = nullptr;
   double* prefix = nullptr;
-  uint32_t* counter = nullptr;       // used when sc == null
-  double* Wbuf = nullptr;
-  double* total = nullptr;           // device scalar: scan total
+  double* Wbuf = nullptr;            // multinomial prefix sums (n)
 };
 
 // Scratch bytes the resampler needs for n weights.

@@ -28,53 +28,9 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "mfma_team.h"
 
 namespace ssm {
-
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ float opget(const float* M, int ld, bool t, int r, int c) {
-  return t ? M[c * ld + r] : M[r * ld + c];
-}
-
-// C (M x N) = alpha * op(A) op(B) [+ op(A2) op(B2)] [+ Add] [+ I on the first `eye`
-// diagonal entries]; matrices row-major in LDS, all sizes multiples of 16.  Output 16x16
-// tiles are dealt round-robin to the NW waves; each tile accumulates K/4 MFMAs.
-template <int NW>
-__device__ __forceinline__ void gemm16(float* C, int ldc, int M, int N, float alpha,
-                                       const float* A, int lda, bool tA, const float* B,
-                                       int ldb, bool tB, int K, const float* A2, int lda2,
-                                       bool tA2, const float* B2, int ldb2, bool tB2, int K2,
-                                       const float* Add, int ldadd, int eye) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int li = lane & 15, lk = lane >> 4;
-  const int tn = N >> 4, ntiles = (M >> 4) * tn;
-  for (int tile = wave; tile < ntiles; tile += NW) {
-    const int r0 = (tile / tn) << 4, c0 = (tile % tn) << 4;
-    f4v acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const float a = opget(A, lda, tA, r0 + li, k0 + lk);
-      const float b = opget(B, ldb, tB, k0 + lk, c0 + li);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-    }
-    if (alpha != 1.f) acc *= alpha;
-    if (A2) {
-      for (int k0 = 0; k0 < K2; k0 += 4) {
-        const float a = opget(A2, lda2, tA2, r0 + li, k0 + lk);
-        const float b = opget(B2, ldb2, tB2, k0 + lk, c0 + li);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = r0 + 4 * lk + r, j = c0 + li;
-      float v = acc[r];
-      if (Add) v += Add[i * ldadd + j];
-      if (i == j && i < eye) v += 1.f;
-      C[i * ldc + j] = v;
-    }
-  }
-}
 
 struct KfBatchParams {
   int T;

@@ -1,106 +1,289 @@
-// pf_common.h -- the end of a particle-filter step shared by the filter kernels: reduce the
-// (max, sum exp, sum exp^2) partials of the pre-update and updated log-weights across the
-// block, hand them to the last block to finish, and let that block fold all partials (FP64,
-// fixed order) into the log-likelihood increment, the normaliser of the stored
-// log-weights and the ESS that decides the next step's resampling.
-//   predict baseline  GF/src/algorithms/particles.jl:71-73 (logsumexp(logw) + ll_baseline)
-//   marginalise!      GF/src/containers.jl:160-173 (ll = LSE_after - baseline)
-//   ESS               GF/src/resamplers.jl:179-183 (1 / sum(w^2))
+// pf_common.h -- tiles, per-tile weight partials and the one-block finish kernel shared by
+// the particle filters and the resampler.
+//
+// Particles are processed in tiles of kTile = 1024 (256 threads x 4, lane-striped: item k
+// of thread t is particle tile0 + 256 k + t, so every load and store of a wave is one
+// contiguous 256-byte run).  At the end of a filter step every tile b leaves a partial:
+//   m0_b, s0_b  max and sum exp(nl - m0) of the pre-update normalised log-weights
+//               (the predict baseline, GF/src/algorithms/particles.jl:71-73);
+//   m1_b        max of the updated log-weights l (in the filter's type T);
+//   tau_b       sum_i e_i in FP64 with e_i = exp(l_i - m1_b) evaluated in T;
+//   s2_b        sum_i e_i^2 in FP64.
+// The finish kernel (one block) folds them in a fixed order into
+//   lse = M + log S (M = max m1_b, S = sum tau_b exp(m1_b - M)),  ll = lse - baseline
+//   (marginalise!, GF/src/containers.jl:160-173),  ESS = S^2 / sum s2_b exp(2(m1_b - M))
+//   (GF/src/resamplers.jl:179-183),
+// and the resampling weights of the next step: w_i = e_i * c_b with c_b = exp(m1_b - M)/S
+// (softmax of the log-weights, containers.jl:140-141), whose tile sums c_b tau_b it scans
+// into tile prefixes.  The resampler recomputes e_i from (l_i, m1_b) with the same FP32
+// operations, so the weights it scans are exactly the ones these sums describe.
 #pragma once
 #include "device_common.h"
 #include "internal.h"
 
 namespace ssm {
 
-// Call with every thread of the block.  a0: partial over the pre-update normalised
-// log-weights; a1: over the updated ones (threads without particles pass lse_empty()).
-// `fresh` means the pre-update weights were all zero (just resampled, or TypelessZero).
-template <typename T, int NTHR>
-__device__ void pf_step_finish(LseAcc<T> a0, LseAcc<T> a1, double* partials, int nblocks,
-                               BfScalars* sc, bool fresh, bool rs, int t, double logN, int N,
-                               double* ll_steps, int32_t* rs_steps) {
-  constexpr int NW = NTHR / kWave;
-  __shared__ LseAcc<double> w0[NW], w1[NW];
-  __shared__ int is_last;
+constexpr int kTileThreads = 256;
+constexpr int kTileItems = 4;
+constexpr int kTile = kTileThreads * kTileItems;
+constexpr int kPartial = 6;  // doubles per tile partial
+
+// Per-filter device scratch for tiles (all sized by the tile count nb).
+struct TileScratch {
+  double* partials;  // nb * kPartial
+  double* blk_m;     // nb: m1_b (exactly representable in T)
+  double* blk_c;     // nb: c_b
+  double* prefix;    // nb + 1: exclusive tile prefixes of the weights, prefix[nb] = total
+};
+
+__device__ __forceinline__ double block_sum_fixed(double v, double* lds) {
+  // wave butterfly, then the waves in index order; result broadcast to every thread
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  __syncthreads();
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) r += lds[k];
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* lds) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const T o = wave_shfl_xor(v, m);
+    v = o > v ? o : v;
+  }
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  __syncthreads();
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  T r = (T)-INFINITY;
+  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) r = lds[k] > r ? lds[k] : r;
+  return r;
+}
+
+// Accurate exponential of the weights (the resampler recomputes it bit for bit).
+__device__ __forceinline__ float exp_w(float v) { return expf(v); }
+__device__ __forceinline__ double exp_w(double v) { return exp(v); }
+
+// Writes tile b's partial from each thread's 4 items.  nl: pre-update normalised
+// log-weights (ignored when `fresh`), l: updated log-weights, valid: item < N.
+template <typename T>
+__device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
+                             const bool valid[kTileItems], bool fresh, double* partial) {
+  __shared__ T lds_t[kTileThreads / kWave];
+  __shared__ double lds_d[kTileThreads / kWave];
+  // baseline partial (pre-update), FP32 (max, sum) per thread -> FP64 across waves
+  LseAcc<T> a0 = lse_empty<T>();
+  if (!fresh) {
+    T m = (T)-INFINITY;
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k)
+      if (valid[k]) m = nl[k] > m ? nl[k] : m;
+    T s = 0;
+    if (m != (T)-INFINITY) {
+#pragma unroll
+      for (int k = 0; k < kTileItems; ++k)
+        if (valid[k]) s += exp_t(nl[k] - m);
+    }
+    a0 = LseAcc<T>{m, s, (T)0};
+  }
   a0 = wave_lse(a0);
-  a1 = wave_lse(a1);
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) {
-    w0[wv] = LseAcc<double>{(double)a0.m, (double)a0.s, 0.0};
-    w1[wv] = LseAcc<double>{(double)a1.m, (double)a1.s, (double)a1.s2};
+  // tile max of the updated log-weights
+  T mt = (T)-INFINITY;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k)
+    if (valid[k]) mt = l[k] > mt ? l[k] : mt;
+  const T m1 = block_max<T>(mt, lds_t);
+  double tau = 0.0, s2 = 0.0;
+  if (m1 != (T)-INFINITY) {
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+      if (valid[k]) {
+        const T e = exp_w(l[k] - m1);
+        tau += (double)e;
+        s2 += (double)e * (double)e;
+      }
+    }
   }
+  tau = block_sum_fixed(tau, lds_d);
+  s2 = block_sum_fixed(s2, lds_d);
+  __shared__ LseAcc<double> w0[kTileThreads / kWave];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (lane == 0) w0[w] = LseAcc<double>{(double)a0.m, (double)a0.s, 0.0};
   __syncthreads();
   if (threadIdx.x == 0) {
-    LseAcc<double> b0 = w0[0], b1 = w1[0];
+    LseAcc<double> b0 = w0[0];
+    for (int k = 1; k < kTileThreads / kWave; ++k) b0 = lse_combine(b0, w0[k]);
+    partial[0] = b0.m;
+    partial[1] = b0.s;
+    partial[2] = (double)m1;
+    partial[3] = tau;
+    partial[4] = s2;
+    partial[5] = 0.0;
+  }
+}
+
+// Partials of stored log-weights (filters whose step kernel is not tile-shaped): one block
+// per tile.  nl_pre holds the pre-update normalised log-weights; null means there is no
+// baseline to form (imported weights).
+template <typename T>
+__global__ void __launch_bounds__(kTileThreads)
+    k_tile_partials(int N, const T* __restrict__ l, const T* __restrict__ nl_pre,
+                    const BfScalars* sc, double threshold_n, double* partials) {
+  const bool rs = threshold_n >= sc->ess;
+  const bool fresh = nl_pre == nullptr || rs || sc->typeless;
+  const int tile0 = blockIdx.x * kTile;
+  T nl[kTileItems], lv[kTileItems];
+  bool valid[kTileItems];
 #pragma unroll
-    for (int k = 1; k < NW; ++k) {
-      b0 = lse_combine(b0, w0[k]);
-      b1 = lse_combine(b1, w1[k]);
-    }
-    double* P = partials + (size_t)blockIdx.x * 5;
-    P[0] = b0.m;
-    P[1] = b0.s;
-    P[2] = b1.m;
-    P[3] = b1.s;
-    P[4] = b1.s2;
-    __threadfence();
-    is_last = (atomicAdd(&sc->cnt_step, 1u) == (uint32_t)(nblocks - 1));
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    valid[k] = i < N;
+    lv[k] = valid[k] ? l[i] : (T)0;
+    nl[k] = (valid[k] && !fresh) ? nl_pre[i] : (T)0;
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  LseAcc<double> c0 = lse_empty<double>(), c1 = lse_empty<double>();
-  for (int b = threadIdx.x; b < nblocks; b += NTHR) {
-    const double* P = partials + (size_t)b * 5;
-    c0 = lse_combine(c0, LseAcc<double>{__builtin_nontemporal_load(P + 0),
-                                        __builtin_nontemporal_load(P + 1), 0.0});
-    c1 = lse_combine(c1, LseAcc<double>{__builtin_nontemporal_load(P + 2),
-                                        __builtin_nontemporal_load(P + 3),
-                                        __builtin_nontemporal_load(P + 4)});
+  tile_partial<T>(nl, lv, valid, fresh, partials + (size_t)blockIdx.x * kPartial);
+}
+
+// One block: fold the tile partials, finish the step, prepare the next resampling.
+// T is the log-weight type (its rounding of max(l) - lse defines the stored normaliser).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
+                double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
+                int record) {
+  __shared__ LseAcc<double> w0[4];
+  __shared__ double wM[4], wS[4], wS2[4], wtot[4];
+  __shared__ double Mshared, Sshared;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const bool rs = threshold_n >= sc_in->ess;
+  const bool fresh = rs || sc_in->typeless;
+  // pass 1: M = max m1_b, and the baseline combine
+  double M = -INFINITY;
+  LseAcc<double> c0 = lse_empty<double>();
+  for (int b = tid; b < nb; b += 256) {
+    const double* P = ts.partials + (size_t)b * kPartial;
+    M = fmax(M, P[2]);
+    c0 = lse_combine(c0, LseAcc<double>{P[0], P[1], 0.0});
   }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) M = fmax(M, wave_shfl_xor(M, m));
   c0 = wave_lse(c0);
-  c1 = wave_lse(c1);
-  __syncthreads();
   if (lane == 0) {
+    wM[wv] = M;
     w0[wv] = c0;
-    w1[wv] = c1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    LseAcc<double> b0 = w0[0], b1 = w1[0];
+  if (tid == 0) {
+    double mm = wM[0];
+    for (int k = 1; k < 4; ++k) mm = fmax(mm, wM[k]);
+    Mshared = mm;
+  }
+  __syncthreads();
+  M = Mshared;
+  // pass 2: S = sum tau_b exp(m1_b - M), S2 likewise (fixed order)
+  double S = 0.0, S2 = 0.0;
+  for (int b = tid; b < nb; b += 256) {
+    const double* P = ts.partials + (size_t)b * kPartial;
+    if (P[2] != -INFINITY) {
+      const double f = exp(P[2] - M);
+      S += P[3] * f;
+      S2 += P[4] * f * f;
+    }
+  }
+  S = wave_sum(S);
+  S2 = wave_sum(S2);
+  if (lane == 0) {
+    wS[wv] = S;
+    wS2[wv] = S2;
+  }
+  __syncthreads();
+  if (tid == 0) Sshared = ((wS[0] + wS[1]) + wS[2]) + wS[3];
+  __syncthreads();
+  S = Sshared;
+  S2 = ((wS2[0] + wS2[1]) + wS2[2]) + wS2[3];
+  const bool dead = !(M > -INFINITY) || !(S > 0.0);
+  // pass 3: c_b and the exclusive scan of c_b tau_b (thread-contiguous chunks)
+  const int chunk = (nb + 255) / 256;
+  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  double cs = 0.0;
+  for (int b = lo; b < hi; ++b) {
+    const double* P = ts.partials + (size_t)b * kPartial;
+    const double c = (dead || P[2] == -INFINITY) ? 0.0 : exp(P[2] - M) / S;
+    ts.blk_c[b] = c;
+    ts.blk_m[b] = P[2];
+    cs += c * P[3];
+  }
+  const double incl = wave_inclusive_scan(cs);
+  double excl = wave_shfl_up(incl, 1);
+  if (lane == 0) excl = 0.0;
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  double run = excl;
+  for (int k = 0; k < wv; ++k) run += wtot[k];
+  // Write the prefixes, then make them nondecreasing with an exact prefix max: the
+  // resampler's tiles meet at these values, so they must never step backwards.
+  double pmax = -INFINITY;
+  for (int b = lo; b < hi; ++b) {
+    pmax = fmax(pmax, run);
+    ts.prefix[b] = pmax;
+    run += ts.blk_c[b] * ts.partials[(size_t)b * kPartial + 3];
+  }
+  double cm = pmax;  // chunk max (-inf for empty chunks)
 #pragma unroll
-    for (int k = 1; k < NW; ++k) {
-      b0 = lse_combine(b0, w0[k]);
-      b1 = lse_combine(b1, w1[k]);
-    }
-    const double baseline = fresh ? logN : b0.m + log(b0.s);
-    const int idx = t - 1 - sc->ybase;
-    if (b1.m == -INFINITY) {
-      // Every particle has zero weight: the increment is -Inf; weights restart uniform.
-      if (ll_steps) ll_steps[idx] = -INFINITY;
-      sc->ll_total += -INFINITY;
-      sc->lse = 0.0;
-      sc->mxn = 0.0;
-      sc->s1 = (double)N;
-      sc->ess = (double)N;
-      sc->typeless = 1;
+  for (int d = 1; d < 64; d <<= 1) {
+    const double o = wave_shfl_up(cm, d);
+    if (lane >= d) cm = fmax(cm, o);
+  }
+  __shared__ double wmax[4];
+  if (lane == 63) wmax[wv] = cm;
+  double carry = wave_shfl_up(cm, 1);
+  if (lane == 0) carry = -INFINITY;
+  __syncthreads();
+  for (int k = 0; k < wv; ++k) carry = fmax(carry, wmax[k]);
+  for (int b = lo; b < hi; ++b) ts.prefix[b] = fmax(ts.prefix[b], carry);
+  __syncthreads();
+  if (tid == 0) {
+    double tot = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
+    for (int k = 0; k < 4; ++k) tot = fmax(tot, wmax[k]);
+    ts.prefix[nb] = tot;
+    w0[0] = lse_combine(lse_combine(w0[0], w0[1]), lse_combine(w0[2], w0[3]));
+    const double baseline = fresh ? logN : w0[0].m + log(w0[0].s);
+    const int t = sc_in->step;
+    const int idx = t - 1 - sc_in->ybase;
+    BfScalars s = *sc_in;
+    if (dead) {
+      // every particle has zero weight: the increment is -Inf; weights restart uniform
+      if (record && ll_steps) ll_steps[idx] = -INFINITY;
+      if (record) s.ll_total += -INFINITY;
+      s.lse = 0.0;
+      s.ess = (double)N;
+      s.typeless = 1;
     } else {
-      const double lse_after = b1.m + log(b1.s);
-      const double ll = lse_after - baseline;
-      double ess = (b1.s * b1.s) / b1.s2;
+      const double lse = M + log(S);
+      double ess = (S * S) / S2;
       if (!(ess <= (double)N)) ess = (double)N;  // ESS <= N exactly; clamp rounding
-      sc->lse = lse_after;
-      sc->mxn = (double)((T)b1.m - (T)lse_after);
-      sc->s1 = b1.s;
-      sc->ess = ess;
-      sc->ll_total += ll;
-      sc->typeless = 0;
-      if (ll_steps) ll_steps[idx] = ll;
+      s.lse = lse;
+      s.mxn = (double)((T)M - (T)lse);
+      s.s1 = S;
+      s.ess = ess;
+      s.typeless = 0;
+      if (record) {
+        const double ll = lse - baseline;
+        s.ll_total += ll;
+        if (ll_steps) ll_steps[idx] = ll;
+      }
     }
-    if (rs_steps) rs_steps[idx] = rs ? 1 : 0;
-    sc->last_resampled = rs ? 1 : 0;
-    sc->step = t + 1;
-    sc->cnt_step = 0u;
+    s.total_w = ts.prefix[nb];
+    if (record) {
+      if (rs_steps) rs_steps[idx] = rs ? 1 : 0;
+      s.last_resampled = rs ? 1 : 0;
+      s.step = t + 1;
+    }
+    *sc = s;
   }
 }
 

@@ -64,6 +64,7 @@ struct RbStepParams {
   const float* __restrict__ Sig_in;
   float* __restrict__ Sig_out;
   float* __restrict__ logw;
+  float* __restrict__ nl_pre;  // pre-update normalised log-weights (for the baseline)
   const int32_t* __restrict__ anc;
   const RbModelDev* __restrict__ model;
   const double* __restrict__ ys;
@@ -71,9 +72,6 @@ struct RbStepParams {
   double threshold_n, logN;
   uint64_t seed;
   const float* __restrict__ noise;  // optional outer noise (parity mode)
-  double* __restrict__ partials;
-  double* __restrict__ ll_steps;
-  int32_t* __restrict__ rs_steps;
   int32_t* __restrict__ err;  // set to 1 if an innovation covariance is not PD
 };
 
@@ -110,7 +108,6 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
   __syncthreads();
 
   const double log2pi = 1.8378770664093454835606594728112;
-  LseAcc<float> a0 = lse_empty<float>(), a1 = lse_empty<float>();
   const int G = gridDim.x * kRbWaves;
   for (int i = blockIdx.x * kRbWaves + wave; i < p.N; i += G) {
     const int a = rs ? p.anc[i] : i;
@@ -198,16 +195,10 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
     if (lane == 0) {
       p.xo_out[i] = xnew;
       p.logw[i] = lnew;
+      p.nl_pre[i] = nl;
     }
-    // keep this wave's LDS Sigma symmetric for the next particle's load (overwritten anyway)
     wave_lds_sync();
-    if (lane == 0) {
-      if (!fresh) a0 = lse_combine(a0, LseAcc<float>{nl, 1.f, 0.f});
-      a1 = lse_combine(a1, LseAcc<float>{lnew, 1.f, 1.f});
-    }
   }
-  pf_step_finish<float, kRbThreads>(a0, a1, p.partials, p.nblocks, sc, fresh, rs, t, p.logN,
-                                    p.N, p.ll_steps, p.rs_steps);
 }
 
 // initialise_particle (rbpf.jl:16-22): x ~ N(om0, os0), inner (mu0, Sigma0).
@@ -246,10 +237,10 @@ struct ssm_rbpf {
   float* Sig[2] = {nullptr, nullptr};
   int cur = 0;
   float* logw = nullptr;
+  float* nl_pre = nullptr;
   int32_t* anc = nullptr;
   RbModelDev* model = nullptr;
   BfScalars* sc = nullptr;
-  double* partials = nullptr;
   int nblocks = 0;
   void* rs_scratch = nullptr;
   double* w_dbg = nullptr;
@@ -357,6 +348,7 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   resample_bind_scratch(a, rb->rs_scratch, rb->N);
   int rc = launch_resample(rb->ctx, s, a);
   if (rc) return rc;
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
   RbStepParams p;
   p.N = rb->N;
   p.nblocks = rb->nblocks;
@@ -367,6 +359,7 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   p.Sig_in = rb->Sig[rb->cur];
   p.Sig_out = rb->Sig[rb->cur ^ 1];
   p.logw = rb->logw;
+  p.nl_pre = rb->nl_pre;
   p.anc = rb->anc;
   p.model = rb->model;
   p.ys = rb->ys;
@@ -375,11 +368,15 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   p.logN = std::log((double)rb->N);
   p.seed = rb->seed;
   p.noise = noise_dev;
-  p.partials = rb->partials;
-  p.ll_steps = rb->ll_steps;
-  p.rs_steps = rb->rs_steps;
   p.err = rb->err;
   hipLaunchKernelGGL(k_rbpf_step, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s, p);
+  const int nb = (rb->N + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_tile_partials<float>, dim3(nb), dim3(kTileThreads), 0, s, rb->N,
+                     (const float*)rb->logw, (const float*)rb->nl_pre, (const BfScalars*)rb->sc,
+                     a.threshold_n, ts.partials);
+  hipLaunchKernelGGL(k_pf_finish<float>, dim3(1), dim3(256), 0, s, rb->N, nb,
+                     (const BfScalars*)rb->sc, rb->sc, a.threshold_n, p.logN, ts,
+                     rb->ll_steps, rb->rs_steps, 1);
   SSM_HIP_CHECK(rb->ctx, hipGetLastError());
   rb->cur ^= 1;
   return SSM_OK;
@@ -444,7 +441,7 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
        hipMalloc(&rb->anc, sizeof(int32_t) * Npad) == hipSuccess &&
        hipMalloc(&rb->model, sizeof(RbModelDev)) == hipSuccess &&
        hipMalloc(&rb->sc, sizeof(BfScalars)) == hipSuccess &&
-       hipMalloc(&rb->partials, sizeof(double) * 5 * rb->nblocks) == hipSuccess &&
+       hipMalloc(&rb->nl_pre, sizeof(float) * Npad) == hipSuccess &&
        hipMalloc(&rb->rs_scratch, resample_scratch_bytes(N)) == hipSuccess &&
        hipMalloc(&rb->w_dbg, sizeof(double) * Npad) == hipSuccess &&
        hipMalloc(&rb->err, sizeof(int32_t)) == hipSuccess &&
@@ -456,7 +453,7 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
   (void)hipMemset(rb->logw, 0, sizeof(float) * Npad);
   (void)hipMemset(rb->anc, 0, sizeof(int32_t) * Npad);
   (void)hipMemset(rb->sc, 0, sizeof(BfScalars));
-  (void)hipMemset(rb->rs_scratch, 0, 256);
+  (void)hipMemset(rb->rs_scratch, 0, resample_scratch_bytes(N));
   (void)hipMemset(rb->err, 0, sizeof(int32_t));
   int rc = rb_upload(rb, model);
   if (!rc) rc = rb_steps_cap(rb, 1);
@@ -476,10 +473,10 @@ extern "C" int32_t ssm_rbpf_destroy(ssm_rbpf* rb) {
   if (!rb) return SSM_ERR_ARG;
   (void)hipSetDevice(rb->ctx->device);
   (void)hipStreamSynchronize(rb->ctx->stream);
-  void* ptrs[] = {rb->xo[0], rb->xo[1], rb->mu[0],      rb->mu[1],    rb->Sig[0],
-                  rb->Sig[1], rb->logw, rb->anc,        rb->model,    rb->sc,
-                  rb->partials, rb->rs_scratch, rb->w_dbg, rb->ys,   rb->ll_steps,
-                  rb->rs_steps, rb->err, rb->noise,     rb->u};
+  void* ptrs[] = {rb->xo[0],  rb->xo[1], rb->mu[0],      rb->mu[1],   rb->Sig[0],
+                  rb->Sig[1], rb->logw,  rb->nl_pre,     rb->anc,     rb->model,
+                  rb->sc,     rb->rs_scratch, rb->w_dbg, rb->ys,      rb->ll_steps,
+                  rb->rs_steps, rb->err, rb->noise,      rb->u};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (rb->ev0) (void)hipEventDestroy(rb->ev0);

@@ -9,65 +9,61 @@
 //   offspring->ancestors CUDAExt.jl:90-117, ancestors->offspring CUDAExt.jl:119-142
 // with W the inclusive prefix sum of the weights, taken here in FP64.
 //
-// Design: tiles of kTile = 2048 weights (256 threads x 8).  Kernel 1 reduces each tile in
-// FP64 and the last block to finish scans the tile sums (fixed order: deterministic).
-// Kernel 2 rescans each tile with its prefix, turns prefix sums into cumulative offspring
-// counts O_j, keeps them in LDS and fills the tile's contiguous output range
-// [O_{tile start}, O_{tile end}) with one thread per output slot (binary search in LDS),
-// so the ancestor write is balanced and coalesced however uneven the weights are.  The
-// FP64 prefix sums never reach HBM except for Multinomial, whose queries are unordered and
-// need a global binary search.
+// Design.  Weights are cut into tiles of 1024 (see pf_common.h).  Tile prefixes come either
+// from a particle filter's finish kernel (its tile partials already hold each tile's
+// weight sum) or, for explicit weight vectors, from a tile-sum kernel plus a one-block
+// scan.  The main kernel rescans one tile in FP64 on top of its prefix, turns the prefix
+// sums into cumulative offspring counts O_j in LDS, and fills the tile's contiguous output
+// range [O_{tile start}, O_{tile end}) with one thread per output slot (binary search in
+// LDS), so the ancestor write is balanced and coalesced however uneven the weights are.
+// The FP64 prefix sums never reach HBM except for Multinomial, whose queries are unordered
+// and need a global binary search.  Neighbouring tiles compute their shared boundary count
+// from the same stored prefix, and every prefix sequence is made nondecreasing with an exact
+// running max, so the tiles' output ranges partition [0, n) exactly.
 #include "device_common.h"
 #include "internal.h"
+#include "pf_common.h"
 
 namespace ssm {
 
-constexpr int kThreads = 256;
-constexpr int kItems = 8;
-constexpr int kTile = kThreads * kItems;
+constexpr int kRsThreads = kTileThreads;
 
 size_t resample_scratch_bytes(int n) {
   const size_t nb = (size_t)(n + kTile - 1) / kTile;
-  size_t b = 0;
-  b += 256;                         // counter + total
-  b += ((nb * 8 + 255) / 256) * 256;        // tile_sum
-  b += (((nb + 1) * 8 + 255) / 256) * 256;  // prefix
-  b += (((size_t)n * 8 + 255) / 256) * 256;  // W (multinomial)
-  return b;
+  auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+  return r(nb * kPartial * 8) + r(nb * 8) + r(nb * 8) + r((nb + 1) * 8) + r((size_t)n * 8);
 }
 
 void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   const size_t nb = (size_t)(n + kTile - 1) / kTile;
+  auto r = [](size_t b) { return (b + 255) / 256 * 256; };
   char* p = (char*)scratch;
-  a.counter = (uint32_t*)p;
-  a.total = (double*)(p + 8);
-  p += 256;
-  a.tile_sum = (double*)p;
-  p += ((nb * 8 + 255) / 256) * 256;
+  a.partials = (double*)p;
+  p += r(nb * kPartial * 8);
+  a.blk_m = (double*)p;
+  p += r(nb * 8);
+  a.blk_c = (double*)p;
+  p += r(nb * 8);
   a.prefix = (double*)p;
-  p += (((nb + 1) * 8 + 255) / 256) * 256;
+  p += r((nb + 1) * 8);
   a.Wbuf = (double*)p;
 }
 
-// Kernel parameters passed by value.
 struct RsParams {
-  int kind;
-  int n;
-  int mode;
+  int kind, n, mode;
   const void* w;
   const double* u;
   uint64_t seed;
   uint32_t step_host;
-  BfScalars* sc;
+  const BfScalars* sc;
   double threshold_n;
   int32_t* anc;
-  float* w_debug;
   double* w_debug64;
-  double* tile_sum;
+  const double* blk_m;
+  const double* blk_c;
+  double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
   double* prefix;
-  uint32_t* counter;
   double* Wbuf;
-  double* total;
 };
 
 __device__ __forceinline__ bool rs_active(const RsParams& p) {
@@ -78,112 +74,107 @@ __device__ __forceinline__ uint32_t rs_step(const RsParams& p) {
   return p.sc ? (uint32_t)p.sc->step : p.step_host;
 }
 
-// Weight j as the resampler consumes it (FP64).  For particle-filter log-weights this is
-// softmax of the normalised log-weights: exp(nl_j - max nl) / sum_k exp(nl_k - max nl)
-// with the normalisers kept in BfScalars (containers.jl:140-141), or 1/n for the lazy
-// TypelessZero weights (containers.jl:62-65).
+// Weight of particle j in tile b as the resampler consumes it (FP64).  Particle-filter
+// modes: softmax of the log-weights as w_j = exp(l_j - m_b) c_b (containers.jl:140-141),
+// or 1/n for the lazy TypelessZero weights of initialise (containers.jl:62-65).
 template <int MODE>
-__device__ __forceinline__ double rs_weight(const RsParams& p, int j, bool write_dbg) {
+__device__ __forceinline__ double rs_weight(const RsParams& p, int j, int b) {
   if (MODE == kWeightsF64) return ((const double*)p.w)[j];
   if (MODE == kWeightsF32) return (double)((const float*)p.w)[j];
-  const BfScalars* sc = p.sc;
+  if (p.sc->typeless) return 1.0 / (double)p.n;
   if (MODE == kWeightsLogF32) {
-    float wf;
-    if (sc->typeless) {
-      const double w = 1.0 / (double)p.n;
-      if (write_dbg && p.w_debug64) p.w_debug64[j] = w;
-      return w;
-    }
-    const float nl = ((const float*)p.w)[j] - (float)sc->lse;
-    wf = expf(nl - (float)sc->mxn) / (float)sc->s1;
-    if (write_dbg && p.w_debug64) p.w_debug64[j] = (double)wf;
-    return (double)wf;
+    const float e = exp_w(((const float*)p.w)[j] - (float)p.blk_m[b]);
+    return (double)e * p.blk_c[b];
   } else {
-    double w;
-    if (sc->typeless) {
-      w = 1.0 / (double)p.n;
-    } else {
-      const double nl = ((const double*)p.w)[j] - sc->lse;
-      w = exp(nl - sc->mxn) / sc->s1;
-    }
-    if (write_dbg && p.w_debug64) p.w_debug64[j] = w;
-    return w;
+    const double e = exp_w(((const double*)p.w)[j] - p.blk_m[b]);
+    return e * p.blk_c[b];
   }
 }
 
-// Block-wide FP64 sum (fixed order), result valid in thread 0.
-__device__ __forceinline__ double block_sum_d(double v, double* lds4) {
-  v = wave_sum(v);
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) lds4[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = ((lds4[0] + lds4[1]) + (lds4[2] + lds4[3]));
-  return r;
-}
-
-// Block exclusive scan of one FP64 value per thread; returns the exclusive prefix and
-// writes the block total to *tot.  Fixed order.
-__device__ __forceinline__ double block_exclusive_scan_d(double v, double* lds, double* tot) {
+// Inclusive FP64 scan across the block (thread order), fixed order; *tot = block total.
+__device__ __forceinline__ double block_incl_scan_d(double v, double* lds, double* tot) {
   const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   const double incl = wave_inclusive_scan(v);
-  double excl = wave_shfl_up(incl, 1);
-  if (lane == 0) excl = 0.0;
+  __syncthreads();
   if (lane == kWave - 1) lds[w] = incl;
   __syncthreads();
   double wp = 0.0;
   for (int k = 0; k < w; ++k) wp += lds[k];
   *tot = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  __syncthreads();
-  return wp + excl;
+  return wp + incl;
 }
 
-// Kernel 1: FP64 tile sums; the last block scans them into prefix[0..nb] and total.
+// Inclusive max-scan across the block (thread order); exact.  *tot = block max.
+__device__ __forceinline__ double block_incl_max_d(double v, double* lds, double* tot) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = wave_shfl_up(v, d);
+    if (lane >= d) v = fmax(v, o);
+  }
+  __syncthreads();
+  if (lane == kWave - 1) lds[w] = v;
+  __syncthreads();
+  for (int k = 0; k < w; ++k) v = fmax(v, lds[k]);
+  *tot = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  return v;
+}
+
+// Explicit weights, kernel 1: FP64 tile sums (thread-serial over its striped items, then
+// the block in fixed order).
 template <int MODE>
-__global__ void __launch_bounds__(kThreads) k_tile_sums(RsParams p) {
-  if (!rs_active(p)) return;
-  __shared__ double lds[kThreads / kWave];
-  __shared__ int is_last;
-  const int nb = (p.n + kTile - 1) / kTile;
-  const int b = blockIdx.x;
-  const int base = b * kTile + threadIdx.x * kItems;
+__global__ void __launch_bounds__(kRsThreads) k_tile_sums(RsParams p) {
+  __shared__ double lds[kRsThreads / kWave];
+  const int b = blockIdx.x, tile0 = b * kTile;
   double s = 0.0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int j = base + k;
-    if (j < p.n) s += rs_weight<MODE>(p, j, true);
+  for (int k = 0; k < kTileItems; ++k) {
+    const int j = tile0 + k * kRsThreads + threadIdx.x;
+    if (j < p.n) s += rs_weight<MODE>(p, j, b);
   }
-  const double bs = block_sum_d(s, lds);
-  uint32_t* counter = p.sc ? &p.sc->cnt_scan : p.counter;
-  if (threadIdx.x == 0) {
-    p.tile_sum[b] = bs;
-    __threadfence();
-    is_last = (atomicAdd(counter, 1u) == (uint32_t)(nb - 1));
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // Exclusive scan of the nb tile sums: each thread owns a contiguous chunk.
-  const int chunk = (nb + kThreads - 1) / kThreads;
-  const int lo = min(nb, (int)threadIdx.x * chunk), hi = min(nb, lo + chunk);
-  double cs = 0.0;
-  for (int k = lo; k < hi; ++k) cs += __builtin_nontemporal_load(&p.tile_sum[k]);
-  __shared__ double lds2[kThreads / kWave];
   double tot;
-  double run = block_exclusive_scan_d(cs, lds2, &tot);
-  for (int k = lo; k < hi; ++k) {
-    p.prefix[k] = run;
-    run += __builtin_nontemporal_load(&p.tile_sum[k]);
-  }
-  if (threadIdx.x == 0) {
-    p.prefix[nb] = tot;
-    *p.total = tot;
-    if (p.sc) p.sc->total_w = tot;
-    *counter = 0u;
-  }
+  (void)block_incl_scan_d(s, lds, &tot);
+  if (threadIdx.x == 0) p.tile_sum[b] = tot;
 }
 
-// Cumulative offspring count for an inclusive prefix W (GPU forms use scan total Wn).
+// Explicit weights, kernel 2 (one block): exclusive scan of the tile sums into
+// nondecreasing prefixes, prefix[nb] = total.
+__global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_sum,
+                                                     double* prefix) {
+  __shared__ double lds[4];
+  const int tid = threadIdx.x;
+  const int chunk = (nb + 255) / 256;
+  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  double cs = 0.0;
+  for (int k = lo; k < hi; ++k) cs += tile_sum[k];
+  double tot;
+  const double incl = block_incl_scan_d(cs, lds, &tot);
+  double run = incl - cs;  // exclusive start of this chunk
+  // recompute the exclusive start exactly as the scan's own prefix of the previous threads
+  {
+    const int w = tid / kWave, lane = tid & (kWave - 1);
+    double ex = wave_shfl_up(wave_inclusive_scan(cs), 1);
+    if (lane == 0) ex = 0.0;
+    for (int k = 0; k < w; ++k) ex += lds[k];
+    run = ex;
+  }
+  double pm = -INFINITY;
+  for (int k = lo; k < hi; ++k) {
+    pm = fmax(pm, run);
+    prefix[k] = pm;
+    run += tile_sum[k];
+  }
+  double mx;
+  __syncthreads();
+  const double carry_incl = block_incl_max_d(pm, lds, &mx);
+  double carry = wave_shfl_up(carry_incl, 1);
+  if ((tid & 63) == 0) carry = -INFINITY;
+  for (int k = 0; k < tid / kWave; ++k) carry = fmax(carry, lds[k]);
+  for (int k = lo; k < hi; ++k) prefix[k] = fmax(prefix[k], carry);
+  if (tid == 0) prefix[nb] = fmax(tot, mx);
+}
+
+// Cumulative offspring count for an inclusive prefix W.
 // SYSTEMATIC: O = #{s in [0,n) : fl(u0 + s) < fl(n W)} (strict, resamplers.jl:231).
 __device__ __forceinline__ int offspring_systematic(double W, double u0, int n) {
   const double vs = W * (double)n;
@@ -216,67 +207,84 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
   return clamp_floor(r + uk, n);
 }
 
-// Kernel 2: tile rescan -> cumulative offspring in LDS -> balanced ancestor fill.
-// For MULTINOMIAL it only writes the prefix sums W to Wbuf.
+// Main kernel: tile rescan -> cumulative offspring in LDS -> balanced ancestor fill.
+// For MULTINOMIAL it writes the (nondecreasing) prefix sums W to Wbuf instead.
 template <int MODE>
-__global__ void __launch_bounds__(kThreads) k_scan_offspring(RsParams p) {
+__global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
   if (!rs_active(p)) return;
-  __shared__ double lds[kThreads / kWave];
+  __shared__ double lds[kRsThreads / kWave];
   __shared__ int O[kTile];
   const int n = p.n;
   const int b = blockIdx.x;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
   const int cnt = min(kTile, n - tile0);
-  const int base = tile0 + threadIdx.x * kItems;
-  double w[kItems];
-  double ts = 0.0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int j = base + k;
-    w[k] = (j < n) ? rs_weight<MODE>(p, j, false) : 0.0;
-    ts += w[k];
+  const bool typeless = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64) && p.sc->typeless;
+  double P, Pnext, Wn;
+  if (typeless) {
+    P = (double)tile0 / (double)n;
+    Pnext = (b == nb - 1) ? 1.0 : (double)(tile0 + kTile) / (double)n;
+    Wn = 1.0;
+  } else {
+    P = p.prefix[b];
+    Pnext = p.prefix[b + 1];
+    Wn = p.prefix[nb];
   }
-  double tot;
-  const double excl = block_exclusive_scan_d(ts, lds, &tot);
-  (void)tot;
-  const double P = p.prefix[b];
-  const double Pnext = p.prefix[b + 1];
-  const double Wn = *p.total;
+  // FP64 scan of the tile in particle order: rows of 256 (striped items), row by row.
+  double W[kTileItems];
+  double rowoff = 0.0;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int j = tile0 + k * kRsThreads + threadIdx.x;
+    double wj = 0.0;
+    if (j < n) {
+      wj = rs_weight<MODE>(p, j, b);
+      if (p.w_debug64) p.w_debug64[j] = wj;
+    }
+    double rowtot;
+    const double incl = block_incl_scan_d(wj, lds, &rowtot);
+    W[k] = P + (rowoff + incl);
+    rowoff += rowtot;
+  }
+  // make W nondecreasing across the tile (exact running max in particle order)
+  double carry = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    double rowmax;
+    const double m = block_incl_max_d(W[k], lds, &rowmax);
+    W[k] = fmax(m, carry);
+    carry = fmax(carry, rowmax);
+  }
   const uint32_t step = rs_step(p);
-  double acc = P + excl;
   if (p.kind == SSM_MULTINOMIAL) {
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      const int j = base + k;
-      acc += w[k];
-      if (j < n) p.Wbuf[j] = (j == tile0 + cnt - 1) ? Pnext : acc;
+    for (int k = 0; k < kTileItems; ++k) {
+      const int pos = k * kRsThreads + threadIdx.x;
+      const int j = tile0 + pos;
+      if (j < n) p.Wbuf[j] = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
     }
     return;
   }
   const double u0 = (p.kind == SSM_STRATIFIED)
                         ? 0.0
                         : (p.u ? p.u[0] : resample_uniform(p.seed, step, 0u));
-  // Tile boundary counts are computed from the shared prefixes so neighbouring tiles
-  // agree exactly on where one's output range ends and the next one's begins.
   const int o_prev = (b == 0) ? 0 : offspring_of(p, P, Wn, u0, step);
   const int o_end = (b == nb - 1) ? n : offspring_of(p, Pnext, Wn, u0, step);
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int j = base + k;
-    acc += w[k];
-    if (j < n) {
+  for (int k = 0; k < kTileItems; ++k) {
+    const int pos = k * kRsThreads + threadIdx.x;
+    if (pos < cnt) {
       int o;
-      if (j == tile0 + cnt - 1)
+      if (pos == cnt - 1)
         o = o_end;
       else
-        o = min(max(offspring_of(p, acc, Wn, u0, step), o_prev), o_end);
-      O[j - tile0] = o;
+        o = min(max(offspring_of(p, W[k], Wn, u0, step), o_prev), o_end);
+      O[pos] = o;
     }
   }
   __syncthreads();
-  // Slot s belongs to the first local j with O[j] > s.
-  for (int s = o_prev + (int)threadIdx.x; s < o_end; s += kThreads) {
+  // Slot s belongs to the first position with O > s.
+  for (int s = o_prev + (int)threadIdx.x; s < o_end; s += kRsThreads) {
     int lo = 0, hi = cnt - 1;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
@@ -289,13 +297,15 @@ __global__ void __launch_bounds__(kThreads) k_scan_offspring(RsParams p) {
   }
 }
 
-// Kernel 3 (multinomial): a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.
-__global__ void __launch_bounds__(kThreads) k_multinomial_search(RsParams p) {
+// Multinomial: a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.
+__global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
   if (!rs_active(p)) return;
-  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const int k = blockIdx.x * kRsThreads + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t step = rs_step(p);
-  const double Wn = *p.total;
+  const int nb = (p.n + kTile - 1) / kTile;
+  const bool typeless = (p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64) && p.sc->typeless;
+  const double Wn = typeless ? 1.0 : p.prefix[nb];
   const double uk = p.u ? p.u[k] : resample_uniform(p.seed, step, (uint32_t)k);
   const double q = uk * Wn;
   int lo = 0, hi = p.n;
@@ -307,12 +317,6 @@ __global__ void __launch_bounds__(kThreads) k_multinomial_search(RsParams p) {
       hi = mid;
   }
   p.anc[k] = min(lo, p.n - 1);
-}
-
-template <int MODE>
-static void launch_mode(hipStream_t s, const RsParams& rp, int nb) {
-  hipLaunchKernelGGL(k_tile_sums<MODE>, dim3(nb), dim3(kThreads), 0, s, rp);
-  hipLaunchKernelGGL(k_scan_offspring<MODE>, dim3(nb), dim3(kThreads), 0, s, rp);
 }
 
 int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
@@ -330,24 +334,38 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
   rp.sc = a.sc;
   rp.threshold_n = a.threshold_n;
   rp.anc = a.anc;
-  rp.w_debug = a.w_debug;
   rp.w_debug64 = a.w_debug64;
-  rp.tile_sum = a.tile_sum;
+  rp.blk_m = a.blk_m;
+  rp.blk_c = a.blk_c;
+  rp.tile_sum = a.partials;
   rp.prefix = a.prefix;
-  rp.counter = a.counter;
   rp.Wbuf = a.Wbuf;
-  rp.total = a.total;
   const int nb = (a.n + kTile - 1) / kTile;
+  const dim3 g(nb), blk(kRsThreads);
   switch (a.weight_mode) {
-    case kWeightsF64: launch_mode<kWeightsF64>(stream, rp, nb); break;
-    case kWeightsF32: launch_mode<kWeightsF32>(stream, rp, nb); break;
-    case kWeightsLogF32: launch_mode<kWeightsLogF32>(stream, rp, nb); break;
-    case kWeightsLogF64: launch_mode<kWeightsLogF64>(stream, rp, nb); break;
+    case kWeightsF64:
+      hipLaunchKernelGGL(k_tile_sums<kWeightsF64>, g, blk, 0, stream, rp);
+      hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(256), 0, stream, nb, rp.tile_sum,
+                         rp.prefix);
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsF64>, g, blk, 0, stream, rp);
+      break;
+    case kWeightsF32:
+      hipLaunchKernelGGL(k_tile_sums<kWeightsF32>, g, blk, 0, stream, rp);
+      hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(256), 0, stream, nb, rp.tile_sum,
+                         rp.prefix);
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsF32>, g, blk, 0, stream, rp);
+      break;
+    case kWeightsLogF32:
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsLogF32>, g, blk, 0, stream, rp);
+      break;
+    case kWeightsLogF64:
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsLogF64>, g, blk, 0, stream, rp);
+      break;
     default: return set_error(ctx, SSM_ERR_ARG, "resample: bad weight mode");
   }
   if (a.kind == SSM_MULTINOMIAL)
-    hipLaunchKernelGGL(k_multinomial_search, dim3((a.n + kThreads - 1) / kThreads),
-                       dim3(kThreads), 0, stream, rp);
+    hipLaunchKernelGGL(k_multinomial_search, dim3((a.n + kRsThreads - 1) / kRsThreads),
+                       dim3(kRsThreads), 0, stream, rp);
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
 }
@@ -355,9 +373,9 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
 // ---- offspring <-> ancestors ------------------------------------------------------
 // One thread per output slot s: ancestor = first i with O_i > s (O nondecreasing), which
 // equals the reference's range scatter for valid input and needs no load balancing.
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kRsThreads)
     k_offspring_to_ancestors(int n, const int32_t* __restrict__ O, int32_t* __restrict__ anc) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const int s = blockIdx.x * kRsThreads + threadIdx.x;
   if (s >= n) return;
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -370,14 +388,14 @@ __global__ void __launch_bounds__(kThreads)
   anc[s] = min(lo, n - 1);
 }
 
-__global__ void __launch_bounds__(kThreads) k_fill_i32(int n, int32_t* p, int32_t v) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
+__global__ void __launch_bounds__(kRsThreads) k_fill_i32(int n, int32_t* p, int32_t v) {
+  const int i = blockIdx.x * kRsThreads + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kRsThreads)
     k_ancestors_to_offspring(int n, const int32_t* __restrict__ anc, int32_t* off) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
+  const int i = blockIdx.x * kRsThreads + threadIdx.x;
   if (i >= n) return;
   const int a = anc[i];
   if (a >= 0 && a < n) atomicAdd(&off[a], 1);
@@ -398,14 +416,13 @@ extern "C" int32_t ssm_sample_ancestors(ssm_ctx* ctx, int32_t kind, int32_t n, c
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   const size_t need = resample_scratch_bytes(n);
   if (need > ctx->rs_scratch_bytes) {
+    SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->rs_scratch) (void)hipFree(ctx->rs_scratch);
     ctx->rs_scratch = nullptr;
     ctx->rs_scratch_bytes = 0;
     if (hipMalloc(&ctx->rs_scratch, need) != hipSuccess)
       return set_error(ctx, SSM_ERR_OOM, "ssm_sample_ancestors: cannot allocate %zu B", need);
     ctx->rs_scratch_bytes = need;
-    // The counter must start at zero; the kernels reset it after each use.
-    SSM_HIP_CHECK(ctx, hipMemset(ctx->rs_scratch, 0, 256));
   }
   ResampleArgs a;
   a.kind = kind;
@@ -428,6 +445,7 @@ extern "C" int32_t ssm_sample_ancestors_host(ssm_ctx* ctx, int32_t kind, int32_t
     return set_error(ctx, SSM_ERR_ARG, "ssm_sample_ancestors_host: bad arguments (n=%d)", n);
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   if ((size_t)n > ctx->rs_host_cap) {
+    SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     (void)hipFree(ctx->rs_w);
     (void)hipFree(ctx->rs_u);
     (void)hipFree(ctx->rs_a);
@@ -461,8 +479,8 @@ extern "C" int32_t ssm_offspring_to_ancestors(ssm_ctx* ctx, int32_t n,
   if (n < 1 || !d_offspring || !d_ancestors)
     return set_error(ctx, SSM_ERR_ARG, "ssm_offspring_to_ancestors: bad arguments");
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  hipLaunchKernelGGL(k_offspring_to_ancestors, dim3((n + kThreads - 1) / kThreads),
-                     dim3(kThreads), 0, ctx->stream, n, d_offspring, d_ancestors);
+  hipLaunchKernelGGL(k_offspring_to_ancestors, dim3((n + kRsThreads - 1) / kRsThreads),
+                     dim3(kRsThreads), 0, ctx->stream, n, d_offspring, d_ancestors);
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
 }
@@ -474,9 +492,9 @@ extern "C" int32_t ssm_ancestors_to_offspring(ssm_ctx* ctx, int32_t n,
   if (n < 1 || !d_offspring || !d_ancestors)
     return set_error(ctx, SSM_ERR_ARG, "ssm_ancestors_to_offspring: bad arguments");
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  const dim3 g((n + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_fill_i32, g, dim3(kThreads), 0, ctx->stream, n, d_offspring, 0);
-  hipLaunchKernelGGL(k_ancestors_to_offspring, g, dim3(kThreads), 0, ctx->stream, n,
+  const dim3 g((n + kRsThreads - 1) / kRsThreads);
+  hipLaunchKernelGGL(k_fill_i32, g, dim3(kRsThreads), 0, ctx->stream, n, d_offspring, 0);
+  hipLaunchKernelGGL(k_ancestors_to_offspring, g, dim3(kRsThreads), 0, ctx->stream, n,
                      d_ancestors, d_offspring);
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
