@@ -113,6 +113,69 @@ __device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t stream, uin
   }
 }
 
+// Read access to the model constants inside the step kernel.  REG: copied to registers
+// (compile-time dimensions); otherwise read from the device struct (column-major).
+template <typename T, int MD, int MDY, bool REG>
+struct ModelView;
+
+template <typename T, int MD, int MDY>
+struct ModelView<T, MD, MDY, true> {
+  T a[MD * MD], lq[MD * MD], bb[MD], h[MDY * MD], cc[MDY], lri[MDY * MDY];
+  T cst;
+  __device__ __forceinline__ ModelView(const BfModelDev<T>& m, int D, int Dy) {
+#pragma unroll
+    for (int r = 0; r < MD; ++r) {
+      bb[r] = m.b[r];
+#pragma unroll
+      for (int c = 0; c < MD; ++c) {
+        a[r * MD + c] = m.A[r + c * MD];
+        lq[r * MD + c] = (c <= r) ? m.LQ[r + c * MD] : (T)0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < MDY; ++q) {
+      cc[q] = m.c[q];
+#pragma unroll
+      for (int c = 0; c < MD; ++c) h[q * MD + c] = m.H[q + c * MDY];
+#pragma unroll
+      for (int c = 0; c < MDY; ++c) lri[q * MDY + c] = (c <= q) ? m.LRi[q + c * MDY] : (T)0;
+    }
+    cst = m.cst;
+  }
+  __device__ __forceinline__ T A(int r, int c) const { return a[r * MD + c]; }
+  __device__ __forceinline__ T LQ(int r, int c) const { return lq[r * MD + c]; }
+  __device__ __forceinline__ T b(int r) const { return bb[r]; }
+  __device__ __forceinline__ T H(int q, int c) const { return h[q * MD + c]; }
+  __device__ __forceinline__ T c(int q) const { return cc[q]; }
+  __device__ __forceinline__ T LRi(int q, int c) const { return lri[q * MDY + c]; }
+};
+
+template <typename T, int MD, int MDY>
+struct ModelView<T, MD, MDY, false> {
+  const BfModelDev<T>& m;  // a copy in LDS (see k_bf_step)
+  int D, Dy;
+  T cst;
+  __device__ __forceinline__ ModelView(const BfModelDev<T>& mm, int D_, int Dy_)
+      : m(mm), D(D_), Dy(Dy_), cst(mm.cst) {}
+  __device__ __forceinline__ T A(int r, int c) const { return m.A[r + c * D]; }
+  __device__ __forceinline__ T LQ(int r, int c) const { return m.LQ[r + c * D]; }
+  __device__ __forceinline__ T b(int r) const { return m.b[r]; }
+  __device__ __forceinline__ T H(int q, int c) const { return m.H[q + c * Dy]; }
+  __device__ __forceinline__ T c(int q) const { return m.c[q]; }
+  __device__ __forceinline__ T LRi(int q, int c) const { return m.LRi[q + c * Dy]; }
+};
+
+// Copies the model constants to LDS (runtime-dimension kernels read them many times).
+template <typename T>
+__device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>* g,
+                                                             BfModelDev<T>* s) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(g);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(s);
+  for (int e = threadIdx.x; e < (int)(sizeof(BfModelDev<T>) / 4); e += blockDim.x) dst[e] = src[e];
+  __syncthreads();
+  return *s;
+}
+
 // The fused predict + update + partial-reduction kernel.  DT/DYT > 0 fix the dimensions
 // at compile time; 0 means runtime dimensions up to kMaxD / kMaxDy.
 template <typename T, int DT, int DYT>
@@ -127,7 +190,14 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
   const T lse = (T)sc->lse;
   const int t = sc->step;
   const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
-  const BfModelDev<T>& m = *p.model;
+  // Model constants: registers for compile-time dimensions (read before any store, so the
+  // loads are uniform scalar loads), an LDS copy otherwise.
+  const BfModelDev<T>* msrc = p.model;
+  if constexpr (DT == 0) {
+    __shared__ BfModelDev<T> lds_model;
+    msrc = &model_to_lds<T>(p.model, &lds_model);
+  }
+  const ModelView<T, MD, MDY, (DT > 0)> mv(*msrc, D, Dy);
   T y[MDY];
 #pragma unroll
   for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
@@ -135,16 +205,18 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
   const int tile0 = blockIdx.x * kTile;
   T nl[kStepItems], lnew[kStepItems];
   bool valid[kStepItems];
-#pragma unroll
-  for (int k = 0; k < kStepItems; ++k) {
+
+  auto load_item = [&](int k, T* xk) {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
     valid[k] = i < p.N;
     const int src = (rs && valid[k]) ? p.anc[i] : i;
-    T x[MD];
 #pragma unroll
     for (int d = 0; d < MD; ++d)
-      if (d < D) x[d] = p.xin[(size_t)d * p.Npad + src];
+      if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
     nl[k] = fresh ? (T)0 : p.logw[i] - lse;
+  };
+  auto compute_item = [&](int k, const T* xk) {
+    const int i = tile0 + k * kStepThreads + threadIdx.x;
     T z[MD];
     if (p.noise) {
 #pragma unroll
@@ -158,12 +230,12 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
 #pragma unroll
     for (int r = 0; r < MD; ++r) {
       if (r < D) {
-        T s = m.b[r];
+        T s = mv.b(r);
 #pragma unroll
         for (int c = 0; c < MD; ++c)
-          if (c < D) s += m.A[r + c * D] * x[c];
+          if (c < D) s += mv.A(r, c) * xk[c];
 #pragma unroll
-        for (int c = 0; c <= r; ++c) s += m.LQ[r + c * D] * z[c];
+        for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
         xn[r] = s;
         p.xout[(size_t)r * p.Npad + i] = s;
       }
@@ -173,10 +245,10 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
 #pragma unroll
     for (int q = 0; q < MDY; ++q) {
       if (q < Dy) {
-        T s = m.c[q];
+        T s = mv.c(q);
 #pragma unroll
         for (int c = 0; c < MD; ++c)
-          if (c < D) s += m.H[q + c * Dy] * xn[c];
+          if (c < D) s += mv.H(q, c) * xn[c];
         res[q] = y[q] - s;
       }
     }
@@ -186,14 +258,28 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
       if (q < Dy) {
         T v = 0;
 #pragma unroll
-        for (int c = 0; c <= q; ++c) v += m.LRi[q + c * Dy] * res[c];
+        for (int c = 0; c <= q; ++c) v += mv.LRi(q, c) * res[c];
         quad += v * v;
       }
     }
-    lnew[k] = nl[k] + (m.cst - (T)0.5 * quad);
+    lnew[k] = nl[k] + (mv.cst - (T)0.5 * quad);
     p.logw[i] = lnew[k];
+  };
+  if constexpr (DT > 0) {
+    // all loads first (maximum memory-level parallelism), then the arithmetic
+    T x[kStepItems][MD];
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) load_item(k, x[k]);
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) compute_item(k, x[k]);
+  } else {
+    for (int k = 0; k < kStepItems; ++k) {
+      T xk[MD];
+      load_item(k, xk);
+      compute_item(k, xk);
+    }
   }
-  tile_partial<T>(nl, lnew, valid, fresh, p.partials + (size_t)blockIdx.x * kPartial);
+  tile_partial<T>(nl, lnew, valid, fresh, p.partials, p.nblocks);
 }
 
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).

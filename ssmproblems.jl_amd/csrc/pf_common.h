@@ -26,7 +26,7 @@ namespace ssm {
 constexpr int kTileThreads = 256;
 constexpr int kTileItems = 4;
 constexpr int kTile = kTileThreads * kTileItems;
-constexpr int kPartial = 6;  // doubles per tile partial
+constexpr int kPartial = 5;  // doubles per tile partial (field-major)
 
 // Per-filter device scratch for tiles (all sized by the tile count nb).
 struct TileScratch {
@@ -70,9 +70,11 @@ __device__ __forceinline__ double exp_w(double v) { return exp(v); }
 
 // Writes tile b's partial from each thread's 4 items.  nl: pre-update normalised
 // log-weights (ignored when `fresh`), l: updated log-weights, valid: item < N.
+// Partials are stored field-major: field k of tile b at partials[k * nb + b].
 template <typename T>
 __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
-                             const bool valid[kTileItems], bool fresh, double* partial) {
+                             const bool valid[kTileItems], bool fresh, double* partials,
+                             int nb) {
   __shared__ T lds_t[kTileThreads / kWave];
   __shared__ double lds_d[kTileThreads / kWave];
   // baseline partial (pre-update), FP32 (max, sum) per thread -> FP64 across waves
@@ -117,12 +119,12 @@ __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
   if (threadIdx.x == 0) {
     LseAcc<double> b0 = w0[0];
     for (int k = 1; k < kTileThreads / kWave; ++k) b0 = lse_combine(b0, w0[k]);
-    partial[0] = b0.m;
-    partial[1] = b0.s;
-    partial[2] = (double)m1;
-    partial[3] = tau;
-    partial[4] = s2;
-    partial[5] = 0.0;
+    const int b = blockIdx.x;
+    partials[b] = b0.m;
+    partials[nb + b] = b0.s;
+    partials[2 * nb + b] = (double)m1;
+    partials[3 * nb + b] = tau;
+    partials[4 * nb + b] = s2;
   }
 }
 
@@ -145,84 +147,92 @@ __global__ void __launch_bounds__(kTileThreads)
     lv[k] = valid[k] ? l[i] : (T)0;
     nl[k] = (valid[k] && !fresh) ? nl_pre[i] : (T)0;
   }
-  tile_partial<T>(nl, lv, valid, fresh, partials + (size_t)blockIdx.x * kPartial);
+  tile_partial<T>(nl, lv, valid, fresh, partials, gridDim.x);
 }
 
 // One block: fold the tile partials, finish the step, prepare the next resampling.
-// T is the log-weight type (its rounding of max(l) - lse defines the stored normaliser).
+// T is the filter's type: the per-tile scale factors exp(m1_b - M) are evaluated in T.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
                 double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
                 int record) {
-  __shared__ LseAcc<double> w0[4];
-  __shared__ double wM[4], wS[4], wS2[4], wtot[4];
-  __shared__ double Mshared, Sshared;
+  __shared__ double red[4][4];
+  __shared__ double wtot[4], wmax[4];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const bool rs = threshold_n >= sc_in->ess;
   const bool fresh = rs || sc_in->typeless;
-  // pass 1: M = max m1_b, and the baseline combine
-  double M = -INFINITY;
-  LseAcc<double> c0 = lse_empty<double>();
+  const double* pm0 = ts.partials;
+  const double* ps0 = ts.partials + nb;
+  const double* pm1 = ts.partials + 2 * nb;
+  const double* ptau = ts.partials + 3 * nb;
+  const double* ps2 = ts.partials + 4 * nb;
+  // maxima of the tile maxima (exact, any order)
+  double M1 = -INFINITY, M0 = -INFINITY;
   for (int b = tid; b < nb; b += 256) {
-    const double* P = ts.partials + (size_t)b * kPartial;
-    M = fmax(M, P[2]);
-    c0 = lse_combine(c0, LseAcc<double>{P[0], P[1], 0.0});
+    M1 = fmax(M1, pm1[b]);
+    M0 = fmax(M0, pm0[b]);
   }
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) M = fmax(M, wave_shfl_xor(M, m));
-  c0 = wave_lse(c0);
+  for (int m = 32; m >= 1; m >>= 1) {
+    M1 = fmax(M1, wave_shfl_xor(M1, m));
+    M0 = fmax(M0, wave_shfl_xor(M0, m));
+  }
   if (lane == 0) {
-    wM[wv] = M;
-    w0[wv] = c0;
+    red[0][wv] = M1;
+    red[1][wv] = M0;
   }
   __syncthreads();
-  if (tid == 0) {
-    double mm = wM[0];
-    for (int k = 1; k < 4; ++k) mm = fmax(mm, wM[k]);
-    Mshared = mm;
-  }
+  M1 = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+  M0 = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
   __syncthreads();
-  M = Mshared;
-  // pass 2: S = sum tau_b exp(m1_b - M), S2 likewise (fixed order)
-  double S = 0.0, S2 = 0.0;
-  for (int b = tid; b < nb; b += 256) {
-    const double* P = ts.partials + (size_t)b * kPartial;
-    if (P[2] != -INFINITY) {
-      const double f = exp(P[2] - M);
-      S += P[3] * f;
-      S2 += P[4] * f * f;
+  // scaled sums: one exponential per tile, sums in FP64, fixed order.  Each thread owns
+  // a contiguous chunk of tiles here and in the scan below, so it only rereads its own
+  // scale factors.
+  const int chunk = (nb + 255) / 256;
+  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  double S = 0.0, S2 = 0.0, S0 = 0.0;
+  for (int b = lo; b < hi; ++b) {
+    const double m1 = pm1[b];
+    if (m1 != -INFINITY) {
+      const double f = (double)exp_w((T)(m1 - M1));
+      ts.blk_c[b] = f;  // scale factor; divided by S below
+      S += ptau[b] * f;
+      S2 += ps2[b] * f * f;
+    } else {
+      ts.blk_c[b] = 0.0;
     }
+    ts.blk_m[b] = m1;
+    const double m0 = pm0[b];
+    if (!fresh && m0 != -INFINITY) S0 += ps0[b] * (double)exp_w((T)(m0 - M0));
   }
   S = wave_sum(S);
   S2 = wave_sum(S2);
+  S0 = wave_sum(S0);
   if (lane == 0) {
-    wS[wv] = S;
-    wS2[wv] = S2;
+    red[0][wv] = S;
+    red[1][wv] = S2;
+    red[2][wv] = S0;
   }
   __syncthreads();
-  if (tid == 0) Sshared = ((wS[0] + wS[1]) + wS[2]) + wS[3];
+  S = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+  S2 = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+  S0 = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
+  const double invS = dead ? 0.0 : 1.0 / S;
   __syncthreads();
-  S = Sshared;
-  S2 = ((wS2[0] + wS2[1]) + wS2[2]) + wS2[3];
-  const bool dead = !(M > -INFINITY) || !(S > 0.0);
-  // pass 3: c_b and the exclusive scan of c_b tau_b (thread-contiguous chunks)
-  const int chunk = (nb + 255) / 256;
-  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  // c_b = f_b / S, then the exclusive scan of c_b tau_b over the same chunks
   double cs = 0.0;
   for (int b = lo; b < hi; ++b) {
-    const double* P = ts.partials + (size_t)b * kPartial;
-    const double c = (dead || P[2] == -INFINITY) ? 0.0 : exp(P[2] - M) / S;
+    const double c = ts.blk_c[b] * invS;
     ts.blk_c[b] = c;
-    ts.blk_m[b] = P[2];
-    cs += c * P[3];
+    cs += c * ptau[b];
   }
   const double incl = wave_inclusive_scan(cs);
-  double excl = wave_shfl_up(incl, 1);
-  if (lane == 0) excl = 0.0;
+  double run = wave_shfl_up(incl, 1);
+  if (lane == 0) run = 0.0;
   if (lane == 63) wtot[wv] = incl;
   __syncthreads();
-  double run = excl;
   for (int k = 0; k < wv; ++k) run += wtot[k];
   // Write the prefixes, then make them nondecreasing with an exact prefix max: the
   // resampler's tiles meet at these values, so they must never step backwards.
@@ -230,28 +240,25 @@ __global__ void __launch_bounds__(256)
   for (int b = lo; b < hi; ++b) {
     pmax = fmax(pmax, run);
     ts.prefix[b] = pmax;
-    run += ts.blk_c[b] * ts.partials[(size_t)b * kPartial + 3];
+    run += ts.blk_c[b] * ptau[b];
   }
-  double cm = pmax;  // chunk max (-inf for empty chunks)
+  double cm = pmax;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const double o = wave_shfl_up(cm, d);
     if (lane >= d) cm = fmax(cm, o);
   }
-  __shared__ double wmax[4];
   if (lane == 63) wmax[wv] = cm;
   double carry = wave_shfl_up(cm, 1);
   if (lane == 0) carry = -INFINITY;
   __syncthreads();
   for (int k = 0; k < wv; ++k) carry = fmax(carry, wmax[k]);
   for (int b = lo; b < hi; ++b) ts.prefix[b] = fmax(ts.prefix[b], carry);
-  __syncthreads();
   if (tid == 0) {
     double tot = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
     for (int k = 0; k < 4; ++k) tot = fmax(tot, wmax[k]);
     ts.prefix[nb] = tot;
-    w0[0] = lse_combine(lse_combine(w0[0], w0[1]), lse_combine(w0[2], w0[3]));
-    const double baseline = fresh ? logN : w0[0].m + log(w0[0].s);
+    const double baseline = fresh ? logN : M0 + log(S0);
     const int t = sc_in->step;
     const int idx = t - 1 - sc_in->ybase;
     BfScalars s = *sc_in;
@@ -263,11 +270,11 @@ __global__ void __launch_bounds__(256)
       s.ess = (double)N;
       s.typeless = 1;
     } else {
-      const double lse = M + log(S);
+      const double lse = M1 + log(S);
       double ess = (S * S) / S2;
       if (!(ess <= (double)N)) ess = (double)N;  // ESS <= N exactly; clamp rounding
       s.lse = lse;
-      s.mxn = (double)((T)M - (T)lse);
+      s.mxn = (double)((T)M1 - (T)lse);
       s.s1 = S;
       s.ess = ess;
       s.typeless = 0;
@@ -277,7 +284,7 @@ __global__ void __launch_bounds__(256)
         if (ll_steps) ll_steps[idx] = ll;
       }
     }
-    s.total_w = ts.prefix[nb];
+    s.total_w = tot;
     if (record) {
       if (rs_steps) rs_steps[idx] = rs ? 1 : 0;
       s.last_resampled = rs ? 1 : 0;
