@@ -494,11 +494,11 @@ void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
   const double logN = std::log((double)bf->N);
   const TileScratch ts = tile_scratch(bf);
   if (bf->prec == SSM_F32)
-    hipLaunchKernelGGL(k_pf_finish<float>, dim3(1), dim3(256), 0, s, bf->N, nb, bf->sc, bf->sc,
-                       threshold_n, logN, ts, bf->ll_steps, bf->rs_steps, record);
+    launch_pf_finish<float>(s, bf->N, nb, bf->sc, threshold_n, logN, ts, bf->ll_steps,
+                            bf->rs_steps, record);
   else
-    hipLaunchKernelGGL(k_pf_finish<double>, dim3(1), dim3(256), 0, s, bf->N, nb, bf->sc, bf->sc,
-                       threshold_n, logN, ts, bf->ll_steps, bf->rs_steps, record);
+    launch_pf_finish<double>(s, bf->N, nb, bf->sc, threshold_n, logN, ts, bf->ll_steps,
+                             bf->rs_steps, record);
 }
 
 // Enqueues one full step (resample kernel, step kernel, finish kernel) on stream s.

@@ -152,26 +152,38 @@ __global__ void __launch_bounds__(kTileThreads)
 
 // One block: fold the tile partials, finish the step, prepare the next resampling.
 // T is the filter's type: the per-tile scale factors exp(m1_b - M) are evaluated in T.
-template <typename T>
-__global__ void __launch_bounds__(256)
+// Each thread owns a contiguous chunk of at most CH tiles held in registers (one load
+// phase), so the kernel handles nb <= 256 * CH tiles; the host picks CH.
+template <typename T, int CH, int NT>
+__global__ void __launch_bounds__(NT)
     k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
                 double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
                 int record) {
-  __shared__ double red[4][4];
-  __shared__ double wtot[4], wmax[4];
+  constexpr int NW = NT / 64;
+  __shared__ double red[3][NW];
+  __shared__ double wtot[NW], wmax[NW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const bool rs = threshold_n >= sc_in->ess;
   const bool fresh = rs || sc_in->typeless;
-  const double* pm0 = ts.partials;
-  const double* ps0 = ts.partials + nb;
-  const double* pm1 = ts.partials + 2 * nb;
-  const double* ptau = ts.partials + 3 * nb;
-  const double* ps2 = ts.partials + 4 * nb;
+  const int chunk = (nb + NT - 1) / NT;
+  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  double m0[CH], s0[CH], m1[CH], tau[CH], s2[CH];
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    const int b = lo + q;
+    const bool in = b < hi;
+    m0[q] = in ? ts.partials[b] : -INFINITY;
+    s0[q] = in ? ts.partials[nb + b] : 0.0;
+    m1[q] = in ? ts.partials[2 * nb + b] : -INFINITY;
+    tau[q] = in ? ts.partials[3 * nb + b] : 0.0;
+    s2[q] = in ? ts.partials[4 * nb + b] : 0.0;
+  }
   // maxima of the tile maxima (exact, any order)
   double M1 = -INFINITY, M0 = -INFINITY;
-  for (int b = tid; b < nb; b += 256) {
-    M1 = fmax(M1, pm1[b]);
-    M0 = fmax(M0, pm0[b]);
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    M1 = fmax(M1, m1[q]);
+    M0 = fmax(M0, m0[q]);
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) {
@@ -183,28 +195,22 @@ __global__ void __launch_bounds__(256)
     red[1][wv] = M0;
   }
   __syncthreads();
-  M1 = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-  M0 = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+  M1 = red[0][0];
+  M0 = red[1][0];
+  for (int k = 1; k < NW; ++k) {
+    M1 = fmax(M1, red[0][k]);
+    M0 = fmax(M0, red[1][k]);
+  }
   __syncthreads();
-  // scaled sums: one exponential per tile, sums in FP64, fixed order.  Each thread owns
-  // a contiguous chunk of tiles here and in the scan below, so it only rereads its own
-  // scale factors.
-  const int chunk = (nb + 255) / 256;
-  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
+  // scaled sums: one exponential per tile, sums in FP64, fixed order
+  double f[CH];
   double S = 0.0, S2 = 0.0, S0 = 0.0;
-  for (int b = lo; b < hi; ++b) {
-    const double m1 = pm1[b];
-    if (m1 != -INFINITY) {
-      const double f = (double)exp_w((T)(m1 - M1));
-      ts.blk_c[b] = f;  // scale factor; divided by S below
-      S += ptau[b] * f;
-      S2 += ps2[b] * f * f;
-    } else {
-      ts.blk_c[b] = 0.0;
-    }
-    ts.blk_m[b] = m1;
-    const double m0 = pm0[b];
-    if (!fresh && m0 != -INFINITY) S0 += ps0[b] * (double)exp_w((T)(m0 - M0));
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    f[q] = (m1[q] != -INFINITY) ? (double)exp_w((T)(m1[q] - M1)) : 0.0;
+    S += tau[q] * f[q];
+    S2 += s2[q] * f[q] * f[q];
+    if (!fresh && m0[q] != -INFINITY) S0 += s0[q] * (double)exp_w((T)(m0[q] - M0));
   }
   S = wave_sum(S);
   S2 = wave_sum(S2);
@@ -215,18 +221,22 @@ __global__ void __launch_bounds__(256)
     red[2][wv] = S0;
   }
   __syncthreads();
-  S = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-  S2 = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
-  S0 = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+  S = red[0][0];
+  S2 = red[1][0];
+  S0 = red[2][0];
+  for (int k = 1; k < NW; ++k) {
+    S += red[0][k];
+    S2 += red[1][k];
+    S0 += red[2][k];
+  }
   const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
   const double invS = dead ? 0.0 : 1.0 / S;
-  __syncthreads();
-  // c_b = f_b / S, then the exclusive scan of c_b tau_b over the same chunks
+  // c_b = f_b / S and the exclusive scan of c_b tau_b over the chunks
   double cs = 0.0;
-  for (int b = lo; b < hi; ++b) {
-    const double c = ts.blk_c[b] * invS;
-    ts.blk_c[b] = c;
-    cs += c * ptau[b];
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    f[q] *= invS;  // now c_b
+    cs += f[q] * tau[q];
   }
   const double incl = wave_inclusive_scan(cs);
   double run = wave_shfl_up(incl, 1);
@@ -234,15 +244,17 @@ __global__ void __launch_bounds__(256)
   if (lane == 63) wtot[wv] = incl;
   __syncthreads();
   for (int k = 0; k < wv; ++k) run += wtot[k];
-  // Write the prefixes, then make them nondecreasing with an exact prefix max: the
-  // resampler's tiles meet at these values, so they must never step backwards.
+  // prefixes, made nondecreasing with an exact prefix max: the resampler's tiles meet at
+  // these values, so they must never step backwards
+  double pre[CH];
   double pmax = -INFINITY;
-  for (int b = lo; b < hi; ++b) {
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
     pmax = fmax(pmax, run);
-    ts.prefix[b] = pmax;
-    run += ts.blk_c[b] * ptau[b];
+    pre[q] = pmax;
+    run += f[q] * tau[q];
   }
-  double cm = pmax;
+  double cm = (lo < hi) ? pmax : -INFINITY;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const double o = wave_shfl_up(cm, d);
@@ -253,10 +265,19 @@ __global__ void __launch_bounds__(256)
   if (lane == 0) carry = -INFINITY;
   __syncthreads();
   for (int k = 0; k < wv; ++k) carry = fmax(carry, wmax[k]);
-  for (int b = lo; b < hi; ++b) ts.prefix[b] = fmax(ts.prefix[b], carry);
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    const int b = lo + q;
+    if (b < hi) {
+      ts.prefix[b] = fmax(pre[q], carry);
+      ts.blk_c[b] = f[q];
+      ts.blk_m[b] = m1[q];
+    }
+  }
   if (tid == 0) {
-    double tot = ((wtot[0] + wtot[1]) + wtot[2]) + wtot[3];
-    for (int k = 0; k < 4; ++k) tot = fmax(tot, wmax[k]);
+    double tot = wtot[0];
+    for (int k = 1; k < NW; ++k) tot += wtot[k];
+    for (int k = 0; k < NW; ++k) tot = fmax(tot, wmax[k]);
     ts.prefix[nb] = tot;
     const double baseline = fresh ? logN : M0 + log(S0);
     const int t = sc_in->step;
@@ -292,6 +313,27 @@ __global__ void __launch_bounds__(256)
     }
     *sc = s;
   }
+}
+
+// Enqueues the finish kernel with a register chunk that fits nb tiles.
+template <typename T>
+inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double threshold_n,
+                             double logN, const TileScratch& ts, double* ll_steps,
+                             int32_t* rs_steps, int record) {
+#define SSM_FINISH(CH, NT)                                                                  \
+  hipLaunchKernelGGL((k_pf_finish<T, CH, NT>), dim3(1), dim3(NT), 0, s, N, nb, sc, sc,     \
+                     threshold_n, logN, ts, ll_steps, rs_steps, record)
+  if (nb <= 256)
+    SSM_FINISH(1, 256);
+  else if (nb <= 1024)
+    SSM_FINISH(4, 256);
+  else if (nb <= 4096)
+    SSM_FINISH(4, 1024);
+  else if (nb <= 16384)
+    SSM_FINISH(16, 1024);
+  else
+    SSM_FINISH(64, 1024);
+#undef SSM_FINISH
 }
 
 }  // namespace ssm

@@ -374,9 +374,8 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   hipLaunchKernelGGL(k_tile_partials<float>, dim3(nb), dim3(kTileThreads), 0, s, rb->N,
                      (const float*)rb->logw, (const float*)rb->nl_pre, (const BfScalars*)rb->sc,
                      a.threshold_n, ts.partials);
-  hipLaunchKernelGGL(k_pf_finish<float>, dim3(1), dim3(256), 0, s, rb->N, nb,
-                     (const BfScalars*)rb->sc, rb->sc, a.threshold_n, p.logN, ts,
-                     rb->ll_steps, rb->rs_steps, 1);
+  launch_pf_finish<float>(s, rb->N, nb, rb->sc, a.threshold_n, p.logN, ts, rb->ll_steps,
+                          rb->rs_steps, 1);
   SSM_HIP_CHECK(rb->ctx, hipGetLastError());
   rb->cur ^= 1;
   return SSM_OK;

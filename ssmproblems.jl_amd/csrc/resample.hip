@@ -12,10 +12,11 @@
 // Design.  Weights are cut into tiles of 1024 (see pf_common.h).  Tile prefixes come either
 // from a particle filter's finish kernel (its tile partials already hold each tile's
 // weight sum) or, for explicit weight vectors, from a tile-sum kernel plus a one-block
-// scan.  The main kernel rescans one tile in FP64 on top of its prefix, turns the prefix
-// sums into cumulative offspring counts O_j in LDS, and fills the tile's contiguous output
-// range [O_{tile start}, O_{tile end}) with one thread per output slot (binary search in
-// LDS), so the ancestor write is balanced and coalesced however uneven the weights are.
+// scan.  The main kernel rescans one tile in FP64 on top of its prefix (thread-serial over
+// 4 consecutive particles, then one block scan), turns the prefix sums into cumulative
+// offspring counts O_j, and writes each particle's output slots [O_{j-1}, O_j) from the
+// thread that owns it; ranges longer than 32 slots are written by the whole block, so
+// degenerate weights stay balanced.
 // The FP64 prefix sums never reach HBM except for Multinomial, whose queries are unordered
 // and need a global binary search.  Neighbouring tiles compute their shared boundary count
 // from the same stored prefix, and every prefix sequence is made nondecreasing with an exact
@@ -148,9 +149,9 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
   double cs = 0.0;
   for (int k = lo; k < hi; ++k) cs += tile_sum[k];
   double tot;
-  const double incl = block_incl_scan_d(cs, lds, &tot);
-  double run = incl - cs;  // exclusive start of this chunk
-  // recompute the exclusive start exactly as the scan's own prefix of the previous threads
+  (void)block_incl_scan_d(cs, lds, &tot);  // leaves the wave totals in lds
+  // exclusive start of this thread's chunk, from the same wave scan and wave totals
+  double run;
   {
     const int w = tid / kWave, lane = tid & (kWave - 1);
     double ex = wave_shfl_up(wave_inclusive_scan(cs), 1);
@@ -207,18 +208,61 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
   return clamp_floor(r + uk, n);
 }
 
-// Main kernel: tile rescan -> cumulative offspring in LDS -> balanced ancestor fill.
-// For MULTINOMIAL it writes the (nondecreasing) prefix sums W to Wbuf instead.
+// Exclusive FP64 scan of one value per thread across the block (thread order, fixed
+// order); *tot = block total.
+__device__ __forceinline__ double block_excl_scan_d(double v, double* lds, double* tot) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  const double incl = wave_inclusive_scan(v);
+  double ex = wave_shfl_up(incl, 1);
+  if (lane == 0) ex = 0.0;
+  __syncthreads();
+  if (lane == kWave - 1) lds[w] = incl;
+  __syncthreads();
+  for (int k = 0; k < w; ++k) ex += lds[k];
+  *tot = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+  return ex;
+}
+
+// Exclusive max-scan of one value per thread across the block (exact).
+__device__ __forceinline__ double block_excl_max_d(double v, double* lds) {
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double o = wave_shfl_up(v, d);
+    if (lane >= d) v = fmax(v, o);
+  }
+  double ex = wave_shfl_up(v, 1);
+  if (lane == 0) ex = -INFINITY;
+  __syncthreads();
+  if (lane == kWave - 1) lds[w] = v;
+  __syncthreads();
+  for (int k = 0; k < w; ++k) ex = fmax(ex, lds[k]);
+  return ex;
+}
+
+// Slots a thread writes itself; threads with more hand their range to the whole block.
+constexpr int kDirectSlots = 32;
+
+// Main kernel: tile rescan -> cumulative offspring counts -> ancestor fill.  Thread t owns
+// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial
+// scan plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by
+// its own thread (neighbouring threads write neighbouring ranges); a thread whose range
+// exceeds kDirectSlots hands it to the whole block, which keeps degenerate weights (one
+// particle taking every slot) balanced.  For MULTINOMIAL the kernel writes the
+// nondecreasing prefix sums W to Wbuf instead.
 template <int MODE>
 __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
   if (!rs_active(p)) return;
   __shared__ double lds[kRsThreads / kWave];
-  __shared__ int O[kTile];
+  __shared__ int olast[kRsThreads];
+  __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
+  __shared__ int nheavy;
   const int n = p.n;
   const int b = blockIdx.x;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
   const int cnt = min(kTile, n - tile0);
+  const int t = threadIdx.x;
   const bool typeless = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64) && p.sc->typeless;
   double P, Pnext, Wn;
   if (typeless) {
@@ -230,38 +274,38 @@ __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
     Pnext = p.prefix[b + 1];
     Wn = p.prefix[nb];
   }
-  // FP64 scan of the tile in particle order: rows of 256 (striped items), row by row.
+  if (t == 0) nheavy = 0;
+  // thread-serial scan of 4 consecutive weights, then one block scan of the thread sums
   double W[kTileItems];
-  double rowoff = 0.0;
+  double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
-    const int j = tile0 + k * kRsThreads + threadIdx.x;
+    const int j = tile0 + kTileItems * t + k;
     double wj = 0.0;
     if (j < n) {
       wj = rs_weight<MODE>(p, j, b);
       if (p.w_debug64) p.w_debug64[j] = wj;
     }
-    double rowtot;
-    const double incl = block_incl_scan_d(wj, lds, &rowtot);
-    W[k] = P + (rowoff + incl);
-    rowoff += rowtot;
+    acc += wj;
+    W[k] = acc;
   }
-  // make W nondecreasing across the tile (exact running max in particle order)
-  double carry = -INFINITY;
+  double tot;
+  const double ex = block_excl_scan_d(acc, lds, &tot);
+  (void)tot;
 #pragma unroll
-  for (int k = 0; k < kTileItems; ++k) {
-    double rowmax;
-    const double m = block_incl_max_d(W[k], lds, &rowmax);
-    W[k] = fmax(m, carry);
-    carry = fmax(carry, rowmax);
-  }
+  for (int k = 0; k < kTileItems; ++k) W[k] = P + (ex + W[k]);
+  // nondecreasing in particle order (exact running max)
+#pragma unroll
+  for (int k = 1; k < kTileItems; ++k) W[k] = fmax(W[k], W[k - 1]);
+  const double carry = block_excl_max_d(W[kTileItems - 1], lds);
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
   const uint32_t step = rs_step(p);
   if (p.kind == SSM_MULTINOMIAL) {
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
-      const int pos = k * kRsThreads + threadIdx.x;
-      const int j = tile0 + pos;
-      if (j < n) p.Wbuf[j] = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
+      const int pos = kTileItems * t + k;
+      if (pos < cnt) p.Wbuf[tile0 + pos] = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
     }
     return;
   }
@@ -270,30 +314,43 @@ __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
                         : (p.u ? p.u[0] : resample_uniform(p.seed, step, 0u));
   const int o_prev = (b == 0) ? 0 : offspring_of(p, P, Wn, u0, step);
   const int o_end = (b == nb - 1) ? n : offspring_of(p, Pnext, Wn, u0, step);
+  int O[kTileItems];
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
-    const int pos = k * kRsThreads + threadIdx.x;
-    if (pos < cnt) {
-      int o;
-      if (pos == cnt - 1)
-        o = o_end;
-      else
-        o = min(max(offspring_of(p, W[k], Wn, u0, step), o_prev), o_end);
-      O[pos] = o;
+    const int pos = kTileItems * t + k;
+    if (pos >= cnt - 1)
+      O[k] = o_end;  // the tile's last particle closes its range; positions past n own none
+    else
+      O[k] = min(max(offspring_of(p, W[k], Wn, u0, step), o_prev), o_end);
+  }
+  olast[t] = O[kTileItems - 1];
+  __syncthreads();
+  const int start = (t == 0) ? o_prev : olast[t - 1];
+  if (O[kTileItems - 1] - start <= kDirectSlots) {
+    int s0 = start;
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+      for (int s = s0; s < O[k]; ++s) p.anc[s] = tile0 + kTileItems * t + k;
+      s0 = max(s0, O[k]);
     }
+  } else {
+    const int h = atomicAdd(&nheavy, 1);
+    heavy[h][0] = start;
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) heavy[h][k + 1] = O[k];
+    heavy[h][kTileItems + 1] = t;
   }
   __syncthreads();
-  // Slot s belongs to the first position with O > s.
-  for (int s = o_prev + (int)threadIdx.x; s < o_end; s += kRsThreads) {
-    int lo = 0, hi = cnt - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (O[mid] > s)
-        hi = mid;
-      else
-        lo = mid + 1;
+  // heavy ranges: the whole block strides over each one
+  const int nh = nheavy;
+  for (int h = 0; h < nh; ++h) {
+    const int hs = heavy[h][0], he = heavy[h][kTileItems];
+    const int owner = heavy[h][kTileItems + 1];
+    for (int s = hs + t; s < he; s += kRsThreads) {
+      int k = 0;
+      while (k < kTileItems - 1 && heavy[h][k + 1] <= s) ++k;
+      p.anc[s] = tile0 + kTileItems * owner + k;
     }
-    p.anc[s] = tile0 + lo;
   }
 }
 

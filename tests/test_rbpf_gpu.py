@@ -20,7 +20,7 @@ TOL_X = 2e-5        # outer state, rtol/atol
 TOL_MU = 6e-4       # inner mean, relative to max(1, |mu|)
 TOL_SIG = 4e-4      # inner covariance, relative to its largest entry
 TOL_LW = (5e-4, 1e-4)  # live log-weights: atol + max(rtol, 4e-6 D Dy) * |logw|
-TOL_LL = 3e-4
+TOL_LL = 3e-4       # plus 4e-6 D Dy relative
 
 
 def rb_pair(orc, om0, os0, oa, ob, oq, mu0, S0, A, b, Cv, Q, H, c, R0, rs):
@@ -67,7 +67,7 @@ def _teacher_forced(ctx, orc, om, dm, N, kind, thresholds, ys, seed=2):
         live = o.logw > -30
         r_lw = max(TOL_LW[1], 4e-6 * om.D * om.Dy)
         np.testing.assert_allclose(lw[live], o.logw[live], atol=TOL_LW[0], rtol=r_lw)
-        assert ll == pytest.approx(ll_o, abs=TOL_LL), f"t={t}"
+        assert ll == pytest.approx(ll_o, abs=TOL_LL, rel=4e-6 * om.D * om.Dy), f"t={t}"
         typeless = False
 
 
