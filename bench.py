@@ -91,6 +91,33 @@ def cpu_baseline(p, ys, N, kind, threshold, budget_s=12.0):
     return N * steps / el, steps, el
 
 
+def copy_calibration(nbytes=256 << 20, reps=5):
+    """A plain device-to-device copy of known size: the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes nbytes)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": reps}))
+
+
+def traffic_from_profile(N, resampler, threshold):
+    """HBM bytes per launch of the step kernel from the committed counter summary for this
+    workload (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled by the copy
+    calibration), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "k_bf_step_lg4_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    want = f"N={N} resampler={resampler} threshold={threshold}"
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch") and want in line:
+                return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,8 +128,15 @@ def main():
     ap.add_argument("--resampler", default="systematic", choices=sorted(RESAMPLERS))
     ap.add_argument("--threshold", type=float, default=1.0)
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--reps", type=int, default=200,
+                    help="back-to-back launches per kernel for the per-launch timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the filtering "
               "path runs on a single MI355X; see DESIGN.md for how it would shard)",
@@ -147,13 +181,17 @@ def main():
     # Per-kernel timing with HIP events (plain launches, same kernels and stream).
     import ctypes as C
     ctx.check(lib.ssm_bf_initialise(st.ptr, None))
-    ms = (C.c_double * 3)()
+    ms = (C.c_double * 6)()
     llp = C.c_double()
     TP = min(args.profile_steps, args.T)
     yp = np.ascontiguousarray(ys[:TP])
     ctx.check(lib.ssm_bf_profile(st.ptr, 1, TP, yp.ctypes.data_as(C.POINTER(C.c_double)),
-                                 args.threshold, kind, ms, C.byref(llp)))
-    ms_rs, ms_st, ms_fin = ms[0], ms[1], ms[2]
+                                 args.threshold, kind, args.reps, ms, C.byref(llp)))
+    # per-launch durations from back-to-back launches (what a profiler reports per
+    # dispatch); the in-sequence figures also carry the gap an event pair adds
+    ms_rs, ms_st, ms_fin = ms[3], ms[4], ms[5]
+    log(f"in-sequence (event pair per kernel): resample {ms[0] * 1e3:.2f} us, step "
+        f"{ms[1] * 1e3:.2f} us, finish {ms[2] * 1e3:.2f} us")
     # Algorithmic bytes per particle and launch: step kernel 40 B
     # (ancestor 4 + gathered state 16 + new state 16 + log-weight 4 when resampling;
     # state 16 + log-weight 4 read and 16 + 4 written otherwise); resampler 8 B
@@ -199,10 +237,13 @@ def main():
         "loglik_kalman": ll_kf,
         "loglik_rel_err": ll_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(args.N, args.resampler, args.threshold),
                      "kernel": "k_bf_step<float,4,2>", "bytes_per_launch": step_bytes,
                      "us_per_launch": ms_st * 1e3,
-                     "resample_us_per_step": ms_rs * 1e3, "finish_us_per_step": ms_fin * 1e3},
+                     "us_per_launch_in_sequence": ms[1] * 1e3,
+                     "resample_us_per_launch": ms_rs * 1e3, "finish_us_per_launch": ms_fin * 1e3,
+                     "step_total_us": ms_per_step * 1e3 / args.T},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

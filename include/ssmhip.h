@@ -159,12 +159,16 @@ int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw);
 /* Device time of the last ssm_bf_run, from HIP events around it (ms). */
 int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
 /*
- * Same steps as ssm_bf_run, launched one kernel at a time with HIP events around each:
- * ms_kernels[0] resampler, [1] fused predict/update step kernel, [2] finish kernel, each
- * the mean duration per step (ms); ll_total as in ssm_bf_run.
+ * Kernel timing with HIP events on the filter's stream.  Runs the steps of ssm_bf_run one
+ * kernel at a time with events around each: ms_kernels[0] resampler, [1] fused
+ * predict/update step kernel, [2] finish kernel, each the mean per step (ms; includes the
+ * launch gap an event pair adds).  Then, on the final state, launches each of the three
+ * kernels `reps` times back to back between two events: ms_kernels[3..5] are the mean
+ * durations per launch (ms), comparable to a profiler's per-dispatch duration (reps = 0
+ * skips this and writes 0).  ll_total as in ssm_bf_run.  ms_kernels holds 6 doubles.
  */
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
-                       int32_t resampler, double* ms_kernels, double* ll_total);
+                       int32_t resampler, int32_t reps, double* ms_kernels, double* ll_total);
 
 /* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */
 /*

@@ -130,6 +130,87 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
   return v;
 }
 
+// ---- DPP wave primitives (gfx9 wave64) ---------------------------------------------
+// Cross-lane moves on the VALU: row_shr:k within rows of 16, then row_bcast:15 / :31 to
+// carry row totals across the wave.  Lanes without a source keep `old` (the identity).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v),
+                                                    CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_d(double old, double v) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL,
+                                             ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL,
+                                             ROWS, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+// Inclusive sum scan over the 64 lanes (fixed association).
+template <typename T>
+__device__ __forceinline__ T wave_scan_dpp(T v);
+template <>
+__device__ __forceinline__ double wave_scan_dpp<double>(double v) {
+  v += dpp_d<kDppRowShr1>(0.0, v);
+  v += dpp_d<kDppRowShr2>(0.0, v);
+  v += dpp_d<kDppRowShr4>(0.0, v);
+  v += dpp_d<kDppRowShr8>(0.0, v);
+  v += dpp_d<kDppRowBcast15, 0xA>(0.0, v);
+  v += dpp_d<kDppRowBcast31, 0xC>(0.0, v);
+  return v;
+}
+template <>
+__device__ __forceinline__ float wave_scan_dpp<float>(float v) {
+  v += dpp_f<kDppRowShr1>(0.f, v);
+  v += dpp_f<kDppRowShr2>(0.f, v);
+  v += dpp_f<kDppRowShr4>(0.f, v);
+  v += dpp_f<kDppRowShr8>(0.f, v);
+  v += dpp_f<kDppRowBcast15, 0xA>(0.f, v);
+  v += dpp_f<kDppRowBcast31, 0xC>(0.f, v);
+  return v;
+}
+// Inclusive max scan over the 64 lanes (exact).
+__device__ __forceinline__ double wave_scan_max_dpp(double v) {
+  const double ninf = -INFINITY;
+  v = fmax(v, dpp_d<kDppRowShr1>(ninf, v));
+  v = fmax(v, dpp_d<kDppRowShr2>(ninf, v));
+  v = fmax(v, dpp_d<kDppRowShr4>(ninf, v));
+  v = fmax(v, dpp_d<kDppRowShr8>(ninf, v));
+  v = fmax(v, dpp_d<kDppRowBcast15, 0xA>(ninf, v));
+  v = fmax(v, dpp_d<kDppRowBcast31, 0xC>(ninf, v));
+  return v;
+}
+__device__ __forceinline__ float wave_scan_max_dpp(float v) {
+  const float ninf = -INFINITY;
+  v = fmaxf(v, dpp_f<kDppRowShr1>(ninf, v));
+  v = fmaxf(v, dpp_f<kDppRowShr2>(ninf, v));
+  v = fmaxf(v, dpp_f<kDppRowShr4>(ninf, v));
+  v = fmaxf(v, dpp_f<kDppRowShr8>(ninf, v));
+  v = fmaxf(v, dpp_f<kDppRowBcast15, 0xA>(ninf, v));
+  v = fmaxf(v, dpp_f<kDppRowBcast31, 0xC>(ninf, v));
+  return v;
+}
+// Lane 63's value (the scan total) broadcast to the wave.
+__device__ __forceinline__ double lane63(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+// Wave reductions in a fixed order, result in every lane.
+template <typename T>
+__device__ __forceinline__ T wave_sum_dpp(T v) {
+  return lane63(wave_scan_dpp<T>(v));
+}
+template <typename T>
+__device__ __forceinline__ T wave_max_dpp(T v) {
+  return lane63(wave_scan_max_dpp(v));
+}
+
 // (max, sum exp(v - max), sum exp(2 (v - max))) triples: log-sum-exp and ESS partials.
 __device__ __forceinline__ float exp_t(float v) { return __expf(v); }
 __device__ __forceinline__ double exp_t(double v) { return exp(v); }

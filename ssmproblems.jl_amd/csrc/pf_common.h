@@ -36,34 +36,6 @@ struct TileScratch {
   double* prefix;    // nb + 1: exclusive tile prefixes of the weights, prefix[nb] = total
 };
 
-__device__ __forceinline__ double block_sum_fixed(double v, double* lds) {
-  // wave butterfly, then the waves in index order; result broadcast to every thread
-  v = wave_sum(v);
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  __syncthreads();
-  if (lane == 0) lds[w] = v;
-  __syncthreads();
-  double r = 0.0;
-  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) r += lds[k];
-  return r;
-}
-
-template <typename T>
-__device__ __forceinline__ T block_max(T v, T* lds) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const T o = wave_shfl_xor(v, m);
-    v = o > v ? o : v;
-  }
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  __syncthreads();
-  if (lane == 0) lds[w] = v;
-  __syncthreads();
-  T r = (T)-INFINITY;
-  for (int k = 0; k < (int)(blockDim.x / kWave); ++k) r = lds[k] > r ? lds[k] : r;
-  return r;
-}
-
 // Accurate exponential of the weights (the resampler recomputes it bit for bit).
 __device__ __forceinline__ float exp_w(float v) { return expf(v); }
 __device__ __forceinline__ double exp_w(double v) { return exp(v); }
@@ -71,34 +43,42 @@ __device__ __forceinline__ double exp_w(double v) { return exp(v); }
 // Writes tile b's partial from each thread's 4 items.  nl: pre-update normalised
 // log-weights (ignored when `fresh`), l: updated log-weights, valid: item < N.
 // Partials are stored field-major: field k of tile b at partials[k * nb + b].
+// Wave reductions run on DPP (fixed association); the block needs two barriers.
 template <typename T>
 __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
                              const bool valid[kTileItems], bool fresh, double* partials,
                              int nb) {
-  __shared__ T lds_t[kTileThreads / kWave];
-  __shared__ double lds_d[kTileThreads / kWave];
-  // baseline partial (pre-update), FP32 (max, sum) per thread -> FP64 across waves
-  LseAcc<T> a0 = lse_empty<T>();
-  if (!fresh) {
-    T m = (T)-INFINITY;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k)
-      if (valid[k]) m = nl[k] > m ? nl[k] : m;
-    T s = 0;
-    if (m != (T)-INFINITY) {
-#pragma unroll
-      for (int k = 0; k < kTileItems; ++k)
-        if (valid[k]) s += exp_t(nl[k] - m);
-    }
-    a0 = LseAcc<T>{m, s, (T)0};
-  }
-  a0 = wave_lse(a0);
+  constexpr int NW = kTileThreads / kWave;
+  __shared__ T lds_m1[NW];
+  __shared__ double lds_r[4][NW];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   // tile max of the updated log-weights
   T mt = (T)-INFINITY;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k)
     if (valid[k]) mt = l[k] > mt ? l[k] : mt;
-  const T m1 = block_max<T>(mt, lds_t);
+  mt = wave_max_dpp(mt);
+  if (lane == 0) lds_m1[w] = mt;
+  // baseline partial (pre-update): per-wave (max, sum exp(nl - max)); `fresh` is uniform
+  T m0 = (T)-INFINITY, s0 = (T)0;
+  if (!fresh) {
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k)
+      if (valid[k]) m0 = nl[k] > m0 ? nl[k] : m0;
+    const T mw = wave_max_dpp(m0);
+    T s = (T)0;
+    if (mw != (T)-INFINITY) {
+#pragma unroll
+      for (int k = 0; k < kTileItems; ++k)
+        if (valid[k]) s += exp_t(nl[k] - mw);
+    }
+    s0 = wave_sum_dpp(s);
+    m0 = mw;
+  }
+  __syncthreads();
+  T m1 = lds_m1[0];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) m1 = lds_m1[k] > m1 ? lds_m1[k] : m1;
   double tau = 0.0, s2 = 0.0;
   if (m1 != (T)-INFINITY) {
 #pragma unroll
@@ -110,21 +90,30 @@ __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
       }
     }
   }
-  tau = block_sum_fixed(tau, lds_d);
-  s2 = block_sum_fixed(s2, lds_d);
-  __shared__ LseAcc<double> w0[kTileThreads / kWave];
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  if (lane == 0) w0[w] = LseAcc<double>{(double)a0.m, (double)a0.s, 0.0};
+  tau = wave_sum_dpp(tau);
+  s2 = wave_sum_dpp(s2);
+  if (lane == 0) {
+    lds_r[0][w] = tau;
+    lds_r[1][w] = s2;
+    lds_r[2][w] = (double)m0;
+    lds_r[3][w] = (double)s0;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    LseAcc<double> b0 = w0[0];
-    for (int k = 1; k < kTileThreads / kWave; ++k) b0 = lse_combine(b0, w0[k]);
+    LseAcc<double> b0{lds_r[2][0], lds_r[3][0], 0.0};
+    double tt = lds_r[0][0], ss = lds_r[1][0];
+#pragma unroll
+    for (int k = 1; k < NW; ++k) {
+      b0 = lse_combine(b0, LseAcc<double>{lds_r[2][k], lds_r[3][k], 0.0});
+      tt += lds_r[0][k];
+      ss += lds_r[1][k];
+    }
     const int b = blockIdx.x;
     partials[b] = b0.m;
     partials[nb + b] = b0.s;
     partials[2 * nb + b] = (double)m1;
-    partials[3 * nb + b] = tau;
-    partials[4 * nb + b] = s2;
+    partials[3 * nb + b] = tt;
+    partials[4 * nb + b] = ss;
   }
 }
 
@@ -185,11 +174,8 @@ __global__ void __launch_bounds__(NT)
     M1 = fmax(M1, m1[q]);
     M0 = fmax(M0, m0[q]);
   }
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    M1 = fmax(M1, wave_shfl_xor(M1, m));
-    M0 = fmax(M0, wave_shfl_xor(M0, m));
-  }
+  M1 = wave_max_dpp(M1);
+  M0 = wave_max_dpp(M0);
   if (lane == 0) {
     red[0][wv] = M1;
     red[1][wv] = M0;
@@ -212,9 +198,9 @@ __global__ void __launch_bounds__(NT)
     S2 += s2[q] * f[q] * f[q];
     if (!fresh && m0[q] != -INFINITY) S0 += s0[q] * (double)exp_w((T)(m0[q] - M0));
   }
-  S = wave_sum(S);
-  S2 = wave_sum(S2);
-  S0 = wave_sum(S0);
+  S = wave_sum_dpp(S);
+  S2 = wave_sum_dpp(S2);
+  S0 = wave_sum_dpp(S0);
   if (lane == 0) {
     red[0][wv] = S;
     red[1][wv] = S2;
@@ -238,7 +224,7 @@ __global__ void __launch_bounds__(NT)
     f[q] *= invS;  // now c_b
     cs += f[q] * tau[q];
   }
-  const double incl = wave_inclusive_scan(cs);
+  const double incl = wave_scan_dpp(cs);
   double run = wave_shfl_up(incl, 1);
   if (lane == 0) run = 0.0;
   if (lane == 63) wtot[wv] = incl;
@@ -254,12 +240,7 @@ __global__ void __launch_bounds__(NT)
     pre[q] = pmax;
     run += f[q] * tau[q];
   }
-  double cm = (lo < hi) ? pmax : -INFINITY;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double o = wave_shfl_up(cm, d);
-    if (lane >= d) cm = fmax(cm, o);
-  }
+  double cm = wave_scan_max_dpp((lo < hi) ? pmax : -INFINITY);
   if (lane == 63) wmax[wv] = cm;
   double carry = wave_shfl_up(cm, 1);
   if (lane == 0) carry = -INFINITY;

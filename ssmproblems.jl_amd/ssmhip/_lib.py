@@ -116,7 +116,7 @@ def _declare(L):
         "ssm_bf_export": ([_vp, _dp, _dp, _ip, _dp], i32),
         "ssm_bf_import": ([_vp, _dp, _dp], i32),
         "ssm_bf_last_timing": ([_vp, _dp, _dp], i32),
-        "ssm_bf_profile": ([_vp, i32, i32, _dp, dbl, i32, _dp, _dp], i32),
+        "ssm_bf_profile": ([_vp, i32, i32, _dp, dbl, i32, i32, _dp, _dp], i32),
         "ssm_kf_batch_create": ([_vp, i32, _ip, _ip, _dp, C.POINTER(C.c_int64), i32, _dp,
                                  C.POINTER(C.c_int64), C.POINTER(_vp)], i32),
         "ssm_kf_batch_run": ([_vp, _dp, _ip, _dp, _dp], i32),

@@ -57,7 +57,10 @@ def _teacher_forced(ctx, orc, m, N, prec, kind, thresholds, seed=1):
         r_lw = max(tol["lw"][2], 4e-6 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][2]
         a_lw = max(tol["lw"][0], 2e-5 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][0]
         np.testing.assert_allclose(lw_new[live], ost.logw[live], atol=a_lw, rtol=r_lw)
-        np.testing.assert_allclose(lw_new[~live], ost.logw[~live], rtol=tol["lw"][1])
+        # weights below e^-30 of the live ones: FP32 cancellation in the residual
+        # y - Hx - c dominates, growing with Dx * Dy like the live tolerance
+        r_dead = max(tol["lw"][1], 5e-6 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][1]
+        np.testing.assert_allclose(lw_new[~live], ost.logw[~live], rtol=r_dead)
         r_ll = max(1e-7, 4e-6 * m.Dx * m.Dy) if prec == "f32" else 1e-12
         assert ll == pytest.approx(ll_o, abs=tol["ll"], rel=r_ll), f"t={t}"
         x_gpu, lw_gpu = x_new, lw_new
