@@ -118,6 +118,213 @@ def traffic_from_profile(N, resampler, threshold):
     return None
 
 
+def kf_flops(Dx, Dy):
+    """Algorithmic flops of one Kalman predict + Joseph update, each product counted as 2mnk
+    with full outputs: A Sigma A^T 4Dx^3, (I-KH) Sigma (I-KH)^T 4Dx^3, H Sigma H^T
+    2Dx^2Dy + 2DxDy^2, Sigma H^T 2Dx^2Dy, the two solves for K 2DxDy^2, K H 2Dx^2Dy,
+    K R K^T 2DxDy^2 + 2Dx^2Dy."""
+    Dx = np.asarray(Dx, dtype=np.float64)
+    Dy = np.asarray(Dy, dtype=np.float64)
+    return 8 * Dx ** 3 + 8 * Dx ** 2 * Dy + 6 * Dx * Dy ** 2
+
+
+def kf_loglik_general(mu0, S0, A, b, Q, H, c, R, ys):
+    """Exact FP64 Kalman log-likelihood with the Joseph update (numpy)."""
+    mu, S = mu0.copy(), S0.copy()
+    ll = 0.0
+    for y in ys:
+        mu = A @ mu + b
+        S = A @ S @ A.T + Q
+        z = y - H @ mu - c
+        Sy = H @ S @ H.T + R
+        L = np.linalg.cholesky(Sy)
+        v = np.linalg.solve(L, z)
+        ll += -0.5 * (len(y) * np.log(2 * np.pi) + 2 * np.log(np.diag(L)).sum() + v @ v)
+        K = np.linalg.solve(Sy, H @ S).T
+        IKH = np.eye(len(mu)) - K @ H
+        mu = mu + K @ z
+        S = IKH @ S @ IKH.T + K @ R @ K.T
+        S = np.triu(S) + np.triu(S, 1).T
+    return ll
+
+
+def bench_kf(args, T=200, budget_s=10.0):
+    """Batched Kalman filters: B LG models, Dx ~ U{2..64}, Dy = Dx // 2, A = 0.9 x seeded
+    orthogonal, Q and R seeded Wishart, T = 200.  One bench step = one run of all B."""
+    import ssmhip
+    from ssmhip import workloads as W
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    log(f"device {name} CUs={cus}; generating {args.B} models")
+    w = W.batched_lg(args.B, T, seed=1234)
+    kb = ssmhip.KFBatch.from_packed(w["Dx"], w["Dy"], w["params"], w["param_off"], w["ys"],
+                                    w["y_off"], T, ctx=ctx)
+    for _ in range(args.warmup):
+        ll = kb.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(args.steps):
+        ll = kb.run()
+        dev_ms.append(kb.last_ms())
+    el = time.perf_counter() - t0
+    flops = float((kf_flops(w["Dx"], w["Dy"]) * T).sum())
+    ms_dev = float(np.mean(dev_ms))
+    achieved = flops / (ms_dev * 1e-3) / 1e12
+    # accuracy against an exact FP64 filter on a sample spanning the sizes
+    rng = np.random.default_rng(5)
+    sample = np.unique(np.concatenate([rng.choice(args.B, 48, replace=False),
+                                       [0, args.B - 1]]))
+    errs = []
+    for f in sample:
+        prm, ys_f = W.unpack_lg(w, int(f))
+        ref = kf_loglik_general(*prm, ys_f)
+        errs.append(abs(ll[f] - ref) / abs(ref))
+    log(f"{args.steps} runs in {el:.3f}s ({ms_dev:.1f} ms device per run); "
+        f"{achieved:.2f} TFLOP/s algorithmic; max ll rel err {max(errs):.2e}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        done, t1 = 0, time.perf_counter()
+        order = rng.permutation(args.B)
+        while time.perf_counter() - t1 < budget_s and done < args.B:
+            prm, ys_f = W.unpack_lg(w, int(order[done]))
+            oracle.kf_filter(oracle.LG(*prm), ys_f)
+            done += 1
+        secs = time.perf_counter() - t1
+        cpu = {"value": done / secs, "unit": "filters/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle kf_filter (oracle/oracle.c, single thread) on {done} "
+                         f"randomly chosen filters of the same batch, T={T}, {secs:.1f}s"}
+        log(f"cpu baseline {done / secs:.1f} filters/s")
+    out = {
+        "metric": "filters/s at stated loglik err (batched Kalman filters)",
+        "value": args.steps * args.B / el,
+        "unit": "filters/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 Cholesky of S)",
+        "data": "synthetic (seeded LG models and simulated observations)",
+        "config": {"workload": f"batched KalmanFilter, B={args.B}, Dx~U{{2..64}}, Dy=Dx//2, "
+                               f"T={T}, Joseph update", "B": args.B, "T": T},
+        "loglik_max_rel_err": float(max(errs)), "loglik_err_sample": int(len(sample)),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": achieved / 157.3, "traffic": None,
+                     "kernel": "k_kf_batch (3 size classes, one run)",
+                     "flops_per_run": flops, "ms_per_run_device": ms_dev},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def rbpf_model(D=32, Dy=8, seed=1234):
+    """Outer AR(1) x_t = 0.9 x_{t-1} + N(0, 0.3^2), stationary start; inner D-dim LG with
+    A = 0.9 x seeded orthogonal, Q = 0.1 I, H seeded Dy x D, R_t = exp(x_t) 0.2 I."""
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((D, D))
+    Qm, Rm = np.linalg.qr(G)
+    A = 0.9 * Qm * np.sign(np.diag(Rm))
+    H = rng.standard_normal((Dy, D))
+    oq = 0.3 ** 2
+    return dict(om0=0.0, os0=oq / (1 - 0.81), oa=0.9, ob=0.0, oq=oq, mu0=np.zeros(D),
+                Sigma0=np.eye(D), A=A, b=np.zeros(D), Cv=np.zeros(D), Q=0.1 * np.eye(D), H=H,
+                c=np.zeros(Dy), R0=0.2 * np.eye(Dy), rs=1.0)
+
+
+RB_KEYS = ("om0", "os0", "oa", "ob", "oq", "mu0", "Sigma0", "A", "b", "Cv", "Q", "H", "c",
+           "R0", "rs")
+
+
+def rbpf_simulate(m, T, seed):
+    rng = np.random.default_rng(seed)
+    D, Dy = m["A"].shape[0], m["H"].shape[0]
+    xo = m["om0"] + np.sqrt(m["os0"]) * rng.standard_normal()
+    z = m["mu0"] + np.linalg.cholesky(m["Sigma0"]) @ rng.standard_normal(D)
+    LQ = np.linalg.cholesky(m["Q"])
+    ys = np.zeros((T, Dy))
+    for t in range(T):
+        xprev = xo
+        xo = m["oa"] * xo + m["ob"] + np.sqrt(m["oq"]) * rng.standard_normal()
+        z = m["A"] @ z + m["b"] + m["Cv"] * xprev + LQ @ rng.standard_normal(D)
+        R = np.exp(m["rs"] * xo) * m["R0"]
+        ys[t] = m["H"] @ z + m["c"] + np.linalg.cholesky(R) @ rng.standard_normal(Dy)
+    return ys
+
+
+def bench_rbpf(args, N=16384, T=500, threshold=0.5, budget_s=10.0):
+    """RBPF: N particles, each an outer scalar and an inner 32-dim Kalman filter (FP32
+    32x32 covariance), T = 500, systematic resampling at ESS < N/2.  One bench step = one
+    full filter run."""
+    import ssmhip
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    log(f"device {name} CUs={cus}")
+    m = rbpf_model()
+    D, Dy = m["A"].shape[0], m["H"].shape[0]
+    ys = rbpf_simulate(m, T, 99)
+    dm = ssmhip.RBModel(*(m[k] for k in RB_KEYS))
+    st = ssmhip.DeviceRBParticles(ctx, dm, N, 2024)
+    lls = []
+
+    def one():
+        st.initialise()
+        return st.run(ys, threshold, 0)
+
+    for _ in range(args.warmup):
+        one()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(args.steps):
+        lls.append(one())
+        dev.append(st.last_ms())
+    el = time.perf_counter() - t0
+    value = args.steps * N * T / el
+    ms_dev = float(np.mean(dev))
+    fl = float(kf_flops(D, Dy))
+    achieved = fl * N * T / (ms_dev * 1e-3) / 1e12
+    log(f"{args.steps} filters in {el:.3f}s: {value:.3e} particle-steps/s; device "
+        f"{ms_dev:.1f} ms per run; {achieved:.2f} TFLOP/s algorithmic; ll {lls[-1]:.3f}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        om = oracle.RB(*(m[k] for k in RB_KEYS))
+        rng = np.random.default_rng(3)
+        ost = oracle.rbpf_initialise(om, rng.standard_normal(N))
+        steps, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < budget_s and steps < T:
+            oracle.rbpf_step(om, ost, ys[steps], threshold, 0, rng.random(1),
+                             rng.standard_normal(N))
+            steps += 1
+        secs = time.perf_counter() - t1
+        cpu = {"value": N * steps / secs, "unit": "particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"FP64 oracle rbpf_step (oracle/oracle.c, single thread) on the same "
+                         f"model with N={N}: {steps} steps in {secs:.1f}s"}
+        log(f"cpu baseline {N * steps / secs:.3e} particle-steps/s ({steps} steps)")
+    out = {
+        "metric": "particle-steps/s (RBPF, per-particle Kalman on MFMA)",
+        "value": value, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 Cholesky of S)",
+        "data": "synthetic (seeded hierarchical model and simulated observations)",
+        "config": {"workload": f"RBPF, outer AR(1), inner LG D={D} Dy={Dy}, R_t = exp(x_t) R0, "
+                               f"N={N}, T={T}, systematic at threshold {threshold}",
+                   "N": N, "T": T, "D": D, "Dy": Dy},
+        "loglik": lls[-1],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": achieved / 157.3, "traffic": None,
+                     "kernel": "k_rbpf_step + partials + finish + resampler (whole run)",
+                     "flops_per_particle_step": fl, "ms_per_run_device": ms_dev},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,15 +340,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
+    ap.add_argument("--workload", default="bf", choices=["bf", "kf", "rbpf"],
+                    help="bf: bootstrap filter (default); kf: batched Kalman filters; "
+                         "rbpf: Rao-Blackwellised PF")
+    ap.add_argument("--B", type=int, default=65536, help="kf: number of filters")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the filtering "
-              "path runs on a single MI355X; see DESIGN.md for how it would shard)",
+              "path runs on a single MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "kf":
+        return bench_kf(args)
+    if args.workload == "rbpf":
+        return bench_rbpf(args)
 
     import ssmhip
     ctx = ssmhip.default_context()
@@ -192,7 +407,7 @@ def main():
     ms_rs, ms_st, ms_fin = ms[3], ms[4], ms[5]
     log(f"in-sequence (event pair per kernel): resample {ms[0] * 1e3:.2f} us, step "
         f"{ms[1] * 1e3:.2f} us, finish {ms[2] * 1e3:.2f} us")
-    # Algorithmic bytes per particle and launch (see DESIGN.md): step kernel 40 B
+    # Algorithmic bytes per particle and launch: step kernel 40 B
     # (ancestor 4 + gathered state 16 + new state 16 + log-weight 4 when resampling;
     # state 16 + log-weight 4 read and 16 + 4 written otherwise); resampler 8 B
     # (log-weight read 4, ancestor write 4).

@@ -29,288 +29,23 @@
 #include <tuple>
 #include <vector>
 
-#include "device_common.h"
-#include "internal.h"
-#include "pf_common.h"
+#include "bf_dev.h"
 
 namespace ssm {
 
-constexpr int kMaxD = 16;
-constexpr int kMaxDy = 16;
-constexpr int kStepThreads = kTileThreads;
-constexpr int kStepItems = kTileItems;
-constexpr int kStepBlock = kTile;
-constexpr int kPadTo = kTile;
-
-// Per-filter model constants, precomputed on the host in FP64 and stored in `T`.
-// Column-major with leading dimension D (or Dy).
-template <typename T>
-struct BfModelDev {
-  T A[kMaxD * kMaxD];
-  T b[kMaxD];
-  T LQ[kMaxD * kMaxD];     // lower Cholesky factor of Q
-  T H[kMaxDy * kMaxD];
-  T c[kMaxDy];
-  T LRi[kMaxDy * kMaxDy];  // inverse of the lower Cholesky factor of R
-  T mu0[kMaxD];
-  T L0[kMaxD * kMaxD];     // lower Cholesky factor of Sigma0
-  T cst;                   // -1/2 (Dy log 2 pi + log det R)
-};
-
-template <typename T>
-struct StepParams {
-  int N, Npad, D, Dy, nblocks;
-  const T* __restrict__ xin;
-  T* __restrict__ xout;
-  T* __restrict__ logw;
-  const int32_t* __restrict__ anc;
-  const BfModelDev<T>* __restrict__ model;
-  const double* __restrict__ ys;
-  BfScalars* sc;
-  double threshold_n;
-  double logN;
-  uint64_t seed;
-  const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
-  double* __restrict__ partials;
-};
-
-// Standard normals z[0..D) of particle i at step t from the given stream.
-template <typename T, int MD>
-__device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t stream, uint32_t t,
-                                             uint32_t i, int D, T z[MD]);
-template <>
-__device__ __forceinline__ void draw_normals<float, 1>(uint64_t seed, uint32_t stream,
-                                                       uint32_t t, uint32_t i, int D,
-                                                       float z[1]) {
-  const U4 w = philox4x32_10(i, t, stream, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
-  float z1;
-  box_muller_f32(w.x, w.y, z[0], z1);
-}
-template <typename T, int MD>
-__device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t stream, uint32_t t,
-                                             uint32_t i, int D, T z[MD]) {
-  if constexpr (sizeof(T) == 4) {
-#pragma unroll
-    for (int j = 0; j < (MD + 3) / 4; ++j) {
-      if (4 * j < D) {
-        float q[4];
-        normals4_f32(seed, stream, t, i, (uint32_t)j, q);
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (4 * j + e < MD) z[4 * j + e] = q[e];
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < (MD + 1) / 2; ++j) {
-      if (2 * j < D) {
-        double q[2];
-        normals2_f64(seed, stream, t, i, (uint32_t)j, q);
-        z[2 * j] = q[0];
-        if (2 * j + 1 < MD) z[2 * j + 1] = q[1];
-      }
-    }
-  }
-}
-
-// Read access to the model constants inside the step kernel.  REG: copied to registers
-// (compile-time dimensions); otherwise read from the device struct (column-major).
-template <typename T, int MD, int MDY, bool REG>
-struct ModelView;
-
-template <typename T, int MD, int MDY>
-struct ModelView<T, MD, MDY, true> {
-  T a[MD * MD], lq[MD * MD], bb[MD], h[MDY * MD], cc[MDY], lri[MDY * MDY];
-  T cst;
-  __device__ __forceinline__ ModelView(const BfModelDev<T>& m, int D, int Dy) {
-#pragma unroll
-    for (int r = 0; r < MD; ++r) {
-      bb[r] = m.b[r];
-#pragma unroll
-      for (int c = 0; c < MD; ++c) {
-        a[r * MD + c] = m.A[r + c * MD];
-        lq[r * MD + c] = (c <= r) ? m.LQ[r + c * MD] : (T)0;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < MDY; ++q) {
-      cc[q] = m.c[q];
-#pragma unroll
-      for (int c = 0; c < MD; ++c) h[q * MD + c] = m.H[q + c * MDY];
-#pragma unroll
-      for (int c = 0; c < MDY; ++c) lri[q * MDY + c] = (c <= q) ? m.LRi[q + c * MDY] : (T)0;
-    }
-    cst = m.cst;
-  }
-  __device__ __forceinline__ T A(int r, int c) const { return a[r * MD + c]; }
-  __device__ __forceinline__ T LQ(int r, int c) const { return lq[r * MD + c]; }
-  __device__ __forceinline__ T b(int r) const { return bb[r]; }
-  __device__ __forceinline__ T H(int q, int c) const { return h[q * MD + c]; }
-  __device__ __forceinline__ T c(int q) const { return cc[q]; }
-  __device__ __forceinline__ T LRi(int q, int c) const { return lri[q * MDY + c]; }
-};
-
-template <typename T, int MD, int MDY>
-struct ModelView<T, MD, MDY, false> {
-  const BfModelDev<T>& m;  // a copy in LDS (see k_bf_step)
-  int D, Dy;
-  T cst;
-  __device__ __forceinline__ ModelView(const BfModelDev<T>& mm, int D_, int Dy_)
-      : m(mm), D(D_), Dy(Dy_), cst(mm.cst) {}
-  __device__ __forceinline__ T A(int r, int c) const { return m.A[r + c * D]; }
-  __device__ __forceinline__ T LQ(int r, int c) const { return m.LQ[r + c * D]; }
-  __device__ __forceinline__ T b(int r) const { return m.b[r]; }
-  __device__ __forceinline__ T H(int q, int c) const { return m.H[q + c * Dy]; }
-  __device__ __forceinline__ T c(int q) const { return m.c[q]; }
-  __device__ __forceinline__ T LRi(int q, int c) const { return m.LRi[q + c * Dy]; }
-};
-
-// Copies the model constants to LDS (runtime-dimension kernels read them many times).
-template <typename T>
-__device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>* g,
-                                                             BfModelDev<T>* s) {
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(g);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(s);
-  for (int e = threadIdx.x; e < (int)(sizeof(BfModelDev<T>) / 4); e += blockDim.x) dst[e] = src[e];
-  __syncthreads();
-  return *s;
-}
-
-// The fused predict + update + partial-reduction kernel.  DT/DYT > 0 fix the dimensions
-// at compile time; 0 means runtime dimensions up to kMaxD / kMaxDy.
+// The fused predict + update + partial-reduction kernel: one block per tile.
 template <typename T, int DT, int DYT>
 __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  constexpr int MD = DT ? DT : kMaxD;
-  constexpr int MDY = DYT ? DYT : kMaxDy;
-  const int D = DT ? DT : p.D;
-  const int Dy = DYT ? DYT : p.Dy;
-  BfScalars* sc = p.sc;
-  const bool rs = p.threshold_n >= sc->ess;
-  const bool fresh = rs || sc->typeless;  // normalised pre-update log-weights are all 0
-  const T lse = (T)sc->lse;
-  const int t = sc->step;
-  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
-  // Model constants: registers for compile-time dimensions (read before any store, so the
-  // loads are uniform scalar loads), an LDS copy otherwise.
-  const BfModelDev<T>* msrc = p.model;
-  if constexpr (DT == 0) {
-    __shared__ BfModelDev<T> lds_model;
-    msrc = &model_to_lds<T>(p.model, &lds_model);
-  }
-  const ModelView<T, MD, MDY, (DT > 0)> mv(*msrc, D, Dy);
-  T y[MDY];
-#pragma unroll
-  for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
-
-  const int tile0 = blockIdx.x * kTile;
-  T nl[kStepItems], lnew[kStepItems];
-  bool valid[kStepItems];
-
-  auto load_item = [&](int k, T* xk) {
-    const int i = tile0 + k * kStepThreads + threadIdx.x;
-    valid[k] = i < p.N;
-    const int src = (rs && valid[k]) ? p.anc[i] : i;
-#pragma unroll
-    for (int d = 0; d < MD; ++d)
-      if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
-    nl[k] = fresh ? (T)0 : p.logw[i] - lse;
-  };
-  auto compute_item = [&](int k, const T* xk) {
-    const int i = tile0 + k * kStepThreads + threadIdx.x;
-    T z[MD];
-    if (p.noise) {
-#pragma unroll
-      for (int d = 0; d < MD; ++d)
-        if (d < D) z[d] = p.noise[(size_t)d * p.Npad + i];
-    } else {
-      draw_normals<T, MD>(p.seed, kStreamPropagate, (uint32_t)t, (uint32_t)i, D, z);
-    }
-    // predict: x' = A x + b + L_Q z
-    T xn[MD];
-#pragma unroll
-    for (int r = 0; r < MD; ++r) {
-      if (r < D) {
-        T s = mv.b(r);
-#pragma unroll
-        for (int c = 0; c < MD; ++c)
-          if (c < D) s += mv.A(r, c) * xk[c];
-#pragma unroll
-        for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
-        xn[r] = s;
-        p.xout[(size_t)r * p.Npad + i] = s;
-      }
-    }
-    // update: log N(y; H x' + c, R) = cst - |L_R^-1 (y - H x' - c)|^2 / 2
-    T res[MDY];
-#pragma unroll
-    for (int q = 0; q < MDY; ++q) {
-      if (q < Dy) {
-        T s = mv.c(q);
-#pragma unroll
-        for (int c = 0; c < MD; ++c)
-          if (c < D) s += mv.H(q, c) * xn[c];
-        res[q] = y[q] - s;
-      }
-    }
-    T quad = 0;
-#pragma unroll
-    for (int q = 0; q < MDY; ++q) {
-      if (q < Dy) {
-        T v = 0;
-#pragma unroll
-        for (int c = 0; c <= q; ++c) v += mv.LRi(q, c) * res[c];
-        quad += v * v;
-      }
-    }
-    lnew[k] = nl[k] + (mv.cst - (T)0.5 * quad);
-    p.logw[i] = lnew[k];
-  };
-  if constexpr (DT > 0) {
-    // all loads first (maximum memory-level parallelism), then the arithmetic
-    T x[kStepItems][MD];
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) load_item(k, x[k]);
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) compute_item(k, x[k]);
-  } else {
-    for (int k = 0; k < kStepItems; ++k) {
-      T xk[MD];
-      load_item(k, xk);
-      compute_item(k, xk);
-    }
-  }
-  tile_partial<T>(nl, lnew, valid, fresh, p.partials, p.nblocks);
+  bf_step_tile<T, DT, DYT>(p, blockIdx.x);
 }
 
-// initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).
 template <typename T, int DT>
 __global__ void __launch_bounds__(kStepThreads) k_bf_init(int N, int Npad, int D,
                                                           const BfModelDev<T>* model, T* x,
                                                           T* logw, const T* noise,
                                                           uint64_t seed) {
-  constexpr int MD = DT ? DT : kMaxD;
-  if (DT) D = DT;
-  const int i = blockIdx.x * kStepThreads + threadIdx.x;
-  if (i >= Npad) return;
-  const BfModelDev<T>& m = *model;
-  T z[MD];
-  if (noise) {
-#pragma unroll
-    for (int d = 0; d < MD; ++d)
-      if (d < D) z[d] = noise[(size_t)d * Npad + i];
-  } else {
-    draw_normals<T, MD>(seed, kStreamInit, 0u, (uint32_t)i, D, z);
-  }
-#pragma unroll
-  for (int r = 0; r < MD; ++r) {
-    if (r < D) {
-      T s = m.mu0[r];
-#pragma unroll
-      for (int c = 0; c <= r; ++c) s += m.L0[r + c * D] * z[c];
-      x[(size_t)r * Npad + i] = (i < N) ? s : (T)0;
-    }
-  }
-  logw[i] = (T)0;
+  bf_init_item<T, DT>(N, Npad, D, model, x, logw, noise, seed,
+                      blockIdx.x * kStepThreads + threadIdx.x);
 }
 
 }  // namespace ssm

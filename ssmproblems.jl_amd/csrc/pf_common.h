@@ -47,7 +47,7 @@ __device__ __forceinline__ double exp_w(double v) { return exp(v); }
 template <typename T>
 __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
                              const bool valid[kTileItems], bool fresh, double* partials,
-                             int nb) {
+                             int nb, int b) {
   constexpr int NW = kTileThreads / kWave;
   __shared__ T lds_m1[NW];
   __shared__ double lds_r[4][NW];
@@ -108,7 +108,6 @@ __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
       tt += lds_r[0][k];
       ss += lds_r[1][k];
     }
-    const int b = blockIdx.x;
     partials[b] = b0.m;
     partials[nb + b] = b0.s;
     partials[2 * nb + b] = (double)m1;
@@ -136,7 +135,7 @@ __global__ void __launch_bounds__(kTileThreads)
     lv[k] = valid[k] ? l[i] : (T)0;
     nl[k] = (valid[k] && !fresh) ? nl_pre[i] : (T)0;
   }
-  tile_partial<T>(nl, lv, valid, fresh, partials, gridDim.x);
+  tile_partial<T>(nl, lv, valid, fresh, partials, gridDim.x, blockIdx.x);
 }
 
 // One block: fold the tile partials, finish the step, prepare the next resampling.
@@ -144,10 +143,10 @@ __global__ void __launch_bounds__(kTileThreads)
 // Each thread owns a contiguous chunk of at most CH tiles held in registers (one load
 // phase), so the kernel handles nb <= 256 * CH tiles; the host picks CH.
 template <typename T, int CH, int NT>
-__global__ void __launch_bounds__(NT)
-    k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
-                double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                int record) {
+__device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* sc_in,
+                                                BfScalars* sc, double threshold_n, double logN,
+                                                const TileScratch& ts, double* ll_steps,
+                                                int32_t* rs_steps, int record) {
   constexpr int NW = NT / 64;
   __shared__ double red[3][NW];
   __shared__ double wtot[NW], wmax[NW];
@@ -294,6 +293,15 @@ __global__ void __launch_bounds__(NT)
     }
     *sc = s;
   }
+}
+
+template <typename T, int CH, int NT>
+__global__ void __launch_bounds__(NT)
+    k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
+                double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
+                int record) {
+  pf_finish_block<T, CH, NT>(N, nb, sc_in, sc, threshold_n, logN, ts, ll_steps, rs_steps,
+                             record);
 }
 
 // Enqueues the finish kernel with a register chunk that fits nb tiles.

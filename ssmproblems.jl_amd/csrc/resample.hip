@@ -21,13 +21,9 @@
 // and need a global binary search.  Neighbouring tiles compute their shared boundary count
 // from the same stored prefix, and every prefix sequence is made nondecreasing with an exact
 // running max, so the tiles' output ranges partition [0, n) exactly.
-#include "device_common.h"
-#include "internal.h"
-#include "pf_common.h"
+#include "resample_dev.h"
 
 namespace ssm {
-
-constexpr int kRsThreads = kTileThreads;
 
 size_t resample_scratch_bytes(int n) {
   const size_t nb = (size_t)(n + kTile - 1) / kTile;
@@ -48,77 +44,6 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   a.prefix = (double*)p;
   p += r((nb + 1) * 8);
   a.Wbuf = (double*)p;
-}
-
-struct RsParams {
-  int kind, n, mode;
-  const void* w;
-  const double* u;
-  uint64_t seed;
-  uint32_t step_host;
-  const BfScalars* sc;
-  double threshold_n;
-  int32_t* anc;
-  double* w_debug64;
-  const double* blk_m;
-  const double* blk_c;
-  double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
-  double* prefix;
-  double* Wbuf;
-};
-
-__device__ __forceinline__ bool rs_active(const RsParams& p) {
-  return p.sc == nullptr || p.threshold_n >= p.sc->ess;
-}
-
-__device__ __forceinline__ uint32_t rs_step(const RsParams& p) {
-  return p.sc ? (uint32_t)p.sc->step : p.step_host;
-}
-
-// Weight of particle j in tile b as the resampler consumes it (FP64).  Particle-filter
-// modes: softmax of the log-weights as w_j = exp(l_j - m_b) c_b (containers.jl:140-141),
-// or 1/n for the lazy TypelessZero weights of initialise (containers.jl:62-65).
-template <int MODE>
-__device__ __forceinline__ double rs_weight(const RsParams& p, int j, int b) {
-  if (MODE == kWeightsF64) return ((const double*)p.w)[j];
-  if (MODE == kWeightsF32) return (double)((const float*)p.w)[j];
-  if (p.sc->typeless) return 1.0 / (double)p.n;
-  if (MODE == kWeightsLogF32) {
-    const float e = exp_w(((const float*)p.w)[j] - (float)p.blk_m[b]);
-    return (double)e * p.blk_c[b];
-  } else {
-    const double e = exp_w(((const double*)p.w)[j] - p.blk_m[b]);
-    return e * p.blk_c[b];
-  }
-}
-
-// Inclusive FP64 scan across the block (thread order), fixed order; *tot = block total.
-__device__ __forceinline__ double block_incl_scan_d(double v, double* lds, double* tot) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  const double incl = wave_inclusive_scan(v);
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = incl;
-  __syncthreads();
-  double wp = 0.0;
-  for (int k = 0; k < w; ++k) wp += lds[k];
-  *tot = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  return wp + incl;
-}
-
-// Inclusive max-scan across the block (thread order); exact.  *tot = block max.
-__device__ __forceinline__ double block_incl_max_d(double v, double* lds, double* tot) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double o = wave_shfl_up(v, d);
-    if (lane >= d) v = fmax(v, o);
-  }
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = v;
-  __syncthreads();
-  for (int k = 0; k < w; ++k) v = fmax(v, lds[k]);
-  *tot = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
-  return v;
 }
 
 // Explicit weights, kernel 1: FP64 tile sums (thread-serial over its striped items, then
@@ -154,7 +79,7 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
   double run;
   {
     const int w = tid / kWave, lane = tid & (kWave - 1);
-    double ex = wave_shfl_up(wave_inclusive_scan(cs), 1);
+    double ex = wave_shfl_up(wave_scan_dpp(cs), 1);
     if (lane == 0) ex = 0.0;
     for (int k = 0; k < w; ++k) ex += lds[k];
     run = ex;
@@ -175,205 +100,14 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
   if (tid == 0) prefix[nb] = fmax(tot, mx);
 }
 
-// Cumulative offspring count for an inclusive prefix W.
-// SYSTEMATIC: O = #{s in [0,n) : fl(u0 + s) < fl(n W)} (strict, resamplers.jl:231).
-__device__ __forceinline__ int offspring_systematic(double W, double u0, int n) {
-  const double vs = W * (double)n;
-  double est = ceil(vs - u0);
-  if (!(est >= 0.0)) est = 0.0;  // also catches NaN
-  if (est > (double)n) est = (double)n;
-  int m = (int)est;
-  while (m > 0 && !(u0 + (double)(m - 1) < vs)) --m;
-  while (m < n && (u0 + (double)m < vs)) ++m;
-  return m;
-}
-
-__device__ __forceinline__ int clamp_floor(double v, int n) {
-  double f = floor(v);
-  if (!(f >= 0.0)) f = 0.0;
-  if (f > (double)n) f = (double)n;
-  return (int)f;
-}
-
-__device__ __forceinline__ int offspring_of(const RsParams& p, double W, double Wn, double u0,
-                                            uint32_t step) {
-  const int n = p.n;
-  if (p.kind == SSM_SYSTEMATIC) return offspring_systematic(W, u0, n);
-  const double r = ((double)n * W) / Wn;
-  if (p.kind == SSM_SYSTEMATIC_MURRAY) return clamp_floor(r + u0, n);
-  // STRATIFIED: k = min(n, floor(r + 1)) is 1-based into u.
-  int k = clamp_floor(r + 1.0, n);
-  if (k < 1) k = 1;
-  const double uk = p.u ? p.u[k - 1] : resample_uniform(p.seed, step, (uint32_t)(k - 1));
-  return clamp_floor(r + uk, n);
-}
-
-// Exclusive FP64 scan of one value per thread across the block (thread order, fixed
-// order); *tot = block total.
-__device__ __forceinline__ double block_excl_scan_d(double v, double* lds, double* tot) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  const double incl = wave_inclusive_scan(v);
-  double ex = wave_shfl_up(incl, 1);
-  if (lane == 0) ex = 0.0;
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = incl;
-  __syncthreads();
-  for (int k = 0; k < w; ++k) ex += lds[k];
-  *tot = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  return ex;
-}
-
-// Exclusive max-scan of one value per thread across the block (exact).
-__device__ __forceinline__ double block_excl_max_d(double v, double* lds) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double o = wave_shfl_up(v, d);
-    if (lane >= d) v = fmax(v, o);
-  }
-  double ex = wave_shfl_up(v, 1);
-  if (lane == 0) ex = -INFINITY;
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = v;
-  __syncthreads();
-  for (int k = 0; k < w; ++k) ex = fmax(ex, lds[k]);
-  return ex;
-}
-
-// Slots a thread writes itself; threads with more hand their range to the whole block.
-constexpr int kDirectSlots = 32;
-
-// Main kernel: tile rescan -> cumulative offspring counts -> ancestor fill.  Thread t owns
-// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial
-// scan plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by
-// its own thread (neighbouring threads write neighbouring ranges); a thread whose range
-// exceeds kDirectSlots hands it to the whole block, which keeps degenerate weights (one
-// particle taking every slot) balanced.  For MULTINOMIAL the kernel writes the
-// nondecreasing prefix sums W to Wbuf instead.
+// Main kernel: one block per tile (see scan_offspring_tile).
 template <int MODE>
 __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
-  if (!rs_active(p)) return;
-  __shared__ double lds[kRsThreads / kWave];
-  __shared__ int olast[kRsThreads];
-  __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
-  __shared__ int nheavy;
-  const int n = p.n;
-  const int b = blockIdx.x;
-  const int nb = (n + kTile - 1) / kTile;
-  const int tile0 = b * kTile;
-  const int cnt = min(kTile, n - tile0);
-  const int t = threadIdx.x;
-  const bool typeless = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64) && p.sc->typeless;
-  double P, Pnext, Wn;
-  if (typeless) {
-    P = (double)tile0 / (double)n;
-    Pnext = (b == nb - 1) ? 1.0 : (double)(tile0 + kTile) / (double)n;
-    Wn = 1.0;
-  } else {
-    P = p.prefix[b];
-    Pnext = p.prefix[b + 1];
-    Wn = p.prefix[nb];
-  }
-  if (t == 0) nheavy = 0;
-  // thread-serial scan of 4 consecutive weights, then one block scan of the thread sums
-  double W[kTileItems];
-  double acc = 0.0;
-#pragma unroll
-  for (int k = 0; k < kTileItems; ++k) {
-    const int j = tile0 + kTileItems * t + k;
-    double wj = 0.0;
-    if (j < n) {
-      wj = rs_weight<MODE>(p, j, b);
-      if (p.w_debug64) p.w_debug64[j] = wj;
-    }
-    acc += wj;
-    W[k] = acc;
-  }
-  double tot;
-  const double ex = block_excl_scan_d(acc, lds, &tot);
-  (void)tot;
-#pragma unroll
-  for (int k = 0; k < kTileItems; ++k) W[k] = P + (ex + W[k]);
-  // nondecreasing in particle order (exact running max)
-#pragma unroll
-  for (int k = 1; k < kTileItems; ++k) W[k] = fmax(W[k], W[k - 1]);
-  const double carry = block_excl_max_d(W[kTileItems - 1], lds);
-#pragma unroll
-  for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
-  const uint32_t step = rs_step(p);
-  if (p.kind == SSM_MULTINOMIAL) {
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) {
-      const int pos = kTileItems * t + k;
-      if (pos < cnt) p.Wbuf[tile0 + pos] = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
-    }
-    return;
-  }
-  const double u0 = (p.kind == SSM_STRATIFIED)
-                        ? 0.0
-                        : (p.u ? p.u[0] : resample_uniform(p.seed, step, 0u));
-  const int o_prev = (b == 0) ? 0 : offspring_of(p, P, Wn, u0, step);
-  const int o_end = (b == nb - 1) ? n : offspring_of(p, Pnext, Wn, u0, step);
-  int O[kTileItems];
-#pragma unroll
-  for (int k = 0; k < kTileItems; ++k) {
-    const int pos = kTileItems * t + k;
-    if (pos >= cnt - 1)
-      O[k] = o_end;  // the tile's last particle closes its range; positions past n own none
-    else
-      O[k] = min(max(offspring_of(p, W[k], Wn, u0, step), o_prev), o_end);
-  }
-  olast[t] = O[kTileItems - 1];
-  __syncthreads();
-  const int start = (t == 0) ? o_prev : olast[t - 1];
-  if (O[kTileItems - 1] - start <= kDirectSlots) {
-    int s0 = start;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) {
-      for (int s = s0; s < O[k]; ++s) p.anc[s] = tile0 + kTileItems * t + k;
-      s0 = max(s0, O[k]);
-    }
-  } else {
-    const int h = atomicAdd(&nheavy, 1);
-    heavy[h][0] = start;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) heavy[h][k + 1] = O[k];
-    heavy[h][kTileItems + 1] = t;
-  }
-  __syncthreads();
-  // heavy ranges: the whole block strides over each one
-  const int nh = nheavy;
-  for (int h = 0; h < nh; ++h) {
-    const int hs = heavy[h][0], he = heavy[h][kTileItems];
-    const int owner = heavy[h][kTileItems + 1];
-    for (int s = hs + t; s < he; s += kRsThreads) {
-      int k = 0;
-      while (k < kTileItems - 1 && heavy[h][k + 1] <= s) ++k;
-      p.anc[s] = tile0 + kTileItems * owner + k;
-    }
-  }
+  scan_offspring_tile<MODE>(p, blockIdx.x);
 }
 
-// Multinomial: a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.
 __global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
-  if (!rs_active(p)) return;
-  const int k = blockIdx.x * kRsThreads + threadIdx.x;
-  if (k >= p.n) return;
-  const uint32_t step = rs_step(p);
-  const int nb = (p.n + kTile - 1) / kTile;
-  const bool typeless = (p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64) && p.sc->typeless;
-  const double Wn = typeless ? 1.0 : p.prefix[nb];
-  const double uk = p.u ? p.u[k] : resample_uniform(p.seed, step, (uint32_t)k);
-  const double q = uk * Wn;
-  int lo = 0, hi = p.n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (p.Wbuf[mid] < q)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  p.anc[k] = min(lo, p.n - 1);
+  multinomial_search_item(p, blockIdx.x * kRsThreads + threadIdx.x);
 }
 
 int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
