@@ -325,6 +325,186 @@ def bench_rbpf(args, N=16384, T=500, threshold=0.5, budget_s=10.0):
     print(json.dumps(out), flush=True)
 
 
+def serve_requests(M, seed=1234):
+    """The request stream: each request is a C1-type 1-D BF (the random-walk model,
+    threshold 0.5), a C2-type LG BF or a KF on a seeded LG model with Dx ~ U{1..16},
+    Dy = max(1, Dx // 2); N log-uniform in [256, 262144]; T ~ U[50, 2000]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(M):
+        cls = int(rng.integers(0, 3))
+        T = int(rng.integers(50, 2001))
+        N = int(np.exp(rng.uniform(np.log(256), np.log(262144))))
+        if cls == 0:
+            p = dict(mu0=np.zeros(1), Sigma0=np.eye(1), A=np.eye(1), b=np.zeros(1),
+                     Q=np.array([[0.01]]), H=np.eye(1), c=np.zeros(1), R=np.array([[0.04]]))
+        else:
+            dx = int(rng.integers(1, 17))
+            dy = max(1, dx // 2)
+            G = rng.standard_normal((dx, dx))
+            Qm, Rm = np.linalg.qr(G)
+            A = 0.9 * Qm * np.sign(np.diag(Rm))
+            Wq = rng.standard_normal((dx, dx + 2))
+            Wr = rng.standard_normal((dy, dy + 2))
+            p = dict(mu0=np.zeros(dx), Sigma0=np.eye(dx), A=A, b=np.zeros(dx),
+                     Q=Wq @ Wq.T / (dx + 2), H=rng.standard_normal((dy, dx)), c=np.zeros(dy),
+                     R=Wr @ Wr.T / (dy + 2))
+        dx, dy = p["A"].shape[0], p["H"].shape[0]
+        L0, LQ, LR = (np.linalg.cholesky(p[k]) for k in ("Sigma0", "Q", "R"))
+        x = p["mu0"] + L0 @ rng.standard_normal(dx)
+        E = rng.standard_normal((T, dx)) @ LQ.T
+        V = rng.standard_normal((T, dy)) @ LR.T
+        ys = np.zeros((T, dy))
+        for t in range(T):
+            x = p["A"] @ x + p["b"] + E[t]
+            ys[t] = p["H"] @ x + p["c"] + V[t]
+        kind = 1 if cls == 2 else 0
+        out.append(dict(kind=kind, cls=cls, params=tuple(p[k] for k in (
+            "mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")), ys=ys, N=N,
+            threshold=0.5 if cls == 0 else 1.0, seed=int(rng.integers(1, 2**62))))
+    return out
+
+
+def plain_run(ctx, q):
+    """The plain per-filter GPU path: one filter object per request, created, run from
+    captured graphs, read back and destroyed."""
+    import ssmhip
+    model = ssmhip.create_homogeneous_linear_gaussian_model(*q["params"])
+    if q["kind"] == 1:
+        return ssmhip.filter(None, model, ssmhip.KalmanFilter(), q["ys"], ctx=ctx)[1]
+    algo = ssmhip.BF(q["N"], threshold=q["threshold"], precision="f32")
+    st = ssmhip.initialise(None, model, algo, ctx=ctx, seed=q["seed"])
+    ll = ssmhip.run(st, algo, q["ys"])
+    st.close()
+    return ll
+
+
+def bench_serve(args, budget=16 << 30, max_active=256, budget_s=10.0):
+    import ssmhip
+    from ssmhip import serve as S
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    M = args.requests
+    log(f"device {name} CUs={cus}; generating {M} requests")
+    reqs = serve_requests(M)
+    K = max(8, M // 4)
+
+    def submit(sv, q):
+        return sv.submit_params(S.REQ_KF if q["kind"] == 1 else S.REQ_BF, q["params"], q["ys"],
+                                q["N"], q["threshold"], 0, q["seed"])
+
+    # warm-up: every code path once (kernels, graphs, server threads)
+    plain_run(ctx, reqs[0])
+    with S.Server(budget, max_active, ctx=ctx) as sv:
+        for q in reqs[:4]:
+            sv.wait(submit(sv, q))
+    # plain per-filter path on the first K requests
+    t0 = time.perf_counter()
+    ll_plain = [plain_run(ctx, q) for q in reqs[:K]]
+    el_plain = time.perf_counter() - t0
+    plain_fps = K / el_plain
+    log(f"plain path: {K} filters in {el_plain:.2f}s = {plain_fps:.1f} filters/s")
+    # served, saturation: the same K requests, then all M, submitted at once
+    with S.Server(budget, max_active, ctx=ctx) as sv:
+        t0 = time.perf_counter()
+        ids = [submit(sv, q) for q in reqs[:K]]
+        res_k = [sv.wait(i) for i in ids]
+        el_k = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ids = [submit(sv, q) for q in reqs]
+        res = [sv.wait(i) for i in ids]
+        el_all = time.perf_counter() - t0
+        st_sat = sv.stats()
+    assert all(r.state == S.DONE for r in res_k + res), "served requests failed"
+    served_fps_k = K / el_k
+    served_fps = M / el_all
+    log(f"served: same {K} in {el_k:.2f}s = {served_fps_k:.1f} filters/s "
+        f"({served_fps_k / plain_fps:.1f}x); all {M} in {el_all:.2f}s = {served_fps:.1f} "
+        f"filters/s")
+    bf_exact = sum(1 for q, a, b in zip(reqs[:K], res_k, ll_plain) if q["kind"] == 0 and a.ll == b)
+    n_bf = sum(1 for q in reqs[:K] if q["kind"] == 0)
+    kf_err = [abs(r.ll - kf_loglik_general(*q["params"], q["ys"])) / abs(r.ll)
+              for q, r in zip(reqs, res) if q["kind"] == 1][:64]
+    # latency under Poisson arrivals at fractions of the measured saturation rate
+    lat = {}
+    rng = np.random.default_rng(99)
+    for frac in (0.25, 0.5, 1.0, 2.0):
+        rate = frac * served_fps
+        gaps = rng.exponential(1.0 / rate, size=M)
+        with S.Server(budget, max_active, ctx=ctx) as sv:
+            ids = []
+            t0 = time.perf_counter()
+            due = t0
+            for q, g in zip(reqs, gaps):
+                due += g
+                dt = due - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(submit(sv, q))
+            rr = [sv.wait(i) for i in ids]
+            el = time.perf_counter() - t0
+        L = np.array([r.latency_ms for r in rr])
+        lat[f"{frac}x"] = {"arrival_rate": rate, "p50_ms": float(np.percentile(L, 50)),
+                           "p99_ms": float(np.percentile(L, 99)), "filters_per_s": M / el}
+        log(f"poisson {frac}x ({rate:.0f}/s): p50 {np.percentile(L, 50):.1f} ms, "
+            f"p99 {np.percentile(L, 99):.1f} ms, {M / el:.1f} filters/s")
+    cpu = None
+    if not args.no_cpu:
+        cpu = serve_cpu_baseline(reqs, budget_s)
+        log(f"cpu baseline {cpu['value']:.3f} filters/s")
+    out = {
+        "metric": "filters/s + p50/p99 at stated loglik err (served request stream)",
+        "value": served_fps, "unit": "filters/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": el_all * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 weight sums, FP64 Cholesky of S)",
+        "data": "synthetic (seeded models and simulated observations)",
+        "config": {"workload": f"{M} requests: 1-D BF / LG BF (Dx 1..16) / KF, N log-uniform "
+                               f"[256, 262144], T ~ U[50, 2000]; budget {budget >> 30} GiB, "
+                               f"{max_active} slots", "requests": M},
+        "plain_filters_per_s": plain_fps,
+        "served_filters_per_s_same_requests": served_fps_k,
+        "speedup_vs_plain": served_fps_k / plain_fps,
+        "bf_served_equal_to_plain": f"{bf_exact}/{n_bf}",
+        "kf_loglik_max_rel_err": float(max(kf_err)) if kf_err else None,
+        "latency": lat,
+        "server_stats": st_sat,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_cpu_baseline(reqs, budget_s):
+    """FP64 oracle, one core: requests in stream order, each run for as many steps as fit
+    its share of the budget and its time extrapolated to the full horizon."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rng = np.random.default_rng(3)
+    total, n, t_start = 0.0, 0, time.perf_counter()
+    for q in reqs:
+        if time.perf_counter() - t_start > budget_s:
+            break
+        m = oracle.LG(*q["params"])
+        T = q["ys"].shape[0]
+        t0 = time.perf_counter()
+        if q["kind"] == 1:
+            oracle.kf_filter(m, q["ys"])
+            total += time.perf_counter() - t0
+        else:
+            dx = q["params"][2].shape[0]
+            st = oracle.BFState(oracle.bf_initialise(m, rng.standard_normal((q["N"], dx))))
+            steps = 0
+            while steps < T and time.perf_counter() - t0 < budget_s / 8:
+                oracle.bf_step(m, st, q["ys"][steps], q["threshold"], 0, rng.random(1),
+                               rng.standard_normal((q["N"], dx)))
+                steps += 1
+            total += (time.perf_counter() - t0) * T / steps
+        n += 1
+    return {"value": n / total, "unit": "filters/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (oracle/oracle.c, single thread) on the first {n} requests "
+                      f"of the stream; BF runs cut short are extrapolated to their full T"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,9 +520,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
-    ap.add_argument("--workload", default="bf", choices=["bf", "kf", "rbpf"],
+    ap.add_argument("--workload", default="bf", choices=["bf", "kf", "rbpf", "serve"],
                     help="bf: bootstrap filter (default); kf: batched Kalman filters; "
-                         "rbpf: Rao-Blackwellised PF")
+                         "rbpf: Rao-Blackwellised PF; serve: request stream through the "
+                         "server against the plain per-filter path")
+    ap.add_argument("--requests", type=int, default=256, help="serve: requests per phase")
     ap.add_argument("--B", type=int, default=65536, help="kf: number of filters")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -357,6 +539,8 @@ def main():
         return bench_kf(args)
     if args.workload == "rbpf":
         return bench_rbpf(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import ssmhip
     ctx = ssmhip.default_context()

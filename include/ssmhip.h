@@ -208,6 +208,62 @@ int32_t ssm_rbpf_export(ssm_rbpf* rb, double* xo, double* mu, double* Sigma, dou
                         int32_t* ancestors, double* w);
 int32_t ssm_rbpf_last_timing(ssm_rbpf* rb, double* ms);
 
+/* ---- serving: continuous batching of whole filter runs ---------------------------- */
+/*
+ * A request is one filter(model, algo, ys) call (GF/src/GeneralisedFilters.jl:58-84) on a
+ * linear-Gaussian model: a BootstrapFilter (particles.jl:36-162; 1 <= Dx, Dy <= 16,
+ * 1 <= N <= 262144, FP32 particles, Philox noise keyed by `seed`) or a KalmanFilter
+ * (kalman.jl:23-79; Dx <= 64, Dy <= 32).  The reference has no server: these entry points
+ * are the batched form of those calls, for hosts that issue many likelihood evaluations
+ * (an outer parameter sampler).  The server owns a device arena of hbm_budget_bytes and a
+ * worker thread per lane; requests are admitted while slots (max_active bootstrap filters)
+ * and arena memory last, keep their state device-resident until they finish, and overlap
+ * on the device with everything else admitted.  A bootstrap request's log-likelihood is
+ * the same bit for bit alone and in any mix.
+ */
+#define SSM_REQ_BF 0
+#define SSM_REQ_KF 1
+/* request states reported by ssm_serve_poll / ssm_serve_wait */
+#define SSM_SERVE_QUEUED 0
+#define SSM_SERVE_RUNNING 1
+#define SSM_SERVE_DONE 2
+#define SSM_SERVE_FAILED 3
+#define SSM_SERVE_CANCELLED 4
+
+typedef struct {
+  int32_t kind;        /* SSM_REQ_BF or SSM_REQ_KF */
+  ssm_lg_model model;  /* parameters; copied at submit */
+  int32_t N;           /* BF: particles */
+  int32_t T;           /* number of observations */
+  const double* ys;    /* T x Dy, row t = observation t; copied at submit */
+  double threshold;    /* BF: ESS threshold (resample iff threshold * N >= ESS) */
+  int32_t resampler;   /* BF: SSM_SYSTEMATIC .. SSM_MULTINOMIAL */
+  uint64_t seed;       /* BF: Philox key */
+} ssm_request;
+
+/* Errors of create and submit are reported on ctx (ssm_last_error). */
+int32_t ssm_serve_create(ssm_ctx* ctx, uint64_t hbm_budget_bytes, int32_t max_active,
+                         ssm_server** out);
+/* Queues a request; SSM_ERR_ADMISSION if it can never fit the budget, SSM_ERR_NOT_PD if a
+ * covariance of a BF model does not factorise.  Thread-safe against the worker threads. */
+int32_t ssm_serve_submit(ssm_server* sv, const ssm_request* req, int64_t* id);
+/* state: SSM_SERVE_*; ll (log-evidence) and latency_ms (submit to completion) once
+ * finished; status: SSM_OK or the request's error code. */
+int32_t ssm_serve_poll(ssm_server* sv, int64_t id, int32_t* state, double* ll,
+                       double* latency_ms, int32_t* status);
+/* Blocks until the request finishes or timeout_ms passes (< 0: no timeout). */
+int32_t ssm_serve_wait(ssm_server* sv, int64_t id, double timeout_ms, int32_t* state, double* ll,
+                       double* latency_ms, int32_t* status);
+/* Cancels a queued request, or a running one at its next batch boundary. */
+int32_t ssm_serve_cancel(ssm_server* sv, int64_t id);
+/* Forgets a finished request (its result can no longer be polled). */
+int32_t ssm_serve_release(ssm_server* sv, int64_t id);
+/* out[0..n): completed, failed, cancelled, batched BF steps, Kalman groups, arena bytes in
+ * use, peak arena bytes, peak concurrent BF filters, arena capacity, queued requests. */
+int32_t ssm_serve_stats(ssm_server* sv, double* out, int32_t n);
+/* Cancels what is queued or running, joins the workers and frees the arena. */
+int32_t ssm_serve_destroy(ssm_server* sv);
+
 /* ---- Kalman filters, batched (GF/src/algorithms/kalman.jl:16-79) ------------------- */
 /*
  * B independent linear-Gaussian filters of uneven size (1 <= Dx <= 64, 1 <= Dy <= 32),

@@ -82,80 +82,19 @@ struct ssm_bf {
 
 namespace {
 
-// Lower Cholesky in place (column-major), FP64.
-bool chol_lower(int n, std::vector<double>& A) {
-  for (int j = 0; j < n; ++j) {
-    double d = A[j + j * n];
-    for (int k = 0; k < j; ++k) d -= A[j + k * n] * A[j + k * n];
-    if (!(d > 0.0)) return false;
-    d = std::sqrt(d);
-    A[j + j * n] = d;
-    for (int i = j + 1; i < n; ++i) {
-      double s = A[i + j * n];
-      for (int k = 0; k < j; ++k) s -= A[i + k * n] * A[j + k * n];
-      A[i + j * n] = s / d;
-    }
-    for (int i = 0; i < j; ++i) A[i + j * n] = 0.0;
-  }
-  return true;
-}
-
-// Inverse of a lower-triangular matrix (column-major).
-std::vector<double> tri_inverse(int n, const std::vector<double>& L) {
-  std::vector<double> X(n * n, 0.0);
-  for (int j = 0; j < n; ++j) {
-    X[j + j * n] = 1.0 / L[j + j * n];
-    for (int i = j + 1; i < n; ++i) {
-      double s = 0.0;
-      for (int k = j; k < i; ++k) s += L[i + k * n] * X[k + j * n];
-      X[i + j * n] = -s / L[i + i * n];
-    }
-  }
-  return X;
-}
-
 template <typename T>
 int upload_model(ssm_bf* bf, const ssm_lg_model* mdl) {
   ssm_ctx* ctx = bf->ctx;
-  const int D = mdl->Dx, Dy = mdl->Dy;
   BfModelDev<T> h;
-  std::memset(&h, 0, sizeof h);
-  std::vector<double> LQ(mdl->Q, mdl->Q + D * D), LR(mdl->R, mdl->R + Dy * Dy),
-      L0(mdl->Sigma0, mdl->Sigma0 + D * D);
-  if (!chol_lower(D, LQ)) return set_error(ctx, SSM_ERR_NOT_PD, "Q is not positive definite");
-  if (!chol_lower(Dy, LR)) return set_error(ctx, SSM_ERR_NOT_PD, "R is not positive definite");
-  if (!chol_lower(D, L0))
-    return set_error(ctx, SSM_ERR_NOT_PD, "Sigma0 is not positive definite");
-  const std::vector<double> LRi = tri_inverse(Dy, LR);
-  double logdet = 0.0;
-  for (int q = 0; q < Dy; ++q) logdet += 2.0 * std::log(LR[q + q * Dy]);
-  for (int e = 0; e < D * D; ++e) {
-    h.A[e] = (T)mdl->A[e];
-    h.LQ[e] = (T)LQ[e];
-    h.L0[e] = (T)L0[e];
-  }
-  for (int e = 0; e < D; ++e) {
-    h.b[e] = (T)mdl->b[e];
-    h.mu0[e] = (T)mdl->mu0[e];
-  }
-  for (int e = 0; e < Dy * D; ++e) h.H[e] = (T)mdl->H[e];
-  for (int e = 0; e < Dy; ++e) h.c[e] = (T)mdl->c[e];
-  for (int e = 0; e < Dy * Dy; ++e) h.LRi[e] = (T)LRi[e];
-  h.cst = (T)(-0.5 * ((double)Dy * 1.8378770664093454835606594728112 + logdet));
+  const int rc = bf_build_model<T>(ctx, mdl, &h);
+  if (rc) return rc;
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->model_dev, &h, sizeof h, hipMemcpyHostToDevice,
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return SSM_OK;
 }
 
-int check_model(ssm_ctx* ctx, const ssm_lg_model* m) {
-  if (!m || m->Dx < 1 || m->Dx > kMaxD || m->Dy < 1 || m->Dy > kMaxDy)
-    return set_error(ctx, SSM_ERR_ARG, "model dims must satisfy 1<=Dx<=%d, 1<=Dy<=%d", kMaxD,
-                     kMaxDy);
-  if (!m->mu0 || !m->Sigma0 || !m->A || !m->b || !m->Q || !m->H || !m->c || !m->R)
-    return set_error(ctx, SSM_ERR_ARG, "model has a null parameter pointer");
-  return SSM_OK;
-}
+int check_model(ssm_ctx* ctx, const ssm_lg_model* m) { return bf_check_model(ctx, m); }
 
 TileScratch tile_scratch(ssm_bf* bf) {
   ResampleArgs a;
@@ -268,14 +207,6 @@ int ensure_steps(ssm_bf* bf, size_t T) {
     return set_error(ctx, SSM_ERR_OOM, "cannot allocate observation buffers for T=%zu", T);
   bf->steps_cap = T;
   return SSM_OK;
-}
-
-double ess_uniform(int N) {
-  // ESS = 1 / sum(w^2) with w = 1/N, accumulated left to right in FP64.
-  const double w = 1.0 / (double)N;
-  double s = 0.0;
-  for (int i = 0; i < N; ++i) s += w * w;
-  return 1.0 / s;
 }
 
 int set_step_base(ssm_bf* bf, int t, int ybase) {

@@ -91,4 +91,29 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n);
 // Enqueues the resampler kernels on `stream`.
 int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a);
 
+// ---- batched Kalman filters for the server (kf.hip) ---------------------------------
+// Device arrays of one group of independent filters with per-filter horizons Tf; the
+// parameter and observation packing is the one of ssm_kf_batch_create.
+struct KfGroupDev {
+  int B = 0;
+  int count[3] = {0, 0, 0};  // filters per size class
+  int start[3] = {0, 0, 0};  // first entry of each class in `order`
+  const int32_t* Dx = nullptr;
+  const int32_t* Dy = nullptr;
+  const int32_t* Tf = nullptr;
+  const int32_t* order = nullptr;
+  const double* params = nullptr;
+  const int64_t* param_off = nullptr;
+  const double* ys = nullptr;
+  const int64_t* y_off = nullptr;
+  double* ll = nullptr;
+  int32_t* status = nullptr;
+  uint32_t* counter = nullptr;  // 64 words
+};
+// Splits filters into the size classes, costliest first (host); order has B entries.
+int kf_group_plan(ssm_ctx* ctx, int B, const int32_t* Dx, const int32_t* Dy, const int32_t* Tf,
+                  int32_t* order, int count[3], int start[3]);
+// Enqueues the persistent class kernels of one group on stream s.
+int kf_group_launch(ssm_ctx* ctx, hipStream_t s, const KfGroupDev& g);
+
 }  // namespace ssm

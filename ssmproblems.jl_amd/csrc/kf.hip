@@ -34,6 +34,7 @@ namespace ssm {
 
 struct KfBatchParams {
   int T;
+  const int32_t* __restrict__ Tf;  // optional per-filter horizons (else T for all)
   int count;                      // filters in this class
   const int32_t* __restrict__ order;  // filter ids of this class, costliest first
   uint32_t* counter;              // work-queue head (zeroed before launch)
@@ -132,7 +133,8 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
     double ll = 0.0;
     int ok = 1;
     const double* yrow = p.ys + p.y_off[f];
-    for (int t = 0; t < p.T; ++t) {
+    const int Tn = p.Tf ? p.Tf[f] : p.T;
+    for (int t = 0; t < Tn; ++t) {
       // P1: mu_hat = A mu + b ; T1 = A Sigma
       for (int i = tid; i < dxp; i += nthr) {
         float s = bv[i];
@@ -326,6 +328,70 @@ int launch_class(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int grid_c
 
 }  // namespace
 
+namespace ssm {
+
+int kf_group_plan(ssm_ctx* ctx, int B, const int32_t* Dx, const int32_t* Dy, const int32_t* Tf,
+                  int32_t* order, int count[3], int start[3]) {
+  std::vector<std::vector<int32_t>> cls(kClasses);
+  for (int f = 0; f < B; ++f) {
+    int c = -1;
+    for (int k = 0; k < kClasses && c < 0; ++k)
+      if (Dx[f] >= 1 && Dy[f] >= 1 && Dx[f] <= kCfg[k].dx_max && Dy[f] <= kCfg[k].dy_max) c = k;
+    if (c < 0)
+      return set_error(ctx, SSM_ERR_ARG, "filter %d: dims Dx=%d Dy=%d outside 1..64 / 1..32", f,
+                       Dx[f], Dy[f]);
+    cls[c].push_back(f);
+  }
+  int n = 0;
+  for (int c = 0; c < kClasses; ++c) {
+    std::stable_sort(cls[c].begin(), cls[c].end(), [&](int a, int b) {
+      const double ca = ((double)Dx[a] * Dx[a] * Dx[a] + (double)Dy[a] * Dy[a] * Dy[a]) * Tf[a];
+      const double cb = ((double)Dx[b] * Dx[b] * Dx[b] + (double)Dy[b] * Dy[b] * Dy[b]) * Tf[b];
+      return ca > cb;
+    });
+    start[c] = n;
+    count[c] = (int)cls[c].size();
+    for (int f : cls[c]) order[n++] = f;
+  }
+  return SSM_OK;
+}
+
+int kf_group_launch(ssm_ctx* ctx, hipStream_t s, const KfGroupDev& g) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, g.counter, 64);
+  for (int c = kClasses - 1; c >= 0; --c) {
+    if (g.count[c] == 0) continue;
+    KfBatchParams p;
+    p.T = 0;
+    p.Tf = g.Tf;
+    p.count = g.count[c];
+    p.order = g.order + g.start[c];
+    p.counter = g.counter + 16 * c;
+    p.Dx = g.Dx;
+    p.Dy = g.Dy;
+    p.params = g.params;
+    p.param_off = g.param_off;
+    p.ys = g.ys;
+    p.y_off = g.y_off;
+    p.ll_out = g.ll;
+    p.status = g.status;
+    p.mu_out = nullptr;
+    p.Sigma_out = nullptr;
+    p.mu_off = nullptr;
+    p.cov_off = nullptr;
+    int rc;
+    if (c == 0)
+      rc = launch_class<16, 16, 1>(ctx, s, p, 0);
+    else if (c == 1)
+      rc = launch_class<32, 16, 2>(ctx, s, p, 0);
+    else
+      rc = launch_class<64, 32, 4>(ctx, s, p, 0);
+    if (rc) return rc;
+  }
+  return SSM_OK;
+}
+
+}  // namespace ssm
+
 struct ssm_kf_batch {
   ssm_ctx* ctx = nullptr;
   int B = 0, T = 0;
@@ -463,6 +529,7 @@ extern "C" int32_t ssm_kf_batch_run(ssm_kf_batch* kb, double* ll_out, int32_t* s
     SSM_HIP_CHECK(ctx, hipStreamWaitEvent(cs, kb->fork, 0));
     KfBatchParams p;
     p.T = kb->T;
+    p.Tf = nullptr;
     p.count = kb->count[c];
     p.order = kb->dorder + kb->start[c];
     p.counter = kb->dcounter + 16 * c;

@@ -15,5 +15,6 @@ from .kalman import KalmanFilter, KF, KFBatch, MvNormalState, pack_lg
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, run, filter,
                       rng_to_seed, last_timing)
+from .serve import Server
 
 __all__ = [n for n in dir() if not n.startswith("_")]

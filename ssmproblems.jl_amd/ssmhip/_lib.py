@@ -60,6 +60,13 @@ class LGModelC(C.Structure):
         (n, _dp) for n in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")]
 
 
+class RequestC(C.Structure):
+    """ssm_request."""
+    _fields_ = [("kind", C.c_int32), ("model", LGModelC), ("N", C.c_int32), ("T", C.c_int32),
+                ("ys", _dp), ("threshold", C.c_double), ("resampler", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
 class RBModelC(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("om0", "os0", "oa", "ob", "oq")] + [
         ("D", C.c_int32), ("Dy", C.c_int32), ("rs", C.c_double)] + [
@@ -129,6 +136,14 @@ def _declare(L):
         "ssm_rbpf_run": ([_vp, i32, i32, _dp, dbl, i32, _dp, _dp], i32),
         "ssm_rbpf_export": ([_vp, _dp, _dp, _dp, _dp, _ip, _dp], i32),
         "ssm_rbpf_last_timing": ([_vp, _dp], i32),
+        "ssm_serve_create": ([_vp, u64, i32, C.POINTER(_vp)], i32),
+        "ssm_serve_submit": ([_vp, C.POINTER(RequestC), C.POINTER(C.c_int64)], i32),
+        "ssm_serve_poll": ([_vp, C.c_int64, _ip, _dp, _dp, _ip], i32),
+        "ssm_serve_wait": ([_vp, C.c_int64, dbl, _ip, _dp, _dp, _ip], i32),
+        "ssm_serve_cancel": ([_vp, C.c_int64], i32),
+        "ssm_serve_release": ([_vp, C.c_int64], i32),
+        "ssm_serve_stats": ([_vp, _dp, i32], i32),
+        "ssm_serve_destroy": ([_vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
