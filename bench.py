@@ -159,6 +159,8 @@ def bench_kf(args, T=200, budget_s=10.0):
     w = W.batched_lg(args.B, T, seed=1234)
     kb = ssmhip.KFBatch.from_packed(w["Dx"], w["Dy"], w["params"], w["param_off"], w["ys"],
                                     w["y_off"], T, ctx=ctx)
+    tuned = kb.tune(T_tune=8)
+    log(f"autotune (waves per workgroup): {tuned}")
     for _ in range(args.warmup):
         ll = kb.run()
     ctx.synchronize()
@@ -209,6 +211,8 @@ def bench_kf(args, T=200, budget_s=10.0):
         "config": {"workload": f"batched KalmanFilter, B={args.B}, Dx~U{{2..64}}, Dy=Dx//2, "
                                f"T={T}, Joseph update", "B": args.B, "T": T},
         "loglik_max_rel_err": float(max(errs)), "loglik_err_sample": int(len(sample)),
+        "autotune": {k: {"waves": v["waves"], "ms": {str(a): b for a, b in v["ms"].items()}}
+                     for k, v in tuned.items()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                      "frac": achieved / 157.3, "traffic": None,
                      "kernel": "k_kf_batch (3 size classes, one run)",

@@ -22,6 +22,7 @@
 //   mid      <= 32   <= 16  2         ~33 KB
 //   large    <= 64   <= 32  4         ~127 KB
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -61,7 +62,9 @@ struct KfLds {
                                3 * DYP;  // float offset of the scalar block
   static_assert(kScal % 2 == 0, "scalar block must be 8-byte aligned");
   static constexpr int kFloats = kScal + 16;
-  static constexpr size_t kBytes = ((size_t)kFloats * 4 + 7) / 8 * 8 + (size_t)DYP * DYP * 8;
+  // FP64 block: the Cholesky factor (ld DYP + 1), the pivot column (2 DYP) and 1 / L_jj
+  static constexpr size_t kBytes =
+      ((size_t)kFloats * 4 + 7) / 8 * 8 + ((size_t)DYP * (DYP + 1) + 3 * DYP) * 8;
 };
 
 template <int DXP, int DYP, int NW>
@@ -88,6 +91,8 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
   float* scal = yv + DYP;  // [0] quad, [1] logdet, [2] ok
   int* iscal = reinterpret_cast<int*>(scal + 8);
   double* Sd = reinterpret_cast<double*>(lds + ((L::kFloats + 1) & ~1));
+  double* colbuf = Sd + DYP * (DYP + 1);
+  double* invd = colbuf + 2 * DYP;
 
   const int tid = threadIdx.x, nthr = 64 * NW;
   const int lane = tid & 63, wave = tid >> 6;
@@ -136,9 +141,14 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
     const int Tn = p.Tf ? p.Tf[f] : p.T;
     for (int t = 0; t < Tn; ++t) {
       // P1: mu_hat = A mu + b ; T1 = A Sigma
+      // matrix-vector products run over the zero padding up to dxp (exact zeros), 16 terms
+      // per unrolled block
       for (int i = tid; i < dxp; i += nthr) {
         float s = bv[i];
-        for (int k = 0; k < dx; ++k) s += Am[i * LX + k] * mu[k];
+        for (int k0 = 0; k0 < dxp; k0 += 16) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) s += Am[i * LX + k0 + u] * mu[k0 + u];
+        }
         muh[i] = (i < dx) ? s : 0.f;
       }
       for (int q = tid; q < dyp; q += nthr) yv[q] = q < dy ? (float)yrow[(size_t)t * dy + q] : 0.f;
@@ -150,7 +160,10 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
                  nullptr, 0, false, 0, Qm, LX, 0);
       for (int q = tid; q < dyp; q += nthr) {
         float s = cv[q];
-        for (int k = 0; k < dx; ++k) s += Hm[q * LX + k] * muh[k];
+        for (int k0 = 0; k0 < dxp; k0 += 16) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) s += Hm[q * LX + k0 + u] * muh[k0 + u];
+        }
         zv[q] = (q < dy) ? yv[q] - s : 0.f;
       }
       __syncthreads();
@@ -162,83 +175,14 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
       gemm16<NW>(KR, LY, dyp, dyp, 1.f, Pm, LX, false, Hm, LX, true, dxp, nullptr, 0, false,
                  nullptr, 0, false, 0, Rm, LY, 0);
       __syncthreads();
-      // P5 (wave 0): FP64 Cholesky of S, log-det, K' = S^-1 P, S^-1 z.
+      // P5: FP64 Cholesky of S and log-det by wave 0; the solves K' = S^-1 P and S^-1 z
+      // one column per thread over the whole workgroup.
       if (wave == 0) {
-        for (int e = lane; e < dy * dy; e += 64) {
-          const int i = e / dy, j = e % dy;
-          Sd[i * DYP + j] = (double)KR[i * LY + j];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        int good = 1;
-        double logdet = 0.0;
-        for (int j = 0; j < dy; ++j) {
-          double d = Sd[j * DYP + j];
-          if (!(d > 0.0)) {
-            good = 0;
-            break;
-          }
-          d = sqrt(d);
-          logdet += 2.0 * log(d);
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          if (lane == 0) Sd[j * DYP + j] = d;
-          for (int i = j + 1 + lane; i < dy; i += 64) Sd[i * DYP + j] /= d;
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          for (int i = j + 1 + lane; i < dy; i += 64) {
-            const double lij = Sd[i * DYP + j];
-            for (int k = j + 1; k <= i; ++k) Sd[i * DYP + k] -= lij * Sd[k * DYP + j];
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        }
-        if (good) {
-          // columns 0..dx-1 of P give K' = S^-1 P; column dx is z (forward solve only).
-          for (int col = lane; col <= dx; col += 64) {
-            double x[DYP];
-#pragma unroll
-            for (int q = 0; q < DYP; ++q)
-              x[q] = (q < dy) ? (double)((col < dx) ? Pm[q * LX + col] : zv[q]) : 0.0;
-#pragma unroll
-            for (int q = 0; q < DYP; ++q) {
-              if (q < dy) {
-                double s = x[q];
-#pragma unroll
-                for (int k = 0; k < q; ++k) s -= Sd[q * DYP + k] * x[k];
-                x[q] = s / Sd[q * DYP + q];
-              }
-            }
-            if (col == dx) {
-              double qd = 0.0;
-#pragma unroll
-              for (int q = 0; q < DYP; ++q)
-                if (q < dy) qd += x[q] * x[q];
-              reinterpret_cast<double*>(scal)[2] = qd;  // scal[4..5]
-            } else {
-#pragma unroll
-              for (int q = DYP - 1; q >= 0; --q) {
-                if (q < dy) {
-                  double s = x[q];
-#pragma unroll
-                  for (int k = q + 1; k < DYP; ++k)
-                    if (k < dy) s -= Sd[k * DYP + q] * x[k];
-                  x[q] = s / Sd[q * DYP + q];
-                }
-              }
-#pragma unroll
-              for (int q = 0; q < DYP; ++q)
-                if (q < dyp) KT[q * LX + col] = (q < dy) ? (float)x[q] : 0.f;
-            }
-          }
-          for (int e = lane; e < dyp * (dxp - dx); e += 64) {
-            const int q = e / (dxp - dx), col = dx + e % (dxp - dx);
-            KT[q * LX + col] = 0.f;
-          }
-        }
+        double ldet = 0.0;
+        const int good = chol_factor_wave<DYP>(KR, LY, dy, Sd, DYP + 1, colbuf, invd, &ldet);
         if (lane == 0) {
           iscal[1] = good;
-          reinterpret_cast<double*>(scal)[3] = logdet;  // scal[6..7]
+          reinterpret_cast<double*>(scal)[3] = ldet;  // scal[6..7]
         }
       }
       __syncthreads();
@@ -246,6 +190,17 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
         ok = 0;
         break;
       }
+      {
+        constexpr int LPC = DYP / 8;  // lanes per right-hand side
+        const int ncol = dx + 1 > dxp ? dx + 1 : dxp;
+        for (int c0 = 0; c0 < ncol; c0 += nthr / LPC) {  // whole groups in every pass
+          const int col = c0 + tid / LPC, r = tid % LPC;
+          const double qd = solve_group<DYP>(col < ncol ? col : dxp, r, Sd, DYP + 1, invd, Pm,
+                                             LX, zv, KT, LX, dx, dxp, dy, dyp);
+          if (col == dx && r == 0) reinterpret_cast<double*>(scal)[2] = qd;  // scal[4..5]
+        }
+      }
+      __syncthreads();
       if (tid == 0) {
         const double qd = reinterpret_cast<double*>(scal)[2];
         const double ld = reinterpret_cast<double*>(scal)[3];
@@ -254,7 +209,10 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
       // P6: mu = mu_hat + K z ; M = I - K H ; KR = K R
       for (int i = tid; i < dxp; i += nthr) {
         float s = muh[i];
-        for (int q = 0; q < dy; ++q) s += KT[q * LX + i] * zv[q];
+        for (int q0 = 0; q0 < dyp; q0 += 16) {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) s += KT[(q0 + u) * LX + i] * zv[q0 + u];
+        }
         mu[i] = (i < dx) ? s : 0.f;
       }
       gemm16<NW>(Mm, LX, dxp, dxp, -1.f, KT, LX, true, Hm, LX, false, dyp, nullptr, 0, false,
@@ -271,9 +229,9 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
                  LX, false, dyp, nullptr, 0, 0);
       __syncthreads();
       // P9: Symmetric(Sigma) keeps the upper triangle
-      for (int e = tid; e < dxp * dxp; e += nthr) {
-        const int i = e / dxp, j = e % dxp;
-        if (i > j) Sig[i * LX + j] = Sig[j * LX + i];
+      for (int e = tid; e < DXP * DXP; e += nthr) {
+        const int i = e / DXP, j = e % DXP;
+        if (i > j && i < dxp) Sig[i * LX + j] = Sig[j * LX + i];  // j < i < dxp
       }
       __syncthreads();
     }
@@ -324,6 +282,33 @@ int launch_class(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int grid_c
   hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * NW), lds, s, p);
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
+}
+
+// Waves per workgroup of each size class: candidates the autotuner times, the default
+// before tuning, and the process-wide tuned choice (0 = not tuned).
+constexpr int kNwCand[kClasses][3] = {{1, 2, 4}, {2, 4, 8}, {4, 8, 16}};
+constexpr int kNwDefault[kClasses] = {1, 2, 4};
+std::atomic<int> g_nw_tuned[kClasses] = {{0}, {0}, {0}};
+
+int class_nw(int c) {
+  const int v = g_nw_tuned[c].load();
+  return v ? v : kNwDefault[c];
+}
+
+int launch_class_nw(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int c, int nw) {
+  switch (c * 100 + nw) {
+    case 1: return launch_class<16, 16, 1>(ctx, s, p, 0);
+    case 2: return launch_class<16, 16, 2>(ctx, s, p, 0);
+    case 4: return launch_class<16, 16, 4>(ctx, s, p, 0);
+    case 102: return launch_class<32, 16, 2>(ctx, s, p, 0);
+    case 104: return launch_class<32, 16, 4>(ctx, s, p, 0);
+    case 108: return launch_class<32, 16, 8>(ctx, s, p, 0);
+    case 204: return launch_class<64, 32, 4>(ctx, s, p, 0);
+    case 208: return launch_class<64, 32, 8>(ctx, s, p, 0);
+    case 216: return launch_class<64, 32, 16>(ctx, s, p, 0);
+    default: break;
+  }
+  return set_error(ctx, SSM_ERR_ARG, "no Kalman kernel for class %d with %d waves", c, nw);
 }
 
 }  // namespace
@@ -378,13 +363,7 @@ int kf_group_launch(ssm_ctx* ctx, hipStream_t s, const KfGroupDev& g) {
     p.Sigma_out = nullptr;
     p.mu_off = nullptr;
     p.cov_off = nullptr;
-    int rc;
-    if (c == 0)
-      rc = launch_class<16, 16, 1>(ctx, s, p, 0);
-    else if (c == 1)
-      rc = launch_class<32, 16, 2>(ctx, s, p, 0);
-    else
-      rc = launch_class<64, 32, 4>(ctx, s, p, 0);
+    const int rc = launch_class_nw(ctx, s, p, c, class_nw(c));
     if (rc) return rc;
   }
   return SSM_OK;
@@ -545,13 +524,7 @@ extern "C" int32_t ssm_kf_batch_run(ssm_kf_batch* kb, double* ll_out, int32_t* s
     p.Sigma_out = want_state ? kb->dSig : nullptr;
     p.mu_off = kb->dmuoff;
     p.cov_off = kb->dcovoff;
-    int rc;
-    if (c == 0)
-      rc = launch_class<16, 16, 1>(ctx, cs, p, 0);
-    else if (c == 1)
-      rc = launch_class<32, 16, 2>(ctx, cs, p, 0);
-    else
-      rc = launch_class<64, 32, 4>(ctx, cs, p, 0);
+    const int rc = launch_class_nw(ctx, cs, p, c, class_nw(c));
     if (rc) return rc;
     SSM_HIP_CHECK(ctx, hipEventRecord(kb->join[c], cs));
     SSM_HIP_CHECK(ctx, hipStreamWaitEvent(s, kb->join[c], 0));
@@ -579,6 +552,77 @@ extern "C" int32_t ssm_kf_batch_run(ssm_kf_batch* kb, double* ll_out, int32_t* s
       return set_error(ctx, SSM_ERR_NOT_PD,
                        "filter %d: innovation covariance S is not positive definite", f);
   return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* nw_out,
+                                     double* ms_out) {
+  if (!kb) return SSM_ERR_ARG;
+  ssm_ctx* ctx = kb->ctx;
+  if (T_tune < 1) return set_error(ctx, SSM_ERR_ARG, "ssm_kf_batch_tune: T_tune must be >= 1");
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int Tt = std::min(T_tune, kb->T);
+  std::vector<int32_t> tt(kb->B, Tt);
+  int32_t* dtf = nullptr;
+  SSM_HIP_CHECK(ctx, hipMalloc(&dtf, sizeof(int32_t) * kb->B));
+  SSM_HIP_CHECK(ctx, hipMemcpy(dtf, tt.data(), sizeof(int32_t) * kb->B, hipMemcpyHostToDevice));
+  const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+  int rc = SSM_OK;
+  for (int c = 0; c < kClasses && rc == SSM_OK; ++c) {
+    for (int k = 0; k < 3; ++k)
+      if (ms_out) ms_out[3 * c + k] = NAN;
+    if (nw_out) nw_out[c] = class_nw(c);
+    if (kb->count[c] == 0) continue;
+    KfBatchParams p;
+    p.T = Tt;
+    p.Tf = dtf;
+    // enough filters to fill the device several times over, costliest first
+    p.count = std::min(kb->count[c], 8 * cus);
+    p.order = kb->dorder + kb->start[c];
+    p.counter = kb->dcounter + 16 * c;
+    p.Dx = kb->dDx;
+    p.Dy = kb->dDy;
+    p.params = kb->dparams;
+    p.param_off = kb->dpoff;
+    p.ys = kb->dys;
+    p.y_off = kb->dyoff;
+    p.ll_out = kb->dll;
+    p.status = kb->dstatus;
+    p.mu_out = nullptr;
+    p.Sigma_out = nullptr;
+    p.mu_off = nullptr;
+    p.cov_off = nullptr;
+    double best = 1e300;
+    int best_nw = kNwDefault[c];
+    for (int k = 0; k < 3 && rc == SSM_OK; ++k) {
+      const int nw = kNwCand[c][k];
+      double t_min = 1e300;
+      for (int rep = 0; rep < 3 && rc == SSM_OK; ++rep) {  // first launch warms up
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, kb->dcounter, 64);
+        if (hipEventRecord(kb->ev0, s) != hipSuccess) rc = SSM_ERR_HIP;
+        if (rc == SSM_OK) rc = launch_class_nw(ctx, s, p, c, nw);
+        if (rc == SSM_OK && (hipEventRecord(kb->ev1, s) != hipSuccess ||
+                             hipEventSynchronize(kb->ev1) != hipSuccess))
+          rc = SSM_ERR_HIP;
+        float ms = 0.f;
+        if (rc == SSM_OK && hipEventElapsedTime(&ms, kb->ev0, kb->ev1) != hipSuccess)
+          rc = SSM_ERR_HIP;
+        if (rc == SSM_OK && rep > 0) t_min = std::min(t_min, (double)ms);
+      }
+      if (ms_out) ms_out[3 * c + k] = t_min;
+      if (t_min < best) {
+        best = t_min;
+        best_nw = nw;
+      }
+    }
+    if (rc == SSM_OK) {
+      g_nw_tuned[c].store(best_nw);
+      if (nw_out) nw_out[c] = best_nw;
+    }
+  }
+  (void)hipFree(dtf);
+  if (rc == SSM_ERR_HIP) return set_error(ctx, rc, "ssm_kf_batch_tune: HIP failure");
+  return rc;
 }
 
 extern "C" int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms) {

@@ -52,7 +52,8 @@ struct RbWave {
   float Sig[kRbDX * kRbLX], T1[kRbDX * kRbLX], M[kRbDX * kRbLX];
   float P[kRbDY * kRbLX], KT[kRbDY * kRbLX], KR[kRbDX * kRbLY], Rt[kRbDY * kRbLY];
   float mu[kRbDX], muh[kRbDX], z[kRbDY], y[kRbDY];
-  double Sd[kRbDY * kRbDY];
+  double Sd[kRbDY * (kRbDY + 1)];
+  double chol_work[3 * kRbDY];
 };
 
 struct RbStepParams {
@@ -158,8 +159,8 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
               nullptr, 0, false, nullptr, 0, false, 0, w.Rt, kRbLY, 0);
     wave_lds_sync();
     double quad = 0.0, logdet = 0.0;
-    const int good = chol_solve_wave<kRbDY>(w.Sd, kRbDY, w.KR, kRbLY, w.P, kRbLX, w.z, w.KT,
-                                            kRbLX, D, kRbDX, Dy, kRbDY, &quad, &logdet);
+    const int good = chol_solve_wave<kRbDY>(w.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, w.KT,
+                                            kRbLX, D, kRbDX, Dy, kRbDY, &quad, &logdet, w.chol_work);
     float lnew;
     if (good) {
       const double ll = -0.5 * ((double)Dy * log2pi + logdet + quad);

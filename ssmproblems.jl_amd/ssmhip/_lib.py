@@ -128,6 +128,7 @@ def _declare(L):
                                  C.POINTER(C.c_int64), C.POINTER(_vp)], i32),
         "ssm_kf_batch_run": ([_vp, _dp, _ip, _dp, _dp], i32),
         "ssm_kf_batch_last_timing": ([_vp, _dp], i32),
+        "ssm_kf_batch_tune": ([_vp, i32, _ip, _dp], i32),
         "ssm_kf_batch_destroy": ([_vp], i32),
         "ssm_rbpf_create": ([_vp, C.POINTER(RBModelC), i32, u64, C.POINTER(_vp)], i32),
         "ssm_rbpf_destroy": ([_vp], i32),

@@ -113,6 +113,22 @@ class KFBatch:
         self.ctx.check(self.ctx.lib.ssm_kf_batch_last_timing(self.ptr, C.byref(ms)))
         return ms.value
 
+    CLASSES = ("Dx<=16", "Dx<=32", "Dx<=64")
+    CANDIDATES = ((1, 2, 4), (2, 4, 8), (4, 8, 16))
+
+    def tune(self, T_tune=8):
+        """Times each workgroup shape of each size class on this batch (T_tune steps) and
+        keeps the fastest for the rest of the process.  Returns {class: {"waves": chosen,
+        "ms": {waves: time}}}."""
+        nw = (C.c_int32 * 3)()
+        ms = (C.c_double * 9)()
+        self.ctx.check(self.ctx.lib.ssm_kf_batch_tune(self.ptr, int(T_tune), nw, ms))
+        out = {}
+        for c, name in enumerate(self.CLASSES):
+            out[name] = {"waves": nw[c],
+                         "ms": {w: ms[3 * c + k] for k, w in enumerate(self.CANDIDATES[c])}}
+        return out
+
     def close(self):
         if getattr(self, "ptr", None):
             self.ctx.lib.ssm_kf_batch_destroy(self.ptr)

@@ -98,3 +98,18 @@ def test_non_pd_reported(ctx):
     kb = ssmhip.KFBatch([p], [np.zeros((3, 1))], ctx=ctx)
     with pytest.raises(ssmhip.PosDefException):
         kb.run()
+
+
+def test_autotune_keeps_results(ctx, orc):
+    # every workgroup shape runs the same per-tile and per-column arithmetic, so the tuned
+    # kernels reproduce the default kernels' log-likelihoods bit for bit
+    w = W.batched_lg(400, 20, seed=21, dx_lo=1, dx_hi=64)
+    kb = ssmhip.KFBatch.from_packed(w["Dx"], w["Dy"], w["params"], w["param_off"], w["ys"],
+                                    w["y_off"], w["T"], ctx=ctx)
+    before = kb.run()
+    tuned = kb.tune(T_tune=4)
+    after = kb.run()
+    kb.close()
+    for name, v in tuned.items():
+        assert v["waves"] in v["ms"], (name, v)
+    assert np.array_equal(before, after)
