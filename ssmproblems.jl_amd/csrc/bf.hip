@@ -427,6 +427,9 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   if (rc) return rc;
   constexpr int kChunk = 16;  // even: the x ping-pong returns to its start
   const int nfull = T / kChunk, rem = T % kChunk;
+  auto one_step = [&]() {
+    return enqueue_step(bf, s, threshold, resampler, nullptr, nullptr, false);
+  };
   hipGraphExec_t exec = nullptr;
   if (nfull > 0) {
     const auto key = std::make_tuple(kChunk, bf->cur, resampler, threshold, 0);
@@ -435,8 +438,7 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
       hipGraph_t g;
       SSM_HIP_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       const int cur0 = bf->cur;
-      for (int k = 0; k < kChunk && rc == SSM_OK; ++k)
-        rc = enqueue_step(bf, s, threshold, resampler, nullptr, nullptr, false);
+      for (int k = 0; k < kChunk && rc == SSM_OK; ++k) rc = one_step();
       bf->cur = cur0;
       hipError_t e = hipStreamEndCapture(s, &g);
       if (rc) return rc;
@@ -454,8 +456,7 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   }
   SSM_HIP_CHECK(ctx, hipEventRecord(bf->ev0, s));
   for (int k = 0; k < nfull; ++k) SSM_HIP_CHECK(ctx, hipGraphLaunch(exec, s));
-  for (int k = 0; k < rem && rc == SSM_OK; ++k)
-    rc = enqueue_step(bf, s, threshold, resampler, nullptr, nullptr, false);
+  for (int k = 0; k < rem && rc == SSM_OK; ++k) rc = one_step();
   if (rc) return rc;
   SSM_HIP_CHECK(ctx, hipEventRecord(bf->ev1, s));
   SSM_HIP_CHECK(ctx, hipGetLastError());
@@ -614,23 +615,25 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
   // step counter back to the last step, so the step kernel reads observation row T - 1
   if (rc == SSM_OK && reps > 0) rc = set_step_base(bf, t0 + T - 1, t0 - 1);
   if (rc == SSM_OK && reps > 0) {
-    hipEvent_t e[4];
+    hipEvent_t e[5];
     for (auto& x : e)
       if (hipEventCreate(&x) != hipSuccess) return set_error(ctx, SSM_ERR_HIP, "event create");
     (void)hipEventRecord(e[0], s);
+    for (int k = 0; k < reps; ++k) enqueue_move(bf, s, threshold, nullptr);
+    (void)hipEventRecord(e[1], s);
     for (int k = 0; k < reps && rc == SSM_OK; ++k)
       rc = enqueue_resample(bf, s, threshold, resampler, nullptr, false);
-    (void)hipEventRecord(e[1], s);
-    for (int k = 0; k < reps; ++k) enqueue_move(bf, s, threshold, nullptr);
     (void)hipEventRecord(e[2], s);
     for (int k = 0; k < reps; ++k) enqueue_finish(bf, s, threshold, 0);
     (void)hipEventRecord(e[3], s);
     (void)hipStreamSynchronize(s);
-    for (int j = 0; j < 3; ++j) {
-      float m = 0.f;
-      (void)hipEventElapsedTime(&m, e[j], e[j + 1]);
-      b2b[j] = m / reps;
-    }
+    float m01 = 0.f, m12 = 0.f, m23 = 0.f;
+    (void)hipEventElapsedTime(&m01, e[0], e[1]);
+    (void)hipEventElapsedTime(&m12, e[1], e[2]);
+    (void)hipEventElapsedTime(&m23, e[2], e[3]);
+    b2b[1] = m01 / reps;
+    b2b[0] = m12 / reps;
+    b2b[2] = m23 / reps;
     for (auto& x : e) (void)hipEventDestroy(x);
   }
   double acc[3] = {0.0, 0.0, 0.0};
@@ -640,14 +643,17 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
       (void)hipEventElapsedTime(&m, ev[4 * k + j], ev[4 * k + j + 1]);
       acc[j] += m;
     }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  if (rc) return rc;
+  if (rc) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+  }
   SSM_HIP_CHECK(ctx, hipGetLastError());
   if (ms_kernels)
     for (int j = 0; j < 3; ++j) {
       ms_kernels[j] = acc[j] / T;
       ms_kernels[3 + j] = b2b[j];
     }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   if (ll_total) *ll_total = tot;
   return SSM_OK;
 }

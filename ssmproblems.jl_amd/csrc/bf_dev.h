@@ -190,7 +190,10 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   auto load_item = [&](int k, T* xk) {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
     valid[k] = i < p.N;
-    const int src = (rs && valid[k]) ? p.anc[i] : i;
+    // the ancestor is read whatever the decision (anc holds Npad entries), so the load
+    // does not wait for the scalar block
+    const int a = p.anc[i];
+    const int src = (rs && valid[k]) ? a : i;
 #pragma unroll
     for (int d = 0; d < MD; ++d)
       if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];

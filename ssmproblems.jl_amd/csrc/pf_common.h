@@ -138,39 +138,49 @@ __global__ void __launch_bounds__(kTileThreads)
   tile_partial<T>(nl, lv, valid, fresh, partials, gridDim.x, blockIdx.x);
 }
 
-// One block: fold the tile partials, finish the step, prepare the next resampling.
-// T is the filter's type: the per-tile scale factors exp(m1_b - M) are evaluated in T.
-// Each thread owns a contiguous chunk of at most CH tiles held in registers (one load
-// phase), so the kernel handles nb <= 256 * CH tiles; the host picks CH.
+// The fold of one step's tile partials by one block (every thread ends with the same
+// block-wide values).  T is the filter's type: the per-tile scale factors exp(m1_b - M) are
+// evaluated in T.  Each thread owns a contiguous chunk [lo, hi) of at most CH tiles held in
+// registers (one load phase), so nb <= NT * CH.  For its chunk a thread keeps the tile
+// maxima m1, the scales c_b = exp(m1_b - M) / S and the exclusive prefixes of c_b tau_b,
+// made nondecreasing by an exact running max (the resampler's tiles meet at these values,
+// so they must never step backwards).
+template <int CH>
+struct TileFold {
+  int lo, hi;
+  double m1[CH], c[CH], pre[CH];
+  double M1, S, S2, M0, S0, tot;
+  bool dead;
+};
+
 template <typename T, int CH, int NT>
-__device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* sc_in,
-                                                BfScalars* sc, double threshold_n, double logN,
-                                                const TileScratch& ts, double* ll_steps,
-                                                int32_t* rs_steps, int record) {
+__device__ __forceinline__ void fold_tiles(int nb, const double* __restrict__ partials,
+                                           bool fresh, TileFold<CH>& F) {
   constexpr int NW = NT / 64;
   __shared__ double red[3][NW];
   __shared__ double wtot[NW], wmax[NW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const bool rs = threshold_n >= sc_in->ess;
-  const bool fresh = rs || sc_in->typeless;
   const int chunk = (nb + NT - 1) / NT;
   const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
-  double m0[CH], s0[CH], m1[CH], tau[CH], s2[CH];
+  F.lo = lo;
+  F.hi = hi;
+  double m0[CH], s0[CH], tau[CH], s2[CH];
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
     const int b = lo + q;
     const bool in = b < hi;
-    m0[q] = in ? ts.partials[b] : -INFINITY;
-    s0[q] = in ? ts.partials[nb + b] : 0.0;
-    m1[q] = in ? ts.partials[2 * nb + b] : -INFINITY;
-    tau[q] = in ? ts.partials[3 * nb + b] : 0.0;
-    s2[q] = in ? ts.partials[4 * nb + b] : 0.0;
+    // the baseline partials are only used when the step was not a fresh start
+    m0[q] = (in && !fresh) ? partials[b] : -INFINITY;
+    s0[q] = (in && !fresh) ? partials[nb + b] : 0.0;
+    F.m1[q] = in ? partials[2 * nb + b] : -INFINITY;
+    tau[q] = in ? partials[3 * nb + b] : 0.0;
+    s2[q] = in ? partials[4 * nb + b] : 0.0;
   }
   // maxima of the tile maxima (exact, any order)
   double M1 = -INFINITY, M0 = -INFINITY;
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
-    M1 = fmax(M1, m1[q]);
+    M1 = fmax(M1, F.m1[q]);
     M0 = fmax(M0, m0[q]);
   }
   M1 = wave_max_dpp(M1);
@@ -188,13 +198,12 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
   }
   __syncthreads();
   // scaled sums: one exponential per tile, sums in FP64, fixed order
-  double f[CH];
   double S = 0.0, S2 = 0.0, S0 = 0.0;
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
-    f[q] = (m1[q] != -INFINITY) ? (double)exp_w((T)(m1[q] - M1)) : 0.0;
-    S += tau[q] * f[q];
-    S2 += s2[q] * f[q] * f[q];
+    F.c[q] = (F.m1[q] != -INFINITY) ? (double)exp_w((T)(F.m1[q] - M1)) : 0.0;
+    S += tau[q] * F.c[q];
+    S2 += s2[q] * F.c[q] * F.c[q];
     if (!fresh && m0[q] != -INFINITY) S0 += s0[q] * (double)exp_w((T)(m0[q] - M0));
   }
   S = wave_sum_dpp(S);
@@ -220,8 +229,8 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
   double cs = 0.0;
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
-    f[q] *= invS;  // now c_b
-    cs += f[q] * tau[q];
+    F.c[q] *= invS;
+    cs += F.c[q] * tau[q];
   }
   const double incl = wave_scan_dpp(cs);
   double run = wave_shfl_up(incl, 1);
@@ -229,15 +238,12 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
   if (lane == 63) wtot[wv] = incl;
   __syncthreads();
   for (int k = 0; k < wv; ++k) run += wtot[k];
-  // prefixes, made nondecreasing with an exact prefix max: the resampler's tiles meet at
-  // these values, so they must never step backwards
-  double pre[CH];
   double pmax = -INFINITY;
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
     pmax = fmax(pmax, run);
-    pre[q] = pmax;
-    run += f[q] * tau[q];
+    F.pre[q] = pmax;
+    run += F.c[q] * tau[q];
   }
   double cm = wave_scan_max_dpp((lo < hi) ? pmax : -INFINITY);
   if (lane == 63) wmax[wv] = cm;
@@ -246,52 +252,94 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
   __syncthreads();
   for (int k = 0; k < wv; ++k) carry = fmax(carry, wmax[k]);
 #pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const int b = lo + q;
-    if (b < hi) {
-      ts.prefix[b] = fmax(pre[q], carry);
-      ts.blk_c[b] = f[q];
-      ts.blk_m[b] = m1[q];
+  for (int q = 0; q < CH; ++q) F.pre[q] = fmax(F.pre[q], carry);
+  double tot = wtot[0];
+  for (int k = 1; k < NW; ++k) tot += wtot[k];
+  for (int k = 0; k < NW; ++k) tot = fmax(tot, wmax[k]);
+  __syncthreads();  // the LDS of this fold may be reused by the caller
+  F.M1 = M1;
+  F.S = S;
+  F.S2 = S2;
+  F.M0 = M0;
+  F.S0 = S0;
+  F.tot = tot;
+  F.dead = dead;
+}
+
+// ESS of a folded step (resamplers.jl:179-183): N for dead weights, clamped to <= N.
+template <int CH>
+__device__ __forceinline__ double fold_ess(const TileFold<CH>& F, int N) {
+  if (F.dead) return (double)N;
+  double ess = (F.S * F.S) / F.S2;
+  if (!(ess <= (double)N)) ess = (double)N;  // ESS <= N exactly; clamp rounding
+  return ess;
+}
+
+// The scalars a folded step leaves behind (marginalise!, containers.jl:160-173; ESS,
+// resamplers.jl:179-183): reads the pre-step block sc_in, returns the new one.  rs and
+// fresh are the decisions the folded step was run with.
+template <typename T, int CH>
+__device__ __forceinline__ BfScalars fold_scalars(const TileFold<CH>& F, const BfScalars& sc_in,
+                                                  int N, double logN, bool rs, bool fresh,
+                                                  double* ll_steps, int32_t* rs_steps,
+                                                  int record) {
+  const double baseline = fresh ? logN : F.M0 + log(F.S0);
+  const int t = sc_in.step;
+  const int idx = t - 1 - sc_in.ybase;
+  BfScalars s = sc_in;
+  if (F.dead) {
+    // every particle has zero weight: the increment is -Inf; weights restart uniform
+    if (record && ll_steps) ll_steps[idx] = -INFINITY;
+    if (record) s.ll_total += -INFINITY;
+    s.lse = 0.0;
+    s.ess = (double)N;
+    s.typeless = 1;
+  } else {
+    const double lse = F.M1 + log(F.S);
+    const double ess = fold_ess(F, N);
+    s.lse = lse;
+    s.mxn = (double)((T)F.M1 - (T)lse);
+    s.s1 = F.S;
+    s.ess = ess;
+    s.typeless = 0;
+    if (record) {
+      const double ll = lse - baseline;
+      s.ll_total += ll;
+      if (ll_steps) ll_steps[idx] = ll;
     }
   }
-  if (tid == 0) {
-    double tot = wtot[0];
-    for (int k = 1; k < NW; ++k) tot += wtot[k];
-    for (int k = 0; k < NW; ++k) tot = fmax(tot, wmax[k]);
-    ts.prefix[nb] = tot;
-    const double baseline = fresh ? logN : M0 + log(S0);
-    const int t = sc_in->step;
-    const int idx = t - 1 - sc_in->ybase;
-    BfScalars s = *sc_in;
-    if (dead) {
-      // every particle has zero weight: the increment is -Inf; weights restart uniform
-      if (record && ll_steps) ll_steps[idx] = -INFINITY;
-      if (record) s.ll_total += -INFINITY;
-      s.lse = 0.0;
-      s.ess = (double)N;
-      s.typeless = 1;
-    } else {
-      const double lse = M1 + log(S);
-      double ess = (S * S) / S2;
-      if (!(ess <= (double)N)) ess = (double)N;  // ESS <= N exactly; clamp rounding
-      s.lse = lse;
-      s.mxn = (double)((T)M1 - (T)lse);
-      s.s1 = S;
-      s.ess = ess;
-      s.typeless = 0;
-      if (record) {
-        const double ll = lse - baseline;
-        s.ll_total += ll;
-        if (ll_steps) ll_steps[idx] = ll;
-      }
+  s.total_w = F.tot;
+  if (record) {
+    if (rs_steps) rs_steps[idx] = rs ? 1 : 0;
+    s.last_resampled = rs ? 1 : 0;
+    s.step = t + 1;
+  }
+  return s;
+}
+
+// One block: fold the tile partials, finish the step, prepare the next resampling (the
+// per-tile scales and prefixes go to ts for the resampler).
+template <typename T, int CH, int NT>
+__device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* sc_in,
+                                                BfScalars* sc, double threshold_n, double logN,
+                                                const TileScratch& ts, double* ll_steps,
+                                                int32_t* rs_steps, int record) {
+  const bool rs = threshold_n >= sc_in->ess;
+  const bool fresh = rs || sc_in->typeless;
+  TileFold<CH> F;
+  fold_tiles<T, CH, NT>(nb, ts.partials, fresh, F);
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    const int b = F.lo + q;
+    if (b < F.hi) {
+      ts.prefix[b] = F.pre[q];
+      ts.blk_c[b] = F.c[q];
+      ts.blk_m[b] = F.m1[q];
     }
-    s.total_w = tot;
-    if (record) {
-      if (rs_steps) rs_steps[idx] = rs ? 1 : 0;
-      s.last_resampled = rs ? 1 : 0;
-      s.step = t + 1;
-    }
-    *sc = s;
+  }
+  if (threadIdx.x == 0) {
+    ts.prefix[nb] = F.tot;
+    *sc = fold_scalars<T, CH>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
   }
 }
 

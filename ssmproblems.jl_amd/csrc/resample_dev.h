@@ -7,6 +7,8 @@
 //   Stratified        GF/ext/CUDAExt.jl:68-78       O_j = min(n, floor(r_j + u[min(n, floor(r_j+1))]))
 //   Multinomial       GF/ext/CUDAExt.jl:31-39       a_k = min{j : W_j >= u_k W_n}
 #pragma once
+#include <type_traits>
+
 #include "device_common.h"
 #include "internal.h"
 #include "pf_common.h"
@@ -54,6 +56,33 @@ __device__ __forceinline__ double rs_weight(const RsParams& p, int j, int b) {
   } else {
     const double e = exp_w(((const double*)p.w)[j] - p.blk_m[b]);
     return e * p.blk_c[b];
+  }
+}
+
+// The raw stored value behind a weight (loaded early), and the weight made from it with
+// the tile's m_b and c_b: exactly what rs_weight returns.
+template <int MODE>
+struct RawW {
+  using type = typename std::conditional<(MODE == kWeightsF64 || MODE == kWeightsLogF64), double,
+                                         float>::type;
+  type v;
+};
+template <int MODE>
+__device__ __forceinline__ RawW<MODE> rs_raw(const RsParams& p, int j) {
+  using V = typename RawW<MODE>::type;
+  return RawW<MODE>{((const V*)p.w)[j]};
+}
+template <int MODE>
+__device__ __forceinline__ double rs_weight_of(const RsParams& p, RawW<MODE> r, double m_b,
+                                               double c_b, bool typeless) {
+  if (MODE == kWeightsF64 || MODE == kWeightsF32) return (double)r.v;
+  if (typeless) return 1.0 / (double)p.n;
+  if (MODE == kWeightsLogF32) {
+    const float e = exp_w((float)r.v - (float)m_b);
+    return (double)e * c_b;
+  } else {
+    const double e = exp_w((double)r.v - m_b);
+    return e * c_b;
   }
 }
 
@@ -153,9 +182,39 @@ constexpr int kDirectSlots = 32;
 // exceeds kDirectSlots hands it to the whole block, which keeps degenerate weights (one
 // particle taking every slot) balanced.  For MULTINOMIAL the kernel writes the
 // nondecreasing prefix sums W to Wbuf instead.
+// What a tile's resample needs besides its weights: the decision, the lazy-uniform flag,
+// the tile's prefix P, the next tile's prefix, the total, the tile's weight scale (m_b, c_b)
+// and the Philox step of the uniforms.
+struct TileIn {
+  bool rs, typeless;
+  double P, Pnext, Wn, m_b, c_b;
+  uint32_t step;
+};
+
+// Loads the raw weights of tile b (clamped in-bounds indices).
 template <int MODE>
-__device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
-  if (!rs_active(p)) return;
+__device__ __forceinline__ void load_tile_raw(const RsParams& p, int b,
+                                              RawW<MODE> raw[kTileItems]) {
+  const int n = p.n, tile0 = b * kTile;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int j = tile0 + kTileItems * threadIdx.x + k;
+    raw[k] = rs_raw<MODE>(p, j < n ? j : n - 1);
+  }
+}
+
+// Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  Thread t owns
+// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial scan
+// plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by its own
+// thread (neighbouring threads write neighbouring ranges); a thread whose range exceeds
+// kDirectSlots hands it to the whole block, which keeps degenerate weights (one particle
+// taking every slot) balanced.  For MULTINOMIAL it writes the nondecreasing prefix sums W to
+// Wbuf instead.  All threads of the block call it; it returns at once when !in.rs.
+template <int MODE>
+__device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
+                                                       const RawW<MODE> raw[kTileItems],
+                                                       const TileIn& in) {
+  if (!in.rs) return;
   __shared__ double lds[kRsThreads / kWave];
   __shared__ int olast[kRsThreads];
   __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
@@ -163,18 +222,17 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   const int n = p.n;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
-  const int cnt = min(kTile, n - tile0);
   const int t = threadIdx.x;
-  const bool typeless = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64) && p.sc->typeless;
+  const int cnt = min(kTile, n - tile0);
   double P, Pnext, Wn;
-  if (typeless) {
+  if (in.typeless) {
     P = (double)tile0 / (double)n;
     Pnext = (b == nb - 1) ? 1.0 : (double)(tile0 + kTile) / (double)n;
     Wn = 1.0;
   } else {
-    P = p.prefix[b];
-    Pnext = p.prefix[b + 1];
-    Wn = p.prefix[nb];
+    P = in.P;
+    Pnext = in.Pnext;
+    Wn = in.Wn;
   }
   if (t == 0) nheavy = 0;
   // thread-serial scan of 4 consecutive weights, then one block scan of the thread sums
@@ -185,7 +243,7 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
     const int j = tile0 + kTileItems * t + k;
     double wj = 0.0;
     if (j < n) {
-      wj = rs_weight<MODE>(p, j, b);
+      wj = rs_weight_of<MODE>(p, raw[k], in.m_b, in.c_b, in.typeless);
       if (p.w_debug64) p.w_debug64[j] = wj;
     }
     acc += wj;
@@ -202,7 +260,7 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   const double carry = block_excl_max_d(W[kTileItems - 1], lds);
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
-  const uint32_t step = rs_step(p);
+  const uint32_t step = in.step;
   if (p.kind == SSM_MULTINOMIAL) {
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
@@ -254,6 +312,28 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
       p.anc[s] = tile0 + kTileItems * owner + k;
     }
   }
+}
+
+// The resample of tile b with its inputs from memory: the decision and flags from the
+// filter's scalars (or always, for explicit weights), the prefixes and scales from the
+// finish kernel.  Every global input is requested before the decision is known (all the
+// addresses are in bounds), so the kernel pays one round trip before its arithmetic.
+template <int MODE>
+__device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
+  const int nb = (p.n + kTile - 1) / kTile;
+  RawW<MODE> raw[kTileItems];
+  load_tile_raw<MODE>(p, b, raw);
+  constexpr bool kLog = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64);
+  TileIn in;
+  in.P = p.prefix[b];
+  in.Pnext = p.prefix[b + 1];
+  in.Wn = p.prefix[nb];
+  in.m_b = kLog ? p.blk_m[b] : 0.0;
+  in.c_b = kLog ? p.blk_c[b] : 0.0;
+  in.rs = rs_active(p);
+  in.typeless = kLog && p.sc->typeless;
+  in.step = rs_step(p);
+  scan_offspring_tile_in<MODE>(p, b, raw, in);
 }
 
 // Multinomial: a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.

@@ -115,16 +115,18 @@ def test_philox_noise_matches_oracle(ctx, orc):
     assert ll == pytest.approx(ll_o, abs=1e-4)
 
 
-@pytest.mark.parametrize("kind", [0, 2, 3])
-def test_run_equals_steps(ctx, kind):
+@pytest.mark.parametrize("kind,N,prec", [(0, 20000, "f32"), (2, 20000, "f32"),
+                                         (3, 20000, "f32"), (0, 300000, "f32"),
+                                         (2, 5000, "f64"), (3, 1 << 20, "f32")])
+def test_run_equals_steps(ctx, kind, N, prec):
     # The graph-replayed run and the step-by-step path execute the same kernels on the
     # same counters: identical log-likelihood increments, bit for bit.
     m = config2_model()
     model = to_ssmhip_model(m)
-    N, T = 20000, 37
+    T = 37
     rng = np.random.default_rng(3)
     ys = rng.standard_normal((T, 2))
-    algo = ssmhip.BF(N, threshold=0.7, resampler=RES[kind], precision="f32")
+    algo = ssmhip.BF(N, threshold=0.7, resampler=RES[kind], precision=prec)
     a = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=99)
     tot, steps = ssmhip.run(a, algo, ys, return_steps=True)
     b = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=99)
