@@ -54,6 +54,7 @@ typedef struct ssm_bf ssm_bf;
 typedef struct ssm_rbpf ssm_rbpf;
 typedef struct ssm_server ssm_server;
 typedef struct ssm_kf_batch ssm_kf_batch;
+typedef struct ssm_tree ssm_tree;
 
 /*
  * Homogeneous linear-Gaussian state-space model, the parameters returned by
@@ -169,6 +170,26 @@ int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
  */
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
                        int32_t resampler, int32_t reps, double* ms_kernels, double* ll_total);
+
+/* ---- path storage of a bootstrap filter (GF/ext/CUDAExt.jl:146-298) --------------- */
+/*
+ * Sparse ancestry storage on the device (Jacob, Murray & Rubenthaler 2015): the device
+ * ParallelParticleTree with the ParallelAncestorCallback behaviour (CUDAExt.jl:273-298) and
+ * the paths of ParticleTree's get_ancestry (GF/src/containers.jl:594-611).  Nodes without
+ * live descendants are pruned and their slots reused; the pool doubles when short of N
+ * free slots (expand!, CUDAExt.jl:210-226).
+ *   ssm_bf_tree_create   roots = the filter's current particles (PostInit); capacity >= N
+ *   ssm_bf_tree_insert   appends the particles after the filter's last step, each linked to
+ *                        its ancestor (PostUpdate; the identity when the step kept them)
+ *   ssm_bf_tree_info     pool size, free slots and depth (number of inserts)
+ *   ssm_bf_tree_ancestry every leaf's path: paths[(i (depth+1) + t) Dx + d], t = 0..depth
+ */
+int32_t ssm_bf_tree_create(ssm_bf* bf, int64_t capacity, ssm_tree** out);
+int32_t ssm_bf_tree_insert(ssm_tree* tree);
+int32_t ssm_bf_tree_info(ssm_tree* tree, int64_t* capacity, int64_t* free_slots,
+                         int32_t* depth);
+int32_t ssm_bf_tree_ancestry(ssm_tree* tree, double* paths);
+int32_t ssm_bf_tree_destroy(ssm_tree* tree);
 
 /* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */
 /*

@@ -658,6 +658,23 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
   return SSM_OK;
 }
 
+namespace ssm {
+int bf_device_view(ssm_bf* bf, BfView* v) {
+  if (!bf || !v) return SSM_ERR_ARG;
+  if (!bf->initialised)
+    return set_error(bf->ctx, SSM_ERR_STATE, "path storage needs an initialised filter");
+  v->ctx = bf->ctx;
+  v->N = bf->N;
+  v->Npad = bf->Npad;
+  v->D = bf->D;
+  v->prec = bf->prec;
+  v->x = bf->x[bf->cur];
+  v->anc = bf->anc;
+  v->sc = bf->sc;
+  return SSM_OK;
+}
+}  // namespace ssm
+
 extern "C" int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step) {
   if (!bf) return SSM_ERR_ARG;
   if (ms_total) *ms_total = bf->last_ms;

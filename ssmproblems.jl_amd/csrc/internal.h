@@ -91,6 +91,16 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n);
 // Enqueues the resampler kernels on `stream`.
 int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a);
 
+// ---- a bootstrap filter's device buffers, for the path storage (bf.hip) -------------
+struct BfView {
+  ssm_ctx* ctx;
+  int N, Npad, D, prec;
+  const void* x;         // current state planes x[d][Npad]
+  const int32_t* anc;    // ancestors of the last step (valid if sc->last_resampled)
+  const BfScalars* sc;
+};
+int bf_device_view(ssm_bf* bf, BfView* v);
+
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the
 // parameter and observation packing is the one of ssm_kf_batch_create.

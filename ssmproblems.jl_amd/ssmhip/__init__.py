@@ -16,5 +16,6 @@ from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, run, filter,
                       rng_to_seed, last_timing)
 from .serve import Server
+from .tree import ParallelParticleTree, ParallelAncestorCallback
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -137,6 +137,11 @@ def _declare(L):
         "ssm_rbpf_run": ([_vp, i32, i32, _dp, dbl, i32, _dp, _dp], i32),
         "ssm_rbpf_export": ([_vp, _dp, _dp, _dp, _dp, _ip, _dp], i32),
         "ssm_rbpf_last_timing": ([_vp, _dp], i32),
+        "ssm_bf_tree_create": ([_vp, C.c_int64, C.POINTER(_vp)], i32),
+        "ssm_bf_tree_insert": ([_vp], i32),
+        "ssm_bf_tree_info": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip], i32),
+        "ssm_bf_tree_ancestry": ([_vp, _dp], i32),
+        "ssm_bf_tree_destroy": ([_vp], i32),
         "ssm_serve_create": ([_vp, u64, i32, C.POINTER(_vp)], i32),
         "ssm_serve_submit": ([_vp, C.POINTER(RequestC), C.POINTER(C.c_int64)], i32),
         "ssm_serve_poll": ([_vp, C.c_int64, _ip, _dp, _dp, _ip], i32),
