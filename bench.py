@@ -509,6 +509,106 @@ def serve_cpu_baseline(reqs, budget_s):
                       f"of the stream; BF runs cut short are extrapolated to their full T"}
 
 
+def bench_tree(args, T=200, budget_s=10.0):
+    """Path storage: the headline bootstrap filter (D=4 LG, N particles, systematic at
+    threshold 1.0) with its particles appended to the device ancestry tree after every step
+    (ParallelAncestorCallback).  One bench step = T filter steps, each followed by an
+    insert; the insert cost is the difference to the same steps without the tree."""
+    import ssmhip
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    log(f"device {name} CUs={cus}")
+    p = make_model()
+    N = args.N
+    ys = simulate(p, T, 1234)
+    model = ssmhip.create_homogeneous_linear_gaussian_model(*(p[k] for k in (
+        "mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")))
+    algo = ssmhip.BF(N, threshold=1.0, resampler=ssmhip.Systematic(), precision="f32")
+    rng = np.random.default_rng(2024)
+    st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=2024)
+
+    def run(with_tree):
+        ctx.check(ctx.lib.ssm_bf_initialise(st.ptr, None))
+        tree = ssmhip.ParallelParticleTree(st) if with_tree else None
+        t_ins = 0.0
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for t in range(1, T + 1):
+            ssmhip.step(rng, model, algo, t, st, ys[t - 1])
+            if tree is not None:
+                t1 = time.perf_counter()
+                tree.insert()
+                ctx.synchronize()
+                t_ins += time.perf_counter() - t1
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        info = tree.info() if tree is not None else None
+        if tree is not None:
+            tree.close()
+        return el, t_ins, info
+
+    for _ in range(args.warmup):
+        run(False)
+        run(True)
+    base, full, ins, info = [], [], [], None
+    for _ in range(args.steps):
+        base.append(run(False)[0])
+        el, t_ins, info = run(True)
+        full.append(el)
+        ins.append(t_ins)
+    us_insert = float(np.mean(ins)) / T * 1e6
+    us_step = float(np.mean(base)) / T * 1e6
+    value = N / (us_insert * 1e-6)
+    M = info["capacity"]
+    # algorithmic bytes of one insert: per particle 4 (clear counts) + 20 (ancestor, leaf
+    # gather, parent write, count atomic read+write) + 12 (leaf, count, offspring scatter)
+    # + 4 (prune test) + 4 (new-leaf write) + 48 (new leaf, parent, state 16 read; parent,
+    # leaf, state 16 written) = 92 B, plus two reads of the M offspring counts
+    bytes_insert = 92.0 * N + 8.0 * M
+    achieved = bytes_insert / (us_insert * 1e-6) / 1e9
+    log(f"step {us_step:.1f} us, insert {us_insert:.1f} us ({achieved:.0f} GB/s); pool "
+        f"{M} nodes, {info['used']} used, depth {info['depth']}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        rng2 = np.random.default_rng(5)
+        x = rng2.standard_normal((N, 4))
+        ot = oracle.PTree(x, 4 * N)
+        n, secs = 0, 0.0
+        while secs < budget_s and n < T:
+            a = np.sort(rng2.integers(0, N, size=N)).astype(np.int32)
+            t1 = time.perf_counter()
+            ot.insert(x, a)
+            secs += time.perf_counter() - t1
+            n += 1
+        cpu = {"value": N * n / secs, "unit": "particle-inserts/s", "cores": 1, "kind": "port",
+               "sample": f"oracle PTree insert (oracle/oracle.c, sequential ParallelParticleTree, "
+                         f"single thread), N={N}, D=4, sorted uniform ancestors: {n} inserts "
+                         f"in {secs:.1f}s"}
+        log(f"cpu baseline {N * n / secs:.3e} particle-inserts/s ({n} inserts)")
+    out = {
+        "metric": "particle-inserts/s (device path storage after each filter step)",
+        "value": value, "unit": "particle-inserts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(np.mean(full)) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded LG simulation, Philox noise on device)",
+        "config": {"workload": f"bootstrap filter D=4 Dy=2 LG, N={N}, T={T}, systematic at "
+                               f"threshold 1.0, ancestry tree insert after every step; one "
+                               f"step = T filter steps with inserts",
+                   "N": N, "T": T},
+        "us_per_filter_step": us_step, "us_per_insert": us_insert,
+        "pool_nodes": M, "nodes_used": info["used"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ssm_bf_tree_insert (9 launches, host-timed incl. launch)",
+                     "bytes_per_insert": bytes_insert},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -524,10 +624,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
-    ap.add_argument("--workload", default="bf", choices=["bf", "kf", "rbpf", "serve"],
+    ap.add_argument("--workload", default="bf",
+                    choices=["bf", "kf", "rbpf", "serve", "tree"],
                     help="bf: bootstrap filter (default); kf: batched Kalman filters; "
                          "rbpf: Rao-Blackwellised PF; serve: request stream through the "
-                         "server against the plain per-filter path")
+                         "server against the plain per-filter path; tree: ancestry path "
+                         "storage after every filter step")
     ap.add_argument("--requests", type=int, default=256, help="serve: requests per phase")
     ap.add_argument("--B", type=int, default=65536, help="kf: number of filters")
     args = ap.parse_args()
@@ -545,6 +647,8 @@ def main():
         return bench_rbpf(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "tree":
+        return bench_tree(args)
 
     import ssmhip
     ctx = ssmhip.default_context()

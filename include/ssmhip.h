@@ -174,21 +174,30 @@ int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, doub
 /* ---- path storage of a bootstrap filter (GF/ext/CUDAExt.jl:146-298) --------------- */
 /*
  * Sparse ancestry storage on the device (Jacob, Murray & Rubenthaler 2015): the device
- * ParallelParticleTree with the ParallelAncestorCallback behaviour (CUDAExt.jl:273-298) and
- * the paths of ParticleTree's get_ancestry (GF/src/containers.jl:594-611).  Nodes without
- * live descendants are pruned and their slots reused; the pool doubles when short of N
- * free slots (expand!, CUDAExt.jl:210-226).
+ * ParallelParticleTree with the ParallelAncestorCallback behaviour (CUDAExt.jl:279-298) and
+ * the paths of get_ancestry (CUDAExt.jl:232-262; GF/src/containers.jl:592-611).  Nodes
+ * without live descendants are pruned and their slots reused; new particle i takes the
+ * i-th free slot in index order (insert!, CUDAExt.jl:187-211); the pool doubles when short
+ * of N free slots (expand!, CUDAExt.jl:213-227).
  *   ssm_bf_tree_create   roots = the filter's current particles (PostInit); capacity >= N
  *   ssm_bf_tree_insert   appends the particles after the filter's last step, each linked to
  *                        its ancestor (PostUpdate; the identity when the step kept them)
  *   ssm_bf_tree_info     pool size, free slots and depth (number of inserts)
  *   ssm_bf_tree_ancestry every leaf's path: paths[(i (depth+1) + t) Dx + d], t = 0..depth
+ *   ssm_bf_tree_path     one leaf's path, path[t Dx + d] (get_ancestry(tree, i, T),
+ *                        CUDAExt.jl:248-262)
+ *   ssm_bf_tree_export   the pool's parents (-1 at roots and unused slots never written),
+ *                        offspring counts (capacity entries each) and the N leaf slots: the
+ *                        tree's fields parents/offspring/leaves (CUDAExt.jl:146-150), 0-based
  */
 int32_t ssm_bf_tree_create(ssm_bf* bf, int64_t capacity, ssm_tree** out);
 int32_t ssm_bf_tree_insert(ssm_tree* tree);
 int32_t ssm_bf_tree_info(ssm_tree* tree, int64_t* capacity, int64_t* free_slots,
                          int32_t* depth);
 int32_t ssm_bf_tree_ancestry(ssm_tree* tree, double* paths);
+int32_t ssm_bf_tree_path(ssm_tree* tree, int32_t i, double* path);
+int32_t ssm_bf_tree_export(ssm_tree* tree, int32_t* parents, int32_t* offspring,
+                           int32_t* leaves);
 int32_t ssm_bf_tree_destroy(ssm_tree* tree);
 
 /* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */

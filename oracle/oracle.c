@@ -787,3 +787,134 @@ done:
   free(Rt);
   return rc;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Path storage (Jacob, Murray & Rubenthaler 2015)                           */
+/* ------------------------------------------------------------------------- */
+/*
+ * Sequential restatement of ParallelParticleTree (GF/ext/CUDAExt.jl:146-262) in the
+ * serial order its parallel loops stand for.  One pool of M nodes holds the roots
+ * (time 0) and every later generation; each node has a state (D doubles), a parent
+ * (-1 for a root) and an offspring count.  A node is free when its count is 0 and it is
+ * not a current leaf.
+ *
+ * The reference's prune loop (update_offspring!, CUDAExt.jl:175-185) decrements without
+ * atomics; its intended result, which this restates, is the serial one: a childless leaf
+ * frees itself, and every ancestor whose count reaches zero is freed in turn.
+ */
+typedef struct {
+  int64_t N, M;
+  int D;
+  int depth;
+  double* states;    /* states[k*D + d] */
+  int32_t* parents;  /* -1 at roots */
+  int32_t* offspring;
+  int32_t* leaves;   /* current leaf node of each particle */
+} orc_ptree;
+
+/* ParallelParticleTree(states, M), CUDAExt.jl:152-164: roots in slots 0..N-1. */
+orc_ptree* orc_ptree_create(int64_t N, int D, int64_t M, const double* x0) {
+  if (N < 1 || D < 1 || M < N) return NULL;
+  orc_ptree* t = (orc_ptree*)calloc(1, sizeof(orc_ptree));
+  t->N = N;
+  t->M = M;
+  t->D = D;
+  t->states = (double*)calloc((size_t)M * D, sizeof(double));
+  t->parents = (int32_t*)malloc(sizeof(int32_t) * M);
+  t->offspring = (int32_t*)calloc((size_t)M, sizeof(int32_t));
+  t->leaves = (int32_t*)malloc(sizeof(int32_t) * N);
+  for (int64_t k = 0; k < M; ++k) t->parents[k] = -1;
+  memcpy(t->states, x0, sizeof(double) * N * D);
+  for (int64_t i = 0; i < N; ++i) t->leaves[i] = (int32_t)i;
+  return t;
+}
+
+void orc_ptree_free(orc_ptree* t) {
+  if (!t) return;
+  free(t->states);
+  free(t->parents);
+  free(t->offspring);
+  free(t->leaves);
+  free(t);
+}
+
+/* expand!, CUDAExt.jl:213-227: the pool doubles, new slots are free. */
+static void ptree_expand(orc_ptree* t) {
+  const int64_t M2 = 2 * t->M;
+  t->states = (double*)realloc(t->states, sizeof(double) * M2 * t->D);
+  t->parents = (int32_t*)realloc(t->parents, sizeof(int32_t) * M2);
+  t->offspring = (int32_t*)realloc(t->offspring, sizeof(int32_t) * M2);
+  for (int64_t k = t->M; k < M2; ++k) {
+    t->parents[k] = -1;
+    t->offspring[k] = 0;
+    for (int d = 0; d < t->D; ++d) t->states[k * t->D + d] = 0.0;
+  }
+  t->M = M2;
+}
+
+/*
+ * insert!(tree, states, ancestors), CUDAExt.jl:187-211, ancestors 0-based:
+ *   b = leaves[anc]; offspring[leaves] = counts(anc); prune; expand when fewer than N
+ *   slots are free; new leaf i = the i-th free slot (searchsortedfirst over the cumsum of
+ *   the free flags); parents[new] = b; states[new] = x.
+ * Returns the number of pool doublings.
+ */
+int orc_ptree_insert(orc_ptree* t, const double* x, const int32_t* anc) {
+  const int64_t N = t->N;
+  int32_t* b = (int32_t*)malloc(sizeof(int32_t) * N);
+  int32_t* cnt = (int32_t*)calloc((size_t)N, sizeof(int32_t));
+  for (int64_t i = 0; i < N; ++i) {
+    b[i] = t->leaves[anc[i]];
+    cnt[anc[i]] += 1;
+  }
+  for (int64_t j = 0; j < N; ++j) t->offspring[t->leaves[j]] = cnt[j];
+  for (int64_t j = 0; j < N; ++j) {
+    int32_t k = t->leaves[j];
+    while (k >= 0 && t->offspring[k] == 0) {
+      k = t->parents[k];
+      if (k >= 0) t->offspring[k] -= 1;
+    }
+  }
+  int64_t nfree = 0;
+  for (int64_t k = 0; k < t->M; ++k) nfree += (t->offspring[k] == 0);
+  int grew = 0;
+  if (nfree < N) {
+    ptree_expand(t);
+    ++grew;
+  }
+  int64_t i = 0;
+  for (int64_t k = 0; k < t->M && i < N; ++k) {
+    if (t->offspring[k] != 0) continue;
+    t->leaves[i] = (int32_t)k;
+    t->parents[k] = b[i];
+    memcpy(t->states + k * t->D, x + i * t->D, sizeof(double) * t->D);
+    ++i;
+  }
+  ++t->depth;
+  free(b);
+  free(cnt);
+  return grew;
+}
+
+/* get_ancestry(tree, T), CUDAExt.jl:232-245: paths[(i*(depth+1) + s)*D + d]. */
+void orc_ptree_ancestry(const orc_ptree* t, double* paths) {
+  const int T = t->depth;
+  for (int64_t i = 0; i < t->N; ++i) {
+    int32_t k = t->leaves[i];
+    for (int s = T; s >= 0; --s) {
+      memcpy(paths + ((size_t)i * (T + 1) + s) * t->D, t->states + (size_t)k * t->D,
+             sizeof(double) * t->D);
+      if (s > 0) k = t->parents[k];
+    }
+  }
+}
+
+/* The pool (M entries) and the leaves, for comparison with the device tree. */
+int64_t orc_ptree_size(const orc_ptree* t) { return t->M; }
+
+void orc_ptree_export(const orc_ptree* t, int32_t* parents, int32_t* offspring,
+                      int32_t* leaves) {
+  if (parents) memcpy(parents, t->parents, sizeof(int32_t) * t->M);
+  if (offspring) memcpy(offspring, t->offspring, sizeof(int32_t) * t->M);
+  if (leaves) memcpy(leaves, t->leaves, sizeof(int32_t) * t->N);
+}

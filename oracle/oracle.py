@@ -27,6 +27,10 @@ Parity pinning of each oracle function (what checks it beyond retyping its formu
 * ``rbpf_step`` (rbpf.jl:16-84): RBPF log-likelihood against the KF of the joint model built
   like GFTest's dummy LG model (test/algorithms/rbpf.jl:5-34), and the degenerate outer-noise-0
   case where every particle is identical and the RBPF equals a KF with R_t = exp(x_t) R0.
+* ``PTree`` (ParallelParticleTree, CUDAExt.jl:146-262): paths against a brute-force walk of
+  the dense ancestor history (DenseParticleContainer's get_ancestry, containers.jl:406-418),
+  nodes in use against the number of distinct (time, particle) pairs on the surviving lines,
+  and a hand-worked insert with pool growth (tests/test_oracle_tree.py).
 * ``logsumexp``: parity unpinned beyond closed cases (LogExpFunctions' exact rounding is
   third-party; only tolerances are claimed).
 """
@@ -105,6 +109,14 @@ def _declare(L):
     L.orc_kf_filter.argtypes = [C.POINTER(LGModel), C.c_int, _dp, C.c_double, _dp, _dp, _dp,
                                 _dp]
     L.orc_kf_filter.restype = C.c_int
+    L.orc_ptree_create.argtypes = [C.c_int64, C.c_int, C.c_int64, _dp]
+    L.orc_ptree_create.restype = C.c_void_p
+    L.orc_ptree_free.argtypes = [C.c_void_p]
+    L.orc_ptree_insert.argtypes = [C.c_void_p, _dp, _ip]
+    L.orc_ptree_ancestry.argtypes = [C.c_void_p, _dp]
+    L.orc_ptree_size.argtypes = [C.c_void_p]
+    L.orc_ptree_size.restype = C.c_int64
+    L.orc_ptree_export.argtypes = [C.c_void_p, _ip, _ip, _ip]
     L.orc_rbpf_initialise.argtypes = [C.POINTER(RBModel), C.c_int64, _dp, _dp, _dp, _dp]
     L.orc_rbpf_step.argtypes = [C.POINTER(RBModel), C.c_int64, _dp, _dp, _dp, _dp,
                                 C.POINTER(C.c_int), C.c_double, C.c_int, _dp, _dp, _dp, _dp,
@@ -406,3 +418,59 @@ def rbpf_step(model: RB, st: RBState, y, threshold, kind, u, zo, w_override=None
         raise ValueError(f"orc_rbpf_step rc={rc}")
     st.typeless = bool(typ.value)
     return anc, ll.value, bool(did.value), w_out
+
+
+class PTree:
+    """Sequential ParallelParticleTree (CUDAExt.jl:146-262): roots at x0 (N, D); ``insert``
+    appends a generation x (N, D) with 0-based ancestors; ``ancestry`` -> (N, depth+1, D)."""
+
+    def __init__(self, x0, capacity):
+        x0 = _f64(x0)
+        self.N, self.D = x0.shape
+        self.depth = 0
+        self.ptr = lib().orc_ptree_create(self.N, self.D, int(capacity), _d(x0))
+        if not self.ptr:
+            raise ValueError("capacity below N")
+
+    def insert(self, x, anc):
+        x = _f64(x).reshape(self.N, self.D)
+        a = np.ascontiguousarray(np.asarray(anc, dtype=np.int32))
+        grew = lib().orc_ptree_insert(self.ptr, _d(x), _i(a))
+        self.depth += 1
+        return grew
+
+    def ancestry(self):
+        out = np.zeros((self.N, self.depth + 1, self.D))
+        lib().orc_ptree_ancestry(self.ptr, _d(out))
+        return out
+
+    def export(self):
+        M = lib().orc_ptree_size(self.ptr)
+        par = np.zeros(M, dtype=np.int32)
+        off = np.zeros(M, dtype=np.int32)
+        leaves = np.zeros(self.N, dtype=np.int32)
+        lib().orc_ptree_export(self.ptr, _i(par), _i(off), _i(leaves))
+        return par, off, leaves
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_ptree_free(self.ptr)
+            self.ptr = None
+
+
+def dense_ancestry(xs, ancs):
+    """Paths by walking the stored ancestors back from the last generation, as
+    DenseParticleContainer's get_ancestry (containers.jl:406-418).  xs[t]: (N, D) states,
+    t = 0..T; ancs[t]: 0-based ancestors of generation t+1.  Returns the paths (N, T+1, D)
+    and the number of distinct (time, particle) pairs on the surviving lines."""
+    T = len(ancs)
+    N, D = np.asarray(xs[0]).shape
+    idx = np.arange(N)
+    paths = np.zeros((N, T + 1, D))
+    distinct = 0
+    for t in range(T, -1, -1):
+        paths[:, t, :] = np.asarray(xs[t])[idx]
+        distinct += len(np.unique(idx))
+        if t > 0:
+            idx = np.asarray(ancs[t - 1])[idx]
+    return paths, distinct

@@ -78,6 +78,10 @@ struct ssm_bf {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   int last_T = 0;
+  // attached path storage: ssm_bf_step's resampler writes offspring counts for it
+  ssm_tree* tree = nullptr;
+  int32_t* tree_counts = nullptr;
+  bool counts_fresh = false;
 };
 
 namespace {
@@ -134,7 +138,7 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
 
 // Enqueues the resampler kernels of one step on stream s.
 int enqueue_resample(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
-                     bool debug_weights) {
+                     bool debug_weights, int32_t* counts_out = nullptr) {
   ResampleArgs a;
   a.kind = kind;
   a.n = bf->N;
@@ -146,6 +150,7 @@ int enqueue_resample(ssm_bf* bf, hipStream_t s, double threshold, int kind, cons
   a.threshold_n = threshold * (double)bf->N;
   a.anc = bf->anc;
   a.w_debug64 = debug_weights ? bf->w_dbg : nullptr;
+  a.counts_out = counts_out;
   resample_bind_scratch(a, bf->rs_scratch, bf->N);
   return launch_resample(bf->ctx, s, a);
 }
@@ -177,8 +182,8 @@ void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
 
 // Enqueues one full step (resample kernel, step kernel, finish kernel) on stream s.
 int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
-                 const void* noise_dev, bool debug_weights) {
-  const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights);
+                 const void* noise_dev, bool debug_weights, int32_t* counts_out = nullptr) {
+  const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights, counts_out);
   if (rc) return rc;
   enqueue_move(bf, s, threshold, noise_dev);
   enqueue_finish(bf, s, threshold, 1);
@@ -281,6 +286,7 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
   (void)hipSetDevice(bf->ctx->device);
   (void)hipStreamSynchronize(bf->ctx->stream);
   destroy_graphs(bf);
+  if (bf->tree) tree_detach_filter(bf->tree);
   void* ptrs[] = {bf->x[0],      bf->x[1],     bf->logw,     bf->anc,   bf->model_dev,
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
                   bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u};
@@ -320,6 +326,7 @@ static int upload_noise(ssm_bf* bf, const double* z) {
 extern "C" int32_t ssm_bf_initialise(ssm_bf* bf, const double* z) {
   if (!bf) return SSM_ERR_ARG;
   ssm_ctx* ctx = bf->ctx;
+  bf->counts_fresh = false;
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   if (z) {
     int rc = (bf->prec == SSM_F32) ? upload_noise<float>(bf, z) : upload_noise<double>(bf, z);
@@ -390,7 +397,9 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
     if (rc) return rc;
     noise_dev = bf->noise;
   }
-  rc = enqueue_step(bf, ctx->stream, threshold, resampler, u_dev, noise_dev, true);
+  bf->counts_fresh = false;
+  rc = enqueue_step(bf, ctx->stream, threshold, resampler, u_dev, noise_dev, true,
+                    bf->tree_counts);
   if (rc) return rc;
   SSM_HIP_CHECK(ctx, hipGetLastError());
   double ll = 0.0;
@@ -400,6 +409,7 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(&did, bf->rs_steps, sizeof did, hipMemcpyDeviceToHost,
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  bf->counts_fresh = bf->tree_counts != nullptr && resampler != SSM_MULTINOMIAL;
   if (ll_inc) *ll_inc = ll;
   if (did_resample) *did_resample = did;
   return SSM_OK;
@@ -412,6 +422,7 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   ssm_ctx* ctx = bf->ctx;
   if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_run before initialise");
   if (T < 1 || t0 < 1 || !ys) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: bad arguments");
+  bf->counts_fresh = false;
   if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: unknown resampler %d", resampler);
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -568,6 +579,7 @@ static int import_t(ssm_bf* bf, const double* x, const double* logw) {
 extern "C" int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw) {
   if (!bf || !x || !logw) return bf ? set_error(bf->ctx, SSM_ERR_ARG, "null input") : SSM_ERR_ARG;
   SSM_HIP_CHECK(bf->ctx, hipSetDevice(bf->ctx->device));
+  bf->counts_fresh = false;
   return (bf->prec == SSM_F32) ? import_t<float>(bf, x, logw) : import_t<double>(bf, x, logw);
 }
 
@@ -582,6 +594,7 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
   if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_profile: unknown resampler %d", resampler);
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  bf->counts_fresh = false;
   int rc = ensure_steps(bf, (size_t)T);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
@@ -671,6 +684,16 @@ int bf_device_view(ssm_bf* bf, BfView* v) {
   v->x = bf->x[bf->cur];
   v->anc = bf->anc;
   v->sc = bf->sc;
+  v->counts_fresh = bf->counts_fresh;
+  return SSM_OK;
+}
+
+int bf_attach_tree(ssm_bf* bf, ssm_tree* tree, int32_t* counts) {
+  if (tree && bf->tree && bf->tree != tree)
+    return set_error(bf->ctx, SSM_ERR_STATE, "the filter already has path storage attached");
+  bf->tree = tree;
+  bf->tree_counts = tree ? counts : nullptr;
+  bf->counts_fresh = false;
   return SSM_OK;
 }
 }  // namespace ssm

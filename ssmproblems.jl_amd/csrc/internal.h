@@ -82,6 +82,8 @@ struct ResampleArgs {
   double* blk_c = nullptr;
   double* prefix = nullptr;
   double* Wbuf = nullptr;            // multinomial prefix sums (n)
+  int32_t* counts_out = nullptr;     // optional: offspring count of each particle
+                                     // (systematic and stratified; ignored otherwise)
 };
 
 // Scratch bytes the resampler needs for n weights.
@@ -98,8 +100,16 @@ struct BfView {
   const void* x;         // current state planes x[d][Npad]
   const int32_t* anc;    // ancestors of the last step (valid if sc->last_resampled)
   const BfScalars* sc;
+  bool counts_fresh;     // the attached counts buffer holds the last step's offspring
+                         // counts (valid if sc->last_resampled)
 };
 int bf_device_view(ssm_bf* bf, BfView* v);
+// Attaches path storage to a filter: from then on ssm_bf_step's systematic and stratified
+// resamples also write each particle's offspring count into `counts` (N entries).  One
+// tree per filter; tree = null detaches.  Returns SSM_ERR_STATE when another is attached.
+int bf_attach_tree(ssm_bf* bf, ssm_tree* tree, int32_t* counts);
+// Called by ssm_bf_destroy: the tree loses its filter (its later calls fail cleanly).
+void tree_detach_filter(ssm_tree* tree);
 
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the

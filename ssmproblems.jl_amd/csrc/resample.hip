@@ -131,6 +131,7 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
   rp.tile_sum = a.partials;
   rp.prefix = a.prefix;
   rp.Wbuf = a.Wbuf;
+  rp.counts_out = (a.kind == SSM_MULTINOMIAL) ? nullptr : a.counts_out;
   const int nb = (a.n + kTile - 1) / kTile;
   const dim3 g(nb), blk(kRsThreads);
   switch (a.weight_mode) {

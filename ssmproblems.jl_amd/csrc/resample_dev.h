@@ -32,6 +32,7 @@ struct RsParams {
   double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
   double* prefix;
   double* Wbuf;
+  int32_t* counts_out;  // optional offspring count per particle (systematic, stratified)
 };
 
 __device__ __forceinline__ bool rs_active(const RsParams& p) {
@@ -286,6 +287,13 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   olast[t] = O[kTileItems - 1];
   __syncthreads();
   const int start = (t == 0) ? o_prev : olast[t - 1];
+  if (p.counts_out) {  // particle k owns the output slots [O[k-1], O[k])
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+      const int pos = kTileItems * t + k;
+      if (pos < cnt) p.counts_out[tile0 + pos] = O[k] - (k == 0 ? start : O[k - 1]);
+    }
+  }
   if (O[kTileItems - 1] - start <= kDirectSlots) {
     int s0 = start;
 #pragma unroll

@@ -77,6 +77,7 @@ __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
   p.tile_sum = nullptr;
   p.prefix = F.ts.prefix;
   p.Wbuf = F.Wbuf;
+  p.counts_out = nullptr;
   return p;
 }
 

@@ -141,6 +141,8 @@ def _declare(L):
         "ssm_bf_tree_insert": ([_vp], i32),
         "ssm_bf_tree_info": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip], i32),
         "ssm_bf_tree_ancestry": ([_vp, _dp], i32),
+        "ssm_bf_tree_path": ([_vp, C.c_int32, _dp], i32),
+        "ssm_bf_tree_export": ([_vp, _ip, _ip, _ip], i32),
         "ssm_bf_tree_destroy": ([_vp], i32),
         "ssm_serve_create": ([_vp, u64, i32, C.POINTER(_vp)], i32),
         "ssm_serve_submit": ([_vp, C.POINTER(RequestC), C.POINTER(C.c_int64)], i32),

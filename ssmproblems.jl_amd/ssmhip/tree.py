@@ -3,7 +3,7 @@
 ``ParallelParticleTree(state, capacity)`` roots a tree at the filter's current particles;
 ``insert()`` appends the particles after the filter's last step (ParallelAncestorCallback,
 CUDAExt.jl:273-298); ``ancestry()`` returns every leaf's path x_0 .. x_depth as an array
-(N, depth + 1, Dx), the trajectories ``get_ancestry`` builds (GF/src/containers.jl:594-611).
+(N, depth + 1, Dx), the trajectories ``get_ancestry`` builds (GF/src/containers.jl:592-611).
 """
 from __future__ import annotations
 
@@ -39,6 +39,26 @@ class ParallelParticleTree:
         out = np.zeros((self.N, depth + 1, self.Dx))
         self.ctx.check(self.ctx.lib.ssm_bf_tree_ancestry(self.ptr, out.ctypes.data_as(_lib._dp)))
         return out
+
+    def path(self, i):
+        """Path of particle i, (depth + 1, Dx) (get_ancestry(tree, i, T), CUDAExt.jl:248-262)."""
+        depth = self.info()["depth"]
+        out = np.zeros((depth + 1, self.Dx))
+        self.ctx.check(self.ctx.lib.ssm_bf_tree_path(self.ptr, int(i),
+                                                     out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def export(self):
+        """(parents, offspring, leaves): the node pool's parent links (-1 at roots) and
+        offspring counts, and each particle's leaf slot, all 0-based int32."""
+        cap = self.info()["capacity"]
+        par = np.zeros(cap, dtype=np.int32)
+        off = np.zeros(cap, dtype=np.int32)
+        leaves = np.zeros(self.N, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.ssm_bf_tree_export(
+            self.ptr, par.ctypes.data_as(_lib._ip), off.ctypes.data_as(_lib._ip),
+            leaves.ctypes.data_as(_lib._ip)))
+        return par, off, leaves
 
     def close(self):
         if getattr(self, "ptr", None):
