@@ -527,36 +527,43 @@ def bench_tree(args, T=200, budget_s=10.0):
     rng = np.random.default_rng(2024)
     st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=2024)
 
-    def run(with_tree):
+    def run(mode):
         ctx.check(ctx.lib.ssm_bf_initialise(st.ptr, None))
-        tree = ssmhip.ParallelParticleTree(st) if with_tree else None
+        store = None
+        if mode == "tree":
+            store = ssmhip.ParallelParticleTree(st)
+        elif mode == "history":
+            store = ssmhip.DenseParticleContainer(st, capacity=T)
+        add = None if store is None else (store.insert if mode == "tree" else store.push)
         t_ins = 0.0
         ctx.synchronize()
         t0 = time.perf_counter()
         for t in range(1, T + 1):
             ssmhip.step(rng, model, algo, t, st, ys[t - 1])
-            if tree is not None:
+            if add is not None:
                 t1 = time.perf_counter()
-                tree.insert()
+                add()
                 ctx.synchronize()
                 t_ins += time.perf_counter() - t1
         ctx.synchronize()
         el = time.perf_counter() - t0
-        info = tree.info() if tree is not None else None
-        if tree is not None:
-            tree.close()
+        info = store.info() if mode == "tree" else None
+        if store is not None:
+            store.close()
         return el, t_ins, info
 
     for _ in range(args.warmup):
-        run(False)
-        run(True)
-    base, full, ins, info = [], [], [], None
+        for mode in ("none", "tree", "history"):
+            run(mode)
+    base, full, ins, push, info = [], [], [], [], None
     for _ in range(args.steps):
-        base.append(run(False)[0])
-        el, t_ins, info = run(True)
+        base.append(run("none")[0])
+        el, t_ins, info = run("tree")
         full.append(el)
         ins.append(t_ins)
+        push.append(run("history")[1])
     us_insert = float(np.mean(ins)) / T * 1e6
+    us_push = float(np.mean(push)) / T * 1e6
     us_step = float(np.mean(base)) / T * 1e6
     value = N / (us_insert * 1e-6)
     M = info["capacity"]
@@ -566,8 +573,12 @@ def bench_tree(args, T=200, budget_s=10.0):
     # leaf, state 16 written) = 92 B, plus two reads of the M offspring counts
     bytes_insert = 92.0 * N + 8.0 * M
     achieved = bytes_insert / (us_insert * 1e-6) / 1e9
+    # dense history push: state 16 B read + written, log-weight 4 B read + 8 B written,
+    # ancestor 4 B read + written
+    bytes_push = 52.0 * N
     log(f"step {us_step:.1f} us, insert {us_insert:.1f} us ({achieved:.0f} GB/s); pool "
-        f"{M} nodes, {info['used']} used, depth {info['depth']}")
+        f"{M} nodes, {info['used']} used, depth {info['depth']}; dense history push "
+        f"{us_push:.1f} us ({bytes_push / (us_push * 1e-6) / 1e9:.0f} GB/s)")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -600,9 +611,12 @@ def bench_tree(args, T=200, budget_s=10.0):
                    "N": N, "T": T},
         "us_per_filter_step": us_step, "us_per_insert": us_insert,
         "pool_nodes": M, "nodes_used": info["used"],
+        "dense_history": {"us_per_push": us_push, "bytes_per_push": bytes_push,
+                          "GB_per_s": bytes_push / (us_push * 1e-6) / 1e9,
+                          "note": "host-timed incl. launch and a stream sync"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ssm_bf_tree_insert (9 launches, host-timed incl. launch)",
+                     "kernel": "ssm_bf_tree_insert (4 launches: link+prune, free counts, scan, assign; host-timed incl. launch)",
                      "bytes_per_insert": bytes_insert},
         "cpu_baseline": cpu,
     }

@@ -55,6 +55,7 @@ typedef struct ssm_rbpf ssm_rbpf;
 typedef struct ssm_server ssm_server;
 typedef struct ssm_kf_batch ssm_kf_batch;
 typedef struct ssm_tree ssm_tree;
+typedef struct ssm_history ssm_history;
 
 /*
  * Homogeneous linear-Gaussian state-space model, the parameters returned by
@@ -199,6 +200,28 @@ int32_t ssm_bf_tree_path(ssm_tree* tree, int32_t i, double* path);
 int32_t ssm_bf_tree_export(ssm_tree* tree, int32_t* parents, int32_t* offspring,
                            int32_t* leaves);
 int32_t ssm_bf_tree_destroy(ssm_tree* tree);
+
+/* ---- dense particle history (GF/src/containers.jl:328-418) ------------------------- */
+/*
+ * DenseParticleContainer on the device, filled as DenseAncestorCallback does
+ * (GF/src/callbacks.jl:53-80): the time-0 states at creation (PostInit), then after each
+ * step (PostUpdate) the states, normalised FP64 log-weights and ancestors (the identity when
+ * the step kept its particles).  (4 Dx + 12) B per particle and step in FP32.
+ *   ssm_bf_history_create     capacity = generations allocated up front (doubles when full)
+ *   ssm_bf_history_push       appends the filter's current generation (push!, :364-374)
+ *   ssm_bf_history_info       generations pushed and allocated
+ *   ssm_bf_history_ancestry   paths of particles i0 .. i0+count-1 (get_ancestry, :406-418):
+ *                             paths[(k (T+1) + t) Dx + d], t = 0..T
+ *   ssm_bf_history_generation generation t: x[i Dx + d], logw[i], ancestors[i] (0-based);
+ *                             t = 0 holds states only (Particle(c, t, i), :396-404)
+ */
+int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_history** out);
+int32_t ssm_bf_history_push(ssm_history* h);
+int32_t ssm_bf_history_info(ssm_history* h, int32_t* T, int32_t* capacity);
+int32_t ssm_bf_history_ancestry(ssm_history* h, int32_t i0, int32_t count, double* paths);
+int32_t ssm_bf_history_generation(ssm_history* h, int32_t t, double* x, double* logw,
+                                  int32_t* ancestors);
+int32_t ssm_bf_history_destroy(ssm_history* h);
 
 /* ---- Rao-Blackwellised particle filter (GF/src/algorithms/rbpf.jl:8-84) ------------ */
 /*

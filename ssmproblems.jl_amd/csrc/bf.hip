@@ -23,6 +23,7 @@
 // resampler's tile prefixes (pf_common.h).  Particles are lane-striped within a tile, so
 // every access of a wave is one contiguous run.  ssm_bf_run captures chunks of steps in
 // HIP graphs, so T steps run without host round trips.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -82,6 +83,7 @@ struct ssm_bf {
   ssm_tree* tree = nullptr;
   int32_t* tree_counts = nullptr;
   bool counts_fresh = false;
+  std::vector<ssm_history*> histories;  // attached dense histories
 };
 
 namespace {
@@ -287,6 +289,7 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
   (void)hipStreamSynchronize(bf->ctx->stream);
   destroy_graphs(bf);
   if (bf->tree) tree_detach_filter(bf->tree);
+  for (ssm_history* h : bf->histories) history_detach_filter(h);
   void* ptrs[] = {bf->x[0],      bf->x[1],     bf->logw,     bf->anc,   bf->model_dev,
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
                   bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u};
@@ -682,9 +685,17 @@ int bf_device_view(ssm_bf* bf, BfView* v) {
   v->D = bf->D;
   v->prec = bf->prec;
   v->x = bf->x[bf->cur];
+  v->logw = bf->logw;
   v->anc = bf->anc;
   v->sc = bf->sc;
   v->counts_fresh = bf->counts_fresh;
+  return SSM_OK;
+}
+
+int bf_attach_history(ssm_bf* bf, ssm_history* h, bool on) {
+  auto& v = bf->histories;
+  v.erase(std::remove(v.begin(), v.end(), h), v.end());
+  if (on) v.push_back(h);
   return SSM_OK;
 }
 

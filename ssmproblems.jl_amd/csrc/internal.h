@@ -98,6 +98,7 @@ struct BfView {
   ssm_ctx* ctx;
   int N, Npad, D, prec;
   const void* x;         // current state planes x[d][Npad]
+  const void* logw;      // un-normalised log-weights (normalised: minus sc->lse)
   const int32_t* anc;    // ancestors of the last step (valid if sc->last_resampled)
   const BfScalars* sc;
   bool counts_fresh;     // the attached counts buffer holds the last step's offspring
@@ -110,6 +111,10 @@ int bf_device_view(ssm_bf* bf, BfView* v);
 int bf_attach_tree(ssm_bf* bf, ssm_tree* tree, int32_t* counts);
 // Called by ssm_bf_destroy: the tree loses its filter (its later calls fail cleanly).
 void tree_detach_filter(ssm_tree* tree);
+// Registers (on = true) or removes a dense history of the filter, so that destroying the
+// filter first leaves the history's later pushes failing cleanly.
+int bf_attach_history(ssm_bf* bf, ssm_history* h, bool on);
+void history_detach_filter(ssm_history* h);
 
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the

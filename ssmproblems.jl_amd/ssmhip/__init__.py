@@ -17,5 +17,6 @@ from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, ru
                       rng_to_seed, last_timing)
 from .serve import Server
 from .tree import ParallelParticleTree, ParallelAncestorCallback
+from .history import DenseParticleContainer, DenseAncestorCallback
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -143,6 +143,12 @@ def _declare(L):
         "ssm_bf_tree_ancestry": ([_vp, _dp], i32),
         "ssm_bf_tree_path": ([_vp, C.c_int32, _dp], i32),
         "ssm_bf_tree_export": ([_vp, _ip, _ip, _ip], i32),
+        "ssm_bf_history_create": ([_vp, i32, C.POINTER(_vp)], i32),
+        "ssm_bf_history_push": ([_vp], i32),
+        "ssm_bf_history_info": ([_vp, _ip, _ip], i32),
+        "ssm_bf_history_ancestry": ([_vp, i32, i32, _dp], i32),
+        "ssm_bf_history_generation": ([_vp, i32, _dp, _dp, _ip], i32),
+        "ssm_bf_history_destroy": ([_vp], i32),
         "ssm_bf_tree_destroy": ([_vp], i32),
         "ssm_serve_create": ([_vp, u64, i32, C.POINTER(_vp)], i32),
         "ssm_serve_submit": ([_vp, C.POINTER(RequestC), C.POINTER(C.c_int64)], i32),
