@@ -35,9 +35,9 @@
 namespace ssm {
 
 // The fused predict + update + partial-reduction kernel: one block per tile.
-template <typename T, int DT, int DYT>
+template <typename T, int DT, int DYT, bool NOISE>
 __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  bf_step_tile<T, DT, DYT>(p, blockIdx.x);
+  bf_step_tile<T, DT, DYT, NOISE>(p, blockIdx.x);
 }
 
 template <typename T, int DT>
@@ -130,12 +130,21 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   const TileScratch ts = tile_scratch(bf);
   p.partials = ts.partials;
   const dim3 g(p.nblocks), b(kStepThreads);
-  if (bf->D == 4 && bf->Dy == 2)
-    hipLaunchKernelGGL((k_bf_step<T, 4, 2>), g, b, 0, s, p);
-  else if (bf->D == 1 && bf->Dy == 1)
-    hipLaunchKernelGGL((k_bf_step<T, 1, 1>), g, b, 0, s, p);
-  else
-    hipLaunchKernelGGL((k_bf_step<T, 0, 0>), g, b, 0, s, p);
+  if (noise) {  // parity mode: pre-drawn normals
+    if (bf->D == 4 && bf->Dy == 2)
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, true>), g, b, 0, s, p);
+    else if (bf->D == 1 && bf->Dy == 1)
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, true>), g, b, 0, s, p);
+    else
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, true>), g, b, 0, s, p);
+  } else {
+    if (bf->D == 4 && bf->Dy == 2)
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, false>), g, b, 0, s, p);
+    else if (bf->D == 1 && bf->Dy == 1)
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, false>), g, b, 0, s, p);
+    else
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, false>), g, b, 0, s, p);
+  }
 }
 
 // Enqueues the resampler kernels of one step on stream s.

@@ -7,6 +7,7 @@
 //   initialise particles.jl:36-50
 #pragma once
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -158,8 +159,16 @@ __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>
 }
 
 // The fused predict + update + partial-reduction kernel.  DT/DYT > 0 fix the dimensions
-// at compile time; 0 means runtime dimensions up to kMaxD / kMaxDy.
-template <typename T, int DT, int DYT>
+// at compile time; 0 means runtime dimensions up to kMaxD / kMaxDy.  NOISE: the normals
+// come from pre-drawn planes (parity mode) instead of Philox.
+//
+// The block-uniform `fresh` (resampled or lazy-uniform weights: the pre-update log-weights
+// are all 0) selects one of two straight-line bodies, so that no load sits under a
+// condition: a conditional load becomes a branch, and at every branch the compiler waits
+// for all outstanding loads, which serialised the items' round trips.  A fresh step reads
+// the ancestors (used when it resampled) and gathers the states; a step that kept its
+// particles reads the states in place and the log-weights.
+template <typename T, int DT, int DYT, bool NOISE = false>
 __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MD = DT ? DT : kMaxD;
   constexpr int MDY = DYT ? DYT : kMaxDy;
@@ -184,25 +193,11 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
 
   const int tile0 = tile * kTile;
-  T nl[kStepItems], lnew[kStepItems];
-  bool valid[kStepItems];
 
-  auto load_item = [&](int k, T* xk) {
-    const int i = tile0 + k * kStepThreads + threadIdx.x;
-    valid[k] = i < p.N;
-    // the ancestor is read whatever the decision (anc holds Npad entries), so the load
-    // does not wait for the scalar block
-    const int a = p.anc[i];
-    const int src = (rs && valid[k]) ? a : i;
-#pragma unroll
-    for (int d = 0; d < MD; ++d)
-      if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
-    nl[k] = fresh ? (T)0 : p.logw[i] - lse;
-  };
-  auto compute_item = [&](int k, const T* xk) {
+  auto compute_item = [&](int k, const T* xk, T nlk) -> T {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
     T z[MD];
-    if (p.noise) {
+    if constexpr (NOISE) {
 #pragma unroll
       for (int d = 0; d < MD; ++d)
         if (d < D) z[d] = p.noise[(size_t)d * p.Npad + i];
@@ -246,24 +241,69 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         quad += v * v;
       }
     }
-    lnew[k] = nl[k] + (mv.cst - (T)0.5 * quad);
-    p.logw[i] = lnew[k];
+    const T l = nlk + (mv.cst - (T)0.5 * quad);
+    p.logw[i] = l;
+    return l;
   };
-  if constexpr (DT > 0) {
-    // all loads first (maximum memory-level parallelism), then the arithmetic
-    T x[kStepItems][MD];
+
+  auto body = [&](auto fresh_c) {
+    constexpr bool FRESH = decltype(fresh_c)::value;
+    T nl[kStepItems], lnew[kStepItems];
+    bool valid[kStepItems];
+    // loads that depend on nothing: ancestors (fresh) or log-weights (kept particles);
+    // anc and logw hold Npad entries, so every index below is in bounds
+    int a[kStepItems];
+    T lw[kStepItems];
 #pragma unroll
-    for (int k = 0; k < kStepItems; ++k) load_item(k, x[k]);
-#pragma unroll
-    for (int k = 0; k < kStepItems; ++k) compute_item(k, x[k]);
-  } else {
     for (int k = 0; k < kStepItems; ++k) {
-      T xk[MD];
-      load_item(k, xk);
-      compute_item(k, xk);
+      const int i = tile0 + k * kStepThreads + threadIdx.x;
+      valid[k] = i < p.N;
+      if constexpr (FRESH)
+        a[k] = p.anc[i];
+      else
+        lw[k] = p.logw[i];
     }
-  }
-  tile_partial<T>(nl, lnew, valid, fresh, p.partials, p.nblocks, tile);
+    auto src_of = [&](int k) {
+      const int i = tile0 + k * kStepThreads + threadIdx.x;
+      if constexpr (FRESH)
+        return (rs && valid[k]) ? a[k] : i;
+      else
+        return i;
+    };
+#pragma unroll
+    for (int k = 0; k < kStepItems; ++k) {
+      if constexpr (FRESH)
+        nl[k] = (T)0;
+      else
+        nl[k] = lw[k] - lse;
+    }
+    if constexpr (DT > 0) {
+      // all loads first (maximum memory-level parallelism), then the arithmetic
+      T x[kStepItems][MD];
+#pragma unroll
+      for (int k = 0; k < kStepItems; ++k) {
+        const int src = src_of(k);
+#pragma unroll
+        for (int d = 0; d < MD; ++d) x[k][d] = p.xin[(size_t)d * p.Npad + src];
+      }
+#pragma unroll
+      for (int k = 0; k < kStepItems; ++k) lnew[k] = compute_item(k, x[k], nl[k]);
+    } else {
+      for (int k = 0; k < kStepItems; ++k) {
+        const int src = src_of(k);
+        T xk[MD];
+#pragma unroll
+        for (int d = 0; d < MD; ++d)
+          if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
+        lnew[k] = compute_item(k, xk, nl[k]);
+      }
+    }
+    tile_partial<T>(nl, lnew, valid, FRESH, p.partials, p.nblocks, tile);
+  };
+  if (fresh)
+    body(std::true_type{});
+  else
+    body(std::false_type{});
 }
 
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).

@@ -100,13 +100,19 @@ __device__ void tile_partial(const T nl[kTileItems], const T l[kTileItems],
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    LseAcc<double> b0{lds_r[2][0], lds_r[3][0], 0.0};
     double tt = lds_r[0][0], ss = lds_r[1][0];
 #pragma unroll
     for (int k = 1; k < NW; ++k) {
-      b0 = lse_combine(b0, LseAcc<double>{lds_r[2][k], lds_r[3][k], 0.0});
       tt += lds_r[0][k];
       ss += lds_r[1][k];
+    }
+    // a fresh step has no baseline (the finish uses log N): the empty partial
+    LseAcc<double> b0{-INFINITY, 0.0, 0.0};
+    if (!fresh) {
+      b0 = LseAcc<double>{lds_r[2][0], lds_r[3][0], 0.0};
+#pragma unroll
+      for (int k = 1; k < NW; ++k)
+        b0 = lse_combine(b0, LseAcc<double>{lds_r[2][k], lds_r[3][k], 0.0});
     }
     partials[b] = b0.m;
     partials[nb + b] = b0.s;
@@ -165,16 +171,27 @@ __device__ __forceinline__ void fold_tiles(int nb, const double* __restrict__ pa
   F.lo = lo;
   F.hi = hi;
   double m0[CH], s0[CH], tau[CH], s2[CH];
+  // every load is issued unconditionally from a clamped, in-bounds index and masked
+  // afterwards, so all 5 CH loads are in flight together (a load under a condition
+  // becomes a branch, one round trip per tile)
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
-    const int b = lo + q;
-    const bool in = b < hi;
+    const int bc = min(lo + q, nb - 1);
+    m0[q] = partials[bc];
+    s0[q] = partials[nb + bc];
+    F.m1[q] = partials[2 * nb + bc];
+    tau[q] = partials[3 * nb + bc];
+    s2[q] = partials[4 * nb + bc];
+  }
+#pragma unroll
+  for (int q = 0; q < CH; ++q) {
+    const bool in = lo + q < hi;
     // the baseline partials are only used when the step was not a fresh start
-    m0[q] = (in && !fresh) ? partials[b] : -INFINITY;
-    s0[q] = (in && !fresh) ? partials[nb + b] : 0.0;
-    F.m1[q] = in ? partials[2 * nb + b] : -INFINITY;
-    tau[q] = in ? partials[3 * nb + b] : 0.0;
-    s2[q] = in ? partials[4 * nb + b] : 0.0;
+    m0[q] = (in && !fresh) ? m0[q] : -INFINITY;
+    s0[q] = (in && !fresh) ? s0[q] : 0.0;
+    F.m1[q] = in ? F.m1[q] : -INFINITY;
+    tau[q] = in ? tau[q] : 0.0;
+    s2[q] = in ? s2[q] : 0.0;
   }
   // maxima of the tile maxima (exact, any order)
   double M1 = -INFINITY, M0 = -INFINITY;

@@ -100,10 +100,26 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
   if (tid == 0) prefix[nb] = fmax(tot, mx);
 }
 
-// Main kernel: one block per tile (see scan_offspring_tile).
-template <int MODE>
+// Main kernel: one block per tile (see scan_offspring_tile).  KIND/PLAIN: the particle
+// filter's specialisations (a fixed resampler, Philox uniforms, no debug outputs).
+template <int MODE, int KIND = -1, bool PLAIN = false>
 __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
-  scan_offspring_tile<MODE>(p, blockIdx.x);
+  scan_offspring_tile<MODE, KIND, PLAIN>(p, blockIdx.x);
+}
+
+template <int MODE>
+void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
+  const dim3 g(nb), blk(kRsThreads);
+  const bool plain = rp.u == nullptr && rp.w_debug64 == nullptr && rp.counts_out == nullptr;
+  if (plain && rp.kind == SSM_SYSTEMATIC)
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC, true>), g, blk, 0, stream, rp);
+  else if (plain && rp.kind == SSM_SYSTEMATIC_MURRAY)
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC_MURRAY, true>), g, blk, 0, stream,
+                       rp);
+  else if (plain && rp.kind == SSM_STRATIFIED)
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_STRATIFIED, true>), g, blk, 0, stream, rp);
+  else
+    hipLaunchKernelGGL((k_scan_offspring<MODE>), g, blk, 0, stream, rp);
 }
 
 __global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
@@ -148,10 +164,10 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
       hipLaunchKernelGGL(k_scan_offspring<kWeightsF32>, g, blk, 0, stream, rp);
       break;
     case kWeightsLogF32:
-      hipLaunchKernelGGL(k_scan_offspring<kWeightsLogF32>, g, blk, 0, stream, rp);
+      launch_scan_offspring<kWeightsLogF32>(stream, nb, rp);
       break;
     case kWeightsLogF64:
-      hipLaunchKernelGGL(k_scan_offspring<kWeightsLogF64>, g, blk, 0, stream, rp);
+      launch_scan_offspring<kWeightsLogF64>(stream, nb, rp);
       break;
     default: return set_error(ctx, SSM_ERR_ARG, "resample: bad weight mode");
   }

@@ -132,16 +132,22 @@ __device__ __forceinline__ int clamp_floor(double v, int n) {
   return (int)f;
 }
 
+// KIND >= 0 fixes the resampler at compile time (-1: p.kind); PLAIN: the uniforms come from
+// Philox (no explicit uniform array), so the specialised kernels have no loads under
+// conditions.
+template <int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double Wn, double u0,
                                             uint32_t step) {
   const int n = p.n;
-  if (p.kind == SSM_SYSTEMATIC) return offspring_systematic(W, u0, n);
+  const int kind = KIND >= 0 ? KIND : p.kind;
+  if (kind == SSM_SYSTEMATIC) return offspring_systematic(W, u0, n);
   const double r = ((double)n * W) / Wn;
-  if (p.kind == SSM_SYSTEMATIC_MURRAY) return clamp_floor(r + u0, n);
+  if (kind == SSM_SYSTEMATIC_MURRAY) return clamp_floor(r + u0, n);
   // STRATIFIED: k = min(n, floor(r + 1)) is 1-based into u.
   int k = clamp_floor(r + 1.0, n);
   if (k < 1) k = 1;
-  const double uk = p.u ? p.u[k - 1] : resample_uniform(p.seed, step, (uint32_t)(k - 1));
+  const double* u = PLAIN ? nullptr : p.u;
+  const double uk = u ? u[k - 1] : resample_uniform(p.seed, step, (uint32_t)(k - 1));
   return clamp_floor(r + uk, n);
 }
 
@@ -204,18 +210,23 @@ __device__ __forceinline__ void load_tile_raw(const RsParams& p, int b,
   }
 }
 
-// Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  Thread t owns
+// Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  KIND/PLAIN as
+// offspring_of; PLAIN also means no debug weights and no offspring counts.  Thread t owns
 // the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial scan
 // plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by its own
 // thread (neighbouring threads write neighbouring ranges); a thread whose range exceeds
 // kDirectSlots hands it to the whole block, which keeps degenerate weights (one particle
 // taking every slot) balanced.  For MULTINOMIAL it writes the nondecreasing prefix sums W to
 // Wbuf instead.  All threads of the block call it; it returns at once when !in.rs.
-template <int MODE>
+template <int MODE, int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
                                                        const RawW<MODE> raw[kTileItems],
                                                        const TileIn& in) {
   if (!in.rs) return;
+  const int kind = KIND >= 0 ? KIND : p.kind;
+  double* const w_debug64 = PLAIN ? nullptr : p.w_debug64;
+  int32_t* const counts_out = PLAIN ? nullptr : p.counts_out;
+  const double* const uvec = PLAIN ? nullptr : p.u;
   __shared__ double lds[kRsThreads / kWave];
   __shared__ int olast[kRsThreads];
   __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
@@ -242,11 +253,10 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
     const int j = tile0 + kTileItems * t + k;
-    double wj = 0.0;
-    if (j < n) {
-      wj = rs_weight_of<MODE>(p, raw[k], in.m_b, in.c_b, in.typeless);
-      if (p.w_debug64) p.w_debug64[j] = wj;
-    }
+    // computed for every position (raw holds clamped in-bounds values), masked past n
+    double wj = rs_weight_of<MODE>(p, raw[k], in.m_b, in.c_b, in.typeless);
+    wj = (j < n) ? wj : 0.0;
+    if (w_debug64 && j < n) w_debug64[j] = wj;
     acc += wj;
     W[k] = acc;
   }
@@ -262,7 +272,7 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
   const uint32_t step = in.step;
-  if (p.kind == SSM_MULTINOMIAL) {
+  if (kind == SSM_MULTINOMIAL) {
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
       const int pos = kTileItems * t + k;
@@ -270,11 +280,11 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
     }
     return;
   }
-  const double u0 = (p.kind == SSM_STRATIFIED)
+  const double u0 = (kind == SSM_STRATIFIED)
                         ? 0.0
-                        : (p.u ? p.u[0] : resample_uniform(p.seed, step, 0u));
-  const int o_prev = (b == 0) ? 0 : offspring_of(p, P, Wn, u0, step);
-  const int o_end = (b == nb - 1) ? n : offspring_of(p, Pnext, Wn, u0, step);
+                        : (uvec ? uvec[0] : resample_uniform(p.seed, step, 0u));
+  const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step);
+  const int o_end = (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step);
   int O[kTileItems];
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
@@ -282,16 +292,16 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
     if (pos >= cnt - 1)
       O[k] = o_end;  // the tile's last particle closes its range; positions past n own none
     else
-      O[k] = min(max(offspring_of(p, W[k], Wn, u0, step), o_prev), o_end);
+      O[k] = min(max(offspring_of<KIND, PLAIN>(p, W[k], Wn, u0, step), o_prev), o_end);
   }
   olast[t] = O[kTileItems - 1];
   __syncthreads();
   const int start = (t == 0) ? o_prev : olast[t - 1];
-  if (p.counts_out) {  // particle k owns the output slots [O[k-1], O[k])
+  if (counts_out) {  // particle k owns the output slots [O[k-1], O[k])
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
       const int pos = kTileItems * t + k;
-      if (pos < cnt) p.counts_out[tile0 + pos] = O[k] - (k == 0 ? start : O[k - 1]);
+      if (pos < cnt) counts_out[tile0 + pos] = O[k] - (k == 0 ? start : O[k - 1]);
     }
   }
   if (O[kTileItems - 1] - start <= kDirectSlots) {
@@ -326,7 +336,7 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
 // filter's scalars (or always, for explicit weights), the prefixes and scales from the
 // finish kernel.  Every global input is requested before the decision is known (all the
 // addresses are in bounds), so the kernel pays one round trip before its arithmetic.
-template <int MODE>
+template <int MODE, int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   const int nb = (p.n + kTile - 1) / kTile;
   RawW<MODE> raw[kTileItems];
@@ -341,7 +351,7 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   in.rs = rs_active(p);
   in.typeless = kLog && p.sc->typeless;
   in.step = rs_step(p);
-  scan_offspring_tile_in<MODE>(p, b, raw, in);
+  scan_offspring_tile_in<MODE, KIND, PLAIN>(p, b, raw, in);
 }
 
 // Multinomial: a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.

@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kTileThreads)
     k_bfb_resample(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ owner) {
   const BfbFilter& F = fs[owner[blockIdx.x]];
   const RsParams p = bfb_rs_params(F);
-  scan_offspring_tile<kWeightsLogF32>(p, blockIdx.x - F.tile_base);
+  scan_offspring_tile<kWeightsLogF32, -1, true>(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)

@@ -133,9 +133,45 @@ def test_run_equals_steps(ctx, kind, N, prec):
     incs = [ssmhip.step(rng, model, algo, t, b, ys[t - 1])[1] for t in range(1, T + 1)]
     assert np.array_equal(np.array(incs), steps)
     assert tot == pytest.approx(sum(incs), rel=1e-12)
-    xa = a.export()[0]
-    xb = b.export()[0]
-    assert np.array_equal(xa, xb)
+    for fa, fb in zip(a.export()[:3], b.export()[:3]):  # states, log-weights, ancestors
+        assert np.array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("kind,N,Dx,Dy,prec,obs_scale", [
+    (1, 30000, 4, 2, "f32", 1.0),      # GPU systematic form, register model
+    (0, 4099, 1, 1, "f32", 1.0),       # 1-D model, ragged last tile
+    (2, 10000, 3, 2, "f32", 1.0),      # generic-shape step
+    (0, 6000, 3, 3, "f64", 1.0),       # FP64 generic-shape step
+    (0, 50000, 2, 2, "f32", 1e-4),     # concentrated weights: heavy ancestors, long ranges
+    (2, 50000, 2, 2, "f32", 1e-4),
+])
+def test_run_equals_steps_shapes(ctx, kind, N, Dx, Dy, prec, obs_scale):
+    # The graph-replayed run against the step-by-step path over the kernel shapes, both
+    # precisions and concentrated weights (heavy ancestors): bit for bit.
+    rng = np.random.default_rng(N + Dx)
+    m = random_lg(rng, Dx, Dy, obs_noise_scale=obs_scale)
+    m.A = 0.9 * m.A / max(abs(np.linalg.eigvals(m.A)))
+    model = to_ssmhip_model(m)
+    T = 21
+    ys = rng.standard_normal((T, Dy))
+    algo = ssmhip.BF(N, threshold=0.8, resampler=RES[kind], precision=prec)
+    a = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=5)
+    tot, steps = ssmhip.run(a, algo, ys, return_steps=True)
+    b = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=5)
+    incs = [ssmhip.step(rng, model, algo, t, b, ys[t - 1])[1] for t in range(1, T + 1)]
+    assert np.array_equal(np.array(incs), steps)
+    for fa, fb in zip(a.export()[:3], b.export()[:3]):
+        assert np.array_equal(fa, fb)
+    # a run that continues from a state an ssm_bf_step call left
+    ys2 = rng.standard_normal((18, Dy))
+    ssmhip.step(rng, model, algo, T + 1, a, ys2[0])
+    ssmhip.step(rng, model, algo, T + 1, b, ys2[0])
+    _, s2 = ssmhip.run(a, algo, ys2[1:], t0=T + 2, return_steps=True)
+    incs2 = [ssmhip.step(rng, model, algo, t, b, ys2[t - T - 1])[1]
+             for t in range(T + 2, T + 19)]
+    assert np.array_equal(np.array(incs2), s2)
+    for fa, fb in zip(a.export()[:3], b.export()[:3]):
+        assert np.array_equal(fa, fb)
 
 
 def test_config1_fp64_statistical(ctx, orc):
