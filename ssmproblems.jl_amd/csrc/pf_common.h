@@ -369,6 +369,191 @@ __global__ void __launch_bounds__(NT)
                              record);
 }
 
+// The finish for many tiles (nb > 1024), in rounds: in round r thread t handles tile
+// r NT + t (NT = 512: 256 registers per lane), so every load of a wave is one coalesced run (a thread-owned chunk of
+// contiguous tiles would make each lane touch its own cache line, and a single block's
+// loads then run at one line per cycle).  Maxima and sums accumulate per thread over the
+// rounds and reduce across the block in a fixed order.  The exclusive prefixes of
+// c_b tau_b are a block scan per round plus the running carry, made nondecreasing by a
+// block max-scan (a parallel scan of nonnegative values is not monotone in floating
+// point).  The loads of R rounds are issued together.
+template <int R>
+__device__ __forceinline__ void load_rounds(const double* __restrict__ a, int r0, int nb, int NT,
+                                            double v[R]) {
+#pragma unroll
+  for (int u = 0; u < R; ++u) v[u] = a[min((r0 + u) * NT + (int)threadIdx.x, nb - 1)];
+}
+
+// Scans across an NT-thread block in thread order (fixed association), with the per-wave
+// totals exchanged through lds[NT / 64]: the exclusive sum, and the inclusive max.
+template <int NT>
+__device__ __forceinline__ double block_scan_sum(double v, double* lds, double* total) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double incl = wave_scan_dpp(v);
+  double excl = wave_shfl_up(incl, 1);
+  if (lane == 0) excl = 0.0;
+  if (lane == 63) lds[wv] = incl;
+  __syncthreads();
+  double before = 0.0, tot = 0.0;
+  for (int k = 0; k < NW; ++k) {
+    if (k < wv) before += lds[k];
+    tot += lds[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + excl;
+}
+template <int NT>
+__device__ __forceinline__ double block_scan_max(double v, double* lds, double* total) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double incl = wave_scan_max_dpp(v);
+  if (lane == 63) lds[wv] = incl;
+  __syncthreads();
+  double tot = -INFINITY;
+  for (int k = 0; k < NW; ++k) {
+    if (k < wv) incl = fmax(incl, lds[k]);
+    tot = fmax(tot, lds[k]);
+  }
+  __syncthreads();
+  *total = tot;
+  return incl;
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void pf_finish_stream(int N, int nb, const BfScalars* sc_in,
+                                                 BfScalars* sc, double threshold_n,
+                                                 double logN, const TileScratch& ts,
+                                                 double* ll_steps, int32_t* rs_steps,
+                                                 int record) {
+  constexpr int NW = NT / 64, G = 4;
+  __shared__ double red[3][NW];
+  __shared__ double lds[NW];
+  const bool rs = threshold_n >= sc_in->ess;
+  const bool fresh = rs || sc_in->typeless;
+  const double* P = ts.partials;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rounds = (nb + NT - 1) / NT;
+  // pass 1: maxima of the tile maxima (exact, any order)
+  double M1 = -INFINITY, M0 = -INFINITY;
+  for (int r0 = 0; r0 < rounds; r0 += G) {
+    double m1[G], m0[G];
+    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
+    load_rounds<G>(P, r0, nb, NT, m0);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const bool in = (r0 + u) * NT + tid < nb;
+      M1 = fmax(M1, in ? m1[u] : -INFINITY);
+      M0 = fmax(M0, (in && !fresh) ? m0[u] : -INFINITY);
+    }
+  }
+  M1 = wave_max_dpp(M1);
+  M0 = wave_max_dpp(M0);
+  if (lane == 0) {
+    red[0][wv] = M1;
+    red[1][wv] = M0;
+  }
+  __syncthreads();
+  M1 = red[0][0];
+  M0 = red[1][0];
+  for (int k = 1; k < NW; ++k) {
+    M1 = fmax(M1, red[0][k]);
+    M0 = fmax(M0, red[1][k]);
+  }
+  __syncthreads();
+  // the scale of tile b: exp(m1_b - M1), in T, 0 for an empty tile
+  auto scale = [&](double m1) {
+    return (m1 != -INFINITY) ? (double)exp_w((T)(m1 - M1)) : 0.0;
+  };
+  // pass 2: scaled sums, per thread over the rounds, then the block in wave order
+  double S = 0.0, S2 = 0.0, S0 = 0.0;
+  for (int r0 = 0; r0 < rounds; r0 += G) {
+    double m1[G], tau[G], s2[G], m0[G], s0[G];
+    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
+    load_rounds<G>(P + 3 * nb, r0, nb, NT, tau);
+    load_rounds<G>(P + 4 * nb, r0, nb, NT, s2);
+    load_rounds<G>(P, r0, nb, NT, m0);
+    load_rounds<G>(P + nb, r0, nb, NT, s0);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if ((r0 + u) * NT + tid < nb) {
+        const double c = scale(m1[u]);
+        S += tau[u] * c;
+        S2 += s2[u] * c * c;
+        if (!fresh && m0[u] != -INFINITY) S0 += s0[u] * (double)exp_w((T)(m0[u] - M0));
+      }
+    }
+  }
+  S = wave_sum_dpp(S);
+  S2 = wave_sum_dpp(S2);
+  S0 = wave_sum_dpp(S0);
+  if (lane == 0) {
+    red[0][wv] = S;
+    red[1][wv] = S2;
+    red[2][wv] = S0;
+  }
+  __syncthreads();
+  S = red[0][0];
+  S2 = red[1][0];
+  S0 = red[2][0];
+  for (int k = 1; k < NW; ++k) {
+    S += red[0][k];
+    S2 += red[1][k];
+    S0 += red[2][k];
+  }
+  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
+  const double invS = dead ? 0.0 : 1.0 / S;
+  // pass 3: per round, scales, exclusive prefixes of c_b tau_b and the outputs
+  double carry = 0.0, cmax = -INFINITY;
+  for (int r0 = 0; r0 < rounds; r0 += G) {
+    double m1[G], tau[G];
+    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
+    load_rounds<G>(P + 3 * nb, r0, nb, NT, tau);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (r0 + u < rounds) {  // block-uniform
+        const int b = (r0 + u) * NT + tid;
+        const bool in = b < nb;
+        const double c = in ? scale(m1[u]) * invS : 0.0;
+        const double v = c * (in ? tau[u] : 0.0);
+        double rtot, rmax;
+        const double excl = block_scan_sum<NT>(v, lds, &rtot);
+        // carry + exclusive prefix, then nondecreasing across the block and the rounds
+        const double pre = block_scan_max<NT>(fmax(carry + excl, cmax), lds, &rmax);
+        if (in) {
+          ts.prefix[b] = pre;
+          ts.blk_c[b] = c;
+          ts.blk_m[b] = m1[u];
+        }
+        carry += rtot;
+        cmax = fmax(cmax, rmax);
+      }
+    }
+  }
+  if (tid == 0) {
+    ts.prefix[nb] = fmax(carry, cmax);
+    TileFold<1> F;
+    F.M1 = M1;
+    F.S = S;
+    F.S2 = S2;
+    F.M0 = M0;
+    F.S0 = S0;
+    F.tot = fmax(carry, cmax);
+    F.dead = dead;
+    *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
+  }
+}
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
+    k_pf_finish_stream(int N, int nb, const BfScalars* sc_in, BfScalars* sc,
+                       double threshold_n, double logN, TileScratch ts, double* ll_steps,
+                       int32_t* rs_steps, int record) {
+  pf_finish_stream<T, NT>(N, nb, sc_in, sc, threshold_n, logN, ts, ll_steps, rs_steps,
+                          record);
+}
+
 // Enqueues the finish kernel with a register chunk that fits nb tiles.
 template <typename T>
 inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double threshold_n,
@@ -381,12 +566,9 @@ inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double
     SSM_FINISH(1, 256);
   else if (nb <= 1024)
     SSM_FINISH(4, 256);
-  else if (nb <= 4096)
-    SSM_FINISH(4, 1024);
-  else if (nb <= 16384)
-    SSM_FINISH(16, 1024);
-  else
-    SSM_FINISH(64, 1024);
+  else  // many tiles: rounds of 512 coalesced tiles
+    hipLaunchKernelGGL((k_pf_finish_stream<T, 512>), dim3(1), dim3(512), 0, s, N, nb, sc, sc,
+                       threshold_n, logN, ts, ll_steps, rs_steps, record);
 #undef SSM_FINISH
 }
 

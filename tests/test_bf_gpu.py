@@ -26,7 +26,11 @@ RES = {0: ssmhip.Systematic(), 1: ssmhip.Systematic(form="gpu"), 2: ssmhip.Strat
        3: ssmhip.Multinomial()}
 
 
-def _teacher_forced(ctx, orc, m, N, prec, kind, thresholds, seed=1):
+def _teacher_forced(ctx, orc, m, N, prec, kind, thresholds, seed=1, anc_flips=0.0):
+    # anc_flips: the fraction of ancestors allowed to differ from the oracle's sequential
+    # cumsum, whose rounding error grows like sqrt(N) ulp: where a stratum boundary falls
+    # between the two cumulative sums the neighbouring particle is chosen.  Such slots are
+    # left out of the state comparison.
     rng = np.random.default_rng(seed)
     model = to_ssmhip_model(m)
     algo = ssmhip.BF(N, threshold=1.0, resampler=RES[kind], precision=prec)
@@ -49,18 +53,28 @@ def _teacher_forced(ctx, orc, m, N, prec, kind, thresholds, seed=1):
         anc_o, ll_o, did_o, w_o = orc.bf_step(m, ost, ys[t - 1], thr, kind, u, z,
                                               w_override=w_gpu if did_gpu else None)
         assert did_gpu == did_o, f"t={t} resample decision"
+        same = np.ones(N, dtype=bool)
         if did_gpu:
             np.testing.assert_allclose(w_gpu, w_o, rtol=tol["w"], atol=1e-12)
-            assert np.array_equal(anc_gpu, anc_o), f"t={t} ancestors differ"
-        np.testing.assert_allclose(x_new, ost.x, rtol=tol["x"][0], atol=tol["x"][1])
-        live = ost.logw > -30.0
+            same = anc_gpu == anc_o
+            flips = np.flatnonzero(~same)
+            assert len(flips) <= anc_flips * N, f"t={t} {len(flips)} ancestors differ"
+            # a flip moves along the cumulative order by less than the rounding of the sums:
+            # the weight strictly between the two choices is below 1e-9 of the total
+            for s_ in flips:
+                lo_, hi_ = sorted((anc_gpu[s_], anc_o[s_]))
+                between = w_gpu[lo_ + 1:hi_].sum()
+                assert between <= 1e-9 * w_gpu.sum(), f"t={t} slot {s_}: {lo_}..{hi_} {between}"
+        np.testing.assert_allclose(x_new[same], ost.x[same], rtol=tol["x"][0], atol=tol["x"][1])
+        live = (ost.logw > -30.0) & same
         r_lw = max(tol["lw"][2], 4e-6 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][2]
         a_lw = max(tol["lw"][0], 2e-5 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][0]
         np.testing.assert_allclose(lw_new[live], ost.logw[live], atol=a_lw, rtol=r_lw)
         # weights below e^-30 of the live ones: FP32 cancellation in the residual
         # y - Hx - c dominates, growing with Dx * Dy like the live tolerance
         r_dead = max(tol["lw"][1], 5e-6 * m.Dx * m.Dy) if prec == "f32" else tol["lw"][1]
-        np.testing.assert_allclose(lw_new[~live], ost.logw[~live], rtol=r_dead)
+        dead = ~live & same
+        np.testing.assert_allclose(lw_new[dead], ost.logw[dead], rtol=r_dead)
         r_ll = max(1e-7, 4e-6 * m.Dx * m.Dy) if prec == "f32" else 1e-12
         assert ll == pytest.approx(ll_o, abs=tol["ll"], rel=r_ll), f"t={t}"
         x_gpu, lw_gpu = x_new, lw_new
@@ -83,6 +97,13 @@ def test_teacher_forced_generic_dims(ctx, orc, Dx, Dy):
     m.A = 0.9 * m.A / max(abs(np.linalg.eigvals(m.A)))
     _teacher_forced(ctx, orc, m, 3001, "f32", 0, [1.0, 0.0, 1.0])
     _teacher_forced(ctx, orc, m, 3001, "f64", 2, [1.0, 0.5, 1.0])
+
+
+def test_teacher_forced_many_tiles(ctx, orc):
+    # 8197 tiles: the finish streams each thread's chunk of tiles from memory in groups
+    # (more than 1024 tiles); resample, keep (baseline sums), then the 0.5 rule.
+    _teacher_forced(ctx, orc, config2_model(), (1 << 23) + 4321, "f32", 0, [1.0, 0.0, 0.5],
+                    anc_flips=1e-5)
 
 
 def test_tiny_particle_counts(ctx, orc):
