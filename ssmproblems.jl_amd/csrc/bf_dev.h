@@ -194,9 +194,9 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 
   const int tile0 = tile * kTile;
 
-  auto compute_item = [&](int k, const T* xk, T nlk) -> T {
+  // the normals of item k: pre-drawn planes, or Philox keyed by (seed, t, i)
+  auto normals = [&](int k, T* z) {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
-    T z[MD];
     if constexpr (NOISE) {
 #pragma unroll
       for (int d = 0; d < MD; ++d)
@@ -204,6 +204,9 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     } else {
       draw_normals<T, MD>(p.seed, kStreamPropagate, (uint32_t)t, (uint32_t)i, D, z);
     }
+  };
+  auto compute_item = [&](int k, const T* xk, const T* z, T nlk) -> T {
+    const int i = tile0 + k * kStepThreads + threadIdx.x;
     // predict: x' = A x + b + L_Q z
     T xn[MD];
 #pragma unroll
@@ -278,7 +281,11 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         nl[k] = lw[k] - lse;
     }
     if constexpr (DT > 0) {
-      // all loads first (maximum memory-level parallelism), then the arithmetic
+      // the normals while the ancestor (or log-weight) loads are in flight, then every
+      // gather, then the arithmetic
+      T z[kStepItems][MD];
+#pragma unroll
+      for (int k = 0; k < kStepItems; ++k) normals(k, z[k]);
       T x[kStepItems][MD];
 #pragma unroll
       for (int k = 0; k < kStepItems; ++k) {
@@ -287,15 +294,16 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         for (int d = 0; d < MD; ++d) x[k][d] = p.xin[(size_t)d * p.Npad + src];
       }
 #pragma unroll
-      for (int k = 0; k < kStepItems; ++k) lnew[k] = compute_item(k, x[k], nl[k]);
+      for (int k = 0; k < kStepItems; ++k) lnew[k] = compute_item(k, x[k], z[k], nl[k]);
     } else {
       for (int k = 0; k < kStepItems; ++k) {
         const int src = src_of(k);
-        T xk[MD];
+        T xk[MD], z[MD];
 #pragma unroll
         for (int d = 0; d < MD; ++d)
           if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
-        lnew[k] = compute_item(k, xk, nl[k]);
+        normals(k, z);
+        lnew[k] = compute_item(k, xk, z, nl[k]);
       }
     }
     tile_partial<T>(nl, lnew, valid, FRESH, p.partials, p.nblocks, tile);
