@@ -32,7 +32,7 @@ namespace ssm {
 
 constexpr int kRbDX = 32;  // inner state dim (padded)
 constexpr int kRbDY = 16;  // observation dim (padded)
-constexpr int kRbWaves = 4;
+constexpr int kRbWaves = 8;  // two waves per SIMD: the block's LDS is sized to fit 160 KB
 constexpr int kRbThreads = 64 * kRbWaves;
 constexpr int kRbLX = kRbDX + 1, kRbLY = kRbDY + 1;
 
@@ -48,12 +48,22 @@ struct RbShared {
   float A[kRbDX * kRbLX], Q[kRbDX * kRbLX], H[kRbDY * kRbLX], R0[kRbDY * kRbLY];
   float b[kRbDX], C[kRbDX], c[kRbDY];
 };
+// Per wave (18 KB): T1 is dead while the innovation covariance is factorised, so the FP64
+// factor and its work space live in its place; the solve writes K' over P column by column
+// (each column is read into registers before it is written), and P is not used after it.
 struct RbWave {
-  float Sig[kRbDX * kRbLX], T1[kRbDX * kRbLX], M[kRbDX * kRbLX];
-  float P[kRbDY * kRbLX], KT[kRbDY * kRbLX], KR[kRbDX * kRbLY], Rt[kRbDY * kRbLY];
+  float Sig[kRbDX * kRbLX];
+  union {
+    float T1[kRbDX * kRbLX];
+    struct {
+      double Sd[kRbDY * (kRbDY + 1)];
+      double chol_work[3 * kRbDY];
+    } ch;
+  };
+  float M[kRbDX * kRbLX];
+  float P[kRbDY * kRbLX];  // H Sigma, then K' (KT)
+  float KR[kRbDX * kRbLY], Rt[kRbDY * kRbLY];
   float mu[kRbDX], muh[kRbDX], z[kRbDY], y[kRbDY];
-  double Sd[kRbDY * (kRbDY + 1)];
-  double chol_work[3 * kRbDY];
 };
 
 struct RbStepParams {
@@ -159,8 +169,10 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
               nullptr, 0, false, nullptr, 0, false, 0, w.Rt, kRbLY, 0);
     wave_lds_sync();
     double quad = 0.0, logdet = 0.0;
-    const int good = chol_solve_wave<kRbDY>(w.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, w.KT,
-                                            kRbLX, D, kRbDX, Dy, kRbDY, &quad, &logdet, w.chol_work);
+    float* const KT = w.P;
+    const int good = chol_solve_wave<kRbDY>(w.ch.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, KT,
+                                            kRbLX, D, kRbDX, Dy, kRbDY, &quad, &logdet,
+                                            w.ch.chol_work);
     float lnew;
     if (good) {
       const double ll = -0.5 * ((double)Dy * log2pi + logdet + quad);
@@ -168,19 +180,19 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
       // mu = mu_hat + K z ; M = I - K H ; KR = K R_t
       for (int r = lane; r < kRbDX; r += 64) {
         float s = w.muh[r];
-        for (int q = 0; q < Dy; ++q) s += w.KT[q * kRbLX + r] * w.z[q];
+        for (int q = 0; q < Dy; ++q) s += KT[q * kRbLX + r] * w.z[q];
         w.mu[r] = r < D ? s : 0.f;
       }
-      gemm16<1>(w.M, kRbLX, kRbDX, kRbDX, -1.f, w.KT, kRbLX, true, sh.H, kRbLX, false, kRbDY,
+      gemm16<1>(w.M, kRbLX, kRbDX, kRbDX, -1.f, KT, kRbLX, true, sh.H, kRbLX, false, kRbDY,
                 nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, D);
-      gemm16<1>(w.KR, kRbLY, kRbDX, kRbDY, 1.f, w.KT, kRbLX, true, w.Rt, kRbLY, false, kRbDY,
+      gemm16<1>(w.KR, kRbLY, kRbDX, kRbDY, 1.f, KT, kRbLX, true, w.Rt, kRbLY, false, kRbDY,
                 nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
       wave_lds_sync();
       gemm16<1>(w.T1, kRbLX, kRbDX, kRbDX, 1.f, w.M, kRbLX, false, w.Sig, kRbLX, false, kRbDX,
                 nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
       wave_lds_sync();
       gemm16<1>(w.Sig, kRbLX, kRbDX, kRbDX, 1.f, w.T1, kRbLX, false, w.M, kRbLX, true, kRbDX,
-                w.KR, kRbLY, false, w.KT, kRbLX, false, kRbDY, nullptr, 0, 0);
+                w.KR, kRbLY, false, KT, kRbLX, false, kRbDY, nullptr, 0, 0);
       wave_lds_sync();
     } else {
       lnew = -INFINITY;
