@@ -97,21 +97,27 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Lane S of each aligned group of W lanes (W = 2 or 4), broadcast to the group (DPP
-// quad_perm: the groups live inside quads).
+// Lane S of each aligned group of W lanes (W = 1, 2 or 4), broadcast to the group (DPP
+// quad_perm: the groups live inside quads; a group of one lane is the lane itself).
 template <int S, int W>
 __device__ __forceinline__ double group_bcast(double v) {
-  static_assert(W == 2 || W == 4, "groups of 2 or 4 lanes");
-  constexpr int s0 = S, s1 = S, s2 = (W == 4) ? S : 2 + S, s3 = s2;
-  constexpr int ctrl = s0 | (s1 << 2) | (s2 << 4) | (s3 << 6);
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
+  static_assert(W == 1 || W == 2 || W == 4, "groups of 1, 2 or 4 lanes");
+  if constexpr (W == 1) {
+    return v;
+  } else {
+    constexpr int s0 = S, s1 = S, s2 = (W == 4) ? S : 2 + S, s3 = s2;
+    constexpr int ctrl = s0 | (s1 << 2) | (s2 << 4) | (s3 << 6);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+  }
 }
 // Sum over each aligned group of W lanes, in every lane of the group (fixed order).
 template <int W>
 __device__ __forceinline__ double group_sum(double v) {
-  if constexpr (W == 4)
+  if constexpr (W == 1)
+    return v;
+  else if constexpr (W == 4)
     return (group_bcast<0, 4>(v) + group_bcast<1, 4>(v)) +
            (group_bcast<2, 4>(v) + group_bcast<3, 4>(v));
   else
@@ -130,7 +136,7 @@ __device__ __forceinline__ double group_sum(double v) {
 template <int DYP>
 __device__ __forceinline__ int chol_factor_wave(const float* Sf, int ldSf, int dy, double* Sd, int ldS,
                                 double* colbuf, double* invd, double* logdet) {
-  static_assert(DYP == 16 || DYP == 32, "DYP must be 16 or 32");
+  static_assert(DYP == 8 || DYP == 16 || DYP == 32, "DYP must be 8, 16 or 32");
   constexpr int G = 64 / DYP, E = DYP / G;
   const int lane = threadIdx.x & 63;
   const int i = lane % DYP, h = lane / DYP;
@@ -201,7 +207,7 @@ __device__ __forceinline__ double solve_group(int col, int r, const double* Sd, 
                                               const double* invd, const float* P, int ldP,
                                               const float* zv, float* KT, int ldK, int dx,
                                               int dxp, int dy, int dyp) {
-  static_assert(DYP == 16 || DYP == 32, "DYP must be 16 or 32");
+  static_assert(DYP == 8 || DYP == 16 || DYP == 32, "DYP must be 8, 16 or 32");
   if (col > dx) {
     if (col < dxp)
 #pragma unroll

@@ -86,6 +86,9 @@ struct RbStepParams {
   int32_t* __restrict__ err;  // set to 1 if an innovation covariance is not PD
 };
 
+// DYP: the compile-time bound of the innovation covariance's Cholesky and solves (8 when
+// Dy <= 8, else 16); the matrix-core operands stay padded to kRbDY rows.
+template <int DYP>
 __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
   extern __shared__ __align__(16) unsigned char rb_lds[];
   RbShared& sh = *reinterpret_cast<RbShared*>(rb_lds);
@@ -170,9 +173,10 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
     wave_lds_sync();
     double quad = 0.0, logdet = 0.0;
     float* const KT = w.P;
-    const int good = chol_solve_wave<kRbDY>(w.ch.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, KT,
-                                            kRbLX, D, kRbDX, Dy, kRbDY, &quad, &logdet,
-                                            w.ch.chol_work);
+    // rows Dy..kRbDY-1 of P (and so of K') are zero: the padded rows of H are
+    const int good = chol_solve_wave<DYP>(w.ch.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, KT,
+                                          kRbLX, D, kRbDX, Dy, DYP, &quad, &logdet,
+                                          w.ch.chol_work);
     float lnew;
     if (good) {
       const double ll = -0.5 * ((double)Dy * log2pi + logdet + quad);
@@ -382,7 +386,12 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   p.seed = rb->seed;
   p.noise = noise_dev;
   p.err = rb->err;
-  hipLaunchKernelGGL(k_rbpf_step, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s, p);
+  if (rb->Dy <= 8)
+    hipLaunchKernelGGL(k_rbpf_step<8>, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s,
+                       p);
+  else
+    hipLaunchKernelGGL(k_rbpf_step<16>, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s,
+                       p);
   const int nb = (rb->N + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_tile_partials<float>, dim3(nb), dim3(kTileThreads), 0, s, rb->N,
                      (const float*)rb->logw, (const float*)rb->nl_pre, (const BfScalars*)rb->sc,
@@ -435,11 +444,11 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
   rb->seed = seed;
   const int Npad = (int)(((int64_t)N + 2047) / 2048 * 2048);
   int per_cu = 0;
-  SSM_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rbpf_step,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+  const void* kstep = (rb->Dy <= 8) ? (const void*)k_rbpf_step<8> : (const void*)k_rbpf_step<16>;
+  SSM_HIP_CHECK(ctx, hipFuncSetAttribute(kstep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)rb_lds_bytes()));
-  SSM_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, k_rbpf_step, kRbThreads, rb_lds_bytes()));
+  SSM_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kstep, kRbThreads,
+                                                                  rb_lds_bytes()));
   if (per_cu < 1) per_cu = 1;
   rb->nblocks = std::max(1, std::min((N + kRbWaves - 1) / kRbWaves,
                                      per_cu * (ctx->cu_count > 0 ? ctx->cu_count : 256)));
