@@ -45,6 +45,10 @@ extern "C" {
 #define SSM_STRATIFIED 2        /* CUDAExt.jl:60-78 (Murray Code 7)                    */
 #define SSM_MULTINOMIAL 3       /* CUDAExt.jl:31-39 (inverse CDF, Murray Code 5)       */
 
+/* Conditional SMC trajectory refreshment (GF/src/algorithms/csmc.jl:13-40). */
+#define SSM_CSMC_PLAIN 0     /* NoRefreshment: the reference keeps ancestor 0 */
+#define SSM_CSMC_ANCESTOR 1  /* AncestorSampling: its ancestor is drawn each step */
+
 /* ---- arithmetic types ------------------------------------------------------------- */
 #define SSM_F32 0
 #define SSM_F64 1
@@ -172,6 +176,32 @@ int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
                        int32_t resampler, int32_t reps, double* ms_kernels, double* ll_total);
 
+/* ---- conditional SMC on the bootstrap filter (GF/src/algorithms/csmc.jl) ---------- */
+/*
+ * Sets (x_ref non-NULL) or clears (x_ref NULL) the reference trajectory of conditional SMC
+ * (the ref_state keyword of initialise / step / resample, particles.jl:36-66 and
+ * resamplers.jl:51-65).  x_ref is (T + 1) x Dx, row t = the reference state at time t
+ * (row 0 the time-0 state, as ReferenceTrajectory indexes from 0).  With a reference,
+ * initialise puts particle 0 at x_ref[0] and every step t <= T puts it at x_ref[t]; steps
+ * past T are refused.  mode SSM_CSMC_PLAIN (NoRefreshment, csmc.jl:239-263): when a step
+ * resamples, particle 0's ancestor is particle 0.  mode SSM_CSMC_ANCESTOR (AncestorSampling,
+ * csmc.jl:265-306): every step resamples, and particle 0's ancestor is drawn on the device
+ * with probability proportional to w_i N(x_ref[t]; A x_i + b, Q) (ancestor_sampling.jl:
+ * 64-130), with a uniform from the filter's Philox key.  Set the model before the
+ * reference: Q's factor is taken at this call (and again by ssm_bf_set_model).
+ */
+int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_ref, int32_t mode);
+/*
+ * StatsBase.sample(rng, Weights(get_weights(state))) of _sample_trajectory (csmc.jl:111-117):
+ * the 0-based index of the first particle whose cumulative normalised weight reaches u
+ * (0 <= u < 1), from the current log-weights (all equal after initialise).
+ */
+int32_t ssm_bf_sample_index(ssm_bf* bf, double u, int32_t* index);
+/* Re-keys the filter's Philox draws (initialise, propagation, resampling, ancestor
+ * sampling) for the calls that follow: a new sweep's randomness, as the Julia rng moves on
+ * between _csmc_sample calls (csmc.jl:450-462).  Not in the reference's API. */
+int32_t ssm_bf_set_seed(ssm_bf* bf, uint64_t seed);
+
 /* ---- path storage of a bootstrap filter (GF/ext/CUDAExt.jl:146-298) --------------- */
 /*
  * Sparse ancestry storage on the device (Jacob, Murray & Rubenthaler 2015): the device
@@ -209,6 +239,9 @@ int32_t ssm_bf_tree_destroy(ssm_tree* tree);
  * the step kept its particles).  (4 Dx + 12) B per particle and step in FP32.
  *   ssm_bf_history_create     capacity = generations allocated up front (doubles when full)
  *   ssm_bf_history_push       appends the filter's current generation (push!, :364-374)
+ *   ssm_bf_history_reset      empties it and stores the filter's current particles as time
+ *                             0, keeping the allocation (a new container per CSMC sweep,
+ *                             csmc.jl:140-148, without reallocating)
  *   ssm_bf_history_info       generations pushed and allocated
  *   ssm_bf_history_ancestry   paths of particles i0 .. i0+count-1 (get_ancestry, :406-418):
  *                             paths[(k (T+1) + t) Dx + d], t = 0..T
@@ -217,6 +250,7 @@ int32_t ssm_bf_tree_destroy(ssm_tree* tree);
  */
 int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_history** out);
 int32_t ssm_bf_history_push(ssm_history* h);
+int32_t ssm_bf_history_reset(ssm_history* h);
 int32_t ssm_bf_history_info(ssm_history* h, int32_t* T, int32_t* capacity);
 int32_t ssm_bf_history_ancestry(ssm_history* h, int32_t i0, int32_t count, double* paths);
 int32_t ssm_bf_history_generation(ssm_history* h, int32_t t, double* x, double* logw,

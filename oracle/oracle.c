@@ -417,10 +417,11 @@ int orc_bf_initialise(const orc_lg_model* m, int64_t N, const double* z, double*
  * softmax computed here (teacher forcing with a device's own weights).
  * Outputs: anc (N), *ll_inc, *did_resample, w_out (optional, the weights used).
  */
-int orc_bf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* typeless,
-                double threshold, int kind, const double* u, const double* z,
-                const double* y, const double* w_override, int32_t* anc, double* ll_inc,
-                int* did_resample, double* w_out) {
+static int bf_step_core(const orc_lg_model* m, int64_t N, double* x, double* logw,
+                        int* typeless, double threshold, int kind, const double* u,
+                        const double* z, const double* y, const double* w_override,
+                        const double* x_ref_t, int32_t ref_anc, int32_t* anc, double* ll_inc,
+                        int* did_resample, double* w_out) {
   const int D = m->Dx, Dy = m->Dy;
   double* w = (double*)malloc(sizeof(double) * (size_t)N);
   double* xn = (double*)malloc(sizeof(double) * (size_t)(N * D));
@@ -453,6 +454,7 @@ int orc_bf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* 
   if (*did_resample) {
     rc = orc_resample(kind, N, w_override ? w_override : w, u, anc);
     if (rc) goto done;
+    if (x_ref_t) anc[0] = ref_anc; /* _set_ref_index! / the drawn ancestor, resamplers.jl:51-65 */
     for (int64_t i = 0; i < N; ++i) {
       const int64_t a = anc[i];
       for (int d = 0; d < D; ++d) xn[i * D + d] = x[a * D + d];
@@ -482,6 +484,9 @@ int orc_bf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* 
       xn[i * D + rr] = s + e;
     }
   memcpy(x, xn, sizeof(double) * (size_t)(N * D));
+  /* conditional SMC: particle 1 (index 0) moves to the reference state (particles.jl:59-66) */
+  if (x_ref_t)
+    for (int d = 0; d < D; ++d) x[d] = x_ref_t[d];
 
   /* update: logw += logpdf(MvNormal(H x + c, R), y) */
   for (int64_t i = 0; i < N; ++i) {
@@ -503,6 +508,65 @@ done:
   free(xn);
   free(LQ);
   free(LR);
+  free(r);
+  free(work);
+  return rc;
+}
+
+int orc_bf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* typeless,
+                double threshold, int kind, const double* u, const double* z,
+                const double* y, const double* w_override, int32_t* anc, double* ll_inc,
+                int* did_resample, double* w_out) {
+  return bf_step_core(m, N, x, logw, typeless, threshold, kind, u, z, y, w_override, NULL, 0,
+                      anc, ll_inc, did_resample, w_out);
+}
+
+/*
+ * One conditional-SMC step (csmc.jl:239-306): the BootstrapFilter step above with the
+ * reference pinned to particle 1 (index 0).  When the step resamples, particle 0's
+ * ancestor is ref_anc (0 for NoRefreshment's _set_ref_index!, resamplers.jl:51-65, or
+ * the index ancestor sampling drew, csmc.jl:276-297); after the predict particle 0 is
+ * x_ref_t, the reference state at this step (particles.jl:59-66).  Ancestor sampling
+ * always resamples (csmc.jl:288 calls resample, not maybe_resample): pass threshold
+ * +Inf for it.  w_override and w_out as in orc_bf_step.
+ */
+int orc_csmc_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* typeless,
+                  double threshold, int kind, const double* u, const double* z,
+                  const double* y, const double* w_override, const double* x_ref_t,
+                  int32_t ref_anc, int32_t* anc, double* ll_inc, int* did_resample,
+                  double* w_out) {
+  if (!x_ref_t || ref_anc < 0 || ref_anc >= N) return ORC_ERR_ARG;
+  return bf_step_core(m, N, x, logw, typeless, threshold, kind, u, z, y, w_override, x_ref_t,
+                      ref_anc, anc, ll_inc, did_resample, w_out);
+}
+
+/*
+ * Ancestor-sampling log-weights (ancestor_weight, ancestor_sampling.jl:125-130, with the
+ * generic future_conditional_density :64-68 = SSMProblems.logdensity of the LG dynamics):
+ * out_i = log w_i + log N(x_ref; A x_i + b, Q), log w_i the current log-weights (0 for the
+ * TypelessZero weights after initialise).  x is N x Dx AoS.
+ */
+int orc_as_logweights(const orc_lg_model* m, int64_t N, const double* x, const double* logw,
+                      int typeless, const double* x_ref, double* out) {
+  const int D = m->Dx;
+  double* LQ = (double*)malloc(sizeof(double) * (size_t)(D * D));
+  double* r = (double*)malloc(sizeof(double) * (size_t)D);
+  double* work = (double*)malloc(sizeof(double) * (size_t)D);
+  int rc = ORC_OK;
+  memcpy(LQ, m->Q, sizeof(double) * (size_t)(D * D));
+  if (chol_lower(D, LQ)) {
+    rc = ORC_ERR_NOT_PD;
+  } else {
+    for (int64_t i = 0; i < N; ++i) {
+      for (int a = 0; a < D; ++a) {
+        double s = m->b[a];
+        for (int k = 0; k < D; ++k) s += m->A[a + k * D] * x[i * D + k];
+        r[a] = x_ref[a] - s;
+      }
+      out[i] = (typeless ? 0.0 : logw[i]) + mvn_logpdf_chol(D, LQ, r, work);
+    }
+  }
+  free(LQ);
   free(r);
   free(work);
   return rc;

@@ -31,6 +31,12 @@ Parity pinning of each oracle function (what checks it beyond retyping its formu
   the dense ancestor history (DenseParticleContainer's get_ancestry, containers.jl:406-418),
   nodes in use against the number of distinct (time, particle) pairs on the surviving lines,
   and a hand-worked insert with pool growth (tests/test_oracle_tree.py).
+* ``csmc_step`` / ``as_logweights`` / ``categorical`` / ``csmc_sweep`` (csmc.jl:239-306,
+  ancestor_sampling.jl:64-130, resamplers.jl:51-65): the ancestor-sampling log-weights
+  against scipy's multivariate normal density, the categorical draw against a binary search
+  of the cumulative weights, the step against ``bf_step`` on every particle but the pinned
+  one, and the reference's statistical test of both sweeps (test/algorithms/csmc.jl:5-95):
+  the CSMC chain's mean of x_2 against the exact smoothing mean from the joint Gaussian.
 * ``logsumexp``: parity unpinned beyond closed cases (LogExpFunctions' exact rounding is
   third-party; only tolerances are claimed).
 """
@@ -46,7 +52,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 SYSTEMATIC, SYSTEMATIC_MURRAY, STRATIFIED, MULTINOMIAL = 0, 1, 2, 3
-STREAM_PROPAGATE, STREAM_RESAMPLE, STREAM_INIT, STREAM_OUTER = 0, 1, 2, 3
+STREAM_PROPAGATE, STREAM_RESAMPLE, STREAM_INIT, STREAM_OUTER, STREAM_ANCESTOR = 0, 1, 2, 3, 4
 
 
 def build() -> str:
@@ -102,6 +108,12 @@ def _declare(L):
                               C.c_double, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
                               C.POINTER(C.c_int), _dp]
     L.orc_bf_step.restype = C.c_int
+    L.orc_csmc_step.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp, C.POINTER(C.c_int),
+                                C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int32, _ip,
+                                _dp, C.POINTER(C.c_int), _dp]
+    L.orc_csmc_step.restype = C.c_int
+    L.orc_as_logweights.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp, C.c_int, _dp, _dp]
+    L.orc_as_logweights.restype = C.c_int
     L.orc_kf_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.orc_kf_update.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp,
                                 _dp]
@@ -165,6 +177,14 @@ def philox_uniforms(seed, t, n):
     u = np.zeros(n, dtype=np.float64)
     lib().orc_philox_uniforms(C.c_uint64(seed), t, n, _d(u))
     return u
+
+
+def stream_uniform(seed, stream, t, k=0):
+    """The [0, 1) uniform the device draws for (seed, stream, step t, index k): Philox
+    counter (k, t, stream, 0), key (low, high word of seed), top 53 bits of words 0-1."""
+    w = philox4x32_10([k, t, stream, 0], [seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF])
+    v = ((int(w[0]) << 32) | int(w[1])) >> 11
+    return v * (1.0 / 9007199254740992.0)
 
 
 # --------------------------------------------------------------------------------------
@@ -270,6 +290,81 @@ def bf_step(model: LG, state: BFState, y, threshold, kind, u, z, w_override=None
     return anc, ll.value, bool(did.value), w_out
 
 
+def csmc_step(model: LG, state: BFState, y, threshold, kind, u, z, x_ref_t, ref_anc,
+              w_override=None):
+    """One conditional-SMC step in place (orc_csmc_step): particle 0 takes ancestor ref_anc
+    when the step resamples and moves to x_ref_t.  Returns (anc, ll_inc, did_resample,
+    w_used)."""
+    N = state.x.shape[0]
+    anc = np.zeros(N, dtype=np.int32)
+    ll = C.c_double()
+    did = C.c_int()
+    typ = C.c_int(int(state.typeless))
+    w_out = np.zeros(N)
+    m, keep = model.cstruct()
+    u = _f64(np.atleast_1d(u))
+    z = _f64(z)
+    y = _f64(np.atleast_1d(y))
+    xr = _f64(x_ref_t)
+    wo = None if w_override is None else _f64(w_override)
+    rc = lib().orc_csmc_step(C.byref(m), N, _d(state.x), _d(state.logw), C.byref(typ),
+                             float(threshold), kind, _d(u), _d(z), _d(y), _d(wo), _d(xr),
+                             int(ref_anc), _i(anc), C.byref(ll), C.byref(did), _d(w_out))
+    if rc:
+        raise ValueError(f"orc_csmc_step rc={rc}")
+    state.typeless = bool(typ.value)
+    return anc, ll.value, bool(did.value), w_out
+
+
+def as_logweights(model: LG, state: BFState, x_ref):
+    """Ancestor-sampling log-weights log w_i + log N(x_ref; A x_i + b, Q) (orc_as_logweights)."""
+    N = state.x.shape[0]
+    out = np.zeros(N)
+    m, keep = model.cstruct()
+    xr = _f64(x_ref)
+    rc = lib().orc_as_logweights(C.byref(m), N, _d(state.x), _d(state.logw),
+                                 int(state.typeless), _d(xr), _d(out))
+    if rc:
+        raise ValueError(f"orc_as_logweights rc={rc}")
+    return out
+
+
+def categorical(logw, u):
+    """StatsBase.sample(Weights(softmax(logw))) with the uniform u: the multinomial rule
+    (first j whose cumulative weight reaches u W_n) for one draw."""
+    lw = _f64(logw)
+    w = np.exp(lw - lw.max())
+    return int(resample(MULTINOMIAL, w, np.full(lw.shape[0], float(u)))[0])
+
+
+def csmc_sweep(model: LG, ys, N, x_ref, ancestor_sampling, threshold, kind, rng):
+    """One CSMC sweep (csmc.jl:239-306) with numpy draws: returns (trajectory (K+1, Dx),
+    log-likelihood).  x_ref None runs the unconditional filter (the first iteration)."""
+    ys = np.asarray(ys, dtype=np.float64).reshape(len(ys), -1)
+    K, D = ys.shape[0], model.Dx
+    st = BFState(bf_initialise(model, rng.standard_normal((N, D))))
+    if x_ref is not None:
+        st.x[0] = x_ref[0]
+    xs, ancs = [st.x.copy()], []
+    ll = 0.0
+    for t in range(1, K + 1):
+        u = rng.random(1 if kind in (SYSTEMATIC, SYSTEMATIC_MURRAY) else N)
+        z = rng.standard_normal((N, D))
+        if x_ref is None:
+            anc, inc, _, _ = bf_step(model, st, ys[t - 1], threshold, kind, u, z)
+        else:
+            thr, a0 = threshold, 0
+            if ancestor_sampling:
+                a0 = categorical(as_logweights(model, st, x_ref[t]), rng.random())
+                thr = np.inf
+            anc, inc, _, _ = csmc_step(model, st, ys[t - 1], thr, kind, u, z, x_ref[t], a0)
+        ll += inc
+        xs.append(st.x.copy())
+        ancs.append(anc)
+    j = categorical(st.logw, rng.random())
+    return dense_ancestry(xs, ancs)[0][j], ll
+
+
 def kf_filter(model: LG, ys, jitter=0.0):
     ys = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(-1, model.Dy))
     T = ys.shape[0]
@@ -353,6 +448,20 @@ def joint_gaussian_loglik(model: LG, ys):
     mu_Z, Sig_Z = joint_gaussian(model, T)
     iy = slice(model.Dx * (T + 1), None)
     return mvn_logpdf(ys.reshape(-1), mu_Z[iy], Sig_Z[iy, iy])
+
+
+def joint_posterior(model: LG, ys):
+    """Mean (T+1, Dx) and covariance of X0..XT given y_1:T, by Gaussian conditioning of the
+    joint Gaussian (the smoothing distribution the reference's CSMC tests compare with)."""
+    ys = np.asarray(ys, dtype=np.float64).reshape(-1, model.Dy)
+    T = ys.shape[0]
+    mu_Z, Sig_Z = joint_gaussian(model, T)
+    nx = model.Dx * (T + 1)
+    Sxx, Sxy, Syy = Sig_Z[:nx, :nx], Sig_Z[:nx, nx:], Sig_Z[nx:, nx:]
+    K = np.linalg.solve(Syy, Sxy.T).T
+    mean = mu_Z[:nx] + K @ (ys.reshape(-1) - mu_Z[nx:])
+    cov = Sxx - K @ Sxy.T
+    return mean.reshape(T + 1, model.Dx), cov
 
 
 # --------------------------------------------------------------------------------------

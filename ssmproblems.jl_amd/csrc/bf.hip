@@ -35,9 +35,32 @@
 namespace ssm {
 
 // The fused predict + update + partial-reduction kernel: one block per tile.
-template <typename T, int DT, int DYT, bool NOISE>
+template <typename T, int DT, int DYT, bool NOISE, bool REF>
 __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  bf_step_tile<T, DT, DYT, NOISE>(p, blockIdx.x);
+  bf_step_tile<T, DT, DYT, NOISE, REF>(p, blockIdx.x);
+}
+
+// Conditional SMC: ancestor-sampling log-weights for the reference state at the filter's
+// current step (thread 0 also hands the step to the scalars of their fold).
+template <typename T, int DT>
+__global__ void __launch_bounds__(kStepThreads)
+    k_as_weights(int N, int Npad, int D, const BfModelDev<T>* model, const RefModelDev<T>* rm,
+                 const T* x, const T* logw, const BfScalars* sc, const T* xref, T* w_out,
+                 BfScalars* as_sc) {
+  const int t = sc->step;
+  const int i = blockIdx.x * kStepThreads + threadIdx.x;
+  if (i == 0) {
+    as_sc->step = t;
+    as_sc->typeless = 0;
+  }
+  bf_as_weight_item<T, DT>(N, Npad, D, model, rm, x, logw, sc, xref + (size_t)t * D, w_out, i);
+}
+
+// Conditional SMC: particle 0 of the initial generation is the reference's time-0 state.
+template <typename T>
+__global__ void k_pin_reference(int D, int Npad, T* x, const T* xref) {
+  const int d = threadIdx.x;
+  if (d < D) x[(size_t)d * Npad] = xref[d];
 }
 
 template <typename T, int DT>
@@ -84,6 +107,19 @@ struct ssm_bf {
   int32_t* tree_counts = nullptr;
   bool counts_fresh = false;
   std::vector<ssm_history*> histories;  // attached dense histories
+  // conditional SMC: the reference trajectory and the ancestor-sampling buffers
+  std::vector<double> Q_host;  // Q of the current model (column-major)
+  std::vector<double> xref_host;  // the reference as given (for a model change)
+  void* xref = nullptr;        // (ref_T + 1) x D, the filter's type
+  size_t xref_cap = 0;         // elements
+  int ref_T = -1;              // -1: no reference
+  int ref_mode = SSM_CSMC_PLAIN;
+  void* ref_model = nullptr;   // RefModelDev<T>
+  void* as_w = nullptr;        // Npad log-weights of the ancestor draw
+  void* as_scratch = nullptr;  // their tile scratch
+  BfScalars* as_sc = nullptr;  // the scalars of their fold
+  int32_t* as_idx = nullptr;   // the drawn ancestor
+  int32_t* draw_idx = nullptr; // ssm_bf_sample_index's result
 };
 
 namespace {
@@ -94,6 +130,7 @@ int upload_model(ssm_bf* bf, const ssm_lg_model* mdl) {
   BfModelDev<T> h;
   const int rc = bf_build_model<T>(ctx, mdl, &h);
   if (rc) return rc;
+  bf->Q_host.assign(mdl->Q, mdl->Q + (size_t)mdl->Dx * mdl->Dx);
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->model_dev, &h, sizeof h, hipMemcpyHostToDevice,
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -101,6 +138,9 @@ int upload_model(ssm_bf* bf, const ssm_lg_model* mdl) {
 }
 
 int check_model(ssm_ctx* ctx, const ssm_lg_model* m) { return bf_check_model(ctx, m); }
+
+template <typename T>
+int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref);
 
 TileScratch tile_scratch(ssm_bf* bf) {
   ResampleArgs a;
@@ -129,21 +169,38 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   p.noise = noise;
   const TileScratch ts = tile_scratch(bf);
   p.partials = ts.partials;
+  p.xref = (const T*)bf->xref;
+  p.ref_anc = (bf->ref_mode == SSM_CSMC_ANCESTOR) ? bf->as_idx : nullptr;
+  p.anc_w = bf->anc;
   const dim3 g(p.nblocks), b(kStepThreads);
+  if (bf->ref_T >= 0) {  // conditional SMC: particle 0 follows the reference
+    if (noise) {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, true>), g, b, 0, s, p);
+    } else {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, true>), g, b, 0, s, p);
+    }
+    return;
+  }
   if (noise) {  // parity mode: pre-drawn normals
     if (bf->D == 4 && bf->Dy == 2)
-      hipLaunchKernelGGL((k_bf_step<T, 4, 2, true>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false>), g, b, 0, s, p);
     else if (bf->D == 1 && bf->Dy == 1)
-      hipLaunchKernelGGL((k_bf_step<T, 1, 1, true>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false>), g, b, 0, s, p);
     else
-      hipLaunchKernelGGL((k_bf_step<T, 0, 0, true>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false>), g, b, 0, s, p);
   } else {
     if (bf->D == 4 && bf->Dy == 2)
-      hipLaunchKernelGGL((k_bf_step<T, 4, 2, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false>), g, b, 0, s, p);
     else if (bf->D == 1 && bf->Dy == 1)
-      hipLaunchKernelGGL((k_bf_step<T, 1, 1, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false>), g, b, 0, s, p);
     else
-      hipLaunchKernelGGL((k_bf_step<T, 0, 0, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false>), g, b, 0, s, p);
   }
 }
 
@@ -191,9 +248,52 @@ void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
                              bf->rs_steps, record);
 }
 
+// Conditional SMC with ancestor sampling: the reference particle's ancestor for this step,
+// drawn from softmax(log w_i + log N(x_ref[t]; A x_i + b, Q)) (GF/src/algorithms/
+// csmc.jl:276-292) into bf->as_idx: the weights, their tile partials, their fold and one
+// categorical draw with a Philox uniform of the ancestor stream.
+template <typename T>
+int enqueue_ancestor_draw_t(ssm_bf* bf, hipStream_t s) {
+  const dim3 g((bf->N + kStepThreads - 1) / kStepThreads), b(kStepThreads);
+  const BfModelDev<T>* md = (const BfModelDev<T>*)bf->model_dev;
+  const RefModelDev<T>* rm = (const RefModelDev<T>*)bf->ref_model;
+  if (bf->D == 4)
+    hipLaunchKernelGGL((k_as_weights<T, 4>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
+                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
+                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc);
+  else
+    hipLaunchKernelGGL((k_as_weights<T, 0>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
+                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
+                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc);
+  ResampleArgs a;
+  a.kind = SSM_MULTINOMIAL;
+  a.n = bf->N;
+  a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
+  a.w = bf->as_w;
+  a.seed = bf->seed;
+  a.sc = bf->as_sc;
+  resample_bind_scratch(a, bf->as_scratch, bf->N);
+  const int nb = (bf->N + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_tile_partials<T>, dim3(nb), dim3(kTileThreads), 0, s, bf->N,
+                     (const T*)bf->as_w, (const T*)nullptr, bf->as_sc, -1.0, a.partials);
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
+  // threshold +Inf: the fold treats the weights as fresh, as their partials were formed;
+  // record 0 leaves the step counter at t (the Philox step of the draw)
+  launch_pf_finish<T>(s, bf->N, nb, bf->as_sc, INFINITY, std::log((double)bf->N), ts, nullptr,
+                      nullptr, 0);
+  return launch_categorical_one(bf->ctx, s, a, -1.0, kStreamAncestor, bf->as_idx);
+}
+
 // Enqueues one full step (resample kernel, step kernel, finish kernel) on stream s.
 int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const double* u_dev,
                  const void* noise_dev, bool debug_weights, int32_t* counts_out = nullptr) {
+  if (bf->ref_T >= 0 && bf->ref_mode == SSM_CSMC_ANCESTOR) {
+    // ancestor sampling resamples at every step (csmc.jl:288)
+    threshold = INFINITY;
+    const int rc = (bf->prec == SSM_F32) ? enqueue_ancestor_draw_t<float>(bf, s)
+                                         : enqueue_ancestor_draw_t<double>(bf, s);
+    if (rc) return rc;
+  }
   const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights, counts_out);
   if (rc) return rc;
   enqueue_move(bf, s, threshold, noise_dev);
@@ -301,7 +401,9 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
   for (ssm_history* h : bf->histories) history_detach_filter(h);
   void* ptrs[] = {bf->x[0],      bf->x[1],     bf->logw,     bf->anc,   bf->model_dev,
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
-                  bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u};
+                  bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u,     bf->xref,
+                  bf->ref_model, bf->as_w,     bf->as_scratch, bf->as_sc, bf->as_idx,
+                  bf->draw_idx};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bf->ev0) (void)hipEventDestroy(bf->ev0);
@@ -316,7 +418,11 @@ extern "C" int32_t ssm_bf_set_model(ssm_bf* bf, const ssm_lg_model* model) {
   if (rc) return rc;
   if (model->Dx != bf->D || model->Dy != bf->Dy)
     return set_error(bf->ctx, SSM_ERR_ARG, "ssm_bf_set_model: dimensions changed");
-  return (bf->prec == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
+  rc = (bf->prec == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
+  if (rc || bf->ref_T < 0) return rc;
+  // the reference's transition constants follow Q
+  return (bf->prec == SSM_F32) ? upload_reference_t<float>(bf, bf->ref_T, bf->xref_host.data())
+                               : upload_reference_t<double>(bf, bf->ref_T, bf->xref_host.data());
 }
 
 template <typename T>
@@ -372,6 +478,14 @@ extern "C" int32_t ssm_bf_initialise(ssm_bf* bf, const double* z) {
                        (const BfModelDev<double>*)bf->model_dev, (double*)bf->x[0],
                        (double*)bf->logw, nz, bf->seed);
   }
+  if (bf->ref_T >= 0) {  // conditional SMC: particle 0 starts at the reference (particles.jl:40-50)
+    if (bf->prec == SSM_F32)
+      hipLaunchKernelGGL(k_pin_reference<float>, dim3(1), dim3(64), 0, ctx->stream, bf->D,
+                         bf->Npad, (float*)bf->x[0], (const float*)bf->xref);
+    else
+      hipLaunchKernelGGL(k_pin_reference<double>, dim3(1), dim3(64), 0, ctx->stream, bf->D,
+                         bf->Npad, (double*)bf->x[0], (const double*)bf->xref);
+  }
   SSM_HIP_CHECK(ctx, hipGetLastError());
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   bf->initialised = true;
@@ -385,6 +499,9 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
   ssm_ctx* ctx = bf->ctx;
   if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_step before initialise");
   if (!y || t < 1) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_step: bad t or null y");
+  if (bf->ref_T >= 0 && t > bf->ref_T)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_step: t = %d past the reference's %d steps", t,
+                     bf->ref_T);
   if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_step: unknown resampler %d", resampler);
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -410,8 +527,11 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
     noise_dev = bf->noise;
   }
   bf->counts_fresh = false;
+  // with a reference, particle 0's ancestor is replaced after the resampler counted
+  // offspring, so the tree recounts from the ancestors instead
+  const bool pass_counts = bf->tree_counts != nullptr && bf->ref_T < 0;
   rc = enqueue_step(bf, ctx->stream, threshold, resampler, u_dev, noise_dev, true,
-                    bf->tree_counts);
+                    pass_counts ? bf->tree_counts : nullptr);
   if (rc) return rc;
   SSM_HIP_CHECK(ctx, hipGetLastError());
   double ll = 0.0;
@@ -421,7 +541,7 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(&did, bf->rs_steps, sizeof did, hipMemcpyDeviceToHost,
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  bf->counts_fresh = bf->tree_counts != nullptr && resampler != SSM_MULTINOMIAL;
+  bf->counts_fresh = pass_counts && resampler != SSM_MULTINOMIAL;
   if (ll_inc) *ll_inc = ll;
   if (did_resample) *did_resample = did;
   return SSM_OK;
@@ -434,6 +554,8 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   ssm_ctx* ctx = bf->ctx;
   if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_run before initialise");
   if (T < 1 || t0 < 1 || !ys) return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: bad arguments");
+  if (bf->ref_T >= 0 && t0 + T - 1 > bf->ref_T)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: steps past the reference's %d", bf->ref_T);
   bf->counts_fresh = false;
   if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_run: unknown resampler %d", resampler);
@@ -455,7 +577,8 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   };
   hipGraphExec_t exec = nullptr;
   if (nfull > 0) {
-    const auto key = std::make_tuple(kChunk, bf->cur, resampler, threshold, 0);
+    const auto key = std::make_tuple(kChunk, bf->cur, resampler, threshold,
+                                     bf->ref_T >= 0 ? 1 + bf->ref_mode : 0);
     auto it = bf->graphs.find(key);
     if (it == bf->graphs.end()) {
       hipGraph_t g;
@@ -605,6 +728,8 @@ extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const doubl
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_profile: bad arguments");
   if (resampler < SSM_SYSTEMATIC || resampler > SSM_MULTINOMIAL)
     return set_error(ctx, SSM_ERR_ARG, "ssm_bf_profile: unknown resampler %d", resampler);
+  if (bf->ref_T >= 0)
+    return set_error(ctx, SSM_ERR_STATE, "ssm_bf_profile: times the unconditional filter only");
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   bf->counts_fresh = false;
   int rc = ensure_steps(bf, (size_t)T);
@@ -722,5 +847,121 @@ extern "C" int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_p
   if (!bf) return SSM_ERR_ARG;
   if (ms_total) *ms_total = bf->last_ms;
   if (ms_per_step) *ms_per_step = bf->last_T ? bf->last_ms / bf->last_T : 0.0;
+  return SSM_OK;
+}
+
+// ---- conditional SMC ----------------------------------------------------------------
+
+namespace {
+
+template <typename T>
+int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref) {
+  ssm_ctx* ctx = bf->ctx;
+  const int D = bf->D;
+  const size_t n = (size_t)(T_ref + 1) * D;
+  if (n > bf->xref_cap) {
+    destroy_graphs(bf);  // graphs hold the old buffer address
+    if (bf->xref) (void)hipFree(bf->xref);
+    bf->xref = nullptr;
+    bf->xref_cap = 0;
+    if (hipMalloc(&bf->xref, sizeof(T) * n) != hipSuccess)
+      return set_error(ctx, SSM_ERR_OOM, "reference trajectory of %d steps", T_ref);
+    bf->xref_cap = n;
+  }
+  std::vector<T> h(n);
+  for (size_t e = 0; e < n; ++e) h[e] = (T)x_ref[e];
+  if (x_ref != bf->xref_host.data()) bf->xref_host.assign(x_ref, x_ref + n);
+  // RefModelDev: L_Q^{-1} and -1/2 (D log 2 pi + log det Q), from the FP64 Q
+  std::vector<double> LQ = bf->Q_host;
+  if (!chol_lower(D, LQ)) return set_error(ctx, SSM_ERR_NOT_PD, "Q is not positive definite");
+  const std::vector<double> LQi = tri_inverse(D, LQ);
+  double logdet = 0.0;
+  for (int q = 0; q < D; ++q) logdet += 2.0 * std::log(LQ[q + q * D]);
+  RefModelDev<T> rm;
+  std::memset(&rm, 0, sizeof rm);
+  for (int e = 0; e < D * D; ++e) rm.LQi[e] = (T)LQi[e];
+  rm.cst = (T)(-0.5 * ((double)D * 1.8378770664093454835606594728112 + logdet));
+  hipStream_t s = ctx->stream;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->xref, h.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ref_model, &rm, sizeof rm, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  return SSM_OK;
+}
+
+int ensure_csmc_buffers(ssm_bf* bf) {
+  if (bf->ref_model) return SSM_OK;
+  ssm_ctx* ctx = bf->ctx;
+  const size_t es = (bf->prec == SSM_F32) ? 4 : 8;
+  const bool ok =
+      hipMalloc(&bf->ref_model, (bf->prec == SSM_F32) ? sizeof(RefModelDev<float>)
+                                                      : sizeof(RefModelDev<double>)) ==
+          hipSuccess &&
+      hipMalloc(&bf->as_w, es * bf->Npad) == hipSuccess &&
+      hipMalloc(&bf->as_scratch, resample_scratch_bytes(bf->N)) == hipSuccess &&
+      hipMalloc(&bf->as_sc, sizeof(BfScalars)) == hipSuccess &&
+      hipMalloc(&bf->as_idx, sizeof(int32_t)) == hipSuccess;
+  if (!ok) return set_error(ctx, SSM_ERR_OOM, "conditional SMC buffers");
+  BfScalars h;
+  std::memset(&h, 0, sizeof h);
+  h.ess = (double)bf->N;
+  SSM_HIP_CHECK(ctx, hipMemcpy(bf->as_sc, &h, sizeof h, hipMemcpyHostToDevice));
+  SSM_HIP_CHECK(ctx, hipMemset(bf->as_idx, 0, sizeof(int32_t)));
+  return SSM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_ref,
+                                        int32_t mode) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!x_ref) {  // back to the unconditional filter
+    if (bf->ref_T >= 0) destroy_graphs(bf);
+    bf->ref_T = -1;
+    return SSM_OK;
+  }
+  if (T < 0) return set_error(ctx, SSM_ERR_ARG, "reference: T must be >= 0 (got %d)", T);
+  if (mode != SSM_CSMC_PLAIN && mode != SSM_CSMC_ANCESTOR)
+    return set_error(ctx, SSM_ERR_ARG, "reference: unknown refreshment %d", mode);
+  int rc = ensure_csmc_buffers(bf);
+  if (rc) return rc;
+  rc = (bf->prec == SSM_F32) ? upload_reference_t<float>(bf, T, x_ref)
+                             : upload_reference_t<double>(bf, T, x_ref);
+  if (rc) return rc;
+  if (bf->ref_T < 0 || bf->ref_mode != mode) destroy_graphs(bf);
+  bf->ref_T = T;
+  bf->ref_mode = mode;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_set_seed(ssm_bf* bf, uint64_t seed) {
+  if (!bf) return SSM_ERR_ARG;
+  if (seed != bf->seed) destroy_graphs(bf);  // captured launches hold the old key
+  bf->seed = seed;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_sample_index(ssm_bf* bf, double u, int32_t* index) {
+  if (!bf || !index) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_sample_index before initialise");
+  if (!(u >= 0.0 && u < 1.0)) return set_error(ctx, SSM_ERR_ARG, "sample_index: u outside [0, 1)");
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!bf->draw_idx && hipMalloc(&bf->draw_idx, sizeof(int32_t)) != hipSuccess)
+    return set_error(ctx, SSM_ERR_OOM, "sample_index");
+  ResampleArgs a;
+  a.kind = SSM_MULTINOMIAL;
+  a.n = bf->N;
+  a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
+  a.w = bf->logw;
+  a.seed = bf->seed;
+  a.sc = bf->sc;
+  resample_bind_scratch(a, bf->rs_scratch, bf->N);
+  hipStream_t s = ctx->stream;
+  int rc = launch_categorical_one(ctx, s, a, u, kStreamAncestor, bf->draw_idx);
+  if (rc) return rc;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(index, bf->draw_idx, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
   return SSM_OK;
 }

@@ -54,6 +54,21 @@ struct StepParams {
   uint64_t seed;
   const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
   double* __restrict__ partials;
+  // conditional SMC (the REF kernels): the reference trajectory, (T_ref + 1) x D row-major,
+  // particle 0's ancestor when the step resamples (null: particle 0), and the ancestor array
+  // particle 0's entry is written back to
+  const T* __restrict__ xref;
+  const int32_t* ref_anc;
+  int32_t* anc_w;
+};
+
+// Per-filter constants of the reference's transition density log N(x'; A x + b, Q) for
+// ancestor sampling: the inverse of Q's lower Cholesky factor (column-major, ld D) and
+// -1/2 (D log 2 pi + log det Q).
+template <typename T>
+struct RefModelDev {
+  T LQi[kMaxD * kMaxD];
+  T cst;
 };
 
 // Standard normals z[0..D) of particle i at step t from the given stream.
@@ -168,7 +183,11 @@ __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>
 // for all outstanding loads, which serialised the items' round trips.  A fresh step reads
 // the ancestors (used when it resampled) and gathers the states; a step that kept its
 // particles reads the states in place and the log-weights.
-template <typename T, int DT, int DYT, bool NOISE = false>
+//
+// REF (conditional SMC, GF/src/algorithms/particles.jl:59-66, resamplers.jl:51-65):
+// particle 0 takes the reference state x_ref[t] instead of a propagated one, and when the
+// step resamples its ancestor is *ref_anc (ancestor sampling) or particle 0.
+template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false>
 __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MD = DT ? DT : kMaxD;
   constexpr int MDY = DYT ? DYT : kMaxDy;
@@ -191,6 +210,14 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   T y[MDY];
 #pragma unroll
   for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
+  // the reference state at t and particle 0's ancestor (uniform loads)
+  T xr[MD];
+  int ref_a = 0;
+  if constexpr (REF) {
+#pragma unroll
+    for (int d = 0; d < MD; ++d) xr[d] = (d < D) ? p.xref[(size_t)t * D + d] : (T)0;
+    ref_a = p.ref_anc ? *p.ref_anc : 0;
+  }
 
   const int tile0 = tile * kTile;
 
@@ -218,6 +245,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
           if (c < D) s += mv.A(r, c) * xk[c];
 #pragma unroll
         for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
+        if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
         xn[r] = s;
         p.xout[(size_t)r * p.Npad + i] = s;
       }
@@ -266,6 +294,12 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
       else
         lw[k] = p.logw[i];
     }
+    if constexpr (FRESH && REF) {
+      if (tile0 + threadIdx.x == 0) {  // item 0 of thread 0 of tile 0: particle 0
+        a[0] = ref_a;
+        if (rs) p.anc_w[0] = ref_a;
+      }
+    }
     auto src_of = [&](int k) {
       const int i = tile0 + k * kStepThreads + threadIdx.x;
       if constexpr (FRESH)
@@ -312,6 +346,40 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     body(std::true_type{});
   else
     body(std::false_type{});
+}
+
+// Ancestor-sampling weight of particle i for the reference state x_ref (conditional SMC,
+// GF/src/ancestor_sampling.jl:64-130 with the generic future_conditional_density): its
+// normalised log-weight (0 for the lazy uniform weights) plus log N(x_ref; A x_i + b, Q).
+template <typename T, int DT>
+__device__ __forceinline__ void bf_as_weight_item(int N, int Npad, int D, const BfModelDev<T>* model,
+                                                  const RefModelDev<T>* rm, const T* x,
+                                                  const T* logw, const BfScalars* sc,
+                                                  const T* xref_t, T* w_out, int i) {
+  constexpr int MD = DT ? DT : kMaxD;
+  if (DT) D = DT;
+  if (i >= N) return;
+  const BfModelDev<T>& m = *model;
+  const T nl = sc->typeless ? (T)0 : logw[i] - (T)sc->lse;
+  T r[MD];
+#pragma unroll
+  for (int a = 0; a < MD; ++a) {
+    if (a < D) {
+      T s = m.b[a];
+      for (int c = 0; c < D; ++c) s += m.A[a + c * D] * x[(size_t)c * Npad + i];
+      r[a] = xref_t[a] - s;
+    }
+  }
+  T quad = 0;
+#pragma unroll
+  for (int a = 0; a < MD; ++a) {
+    if (a < D) {
+      T v = 0;
+      for (int c = 0; c <= a; ++c) v += rm->LQi[a + c * D] * r[c];
+      quad += v * v;
+    }
+  }
+  w_out[i] = nl + (rm->cst - (T)0.5 * quad);
 }
 
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).

@@ -14,6 +14,7 @@ constexpr uint32_t kStreamPropagate = 0u;
 constexpr uint32_t kStreamResample = 1u;
 constexpr uint32_t kStreamInit = 2u;
 constexpr uint32_t kStreamOuter = 3u;
+constexpr uint32_t kStreamAncestor = 4u;  // conditional SMC: the reference's ancestor draw
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter {c0..c3}, key {k0, k1}.
 struct U4 {
@@ -98,6 +99,13 @@ __device__ __forceinline__ void normals2_f64(uint64_t seed, uint32_t stream, uin
 __device__ __forceinline__ double resample_uniform(uint64_t seed, uint32_t t, uint32_t k) {
   const U4 w = philox4x32_10(k, t, kStreamResample, 0u, (uint32_t)seed,
                              (uint32_t)(seed >> 32));
+  return u53_closed0(w.x, w.y);
+}
+
+// A [0, 1) uniform from any stream, keyed like resample_uniform.
+__device__ __forceinline__ double stream_uniform(uint64_t seed, uint32_t stream, uint32_t t,
+                                                uint32_t k) {
+  const U4 w = philox4x32_10(k, t, stream, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
   return u53_closed0(w.x, w.y);
 }
 

@@ -155,6 +155,21 @@ extern "C" int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_histo
   return SSM_OK;
 }
 
+extern "C" int32_t ssm_bf_history_reset(ssm_history* h) {
+  if (!h) return SSM_ERR_ARG;
+  ssm_ctx* ctx = h->ctx;
+  if (!h->bf) return set_error(ctx, SSM_ERR_STATE, "particle history: its filter was destroyed");
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  BfView v;
+  const int rc = bf_device_view(h->bf, &v);
+  if (rc) return rc;
+  h->T = 0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(h->x0, v.x, plane_bytes(h), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return SSM_OK;
+}
+
 extern "C" int32_t ssm_bf_history_push(ssm_history* h) {
   if (!h) return SSM_ERR_ARG;
   ssm_ctx* ctx = h->ctx;

@@ -92,6 +92,11 @@ size_t resample_scratch_bytes(int n);
 void resample_bind_scratch(ResampleArgs& a, void* scratch, int n);
 // Enqueues the resampler kernels on `stream`.
 int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a);
+// One categorical draw from folded log-weights (a.weight_mode a log mode, the scratch of a
+// finish kernel bound in a): the first particle whose cumulative weight reaches u W_n;
+// u < 0 draws u from Philox stream `philox_stream` at the step in a.sc.  Writes *out.
+int launch_categorical_one(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a, double u,
+                           uint32_t philox_stream, int32_t* out);
 
 // ---- a bootstrap filter's device buffers, for the path storage (bf.hip) -------------
 struct BfView {

@@ -126,10 +126,8 @@ __global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
   multinomial_search_item(p, blockIdx.x * kRsThreads + threadIdx.x);
 }
 
-int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
-  if (a.n < 1) return set_error(ctx, SSM_ERR_ARG, "resample: n must be >= 1 (got %d)", a.n);
-  if (a.kind < SSM_SYSTEMATIC || a.kind > SSM_MULTINOMIAL)
-    return set_error(ctx, SSM_ERR_ARG, "resample: unknown resampler kind %d", a.kind);
+// The kernels' parameter block for one resample call.
+static RsParams rs_params_of(const ResampleArgs& a) {
   RsParams rp;
   rp.kind = a.kind;
   rp.n = a.n;
@@ -148,6 +146,35 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
   rp.prefix = a.prefix;
   rp.Wbuf = a.Wbuf;
   rp.counts_out = (a.kind == SSM_MULTINOMIAL) ? nullptr : a.counts_out;
+  return rp;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kRsThreads)
+    k_categorical_one(RsParams p, double u, uint32_t stream, int32_t* out) {
+  categorical_one_block<MODE>(p, u, stream, out);
+}
+
+int launch_categorical_one(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a, double u,
+                           uint32_t philox_stream, int32_t* out) {
+  const RsParams rp = rs_params_of(a);
+  if (a.weight_mode == kWeightsLogF32)
+    hipLaunchKernelGGL(k_categorical_one<kWeightsLogF32>, dim3(1), dim3(kRsThreads), 0, stream,
+                       rp, u, philox_stream, out);
+  else if (a.weight_mode == kWeightsLogF64)
+    hipLaunchKernelGGL(k_categorical_one<kWeightsLogF64>, dim3(1), dim3(kRsThreads), 0, stream,
+                       rp, u, philox_stream, out);
+  else
+    return set_error(ctx, SSM_ERR_ARG, "categorical draw: log-weight modes only");
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  return SSM_OK;
+}
+
+int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
+  if (a.n < 1) return set_error(ctx, SSM_ERR_ARG, "resample: n must be >= 1 (got %d)", a.n);
+  if (a.kind < SSM_SYSTEMATIC || a.kind > SSM_MULTINOMIAL)
+    return set_error(ctx, SSM_ERR_ARG, "resample: unknown resampler kind %d", a.kind);
+  const RsParams rp = rs_params_of(a);
   const int nb = (a.n + kTile - 1) / kTile;
   const dim3 g(nb), blk(kRsThreads);
   switch (a.weight_mode) {

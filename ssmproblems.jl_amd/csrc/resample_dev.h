@@ -210,32 +210,23 @@ __device__ __forceinline__ void load_tile_raw(const RsParams& p, int b,
   }
 }
 
-// Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  KIND/PLAIN as
-// offspring_of; PLAIN also means no debug weights and no offspring counts.  Thread t owns
-// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial scan
-// plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by its own
-// thread (neighbouring threads write neighbouring ranges); a thread whose range exceeds
-// kDirectSlots hands it to the whole block, which keeps degenerate weights (one particle
-// taking every slot) balanced.  For MULTINOMIAL it writes the nondecreasing prefix sums W to
-// Wbuf instead.  All threads of the block call it; it returns at once when !in.rs.
-template <int MODE, int KIND = -1, bool PLAIN = false>
-__device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
-                                                       const RawW<MODE> raw[kTileItems],
-                                                       const TileIn& in) {
-  if (!in.rs) return;
-  const int kind = KIND >= 0 ? KIND : p.kind;
-  double* const w_debug64 = PLAIN ? nullptr : p.w_debug64;
-  int32_t* const counts_out = PLAIN ? nullptr : p.counts_out;
-  const double* const uvec = PLAIN ? nullptr : p.u;
+// The cumulative weights of tile b's particles, W[k] for position kTileItems t + k: the
+// tile's prefix P plus the in-tile sums (a thread-serial scan of kTileItems consecutive
+// weights, then one block scan of the thread sums), made nondecreasing by an exact running
+// max.  Positions past n add zero.  Also returns the tile's prefix, the next tile's prefix
+// and the total (for the lazy TypelessZero weights: tile0/n, the next tile's start over n
+// or 1, and 1).  Optionally records each consumed FP64 weight in w_debug64.
+template <int MODE>
+__device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
+                                                 const RawW<MODE> raw[kTileItems],
+                                                 const TileIn& in, double* w_debug64,
+                                                 double W[kTileItems], double* P_out,
+                                                 double* Pnext_out, double* Wn_out) {
   __shared__ double lds[kRsThreads / kWave];
-  __shared__ int olast[kRsThreads];
-  __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
-  __shared__ int nheavy;
   const int n = p.n;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
   const int t = threadIdx.x;
-  const int cnt = min(kTile, n - tile0);
   double P, Pnext, Wn;
   if (in.typeless) {
     P = (double)tile0 / (double)n;
@@ -246,9 +237,6 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
     Pnext = in.Pnext;
     Wn = in.Wn;
   }
-  if (t == 0) nheavy = 0;
-  // thread-serial scan of 4 consecutive weights, then one block scan of the thread sums
-  double W[kTileItems];
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
@@ -271,6 +259,39 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   const double carry = block_excl_max_d(W[kTileItems - 1], lds);
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
+  *P_out = P;
+  *Pnext_out = Pnext;
+  *Wn_out = Wn;
+}
+
+// Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  KIND/PLAIN as
+// offspring_of; PLAIN also means no debug weights and no offspring counts.  Thread t owns
+// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial scan
+// plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by its own
+// thread (neighbouring threads write neighbouring ranges); a thread whose range exceeds
+// kDirectSlots hands it to the whole block, which keeps degenerate weights (one particle
+// taking every slot) balanced.  For MULTINOMIAL it writes the nondecreasing prefix sums W to
+// Wbuf instead.  All threads of the block call it; it returns at once when !in.rs.
+template <int MODE, int KIND = -1, bool PLAIN = false>
+__device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
+                                                       const RawW<MODE> raw[kTileItems],
+                                                       const TileIn& in) {
+  if (!in.rs) return;
+  const int kind = KIND >= 0 ? KIND : p.kind;
+  double* const w_debug64 = PLAIN ? nullptr : p.w_debug64;
+  int32_t* const counts_out = PLAIN ? nullptr : p.counts_out;
+  const double* const uvec = PLAIN ? nullptr : p.u;
+  __shared__ int olast[kRsThreads];
+  __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
+  __shared__ int nheavy;
+  const int n = p.n;
+  const int nb = (n + kTile - 1) / kTile;
+  const int tile0 = b * kTile;
+  const int t = threadIdx.x;
+  const int cnt = min(kTile, n - tile0);
+  double W[kTileItems], P, Pnext, Wn;
+  if (t == 0) nheavy = 0;
+  tile_cum_weights<MODE>(p, b, raw, in, w_debug64, W, &P, &Pnext, &Wn);
   const uint32_t step = in.step;
   if (kind == SSM_MULTINOMIAL) {
 #pragma unroll
@@ -373,6 +394,84 @@ __device__ __forceinline__ void multinomial_search_item(const RsParams& p, int k
       hi = mid;
   }
   p.anc[k] = min(lo, p.n - 1);
+}
+
+}  // namespace ssm
+
+namespace ssm {
+
+// One categorical draw from the weights a finish kernel has folded (StatsBase.sample with
+// Weights, as the multinomial rule of CUDAExt.jl:31-39 for a single uniform): the first
+// particle j whose cumulative weight W_j reaches u W_n, with the same cumulative weights the
+// multinomial resampler forms (per-tile prefixes from the finish, the in-tile scan of
+// tile_cum_weights, the tile's last particle closing at the next prefix).  One block of
+// kRsThreads threads; *out gets the index.  The uniform is u, or (u < 0) Philox draw k = 0
+// of `stream` at the filter's current step.
+template <int MODE>
+__device__ __forceinline__ void categorical_one_block(const RsParams& p, double u,
+                                                      uint32_t stream, int32_t* out) {
+  __shared__ int cnt_lds[kRsThreads / kWave];
+  __shared__ int tile_sel;
+  const int n = p.n, nb = (n + kTile - 1) / kTile, t = threadIdx.x;
+  constexpr bool kLog = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64);
+  const bool typeless = kLog && p.sc->typeless;
+  const uint32_t step = rs_step(p);
+  const double uu = (u >= 0.0) ? u : stream_uniform(p.seed, stream, step, 0u);
+  const double Wn = typeless ? 1.0 : p.prefix[nb];
+  const double q = uu * Wn;
+  // the tile: the number of tiles whose closing prefix is below q (tile b closes at
+  // prefix[b + 1]; the last tile at Wn), counted over all tiles in parallel
+  int below = 0;
+  for (int b = t; b < nb; b += kRsThreads) {
+    const double close = (b == nb - 1)
+                             ? Wn
+                             : (typeless ? (double)((b + 1) * kTile) / (double)n : p.prefix[b + 1]);
+    below += (close < q) ? 1 : 0;
+  }
+  below = (int)wave_sum_dpp((float)below);  // at most nb < 2^24: exact
+  const int w = t / kWave, lane = t & (kWave - 1);
+  if (lane == 0) cnt_lds[w] = below;
+  __syncthreads();
+  if (t == 0) {
+    int s = 0;
+    for (int k = 0; k < kRsThreads / kWave; ++k) s += cnt_lds[k];
+    tile_sel = min(s, nb - 1);
+  }
+  __syncthreads();
+  const int b = tile_sel;
+  RawW<MODE> raw[kTileItems];
+  load_tile_raw<MODE>(p, b, raw);
+  TileIn in;
+  in.rs = true;
+  in.typeless = typeless;
+  in.P = p.prefix[b];
+  in.Pnext = p.prefix[b + 1];
+  in.Wn = Wn;
+  in.m_b = kLog ? p.blk_m[b] : 0.0;
+  in.c_b = kLog ? p.blk_c[b] : 0.0;
+  in.step = step;
+  double W[kTileItems], P, Pnext, Wt;
+  tile_cum_weights<MODE>(p, b, raw, in, nullptr, W, &P, &Pnext, &Wt);
+  const int tile0 = b * kTile, cnt = min(kTile, n - tile0);
+  // the first position of the tile whose cumulative weight reaches q (the tile's last
+  // particle closes at Pnext, as in the multinomial resampler's W array)
+  int first = kTile;
+#pragma unroll
+  for (int k = kTileItems - 1; k >= 0; --k) {
+    const int pos = kTileItems * t + k;
+    const double Wk = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
+    if (pos < cnt && Wk >= q) first = pos;
+  }
+  int m = first;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) m = min(m, __shfl_xor(m, off, kWave));
+  if (lane == 0) cnt_lds[w] = m;
+  __syncthreads();
+  if (t == 0) {
+    int s = kTile;
+    for (int k = 0; k < kRsThreads / kWave; ++k) s = min(s, cnt_lds[k]);
+    *out = tile0 + min(s, cnt - 1);
+  }
 }
 
 }  // namespace ssm

@@ -82,6 +82,24 @@ class DeviceParticles:
         self.ctx.check(self.ctx.lib.ssm_bf_import(self.ptr, x.ctypes.data_as(_lib._dp),
                                                   lw.ctypes.data_as(_lib._dp)))
 
+    def set_reference(self, x_ref, mode=_lib.SSM_CSMC_PLAIN):
+        """Pins particle 0 to the reference trajectory x_ref ((T + 1) x Dx, row t = time t)
+        for conditional SMC; None returns to the unconditional filter."""
+        if x_ref is None:
+            self.ctx.check(self.ctx.lib.ssm_bf_set_reference(self.ptr, 0, None, 0))
+            self.ref_T = None
+            return
+        xr = np.ascontiguousarray(np.asarray(x_ref, dtype=np.float64).reshape(-1, self.Dx))
+        self.ctx.check(self.ctx.lib.ssm_bf_set_reference(self.ptr, xr.shape[0] - 1,
+                                                         xr.ctypes.data_as(_lib._dp), int(mode)))
+        self.ref_T = xr.shape[0] - 1
+
+    def sample_index(self, u):
+        """0-based index of the first particle whose cumulative normalised weight reaches u."""
+        idx = C.c_int32()
+        self.ctx.check(self.ctx.lib.ssm_bf_sample_index(self.ptr, float(u), C.byref(idx)))
+        return idx.value
+
     @property
     def particles(self):
         return self.export()[0]

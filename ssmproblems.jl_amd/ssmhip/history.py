@@ -29,6 +29,10 @@ class DenseParticleContainer:
     def push(self):
         self.ctx.check(self.ctx.lib.ssm_bf_history_push(self.ptr))
 
+    def reset(self):
+        """Empties the container and stores the filter's current particles as time 0."""
+        self.ctx.check(self.ctx.lib.ssm_bf_history_reset(self.ptr))
+
     def __len__(self):
         T = C.c_int32()
         self.ctx.check(self.ctx.lib.ssm_bf_history_info(self.ptr, C.byref(T), None))
