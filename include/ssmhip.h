@@ -384,6 +384,26 @@ int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms);
  */
 int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* nw_out, double* ms_out);
 int32_t ssm_kf_batch_destroy(ssm_kf_batch* kb);
+/*
+ * kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter) for B filters of one
+ * shape (GF/src/integrations/logdensity.jl:149-161): log p(y_1:T) of each, the Kalman
+ * recursion with the Joseph-form update and, when jitter > 0, jitter I added to every
+ * filtered covariance (kalman.jl:61-66; 0 is the reference's `nothing`).  Host arrays,
+ * column-major FP64 matrices, filter-major:
+ *   mu0 (B x Dx), Sigma0 (B x Dx x Dx), ys (B x T x Dy, row t = observation t + 1);
+ *   A, Q (P x Dx x Dx), b (P x Dx), H (P x Dy x Dx), c (P x Dy), R (P x Dy x Dy) with
+ *   P = B T blocks when time_varying = 1 (block f T + t is step t + 1 of filter f, the
+ *   reference's As[t + 1] ...) or P = B when time_varying = 0 (one block per filter for
+ *   every step).
+ * ll_out (B) receives each log-likelihood (NaN where S lost positive definiteness; its
+ * status_out entry, optional, is then SSM_ERR_NOT_PD and the call returns it).
+ * 1 <= Dx <= 64, 1 <= Dy <= 32.  The inputs are staged in a device block the context keeps.
+ */
+int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                             int32_t time_varying, const double* mu0, const double* Sigma0,
+                             const double* A, const double* b, const double* Q, const double* H,
+                             const double* c, const double* R, const double* ys, double jitter,
+                             double* ll_out, int32_t* status_out);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,10 @@ Parity pinning of each oracle function (what checks it beyond retyping its formu
 * ``kf_filter`` / ``kf_update`` (kalman.jl:42-79, kalman_gradient.jl:12-29): the closed-form
   joint Gaussian (``joint_gaussian_loglik``, restating GFTest/models/linear_gaussian.jl:171-215,
   as test/algorithms/kalman.jl:5-43 does) and the MVN conditional for T=1.
+* ``kf_loglikelihood`` (logdensity.jl:149-161): the joint Gaussian with per-step parameters;
+  with jitter j, the jitter-free joint Gaussian whose Q_t (t >= 2) is Q_t + j A_t A_t'
+  (A (S + j I) A' + Q = A S A' + (Q + j A A')); constant parameters equal ``kf_filter``
+  (the reference's kf_loglikelihood == filter(KF) test, logdensity.jl:93-132).
 * ``bf_step`` (particles.jl:36-162, containers.jl:140-173): special cases (H = 0 makes every
   particle's weight equal, so ll_t is the exact N(y; c, R) density; Q -> 0 makes propagation
   deterministic; uniform weights give ESS = N and an identity systematic resample; the baseline
@@ -412,26 +416,49 @@ def simulate_lg(model: LG, T, rng):
 
 def joint_gaussian(model: LG, T):
     """Joint Gaussian of Z = [X0..XT, Y1..YT]; restates GFTest/models/linear_gaussian.jl:171-215."""
-    Dx, Dy = model.Dx, model.Dy
+    rep = lambda M: [M] * T
+    return joint_gaussian_tv(model.mu0, model.Sigma0, rep(model.A), rep(model.b), rep(model.Q),
+                             rep(model.H), rep(model.c), rep(model.R))
+
+
+def joint_gaussian_tv(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs):
+    """The same joint Gaussian with step-t parameters As[t-1] ... (t = 1..T)."""
+    T = len(As)
+    Dx, Dy = np.asarray(As[0]).shape[0], np.atleast_2d(Hs[0]).shape[0]
     n = Dx + T * (Dx + Dy)
     P = np.zeros((n, n))
     mu_e = np.zeros(n)
     S_e = np.zeros((n, n))
-    mu_e[:Dx] = model.mu0
-    S_e[:Dx, :Dx] = model.Sigma0
+    mu_e[:Dx] = mu0
+    S_e[:Dx, :Dx] = Sigma0
     for t in range(1, T + 1):
         iA, jA = t * Dx, (t - 1) * Dx
-        P[iA:iA + Dx, jA:jA + Dx] = model.A
+        P[iA:iA + Dx, jA:jA + Dx] = As[t - 1]
         iH, jH = Dx * (T + 1) + (t - 1) * Dy, Dx * t
-        P[iH:iH + Dy, jH:jH + Dx] = model.H
-        mu_e[iA:iA + Dx] = model.b
-        mu_e[iH:iH + Dy] = model.c
-        S_e[iA:iA + Dx, iA:iA + Dx] = model.Q
-        S_e[iH:iH + Dy, iH:iH + Dy] = model.R
+        P[iH:iH + Dy, jH:jH + Dx] = Hs[t - 1]
+        mu_e[iA:iA + Dx] = bs[t - 1]
+        mu_e[iH:iH + Dy] = cs[t - 1]
+        S_e[iA:iA + Dx, iA:iA + Dx] = Qs[t - 1]
+        S_e[iH:iH + Dy, iH:iH + Dy] = Rs[t - 1]
     IP = np.eye(n) - P
     mu_Z = np.linalg.solve(IP, mu_e)
     Sig_Z = np.linalg.solve(IP, np.linalg.solve(IP, S_e).T).T
     return mu_Z, Sig_Z
+
+
+def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=0.0):
+    """kf_loglikelihood (GF/src/integrations/logdensity.jl:149-161): kalman_predict then the
+    Joseph-form update with jitter, step parameters As[t] ..., summed increments."""
+    mu = _f64(mu0)
+    S = np.asarray(Sigma0, dtype=np.float64)
+    ys = np.asarray(ys, dtype=np.float64).reshape(len(As), -1)
+    Dx, Dy = mu.shape[0], ys.shape[1]
+    ll = 0.0
+    for t in range(len(As)):
+        mu, S, inc = kf_step(Dx, Dy, mu, S, As[t], bs[t], Qs[t], Hs[t], cs[t], Rs[t], ys[t],
+                             jitter)
+        ll += inc
+    return ll
 
 
 def mvn_logpdf(y, mu, S):

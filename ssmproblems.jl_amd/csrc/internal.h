@@ -22,6 +22,9 @@ struct ssm_ctx {
   double* rs_u = nullptr;
   int32_t* rs_a = nullptr;
   size_t rs_host_cap = 0;
+  // Lazily grown device copy of the inputs of ssm_kf_loglikelihood.
+  void* kfll = nullptr;
+  size_t kfll_bytes = 0;
 };
 
 namespace ssm {

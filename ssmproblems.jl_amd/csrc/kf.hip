@@ -51,6 +51,20 @@ struct KfBatchParams {
   double* __restrict__ Sigma_out;
   const int64_t* __restrict__ mu_off;
   const int64_t* __restrict__ cov_off;
+  // added to the diagonal of every filtered covariance (KalmanFilter's jitter; 0 = none)
+  double jitter = 0.0;
+  // kf_loglikelihood's layout (the TV kernels; uniform Dx, Dy): filter f starts from
+  // mu0 + f Dx and Sigma0 + f Dx^2, and step t (0-based) uses parameter block
+  // f * blocks + (blocks > 1 ? t : 0) of each array (column-major matrices)
+  const double* __restrict__ tv_mu0 = nullptr;
+  const double* __restrict__ tv_Sigma0 = nullptr;
+  const double* __restrict__ tv_A = nullptr;
+  const double* __restrict__ tv_b = nullptr;
+  const double* __restrict__ tv_Q = nullptr;
+  const double* __restrict__ tv_H = nullptr;
+  const double* __restrict__ tv_c = nullptr;
+  const double* __restrict__ tv_R = nullptr;
+  int tv_blocks = 1;
 };
 
 constexpr double kLog2Pi = 1.8378770664093454835606594728112;
@@ -67,7 +81,35 @@ struct KfLds {
       ((size_t)kFloats * 4 + 7) / 8 * 8 + ((size_t)DYP * (DYP + 1) + 3 * DYP) * 8;
 };
 
-template <int DXP, int DYP, int NW>
+// Loads one parameter block (A, b, Q, H, c, R, column-major FP64) into the zero-padded
+// row-major FP32 LDS matrices.
+__device__ __forceinline__ void kf_load_block(int dx, int dy, int dxp, int dyp, int LX, int LY,
+                                              const double* g_A, const double* g_b,
+                                              const double* g_Q, const double* g_H,
+                                              const double* g_c, const double* g_R, float* Am,
+                                              float* Qm, float* Hm, float* Rm, float* bv,
+                                              float* cv, int tid, int nthr) {
+  for (int e = tid; e < dxp * dxp; e += nthr) {
+    const int i = e / dxp, j = e % dxp;
+    const bool in = i < dx && j < dx;
+    Am[i * LX + j] = in ? (float)g_A[i + j * dx] : 0.f;
+    Qm[i * LX + j] = in ? (float)g_Q[i + j * dx] : 0.f;
+  }
+  for (int e = tid; e < dyp * dxp; e += nthr) {
+    const int i = e / dxp, j = e % dxp;
+    Hm[i * LX + j] = (i < dy && j < dx) ? (float)g_H[i + j * dy] : 0.f;
+  }
+  for (int e = tid; e < dyp * dyp; e += nthr) {
+    const int i = e / dyp, j = e % dyp;
+    Rm[i * LY + j] = (i < dy && j < dy) ? (float)g_R[i + j * dy] : 0.f;
+  }
+  for (int e = tid; e < dxp; e += nthr) bv[e] = e < dx ? (float)g_b[e] : 0.f;
+  for (int e = tid; e < dyp; e += nthr) cv[e] = e < dy ? (float)g_c[e] : 0.f;
+}
+
+// TV: the kf_loglikelihood layout (separate arrays, one parameter block per step or one
+// for all steps); otherwise each filter's packed homogeneous parameters.
+template <int DXP, int DYP, int NW, bool TV>
 __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
   using L = KfLds<DXP, DYP>;
   constexpr int LX = L::LX, LY = L::LY;
@@ -105,41 +147,64 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
     const int dx = p.Dx[f], dy = p.Dy[f];
     const int dxp = (dx + 15) & ~15, dyp = (dy + 15) & ~15;
     // ---- load parameters (column-major FP64) into zero-padded row-major FP32 LDS ----
-    const double* g = p.params + p.param_off[f];
-    const double* g_mu0 = g;
-    const double* g_S0 = g_mu0 + dx;
-    const double* g_A = g_S0 + dx * dx;
-    const double* g_b = g_A + dx * dx;
-    const double* g_Q = g_b + dx;
-    const double* g_H = g_Q + dx * dx;
-    const double* g_c = g_H + dy * dx;
-    const double* g_R = g_c + dy;
+    const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
+    const double *g_mu0, *g_S0;
+    // block b of the TV arrays, or the packed block
+    auto block = [&](int64_t b, const double** gA, const double** gb, const double** gQ,
+                     const double** gH, const double** gc, const double** gR) {
+      if constexpr (TV) {
+        *gA = p.tv_A + b * nxx;
+        *gb = p.tv_b + b * dx;
+        *gQ = p.tv_Q + b * nxx;
+        *gH = p.tv_H + b * nyx;
+        *gc = p.tv_c + b * dy;
+        *gR = p.tv_R + b * nyy;
+      } else {
+        (void)b;
+        const double* g = p.params + p.param_off[f] + dx + nxx;
+        *gA = g;
+        *gb = g + nxx;
+        *gQ = g + nxx + dx;
+        *gH = g + 2 * nxx + dx;
+        *gc = g + 2 * nxx + dx + nyx;
+        *gR = g + 2 * nxx + dx + nyx + dy;
+      }
+    };
+    if constexpr (TV) {
+      g_mu0 = p.tv_mu0 + (int64_t)f * dx;
+      g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
+    } else {
+      g_mu0 = p.params + p.param_off[f];
+      g_S0 = g_mu0 + dx;
+    }
+    const int64_t blk0 = TV ? (int64_t)f * p.tv_blocks : 0;
+    {
+      const double *gA, *gb, *gQ, *gH, *gc, *gR;
+      block(blk0, &gA, &gb, &gQ, &gH, &gc, &gR);
+      kf_load_block(dx, dy, dxp, dyp, LX, LY, gA, gb, gQ, gH, gc, gR, Am, Qm, Hm, Rm, bv, cv,
+                    tid, nthr);
+    }
     for (int e = tid; e < dxp * dxp; e += nthr) {
       const int i = e / dxp, j = e % dxp;
-      const bool in = i < dx && j < dx;
-      Sig[i * LX + j] = in ? (float)g_S0[i + j * dx] : 0.f;
-      Am[i * LX + j] = in ? (float)g_A[i + j * dx] : 0.f;
-      Qm[i * LX + j] = in ? (float)g_Q[i + j * dx] : 0.f;
+      Sig[i * LX + j] = (i < dx && j < dx) ? (float)g_S0[i + j * dx] : 0.f;
     }
-    for (int e = tid; e < dyp * dxp; e += nthr) {
-      const int i = e / dxp, j = e % dxp;
-      Hm[i * LX + j] = (i < dy && j < dx) ? (float)g_H[i + j * dy] : 0.f;
-    }
-    for (int e = tid; e < dyp * dyp; e += nthr) {
-      const int i = e / dyp, j = e % dyp;
-      Rm[i * LY + j] = (i < dy && j < dy) ? (float)g_R[i + j * dy] : 0.f;
-    }
-    for (int e = tid; e < dxp; e += nthr) {
-      mu[e] = e < dx ? (float)g_mu0[e] : 0.f;
-      bv[e] = e < dx ? (float)g_b[e] : 0.f;
-    }
-    for (int e = tid; e < dyp; e += nthr) cv[e] = e < dy ? (float)g_c[e] : 0.f;
+    for (int e = tid; e < dxp; e += nthr) mu[e] = e < dx ? (float)g_mu0[e] : 0.f;
     __syncthreads();
+    const float jit = (float)p.jitter;
     double ll = 0.0;
     int ok = 1;
     const double* yrow = p.ys + p.y_off[f];
     const int Tn = p.Tf ? p.Tf[f] : p.T;
     for (int t = 0; t < Tn; ++t) {
+      if constexpr (TV) {
+        if (p.tv_blocks > 1 && t > 0) {  // this step's parameters (the last step ended synced)
+          const double *gA, *gb, *gQ, *gH, *gc, *gR;
+          block(blk0 + t, &gA, &gb, &gQ, &gH, &gc, &gR);
+          kf_load_block(dx, dy, dxp, dyp, LX, LY, gA, gb, gQ, gH, gc, gR, Am, Qm, Hm, Rm, bv,
+                        cv, tid, nthr);
+          __syncthreads();
+        }
+      }
       // P1: mu_hat = A mu + b ; T1 = A Sigma
       // matrix-vector products run over the zero padding up to dxp (exact zeros), 16 terms
       // per unrolled block
@@ -228,10 +293,11 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
       gemm16<NW>(Sig, LX, dxp, dxp, 1.f, T1, LX, false, Mm, LX, true, dxp, KR, LY, false, KT,
                  LX, false, dyp, nullptr, 0, 0);
       __syncthreads();
-      // P9: Symmetric(Sigma) keeps the upper triangle
+      // P9: Symmetric(Sigma + jitter I) keeps the upper triangle (kalman.jl:61-66)
       for (int e = tid; e < DXP * DXP; e += nthr) {
         const int i = e / DXP, j = e % DXP;
         if (i > j && i < dxp) Sig[i * LX + j] = Sig[j * LX + i];  // j < i < dxp
+        if (i == j && i < dx) Sig[i * LX + i] += jit;
       }
       __syncthreads();
     }
@@ -270,7 +336,7 @@ __global__ void k_zero_u32(uint32_t* p, int n) {
 template <int DXP, int DYP, int NW>
 int launch_class(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int grid_cap) {
   const size_t lds = KfLds<DXP, DYP>::kBytes;
-  auto fn = k_kf_batch<DXP, DYP, NW>;
+  auto fn = p.tv_A ? k_kf_batch<DXP, DYP, NW, true> : k_kf_batch<DXP, DYP, NW, false>;
   SSM_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)fn,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = 0;
@@ -623,6 +689,117 @@ extern "C" int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* 
   (void)hipFree(dtf);
   if (rc == SSM_ERR_HIP) return set_error(ctx, rc, "ssm_kf_batch_tune: HIP failure");
   return rc;
+}
+
+// kf_loglikelihood (GF/src/integrations/logdensity.jl:149-161) for B filters of one shape:
+// the inputs are copied into one device block (grown on the context as needed) and run by
+// the size class's persistent kernel in the TV form.
+extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy,
+                                        int32_t T, int32_t time_varying, const double* mu0,
+                                        const double* Sigma0, const double* A, const double* b,
+                                        const double* Q, const double* H, const double* c,
+                                        const double* R, const double* ys, double jitter,
+                                        double* ll_out, int32_t* status_out) {
+  if (!ctx) return SSM_ERR_ARG;
+  if (B < 1 || T < 1 || !mu0 || !Sigma0 || !A || !b || !Q || !H || !c || !R || !ys || !ll_out)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: bad arguments");
+  if (time_varying != 0 && time_varying != 1)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: time_varying must be 0 or 1");
+  if (!(jitter >= 0.0))
+    return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: jitter must be >= 0");
+  int cls = -1;
+  for (int k = 0; k < kClasses && cls < 0; ++k)
+    if (Dx >= 1 && Dy >= 1 && Dx <= kCfg[k].dx_max && Dy <= kCfg[k].dy_max) cls = k;
+  if (cls < 0)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: dims Dx=%d Dy=%d outside 1..64 / 1..32",
+                     Dx, Dy);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const int64_t nb = time_varying ? (int64_t)B * T : (int64_t)B;  // parameter blocks
+  const int64_t nxx = (int64_t)Dx * Dx, nyx = (int64_t)Dy * Dx, nyy = (int64_t)Dy * Dy;
+  const size_t n_mu0 = (size_t)B * Dx, n_S0 = (size_t)B * nxx, n_A = (size_t)(nb * nxx),
+               n_b = (size_t)(nb * Dx), n_H = (size_t)(nb * nyx), n_c = (size_t)(nb * Dy),
+               n_R = (size_t)(nb * nyy), n_y = (size_t)B * T * Dy;
+  const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B;
+  const size_t bytes = 8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256;
+  int rc = ensure_device(ctx, &ctx->kfll, &ctx->kfll_bytes, bytes);
+  if (rc) return rc;
+  double* d = (double*)ctx->kfll;
+  double* d_mu0 = d;
+  double* d_S0 = d_mu0 + n_mu0;
+  double* d_A = d_S0 + n_S0;
+  double* d_b = d_A + n_A;
+  double* d_Q = d_b + n_b;
+  double* d_H = d_Q + n_A;
+  double* d_c = d_H + n_H;
+  double* d_R = d_c + n_c;
+  double* d_y = d_R + n_R;
+  double* d_ll = d_y + n_y;
+  int64_t* d_yoff = (int64_t*)(d_ll + B);
+  int32_t* d_Dx = (int32_t*)(d_yoff + B);
+  int32_t* d_Dy = d_Dx + B;
+  int32_t* d_order = d_Dy + B;
+  int32_t* d_status = d_order + B;
+  uint32_t* d_counter = (uint32_t*)(d_status + B);
+  std::vector<int64_t> yoff(B);
+  std::vector<int32_t> dims(3 * (size_t)B);
+  for (int f = 0; f < B; ++f) {
+    yoff[f] = (int64_t)f * T * Dy;
+    dims[f] = Dx;
+    dims[B + f] = Dy;
+    dims[2 * (size_t)B + f] = f;
+  }
+  hipStream_t s = ctx->stream;
+  struct Up {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } ups[] = {{d_mu0, mu0, 8 * n_mu0}, {d_S0, Sigma0, 8 * n_S0}, {d_A, A, 8 * n_A},
+             {d_b, b, 8 * n_b},       {d_Q, Q, 8 * n_A},         {d_H, H, 8 * n_H},
+             {d_c, c, 8 * n_c},       {d_R, R, 8 * n_R},         {d_y, ys, 8 * n_y},
+             {d_yoff, yoff.data(), 8 * (size_t)B}, {d_Dx, dims.data(), 12 * (size_t)B}};
+  for (const Up& u : ups)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counter, 16);
+  KfBatchParams p;
+  p.T = T;
+  p.Tf = nullptr;
+  p.count = B;
+  p.order = d_order;
+  p.counter = d_counter;
+  p.Dx = d_Dx;
+  p.Dy = d_Dy;
+  p.params = nullptr;
+  p.param_off = nullptr;
+  p.ys = d_y;
+  p.y_off = d_yoff;
+  p.ll_out = d_ll;
+  p.status = d_status;
+  p.mu_out = nullptr;
+  p.Sigma_out = nullptr;
+  p.mu_off = nullptr;
+  p.cov_off = nullptr;
+  p.jitter = jitter;
+  p.tv_mu0 = d_mu0;
+  p.tv_Sigma0 = d_S0;
+  p.tv_A = d_A;
+  p.tv_b = d_b;
+  p.tv_Q = d_Q;
+  p.tv_H = d_H;
+  p.tv_c = d_c;
+  p.tv_R = d_R;
+  p.tv_blocks = time_varying ? T : 1;
+  rc = launch_class_nw(ctx, s, p, cls, class_nw(cls));
+  if (rc) return rc;
+  std::vector<int32_t> st(B);
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_out, d_ll, 8 * (size_t)B, hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(st.data(), d_status, 4 * (size_t)B, hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  if (status_out) std::memcpy(status_out, st.data(), 4 * (size_t)B);
+  for (int f = 0; f < B; ++f)
+    if (st[f] != SSM_OK)
+      return set_error(ctx, SSM_ERR_NOT_PD,
+                       "filter %d: innovation covariance S is not positive definite", f);
+  return SSM_OK;
 }
 
 extern "C" int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms) {

@@ -11,7 +11,8 @@ from .models import (StateSpaceModel, HomogeneousGaussianPrior,
                      create_homogeneous_linear_gaussian_model, PackedLG, HierarchicalSSM)
 from .resamplers import (Systematic, Stratified, Multinomial, ESSResampler, sample_ancestors,
                          offspring_to_ancestors, ancestors_to_offspring)
-from .kalman import KalmanFilter, KF, KFBatch, MvNormalState, pack_lg
+from .kalman import (KalmanFilter, KF, KFBatch, MvNormalState, pack_lg, kf_loglikelihood,
+                     kf_loglikelihood_batch)
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, run, filter,
                       rng_to_seed, last_timing)

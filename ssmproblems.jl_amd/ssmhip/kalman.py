@@ -19,8 +19,8 @@ from .models import StateSpaceModel, lg_params
 
 @dataclass
 class KalmanFilter:
-    """KalmanFilter(; jitter=nothing) -- kalman.jl:16-21 (jitter is not supported on the
-    device path; the reference default is nothing)."""
+    """KalmanFilter(; jitter=nothing) -- kalman.jl:16-21 (``filter`` runs without jitter,
+    the reference default; ``kf_loglikelihood`` takes one)."""
     jitter: float = None
 
 
@@ -156,3 +156,65 @@ def kalman_filter(model: StateSpaceModel, observations, ctx=None):
     ll, mus, sigs = kb.run(return_state=True)
     kb.close()
     return MvNormalState(mus[0], sigs[0]), float(ll[0])
+
+
+def _colmajor_blocks(M, r, c_, lead):
+    """(lead..., r, c) row-major blocks -> flat FP64, each block column-major."""
+    M = np.asarray(M, dtype=np.float64).reshape(*lead, r, c_)
+    return np.ascontiguousarray(np.swapaxes(M, -1, -2)).reshape(-1)
+
+
+def kf_loglikelihood_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=None,
+                           return_status=False):
+    """kf_loglikelihood for B filters of one shape in one device call
+    (GF/src/integrations/logdensity.jl:149-161).
+
+    mu0 (B, Dx), Sigma0 (B, Dx, Dx), ys (B, T, Dy).  The step parameters are either per step,
+    A (B, T, Dx, Dx), b (B, T, Dx), Q (B, T, Dx, Dx), H (B, T, Dy, Dx), c (B, T, Dy),
+    R (B, T, Dy, Dy) -- As[t] of the reference -- or one set per filter without the T axis;
+    a mix is broadcast along T.  Returns the B log-likelihoods (and the status codes)."""
+    ctx = ctx or default_context()
+    mu0 = np.asarray(mu0, dtype=np.float64)
+    B, Dx = mu0.shape
+    ys = np.asarray(ys, dtype=np.float64)
+    T = ys.shape[1]
+    Dy = ys.reshape(B, T, -1).shape[2]
+    shapes = {"A": (Dx, Dx), "b": (Dx,), "Q": (Dx, Dx), "H": (Dy, Dx), "c": (Dy,), "R": (Dy, Dy)}
+    arrs = {k: np.asarray(v, dtype=np.float64) for k, v in
+            dict(A=A, b=b, Q=Q, H=H, c=c, R=R).items()}
+    tv = any(arrs[k].ndim == len(shapes[k]) + 2 for k in shapes)
+    flat = {}
+    for k, shp in shapes.items():
+        a = arrs[k]
+        if tv and a.ndim == len(shp) + 1:
+            a = np.broadcast_to(a[:, None], (B, T) + shp)
+        lead = (B, T) if tv else (B,)
+        if a.shape != lead + shp:
+            raise ValueError(f"{k}: shape {a.shape}, expected {lead + shp}")
+        flat[k] = (_colmajor_blocks(a, *shp, lead) if len(shp) == 2
+                   else np.ascontiguousarray(a.reshape(-1)))
+    S0 = _colmajor_blocks(Sigma0, Dx, Dx, (B,))
+    yv = np.ascontiguousarray(ys.reshape(-1))
+    mu0c = np.ascontiguousarray(mu0.reshape(-1))
+    ll = np.zeros(B)
+    st = np.zeros(B, dtype=np.int32)
+    rc = ctx.lib.ssm_kf_loglikelihood(
+        ctx.ptr, B, Dx, Dy, T, int(tv), mu0c.ctypes.data_as(_lib._dp), S0.ctypes.data_as(_lib._dp),
+        *(flat[k].ctypes.data_as(_lib._dp) for k in ("A", "b", "Q", "H", "c", "R")),
+        yv.ctypes.data_as(_lib._dp), float(jitter or 0.0), ll.ctypes.data_as(_lib._dp),
+        st.ctypes.data_as(_lib._ip))
+    if rc and not return_status:
+        ctx.check(rc)
+    return (ll, st) if return_status else ll
+
+
+def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=None, ctx=None):
+    """kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter) of one filter: the
+    sequences hold one parameter per step (logdensity.jl:149-161)."""
+    ys = np.asarray(ys, dtype=np.float64)
+    T = ys.shape[0]
+    ys = ys.reshape(1, T, -1)
+    st = lambda seq: np.asarray(seq, dtype=np.float64)[None]
+    return float(kf_loglikelihood_batch(np.asarray(mu0, dtype=np.float64)[None], st(Sigma0),
+                                        st(As), st(bs), st(Qs), st(Hs), st(cs), st(Rs), ys,
+                                        jitter=jitter, ctx=ctx)[0])
