@@ -623,6 +623,161 @@ def bench_tree(args, T=200, budget_s=10.0):
     print(json.dumps(out), flush=True)
 
 
+def bench_csmc(args, budget_s=10.0):
+    """Particle Gibbs sweeps: conditional SMC with ancestor sampling (CSMC-AS) on the
+    headline D=4 LG model, N particles, K steps, systematic resampling at every step, the
+    device particle history filled after each step and one trajectory drawn per sweep.  One
+    bench step = one sweep conditioned on the previous sweep's trajectory."""
+    import ssmhip
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    log(f"device {name} CUs={cus}")
+    p = make_model()
+    N, K = args.N, args.K
+    ys = simulate(p, K, 1234)
+    prm = [p[k] for k in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")]
+    model = ssmhip.create_homogeneous_linear_gaussian_model(*prm)
+    csmc = ssmhip.ConditionalSMC(ssmhip.BF(N, threshold=1.0, precision="f32"),
+                                 ssmhip.AncestorSampling())
+    ws = ssmhip.CSMCWorkspace(model, csmc.pf, K, ctx)
+    rng = np.random.default_rng(77)
+    traj, _ = ssmhip.csmc_sample(rng, model, csmc, ys, None, workspace=ws)
+    for _ in range(args.warmup):
+        traj, _ = ssmhip.csmc_sample(rng, model, csmc, ys, traj, workspace=ws)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        traj, ll = ssmhip.csmc_sample(rng, model, csmc, ys, traj, workspace=ws)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    sweeps = args.steps / el
+    # algorithmic bytes per particle and step (FP32): the step kernel 40 (state read and
+    # written 32, log-weight 4, ancestor 4), the ancestor-sampling weights 24 (state 16,
+    # log-weight 4, weight 4) and their tile partials 4, the resampler 8 (log-weight read,
+    # ancestor written), the history push 52
+    bpp = 40 + 24 + 4 + 8 + 52
+    achieved = bpp * N * K * sweeps / 1e9
+    log(f"{args.steps} sweeps in {el:.3f}s: {sweeps:.2f} sweeps/s, "
+        f"{sweeps * N * K:.3e} particle-steps/s, {achieved:.0f} GB/s algorithmic")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        om = oracle.LG(*prm)
+        orng = np.random.default_rng(5)
+        Nc = min(N, 16384)
+        xr = traj.as_array()
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < budget_s:
+            xr, _ = oracle.csmc_sweep(om, ys, Nc, xr, True, 1.0, oracle.SYSTEMATIC, orng)
+            n += 1
+        secs = time.perf_counter() - t1
+        cpu = {"value": n * Nc * K / secs, "unit": "particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle csmc_sweep (oracle/oracle.c steps, single thread), CSMC-AS, "
+                         f"N={Nc}, K={K}: {n} sweeps in {secs:.1f}s"}
+        log(f"cpu baseline {n * Nc * K / secs:.3e} particle-steps/s ({n} sweeps at N={Nc})")
+    out = {
+        "metric": "CSMC-AS sweeps/s (particle Gibbs trajectory updates)",
+        "value": sweeps, "unit": "sweeps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded LG simulation, Philox noise on device)",
+        "config": {"workload": f"conditional SMC with ancestor sampling, D=4 Dy=2 LG, N={N}, "
+                               f"K={K}, systematic every step, dense device history; one "
+                               f"step = one sweep", "N": N, "K": K},
+        "particle_steps_per_s": sweeps * N * K,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one sweep (host-timed: per-step calls with a stream sync each)",
+                     "bytes_per_particle_step": bpp},
+        "cpu_baseline": cpu,
+    }
+    ws.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_kfll(args, Dx=8, Dy=4, T=100, budget_s=10.0):
+    """kf_loglikelihood as a batched likelihood service: B filters of one shape with
+    per-step parameters (As[t], bs[t], ... of logdensity.jl:149-161), seeded stable
+    dynamics.  One bench step = one call evaluating all B likelihoods."""
+    import ssmhip
+    ctx = ssmhip.default_context()
+    name, cus = ctx.device_info()
+    B = args.B
+    log(f"device {name} CUs={cus}; B={B} Dx={Dx} Dy={Dy} T={T}")
+    rng = np.random.default_rng(31)
+    qr = np.linalg.qr(rng.standard_normal((B, T, Dx, Dx)))[0]
+    A = 0.9 * qr
+    b = 0.1 * rng.standard_normal((B, T, Dx))
+    G = rng.standard_normal((B, T, Dx, Dx)) / np.sqrt(Dx)
+    Q = 0.2 * (G @ np.swapaxes(G, -1, -2)) + 0.05 * np.eye(Dx)
+    H = rng.standard_normal((B, T, Dy, Dx))
+    c = 0.1 * rng.standard_normal((B, T, Dy))
+    Gr = rng.standard_normal((B, T, Dy, Dy)) / np.sqrt(Dy)
+    R = 0.3 * (Gr @ np.swapaxes(Gr, -1, -2)) + 0.1 * np.eye(Dy)
+    mu0 = rng.standard_normal((B, Dx))
+    S0 = np.broadcast_to(np.eye(Dx), (B, Dx, Dx)).copy()
+    ys = rng.standard_normal((B, T, Dy))
+    args_ = (mu0, S0, A, b, Q, H, c, R, ys)
+    for _ in range(args.warmup):
+        ll = ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx)
+    t0 = time.perf_counter()
+    dev = []
+    for _ in range(args.steps):
+        ll = ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx)
+        ms = ssmhip._lib.C.c_double()
+        ctx.check(ctx.lib.ssm_kf_loglikelihood_last_timing(ctx.ptr, ssmhip._lib.C.byref(ms)))
+        dev.append(ms.value)
+    el = time.perf_counter() - t0
+    ms_dev = float(np.mean(dev))
+    # algorithmic HBM bytes per filter and step: its parameter block (FP64) and observation
+    psz = (2 * Dx * Dx + Dx + Dy * Dx + Dy + Dy * Dy + Dy) * 8
+    bytes_run = float(B) * T * psz
+    achieved = bytes_run / (ms_dev * 1e-3) / 1e9
+    flops = float(kf_flops(Dx, Dy)) * T * B
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    errs = []
+    for f in range(0, B, max(1, B // 16)):
+        ref = oracle.kf_loglikelihood(mu0[f], S0[f], A[f], b[f], Q[f], H[f], c[f], R[f], ys[f])
+        errs.append(abs(ll[f] - ref) / abs(ref))
+    log(f"{args.steps} calls in {el:.3f}s; device {ms_dev:.3f} ms per call "
+        f"({B / (ms_dev * 1e-3):.3e} evals/s, {achieved:.0f} GB/s, "
+        f"{flops / (ms_dev * 1e-3) / 1e12:.2f} TFLOP/s); wall incl. host packing and copies "
+        f"{el / args.steps * 1e3:.1f} ms; max rel err {max(errs):.2e}")
+    cpu = None
+    if not args.no_cpu:
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < budget_s and n < B:
+            oracle.kf_loglikelihood(mu0[n], S0[n], A[n], b[n], Q[n], H[n], c[n], R[n], ys[n])
+            n += 1
+        secs = time.perf_counter() - t1
+        cpu = {"value": n / secs, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
+               "sample": f"oracle kf_loglikelihood (oracle/oracle.c predict/update per step, "
+                         f"single thread) on the first {n} filters of the batch, {secs:.1f}s"}
+        log(f"cpu baseline {n / secs:.1f} evals/s")
+    out = {
+        "metric": "kf_loglikelihood evals/s (batched, per-step parameters)",
+        "value": B / (ms_dev * 1e-3), "unit": "likelihood evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_dev,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 Cholesky of S)",
+        "data": "synthetic (seeded stable time-varying LG parameters and observations)",
+        "config": {"workload": f"kf_loglikelihood, B={B}, Dx={Dx}, Dy={Dy}, T={T}, per-step "
+                               f"parameters, inputs resident on the device", "B": B, "T": T},
+        "wall_ms_per_call_incl_pcie": el / args.steps * 1e3,
+        "loglik_max_rel_err": float(max(errs)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_kf_batch<16,16,NW,true>", "bytes_per_call": bytes_run},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -639,13 +794,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (counter calibration)")
     ap.add_argument("--workload", default="bf",
-                    choices=["bf", "kf", "rbpf", "serve", "tree"],
+                    choices=["bf", "kf", "rbpf", "serve", "tree", "csmc", "kfll"],
                     help="bf: bootstrap filter (default); kf: batched Kalman filters; "
                          "rbpf: Rao-Blackwellised PF; serve: request stream through the "
                          "server against the plain per-filter path; tree: ancestry path "
-                         "storage after every filter step")
+                         "storage after every filter step; csmc: CSMC-AS sweeps; kfll: "
+                         "batched kf_loglikelihood with per-step parameters")
     ap.add_argument("--requests", type=int, default=256, help="serve: requests per phase")
-    ap.add_argument("--B", type=int, default=65536, help="kf: number of filters")
+    ap.add_argument("--B", type=int, default=None,
+                    help="kf, kfll: number of filters (default 65536 for kf, 4096 for kfll)")
+    ap.add_argument("--K", type=int, default=100, help="csmc: steps per sweep")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -655,6 +813,8 @@ def main():
               "path runs on a single MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    if args.B is None:
+        args.B = 4096 if args.workload == "kfll" else 65536
     if args.workload == "kf":
         return bench_kf(args)
     if args.workload == "rbpf":
@@ -663,6 +823,10 @@ def main():
         return bench_serve(args)
     if args.workload == "tree":
         return bench_tree(args)
+    if args.workload == "csmc":
+        return bench_csmc(args)
+    if args.workload == "kfll":
+        return bench_kfll(args)
 
     import ssmhip
     ctx = ssmhip.default_context()

@@ -404,6 +404,9 @@ int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, in
                              const double* A, const double* b, const double* Q, const double* H,
                              const double* c, const double* R, const double* ys, double jitter,
                              double* ll_out, int32_t* status_out);
+/* Device time of the last ssm_kf_loglikelihood's kernel on this context, from HIP events
+ * (ms; the host-to-device copies of its inputs are outside). */
+int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,8 @@ struct ssm_ctx {
   // Lazily grown device copy of the inputs of ssm_kf_loglikelihood.
   void* kfll = nullptr;
   size_t kfll_bytes = 0;
+  hipEvent_t kfll_ev[2] = {nullptr, nullptr};  // around its kernel
+  double kfll_ms = 0.0;
 };
 
 namespace ssm {

@@ -788,17 +788,30 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   p.tv_c = d_c;
   p.tv_R = d_R;
   p.tv_blocks = time_varying ? T : 1;
+  for (hipEvent_t& e : ctx->kfll_ev)
+    if (!e) SSM_HIP_CHECK(ctx, hipEventCreate(&e));
+  SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[0], s));
   rc = launch_class_nw(ctx, s, p, cls, class_nw(cls));
   if (rc) return rc;
+  SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[1], s));
   std::vector<int32_t> st(B);
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_out, d_ll, 8 * (size_t)B, hipMemcpyDeviceToHost, s));
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(st.data(), d_status, 4 * (size_t)B, hipMemcpyDeviceToHost, s));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  float ms = 0.f;
+  SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->kfll_ev[0], ctx->kfll_ev[1]));
+  ctx->kfll_ms = ms;
   if (status_out) std::memcpy(status_out, st.data(), 4 * (size_t)B);
   for (int f = 0; f < B; ++f)
     if (st[f] != SSM_OK)
       return set_error(ctx, SSM_ERR_NOT_PD,
                        "filter %d: innovation covariance S is not positive definite", f);
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms) {
+  if (!ctx) return SSM_ERR_ARG;
+  if (ms) *ms = ctx->kfll_ms;
   return SSM_OK;
 }
 

@@ -136,6 +136,7 @@ def _declare(L):
         "ssm_kf_batch_destroy": ([_vp], i32),
         "ssm_kf_loglikelihood": ([_vp, i32, i32, i32, i32, i32, _dp, _dp, _dp, _dp, _dp, _dp,
                                   _dp, _dp, _dp, dbl, _dp, _ip], i32),
+        "ssm_kf_loglikelihood_last_timing": ([_vp, _dp], i32),
         "ssm_rbpf_create": ([_vp, C.POINTER(RBModelC), i32, u64, C.POINTER(_vp)], i32),
         "ssm_rbpf_destroy": ([_vp], i32),
         "ssm_rbpf_initialise": ([_vp, _dp], i32),
