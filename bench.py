@@ -626,8 +626,9 @@ def bench_tree(args, T=200, budget_s=10.0):
 def bench_csmc(args, budget_s=10.0):
     """Particle Gibbs sweeps: conditional SMC with ancestor sampling (CSMC-AS) on the
     headline D=4 LG model, N particles, K steps, systematic resampling at every step, the
-    device particle history filled after each step and one trajectory drawn per sweep.  One
-    bench step = one sweep conditioned on the previous sweep's trajectory."""
+    device particle history recorded by the steps themselves and one trajectory drawn per
+    sweep.  A sweep is one graph-replayed run of K steps plus the draw and the path
+    read-back.  One bench step = one sweep conditioned on the previous sweep's trajectory."""
     import ssmhip
     ctx = ssmhip.default_context()
     name, cus = ctx.device_info()
@@ -690,7 +691,8 @@ def bench_csmc(args, budget_s=10.0):
         "particle_steps_per_s": sweeps * N * K,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one sweep (host-timed: per-step calls with a stream sync each)",
+                     "kernel": "one sweep (host-timed: one graph-replayed run of K steps, the "
+                               "trajectory draw and its read-back)",
                      "bytes_per_particle_step": bpp},
         "cpu_baseline": cpu,
     }

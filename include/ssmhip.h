@@ -251,6 +251,13 @@ int32_t ssm_bf_tree_destroy(ssm_tree* tree);
 int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_history** out);
 int32_t ssm_bf_history_push(ssm_history* h);
 int32_t ssm_bf_history_reset(ssm_history* h);
+/* Recording (on = 1): every later step of the filter, per-step or graph-replayed by
+ * ssm_bf_run, appends its generation to h on the device, as DenseAncestorCallback does at
+ * PostUpdate inside filter(...; callback) (callbacks.jl:53-80); the history grows as
+ * needed.  One history records at a time (turning another on replaces it); on = 0 stops.
+ * Do not also call ssm_bf_history_push for the same steps.  ssm_bf_profile's steps are
+ * not recorded. */
+int32_t ssm_bf_history_record(ssm_history* h, int32_t on);
 int32_t ssm_bf_history_info(ssm_history* h, int32_t* T, int32_t* capacity);
 int32_t ssm_bf_history_ancestry(ssm_history* h, int32_t i0, int32_t count, double* paths);
 int32_t ssm_bf_history_generation(ssm_history* h, int32_t t, double* x, double* logw,

@@ -80,6 +80,7 @@ struct ssm_bf {
   ssm_ctx* ctx = nullptr;
   int N = 0, Npad = 0, D = 0, Dy = 0, prec = SSM_F32;
   uint64_t seed = 0;
+  uint64_t* seed_dev = nullptr;  // device copy of the Philox key
   void* x[2] = {nullptr, nullptr};
   int cur = 0;
   void* logw = nullptr;
@@ -107,6 +108,7 @@ struct ssm_bf {
   int32_t* tree_counts = nullptr;
   bool counts_fresh = false;
   std::vector<ssm_history*> histories;  // attached dense histories
+  ssm_history* recording = nullptr;      // the one that takes every step's generation
   // conditional SMC: the reference trajectory and the ancestor-sampling buffers
   std::vector<double> Q_host;  // Q of the current model (column-major)
   std::vector<double> xref_host;  // the reference as given (for a model change)
@@ -166,6 +168,7 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   p.threshold_n = threshold_n;
   p.logN = std::log((double)bf->N);
   p.seed = bf->seed;
+  p.seed_dev = bf->seed_dev;
   p.noise = noise;
   const TileScratch ts = tile_scratch(bf);
   p.partials = ts.partials;
@@ -214,6 +217,7 @@ int enqueue_resample(ssm_bf* bf, hipStream_t s, double threshold, int kind, cons
   a.w = bf->logw;
   a.u = u_dev;
   a.seed = bf->seed;
+  a.seed_dev = bf->seed_dev;
   a.sc = bf->sc;
   a.threshold_n = threshold * (double)bf->N;
   a.anc = bf->anc;
@@ -271,6 +275,7 @@ int enqueue_ancestor_draw_t(ssm_bf* bf, hipStream_t s) {
   a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
   a.w = bf->as_w;
   a.seed = bf->seed;
+  a.seed_dev = bf->seed_dev;
   a.sc = bf->as_sc;
   resample_bind_scratch(a, bf->as_scratch, bf->N);
   const int nb = (bf->N + kTile - 1) / kTile;
@@ -298,6 +303,12 @@ int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const do
   if (rc) return rc;
   enqueue_move(bf, s, threshold, noise_dev);
   enqueue_finish(bf, s, threshold, 1);
+  if (bf->recording) {  // DenseAncestorCallback at PostUpdate, on the device
+    BfView v;
+    const int rc2 = bf_device_view(bf, &v);
+    if (rc2) return rc2;
+    history_record_push(bf->recording, s, v);
+  }
   return SSM_OK;
 }
 
@@ -362,11 +373,13 @@ extern "C" int32_t ssm_bf_create(ssm_ctx* ctx, const ssm_lg_model* model, int32_
                 hipSuccess &&
             hipMalloc(&bf->sc, sizeof(BfScalars)) == hipSuccess &&
             hipMalloc(&bf->rs_scratch, resample_scratch_bytes(N)) == hipSuccess &&
-            hipMalloc(&bf->w_dbg, sizeof(double) * bf->Npad) == hipSuccess;
+            hipMalloc(&bf->w_dbg, sizeof(double) * bf->Npad) == hipSuccess &&
+            hipMalloc(&bf->seed_dev, sizeof(uint64_t)) == hipSuccess;
   if (!ok) {
     ssm_bf_destroy(bf);
     return set_error(ctx, SSM_ERR_OOM, "ssm_bf_create: allocation failed for N=%d", N);
   }
+  (void)hipMemcpy(bf->seed_dev, &bf->seed, sizeof(uint64_t), hipMemcpyHostToDevice);
   (void)hipMemset(bf->x[0], 0, es * bf->Npad * bf->D);
   (void)hipMemset(bf->x[1], 0, es * bf->Npad * bf->D);
   (void)hipMemset(bf->logw, 0, es * bf->Npad);
@@ -403,7 +416,7 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
                   bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u,     bf->xref,
                   bf->ref_model, bf->as_w,     bf->as_scratch, bf->as_sc, bf->as_idx,
-                  bf->draw_idx};
+                  bf->draw_idx,  bf->seed_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bf->ev0) (void)hipEventDestroy(bf->ev0);
@@ -526,6 +539,12 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
     if (rc) return rc;
     noise_dev = bf->noise;
   }
+  if (bf->recording) {
+    bool moved = false;
+    rc = history_record_prepare(bf->recording, ctx->stream, t, 1, &moved);
+    if (rc) return rc;
+    if (moved) destroy_graphs(bf);
+  }
   bf->counts_fresh = false;
   // with a reference, particle 0's ancestor is replaced after the resampler counted
   // offspring, so the tree recounts from the ancestors instead
@@ -542,6 +561,7 @@ extern "C" int32_t ssm_bf_step(ssm_bf* bf, int32_t t, const double* y, double th
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   bf->counts_fresh = pass_counts && resampler != SSM_MULTINOMIAL;
+  if (bf->recording) history_record_commit(bf->recording, 1);
   if (ll_inc) *ll_inc = ll;
   if (did_resample) *did_resample = did;
   return SSM_OK;
@@ -563,6 +583,12 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   int rc = ensure_steps(bf, (size_t)T);
   if (rc) return rc;
   hipStream_t s = ctx->stream;
+  if (bf->recording) {
+    bool moved = false;
+    rc = history_record_prepare(bf->recording, s, t0, T, &moved);
+    if (rc) return rc;
+    if (moved) destroy_graphs(bf);
+  }
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ys, ys, sizeof(double) * T * bf->Dy,
                                     hipMemcpyHostToDevice, s));
   const double zero = 0.0;
@@ -617,6 +643,7 @@ extern "C" int32_t ssm_bf_run(ssm_bf* bf, int32_t t0, int32_t T, const double* y
   SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, bf->ev0, bf->ev1));
   bf->last_ms = ms;
   bf->last_T = T;
+  if (bf->recording) history_record_commit(bf->recording, T);
   if (ll_total) *ll_total = tot;
   return SSM_OK;
 }
@@ -830,6 +857,17 @@ int bf_attach_history(ssm_bf* bf, ssm_history* h, bool on) {
   auto& v = bf->histories;
   v.erase(std::remove(v.begin(), v.end(), h), v.end());
   if (on) v.push_back(h);
+  if (!on && bf->recording == h) {
+    bf->recording = nullptr;
+    destroy_graphs(bf);  // captured pushes write into h
+  }
+  return SSM_OK;
+}
+
+int bf_record_history(ssm_bf* bf, ssm_history* h, bool on) {
+  ssm_history* next = on ? h : (bf->recording == h ? nullptr : bf->recording);
+  if (next != bf->recording) destroy_graphs(bf);
+  bf->recording = next;
   return SSM_OK;
 }
 
@@ -937,7 +975,12 @@ extern "C" int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_r
 
 extern "C" int32_t ssm_bf_set_seed(ssm_bf* bf, uint64_t seed) {
   if (!bf) return SSM_ERR_ARG;
-  if (seed != bf->seed) destroy_graphs(bf);  // captured launches hold the old key
+  ssm_ctx* ctx = bf->ctx;
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // the step and resampler kernels read the key from seed_dev, so captured graphs stay valid
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->seed_dev, &seed, sizeof seed, hipMemcpyHostToDevice,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   bf->seed = seed;
   return SSM_OK;
 }
@@ -956,6 +999,7 @@ extern "C" int32_t ssm_bf_sample_index(ssm_bf* bf, double u, int32_t* index) {
   a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
   a.w = bf->logw;
   a.seed = bf->seed;
+  a.seed_dev = bf->seed_dev;
   a.sc = bf->sc;
   resample_bind_scratch(a, bf->rs_scratch, bf->N);
   hipStream_t s = ctx->stream;

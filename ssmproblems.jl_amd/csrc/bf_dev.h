@@ -52,6 +52,8 @@ struct StepParams {
   double threshold_n;
   double logN;
   uint64_t seed;
+  // when set, the Philox key is read from device memory (graph replays survive a re-key)
+  const uint64_t* seed_dev = nullptr;
   const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
   double* __restrict__ partials;
   // conditional SMC (the REF kernels): the reference trajectory, (T_ref + 1) x D row-major,
@@ -198,6 +200,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   const bool fresh = rs || sc->typeless;  // normalised pre-update log-weights are all 0
   const T lse = (T)sc->lse;
   const int t = sc->step;
+  const uint64_t seed = p.seed_dev ? *p.seed_dev : p.seed;
   const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
   // Model constants: registers for compile-time dimensions (read before any store, so the
   // loads are uniform scalar loads), an LDS copy otherwise.
@@ -229,7 +232,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
       for (int d = 0; d < MD; ++d)
         if (d < D) z[d] = p.noise[(size_t)d * p.Npad + i];
     } else {
-      draw_normals<T, MD>(p.seed, kStreamPropagate, (uint32_t)t, (uint32_t)i, D, z);
+      draw_normals<T, MD>(seed, kStreamPropagate, (uint32_t)t, (uint32_t)i, D, z);
     }
   };
   auto compute_item = [&](int k, const T* xk, const T* z, T nlk) -> T {

@@ -36,6 +36,25 @@ __global__ void __launch_bounds__(kHistThreads)
   an[i] = sc->last_resampled ? anc[i] : i;
 }
 
+// The push of a recording history (also replayed from graphs): generation t (the step the
+// finish kernel just counted, sc->step - 1) goes to slot t + *base, skipped outside
+// [0, cap).
+template <typename T>
+__global__ void __launch_bounds__(kHistThreads)
+    k_hist_push_rec(int N, int Npad, int D, const T* __restrict__ x, const T* __restrict__ logw,
+                    const int32_t* __restrict__ anc, const BfScalars* sc,
+                    const int32_t* __restrict__ base, int cap, T* st, double* lw, int32_t* an) {
+  const int i = blockIdx.x * kHistThreads + threadIdx.x;
+  const int slot = sc->step - 1 + *base;
+  if (i >= N || slot < 0 || slot >= cap) return;
+  const size_t o = (size_t)slot * Npad;
+  for (int d = 0; d < D; ++d)
+    st[o * D + (size_t)d * Npad + i] = x[(size_t)d * Npad + i];
+  const T lse = (T)sc->lse;
+  lw[o + i] = sc->typeless ? 0.0 : (double)(T)(logw[i] - lse);
+  an[o + i] = sc->last_resampled ? anc[i] : i;
+}
+
 // Paths of particles i0 .. i0 + count - 1: out[(k (Tn + 1) + t) D + d], t = 0 .. Tn.
 template <typename T>
 __global__ void __launch_bounds__(kHistThreads)
@@ -68,6 +87,7 @@ struct ssm_history {
   void* states = nullptr;
   double* logw = nullptr;
   int32_t* anc = nullptr;
+  int32_t* base = nullptr;  // device slot base of a recording run
 };
 
 void ssm::history_detach_filter(ssm_history* h) { h->bf = nullptr; }
@@ -78,7 +98,7 @@ size_t elem(const ssm_history* h) { return h->prec == SSM_F32 ? 4 : 8; }
 size_t plane_bytes(const ssm_history* h) { return elem(h) * h->D * h->Npad; }
 
 void hist_free(ssm_history* h) {
-  void* ps[] = {h->x0, h->states, h->logw, h->anc};
+  void* ps[] = {h->x0, h->states, h->logw, h->anc, h->base};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -136,7 +156,9 @@ extern "C" int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_histo
   h->Npad = v.Npad;
   h->D = v.D;
   h->prec = v.prec;
-  if (hipMalloc(&h->x0, plane_bytes(h)) != hipSuccess) {
+  if (hipMalloc(&h->x0, plane_bytes(h)) != hipSuccess ||
+      hipMalloc(&h->base, sizeof(int32_t)) != hipSuccess) {
+    hist_free(h);
     delete h;
     return set_error(ctx, SSM_ERR_OOM, "particle history: time-0 states");
   }
@@ -153,6 +175,49 @@ extern "C" int32_t ssm_bf_history_create(ssm_bf* bf, int32_t capacity, ssm_histo
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *out = h;
   return SSM_OK;
+}
+
+namespace ssm {
+
+int history_record_prepare(ssm_history* h, hipStream_t s, int t0, int T, bool* moved) {
+  ssm_ctx* ctx = h->ctx;
+  *moved = false;
+  const int64_t need = (int64_t)h->T + T;
+  if (need > INT32_MAX)
+    return set_error(ctx, SSM_ERR_ARG, "particle history: 2^31 - 1 generations reached");
+  if (need > h->cap) {
+    int64_t cap2 = std::max<int64_t>(need, 2 * (int64_t)h->cap);
+    const int rc = hist_grow(h, (int)std::min<int64_t>(cap2, INT32_MAX));
+    if (rc) return rc;
+    *moved = true;
+  }
+  const int32_t base = h->T - t0;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(h->base, &base, sizeof base, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));  // `base` is a host local
+  return SSM_OK;
+}
+
+void history_record_push(ssm_history* h, hipStream_t s, const BfView& v) {
+  const dim3 g((h->N + kHistThreads - 1) / kHistThreads), b(kHistThreads);
+  if (h->prec == SSM_F32)
+    hipLaunchKernelGGL(k_hist_push_rec<float>, g, b, 0, s, h->N, h->Npad, h->D,
+                       (const float*)v.x, (const float*)v.logw, v.anc, v.sc, h->base, h->cap,
+                       (float*)h->states, h->logw, h->anc);
+  else
+    hipLaunchKernelGGL(k_hist_push_rec<double>, g, b, 0, s, h->N, h->Npad, h->D,
+                       (const double*)v.x, (const double*)v.logw, v.anc, v.sc, h->base, h->cap,
+                       (double*)h->states, h->logw, h->anc);
+}
+
+void history_record_commit(ssm_history* h, int T) { h->T += T; }
+
+}  // namespace ssm
+
+extern "C" int32_t ssm_bf_history_record(ssm_history* h, int32_t on) {
+  if (!h) return SSM_ERR_ARG;
+  ssm_ctx* ctx = h->ctx;
+  if (!h->bf) return set_error(ctx, SSM_ERR_STATE, "particle history: its filter was destroyed");
+  return bf_record_history(h->bf, h, on != 0);
 }
 
 extern "C" int32_t ssm_bf_history_reset(ssm_history* h) {

@@ -76,6 +76,7 @@ struct ResampleArgs {
   const void* w = nullptr;           // weights or log-weights
   const double* u = nullptr;         // explicit uniforms (device) or null -> Philox
   uint64_t seed = 0;
+  const uint64_t* seed_dev = nullptr;  // device copy of the key (read instead of `seed`)
   uint32_t step_host = 0;            // Philox step when `sc` is null
   BfScalars* sc = nullptr;           // particle-filter scalars, or null (always resample)
   double threshold_n = 0.0;          // threshold * N (particle-filter decision)
@@ -125,6 +126,15 @@ void tree_detach_filter(ssm_tree* tree);
 // filter first leaves the history's later pushes failing cleanly.
 int bf_attach_history(ssm_bf* bf, ssm_history* h, bool on);
 void history_detach_filter(ssm_history* h);
+// A recording history takes every generation the filter's steps produce, also inside
+// captured graphs.  Before steps t0 .. t0 + T - 1: room for T more generations and the
+// device slot base (the push of step t writes slot t + base); *moved when the buffers were
+// reallocated (graphs that hold them must go).  After the steps: the generations count.
+int history_record_prepare(ssm_history* h, hipStream_t s, int t0, int T, bool* moved);
+void history_record_push(ssm_history* h, hipStream_t s, const BfView& v);
+void history_record_commit(ssm_history* h, int T);
+// Makes h the filter's recording history (on) or stops recording into it (off).
+int bf_record_history(ssm_bf* bf, ssm_history* h, bool on);
 
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the

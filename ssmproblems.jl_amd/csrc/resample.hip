@@ -135,6 +135,7 @@ static RsParams rs_params_of(const ResampleArgs& a) {
   rp.w = a.w;
   rp.u = a.u;
   rp.seed = a.seed;
+  rp.seed_dev = a.seed_dev;
   rp.step_host = a.step_host;
   rp.sc = a.sc;
   rp.threshold_n = a.threshold_n;

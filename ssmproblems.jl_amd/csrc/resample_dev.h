@@ -22,6 +22,7 @@ struct RsParams {
   const void* w;
   const double* u;
   uint64_t seed;
+  const uint64_t* seed_dev = nullptr;  // when set, the Philox key is read from here
   uint32_t step_host;
   const BfScalars* sc;
   double threshold_n;
@@ -37,6 +38,10 @@ struct RsParams {
 
 __device__ __forceinline__ bool rs_active(const RsParams& p) {
   return p.sc == nullptr || p.threshold_n >= p.sc->ess;
+}
+
+__device__ __forceinline__ uint64_t rs_seed(const RsParams& p) {
+  return p.seed_dev ? *p.seed_dev : p.seed;
 }
 
 __device__ __forceinline__ uint32_t rs_step(const RsParams& p) {
@@ -147,7 +152,7 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
   int k = clamp_floor(r + 1.0, n);
   if (k < 1) k = 1;
   const double* u = PLAIN ? nullptr : p.u;
-  const double uk = u ? u[k - 1] : resample_uniform(p.seed, step, (uint32_t)(k - 1));
+  const double uk = u ? u[k - 1] : resample_uniform(rs_seed(p), step, (uint32_t)(k - 1));
   return clamp_floor(r + uk, n);
 }
 
@@ -303,7 +308,7 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   }
   const double u0 = (kind == SSM_STRATIFIED)
                         ? 0.0
-                        : (uvec ? uvec[0] : resample_uniform(p.seed, step, 0u));
+                        : (uvec ? uvec[0] : resample_uniform(rs_seed(p), step, 0u));
   const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step);
   const int o_end = (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step);
   int O[kTileItems];
@@ -383,7 +388,7 @@ __device__ __forceinline__ void multinomial_search_item(const RsParams& p, int k
   const int nb = (p.n + kTile - 1) / kTile;
   const bool typeless = (p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64) && p.sc->typeless;
   const double Wn = typeless ? 1.0 : p.prefix[nb];
-  const double uk = p.u ? p.u[k] : resample_uniform(p.seed, step, (uint32_t)k);
+  const double uk = p.u ? p.u[k] : resample_uniform(rs_seed(p), step, (uint32_t)k);
   const double q = uk * Wn;
   int lo = 0, hi = p.n;
   while (lo < hi) {
@@ -416,7 +421,7 @@ __device__ __forceinline__ void categorical_one_block(const RsParams& p, double 
   constexpr bool kLog = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64);
   const bool typeless = kLog && p.sc->typeless;
   const uint32_t step = rs_step(p);
-  const double uu = (u >= 0.0) ? u : stream_uniform(p.seed, stream, step, 0u);
+  const double uu = (u >= 0.0) ? u : stream_uniform(rs_seed(p), stream, step, 0u);
   const double Wn = typeless ? 1.0 : p.prefix[nb];
   const double q = uu * Wn;
   // the tile: the number of tiles whose closing prefix is below q (tile b closes at

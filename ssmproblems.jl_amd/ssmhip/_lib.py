@@ -153,6 +153,7 @@ def _declare(L):
         "ssm_bf_history_create": ([_vp, i32, C.POINTER(_vp)], i32),
         "ssm_bf_history_push": ([_vp], i32),
         "ssm_bf_history_reset": ([_vp], i32),
+        "ssm_bf_history_record": ([_vp, i32], i32),
         "ssm_bf_history_info": ([_vp, _ip, _ip], i32),
         "ssm_bf_history_ancestry": ([_vp, i32, i32, _dp], i32),
         "ssm_bf_history_generation": ([_vp, i32, _dp, _dp, _ip], i32),

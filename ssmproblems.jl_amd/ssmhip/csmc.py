@@ -5,10 +5,11 @@ runs one sweep and returns ``(ReferenceTrajectory, log_likelihood)`` as ``_csmc_
 does (csmc.jl:239-306).  The particles, their history and every draw stay on the device:
 particle 0 is pinned to the reference (initialise and each step), its ancestor is particle
 0 (NoRefreshment) or drawn on the device from the ancestor-sampling weights
-(AncestorSampling), each generation goes into a device DenseParticleContainer, and the
-returned trajectory is the path of one particle drawn from the final weights
-(_sample_trajectory, csmc.jl:111-117).  ``mcmc_step`` is the AbstractMCMC interface
-(csmc.jl:450-462).  The model must be linear-Gaussian; BackwardSimulation and the
+(AncestorSampling), each generation goes into a device DenseParticleContainer that the
+filter's steps fill themselves (a homogeneous model runs the whole sweep as one
+graph-replayed call), and the returned trajectory is the path of one particle drawn from
+the final weights (_sample_trajectory, csmc.jl:111-117).  ``mcmc_step`` is the AbstractMCMC
+interface (csmc.jl:450-462).  The model must be linear-Gaussian; BackwardSimulation and the
 Rao-Blackwellised CSMC are not provided.
 """
 from __future__ import annotations
@@ -20,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import default_context
-from .filters import BootstrapFilter, DeviceParticles, rng_to_seed, step
+from .filters import BootstrapFilter, DeviceParticles, rng_to_seed, run, step
 from .history import DenseParticleContainer
 from .models import StateSpaceModel
 
@@ -82,6 +83,7 @@ class CSMCWorkspace:
         self.state = DeviceParticles(self.ctx, model, pf, 0)
         self.ctx.check(self.ctx.lib.ssm_bf_initialise(self.state.ptr, None))
         self.history = DenseParticleContainer(self.state, capacity=max(int(K), 1))
+        self.history.record(True)  # every step appends its generation on the device
         self.N, self.Dx = pf.N, self.state.Dx
 
     def close(self):
@@ -127,11 +129,13 @@ def csmc_sample(rng, model: StateSpaceModel, csmc: ConditionalSMC, observations,
         st.set_params(PackedLG.from_model(model, 1))
     ws.ctx.check(lib.ssm_bf_initialise(st.ptr, None))
     ws.history.reset()
-    ll = 0.0
-    for t in range(1, K + 1):
-        _, inc = step(rng, model, pf, t, st, ys[t - 1])
-        ll += inc
-        ws.history.push()
+    if model.is_homogeneous:  # all K steps in one call, replayed from HIP graphs
+        ll = run(st, pf, ys)
+    else:  # calc_params per step
+        ll = 0.0
+        for t in range(1, K + 1):
+            _, inc = step(rng, model, pf, t, st, ys[t - 1])
+            ll += inc
     idx = st.sample_index(float(rng.random()))
     path = ws.history.get_ancestry(idx)
     return ReferenceTrajectory(path[0], path[1:]), ll

@@ -29,6 +29,11 @@ class DenseParticleContainer:
     def push(self):
         self.ctx.check(self.ctx.lib.ssm_bf_history_push(self.ptr))
 
+    def record(self, on=True):
+        """While on, every step of the filter (per-step or a graph-replayed run) appends its
+        generation on the device; do not push those steps by hand as well."""
+        self.ctx.check(self.ctx.lib.ssm_bf_history_record(self.ptr, int(bool(on))))
+
     def reset(self):
         """Empties the container and stores the filter's current particles as time 0."""
         self.ctx.check(self.ctx.lib.ssm_bf_history_reset(self.ptr))

@@ -85,3 +85,41 @@ def test_callback_and_errors(ctx):
     assert h.ancestry().shape == (1024, 13, 2)  # the stored history stays readable
     h2.close()
     h.close()
+
+
+@pytest.mark.parametrize("prec,kind,thr", [("f32", 0, 0.7), ("f64", 2, 1.0), ("f32", 3, 0.5)])
+def test_recording_during_graph_run_equals_pushes(ctx, prec, kind, thr):
+    # DenseAncestorCallback inside filter(...): a recording history filled by the filter's own
+    # steps, here 37 graph-replayed steps (two 16-step graphs and 5 single steps) in two runs,
+    # equals the manual pushes after per-step calls with the same Philox key, bit for bit
+    rng = np.random.default_rng(17)
+    m = random_lg(rng, 3, 2)
+    m.A = 0.9 * m.A / max(abs(np.linalg.eigvals(m.A)))
+    model = to_ssmhip_model(m)
+    res = {0: ssmhip.Systematic(), 2: ssmhip.Stratified(), 3: ssmhip.Multinomial()}[kind]
+    algo = ssmhip.BF(6000, threshold=thr, resampler=res, precision=prec)
+    ys = rng.standard_normal((37, 2))
+    a = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=21)
+    ha = ssmhip.DenseParticleContainer(a, capacity=2)  # grows during the first run
+    ha.record(True)
+    ll1 = ssmhip.run(a, algo, ys[:20], t0=1)
+    ll2 = ssmhip.run(a, algo, ys[20:], t0=21)
+    assert len(ha) == 37
+    b = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=21)
+    hb = ssmhip.DenseParticleContainer(b, capacity=64)
+    incs = []
+    for t in range(1, 38):
+        incs.append(ssmhip.step(rng, model, algo, t, b, ys[t - 1])[1])
+        hb.push()
+    assert ll1 + ll2 == pytest.approx(sum(incs), rel=1e-12)
+    for t in (0, 1, 16, 17, 20, 21, 37):
+        for ga, gb in zip(ha.generation(t), hb.generation(t)):
+            if ga is not None:
+                assert np.array_equal(ga, gb), t
+    assert np.array_equal(ha.ancestry(0, 50), hb.ancestry(0, 50))
+    # per-step calls record too; turning recording off stops it
+    ssmhip.step(rng, model, algo, 38, a, ys[0])
+    assert len(ha) == 38
+    ha.record(False)
+    ssmhip.step(rng, model, algo, 39, a, ys[1])
+    assert len(ha) == 38
