@@ -258,6 +258,18 @@ int32_t ssm_bf_history_reset(ssm_history* h);
  * Do not also call ssm_bf_history_push for the same steps.  ssm_bf_profile's steps are
  * not recorded. */
 int32_t ssm_bf_history_record(ssm_history* h, int32_t on);
+/*
+ * Backward simulation over the stored generations (CSMC-BS, GF/src/algorithms/csmc.jl:
+ * 392-446 after its forward pass): index T ~ the last generation's weights, then for
+ * t = T-1 .. 0 index t ~ w_t^i N(x*_{t+1}; A x_t^i + b, Q) (ancestor_weight,
+ * ancestor_sampling.jl:125-130; the time-0 weights are uniform), each the first particle
+ * whose cumulative weight reaches u[t] of the total (StatsBase.sample(Weights(.))).  Uses
+ * the filter's current model (homogeneous dynamics).  u: T + 1 uniforms in [0, 1), u[t]
+ * for time t; indices (T + 1, 0-based, optional) and trajectory ((T + 1) x Dx, row t the
+ * state x*_t, optional) receive the draw.  Every draw runs on the device.
+ */
+int32_t ssm_bf_history_backward_sample(ssm_history* h, const double* u, int32_t* indices,
+                                       double* trajectory);
 int32_t ssm_bf_history_info(ssm_history* h, int32_t* T, int32_t* capacity);
 int32_t ssm_bf_history_ancestry(ssm_history* h, int32_t i0, int32_t count, double* paths);
 int32_t ssm_bf_history_generation(ssm_history* h, int32_t t, double* x, double* logw,

@@ -341,15 +341,33 @@ def categorical(logw, u):
     return int(resample(MULTINOMIAL, w, np.full(lw.shape[0], float(u)))[0])
 
 
-def csmc_sweep(model: LG, ys, N, x_ref, ancestor_sampling, threshold, kind, rng):
-    """One CSMC sweep (csmc.jl:239-306) with numpy draws: returns (trajectory (K+1, Dx),
-    log-likelihood).  x_ref None runs the unconditional filter (the first iteration)."""
+def backward_sample(model: LG, xs, logws, u):
+    """CSMC-BS's backward pass (csmc.jl:415-444): xs[t] (N, Dx) for t = 0..K, logws[t-1] the
+    normalised log-weights of generation t, u[t] the uniform of time t.  Index K ~ the last
+    weights; index t ~ softmax(log w_t + log N(x*_{t+1}; A x_t + b, Q)), the time-0 weights
+    uniform.  Returns (indices (K+1,), trajectory (K+1, Dx))."""
+    K = len(xs) - 1
+    idx = np.zeros(K + 1, dtype=np.int64)
+    idx[K] = categorical(logws[K - 1], u[K])
+    for t in range(K - 1, -1, -1):
+        st = BFState(np.asarray(xs[t], dtype=np.float64),
+                     None if t == 0 else logws[t - 1], typeless=(t == 0))
+        idx[t] = categorical(as_logweights(model, st, np.asarray(xs[t + 1])[idx[t + 1]]), u[t])
+    traj = np.array([np.asarray(xs[t])[idx[t]] for t in range(K + 1)])
+    return idx, traj
+
+
+def csmc_sweep(model: LG, ys, N, x_ref, ancestor_sampling, threshold, kind, rng,
+               backward=False):
+    """One CSMC sweep (csmc.jl:239-306, :392-446 with backward=True) with numpy draws:
+    returns (trajectory (K+1, Dx), log-likelihood).  x_ref None runs the unconditional
+    filter (the first iteration)."""
     ys = np.asarray(ys, dtype=np.float64).reshape(len(ys), -1)
     K, D = ys.shape[0], model.Dx
     st = BFState(bf_initialise(model, rng.standard_normal((N, D))))
     if x_ref is not None:
         st.x[0] = x_ref[0]
-    xs, ancs = [st.x.copy()], []
+    xs, ancs, lws = [st.x.copy()], [], []
     ll = 0.0
     for t in range(1, K + 1):
         u = rng.random(1 if kind in (SYSTEMATIC, SYSTEMATIC_MURRAY) else N)
@@ -365,6 +383,10 @@ def csmc_sweep(model: LG, ys, N, x_ref, ancestor_sampling, threshold, kind, rng)
         ll += inc
         xs.append(st.x.copy())
         ancs.append(anc)
+        lws.append(st.logw.copy())
+    if backward:
+        u = rng.random(K + 1)[::-1]  # the last time's draw first
+        return backward_sample(model, xs, lws, u)[1], ll
     j = categorical(st.logw, rng.random())
     return dense_ancestry(xs, ancs)[0][j], ll
 

@@ -56,6 +56,29 @@ __global__ void __launch_bounds__(kStepThreads)
   bf_as_weight_item<T, DT>(N, Npad, D, model, rm, x, logw, sc, xref + (size_t)t * D, w_out, i);
 }
 
+// CSMC-BS: backward-simulation log-weights of one stored generation (thread 0 also marks
+// the scalars of their fold as explicit weights).
+template <typename T, int DT>
+__global__ void __launch_bounds__(kStepThreads)
+    k_bs_weights(int N, int Npad, int D, const BfModelDev<T>* model, const RefModelDev<T>* rm,
+                 const T* x, const double* logw, const T* x_next, const int32_t* idx_next,
+                 T* w_out, BfScalars* as_sc) {
+  const int i = blockIdx.x * kStepThreads + threadIdx.x;
+  if (i == 0) as_sc->typeless = 0;
+  bf_bs_weight_item<T, DT>(N, Npad, D, model, rm, x, logw, x_next, idx_next, w_out, i);
+}
+
+// The drawn trajectory: traj[t D + d] = generation t's state d of particle idx[t].
+template <typename T>
+__global__ void k_bs_gather(int Tn, int D, int Npad, const T* x0, const T* states,
+                            const int32_t* idx, double* traj) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (Tn + 1) * D) return;
+  const int t = e / D, d = e % D;
+  const T* plane = (t == 0) ? x0 : states + (size_t)(t - 1) * D * Npad;
+  traj[e] = (double)plane[(size_t)d * Npad + idx[t]];
+}
+
 // Conditional SMC: particle 0 of the initial generation is the reference's time-0 state.
 template <typename T>
 __global__ void k_pin_reference(int D, int Npad, T* x, const T* xref) {
@@ -116,6 +139,10 @@ struct ssm_bf {
   size_t xref_cap = 0;         // elements
   int ref_T = -1;              // -1: no reference
   int ref_mode = SSM_CSMC_PLAIN;
+  bool trans_ok = false;       // ref_model holds the current Q's constants
+  int32_t* bs_idx = nullptr;   // backward simulation: drawn indices (bs_cap)
+  double* bs_traj = nullptr;   // and the trajectory (bs_cap x D)
+  int bs_cap = 0;
   void* ref_model = nullptr;   // RefModelDev<T>
   void* as_w = nullptr;        // Npad log-weights of the ancestor draw
   void* as_scratch = nullptr;  // their tile scratch
@@ -143,6 +170,8 @@ int check_model(ssm_ctx* ctx, const ssm_lg_model* m) { return bf_check_model(ctx
 
 template <typename T>
 int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref);
+template <typename T>
+int upload_transition_t(ssm_bf* bf);
 
 TileScratch tile_scratch(ssm_bf* bf) {
   ResampleArgs a;
@@ -416,7 +445,7 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
                   bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u,     bf->xref,
                   bf->ref_model, bf->as_w,     bf->as_scratch, bf->as_sc, bf->as_idx,
-                  bf->draw_idx,  bf->seed_dev};
+                  bf->draw_idx,  bf->seed_dev, bf->bs_idx,   bf->bs_traj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bf->ev0) (void)hipEventDestroy(bf->ev0);
@@ -432,6 +461,7 @@ extern "C" int32_t ssm_bf_set_model(ssm_bf* bf, const ssm_lg_model* model) {
   if (model->Dx != bf->D || model->Dy != bf->Dy)
     return set_error(bf->ctx, SSM_ERR_ARG, "ssm_bf_set_model: dimensions changed");
   rc = (bf->prec == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
+  bf->trans_ok = false;
   if (rc || bf->ref_T < 0) return rc;
   // the reference's transition constants follow Q
   return (bf->prec == SSM_F32) ? upload_reference_t<float>(bf, bf->ref_T, bf->xref_host.data())
@@ -909,7 +939,17 @@ int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref) {
   std::vector<T> h(n);
   for (size_t e = 0; e < n; ++e) h[e] = (T)x_ref[e];
   if (x_ref != bf->xref_host.data()) bf->xref_host.assign(x_ref, x_ref + n);
-  // RefModelDev: L_Q^{-1} and -1/2 (D log 2 pi + log det Q), from the FP64 Q
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->xref, h.data(), sizeof(T) * n, hipMemcpyHostToDevice,
+                                    ctx->stream));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // h is a host local
+  return upload_transition_t<T>(bf);
+}
+
+// RefModelDev: L_Q^{-1} and -1/2 (D log 2 pi + log det Q), from the FP64 Q.
+template <typename T>
+int upload_transition_t(ssm_bf* bf) {
+  ssm_ctx* ctx = bf->ctx;
+  const int D = bf->D;
   std::vector<double> LQ = bf->Q_host;
   if (!chol_lower(D, LQ)) return set_error(ctx, SSM_ERR_NOT_PD, "Q is not positive definite");
   const std::vector<double> LQi = tri_inverse(D, LQ);
@@ -920,9 +960,9 @@ int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref) {
   for (int e = 0; e < D * D; ++e) rm.LQi[e] = (T)LQi[e];
   rm.cst = (T)(-0.5 * ((double)D * 1.8378770664093454835606594728112 + logdet));
   hipStream_t s = ctx->stream;
-  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->xref, h.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->ref_model, &rm, sizeof rm, hipMemcpyHostToDevice, s));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  bf->trans_ok = true;
   return SSM_OK;
 }
 
@@ -1009,3 +1049,99 @@ extern "C" int32_t ssm_bf_sample_index(ssm_bf* bf, double u, int32_t* index) {
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
   return SSM_OK;
 }
+
+// ---- backward simulation over a dense history (CSMC-BS) -------------------------------
+
+namespace ssm {
+
+namespace {
+
+template <typename T>
+int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
+  ssm_ctx* ctx = bf->ctx;
+  hipStream_t s = ctx->stream;
+  const int N = bf->N, Npad = bf->Npad, D = bf->D, Tn = hv.T;
+  const size_t plane = (size_t)D * Npad;
+  const dim3 g((N + kStepThreads - 1) / kStepThreads), b(kStepThreads);
+  const BfModelDev<T>* md = (const BfModelDev<T>*)bf->model_dev;
+  const RefModelDev<T>* rm = (const RefModelDev<T>*)bf->ref_model;
+  const T* x0 = (const T*)hv.x0;
+  const T* st = (const T*)hv.states;
+  const int nb = (N + kTile - 1) / kTile;
+  const double logN = std::log((double)N);
+  ResampleArgs a;
+  a.kind = SSM_MULTINOMIAL;
+  a.n = N;
+  a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
+  a.w = bf->as_w;
+  a.seed = bf->seed;
+  a.seed_dev = bf->seed_dev;
+  a.sc = bf->as_sc;
+  resample_bind_scratch(a, bf->as_scratch, N);
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
+  for (int t = Tn; t >= 0; --t) {
+    const T* x = (t == 0) ? x0 : st + (size_t)(t - 1) * plane;
+    const double* lw = (t == 0) ? nullptr : hv.logw + (size_t)(t - 1) * Npad;
+    const T* xn = (t == Tn) ? nullptr : st + (size_t)t * plane;  // generation t + 1
+    const int32_t* in = bf->bs_idx + t + 1;  // read only when xn is set (t < Tn)
+    if (D == 4)
+      hipLaunchKernelGGL((k_bs_weights<T, 4>), g, b, 0, s, N, Npad, D, md, rm, x, lw, xn, in,
+                         (T*)bf->as_w, bf->as_sc);
+    else
+      hipLaunchKernelGGL((k_bs_weights<T, 0>), g, b, 0, s, N, Npad, D, md, rm, x, lw, xn, in,
+                         (T*)bf->as_w, bf->as_sc);
+    hipLaunchKernelGGL(k_tile_partials<T>, dim3(nb), dim3(kTileThreads), 0, s, N,
+                       (const T*)bf->as_w, (const T*)nullptr, bf->as_sc, -1.0, a.partials);
+    launch_pf_finish<T>(s, N, nb, bf->as_sc, INFINITY, logN, ts, nullptr, nullptr, 0);
+    const int rc = launch_categorical_one(ctx, s, a, u[t], kStreamAncestor, bf->bs_idx + t);
+    if (rc) return rc;
+  }
+  const int ne = (Tn + 1) * D;
+  hipLaunchKernelGGL(k_bs_gather<T>, dim3((ne + 255) / 256), dim3(256), 0, s, Tn, D, Npad, x0,
+                     st, bf->bs_idx, bf->bs_traj);
+  SSM_HIP_CHECK(ctx, hipGetLastError());
+  return SSM_OK;
+}
+
+}  // namespace
+
+int bf_backward_sample(ssm_bf* bf, const BsHistory& hv, const double* u, int32_t* idx,
+                       double* traj) {
+  ssm_ctx* ctx = bf->ctx;
+  for (int t = 0; t <= hv.T; ++t)
+    if (!(u[t] >= 0.0 && u[t] < 1.0))
+      return set_error(ctx, SSM_ERR_ARG, "backward simulation: u[%d] outside [0, 1)", t);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_csmc_buffers(bf);
+  if (rc) return rc;
+  if (!bf->trans_ok) {
+    rc = (bf->prec == SSM_F32) ? upload_transition_t<float>(bf) : upload_transition_t<double>(bf);
+    if (rc) return rc;
+  }
+  if (hv.T + 1 > bf->bs_cap) {
+    if (bf->bs_idx) (void)hipFree(bf->bs_idx);
+    if (bf->bs_traj) (void)hipFree(bf->bs_traj);
+    bf->bs_idx = nullptr;
+    bf->bs_traj = nullptr;
+    bf->bs_cap = 0;
+    const int cap = std::max(hv.T + 1, 64);
+    if (hipMalloc(&bf->bs_idx, sizeof(int32_t) * (cap + 1)) != hipSuccess ||
+        hipMalloc(&bf->bs_traj, sizeof(double) * cap * bf->D) != hipSuccess)
+      return set_error(ctx, SSM_ERR_OOM, "backward simulation buffers");
+    bf->bs_cap = cap;
+  }
+  rc = (bf->prec == SSM_F32) ? backward_sample_t<float>(bf, hv, u)
+                             : backward_sample_t<double>(bf, hv, u);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  if (idx)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(idx, bf->bs_idx, sizeof(int32_t) * (hv.T + 1),
+                                      hipMemcpyDeviceToHost, s));
+  if (traj)
+    SSM_HIP_CHECK(ctx, hipMemcpyAsync(traj, bf->bs_traj, sizeof(double) * (hv.T + 1) * bf->D,
+                                      hipMemcpyDeviceToHost, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  return SSM_OK;
+}
+
+}  // namespace ssm

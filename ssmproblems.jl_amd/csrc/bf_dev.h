@@ -385,6 +385,47 @@ __device__ __forceinline__ void bf_as_weight_item(int N, int Npad, int D, const 
   w_out[i] = nl + (rm->cst - (T)0.5 * quad);
 }
 
+// Backward-simulation weight of stored particle i of generation t (CSMC-BS, csmc.jl:418-437
+// with ancestor_weight, ancestor_sampling.jl:125-130): its stored normalised log-weight
+// (FP64; 0 for the time-0 particles, whose weights are the lazy TypelessZero) plus
+// log N(x*; A x_i + b, Q), x* the state drawn at t + 1 (particle *idx_next of the plane
+// x_next).  x_next null leaves the log-weight alone (the draw at the last time).
+template <typename T, int DT>
+__device__ __forceinline__ void bf_bs_weight_item(int N, int Npad, int D,
+                                                  const BfModelDev<T>* model,
+                                                  const RefModelDev<T>* rm, const T* x,
+                                                  const double* logw, const T* x_next,
+                                                  const int32_t* idx_next, T* w_out, int i) {
+  constexpr int MD = DT ? DT : kMaxD;
+  if (DT) D = DT;
+  if (i >= N) return;
+  T nl = logw ? (T)logw[i] : (T)0;
+  if (x_next) {
+    const BfModelDev<T>& m = *model;
+    const int j = *idx_next;
+    T r[MD];
+#pragma unroll
+    for (int a = 0; a < MD; ++a) {
+      if (a < D) {
+        T s = m.b[a];
+        for (int c = 0; c < D; ++c) s += m.A[a + c * D] * x[(size_t)c * Npad + i];
+        r[a] = x_next[(size_t)a * Npad + j] - s;
+      }
+    }
+    T quad = 0;
+#pragma unroll
+    for (int a = 0; a < MD; ++a) {
+      if (a < D) {
+        T v = 0;
+        for (int c = 0; c <= a; ++c) v += rm->LQi[a + c * D] * r[c];
+        quad += v * v;
+      }
+    }
+    nl += rm->cst - (T)0.5 * quad;
+  }
+  w_out[i] = nl;
+}
+
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).
 template <typename T, int DT>
 __device__ __forceinline__ void bf_init_item(int N, int Npad, int D, const BfModelDev<T>* model,

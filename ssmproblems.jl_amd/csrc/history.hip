@@ -220,6 +220,16 @@ extern "C" int32_t ssm_bf_history_record(ssm_history* h, int32_t on) {
   return bf_record_history(h->bf, h, on != 0);
 }
 
+extern "C" int32_t ssm_bf_history_backward_sample(ssm_history* h, const double* u,
+                                                  int32_t* indices, double* trajectory) {
+  if (!h || !u) return SSM_ERR_ARG;
+  ssm_ctx* ctx = h->ctx;
+  if (!h->bf) return set_error(ctx, SSM_ERR_STATE, "particle history: its filter was destroyed");
+  if (h->T < 1) return set_error(ctx, SSM_ERR_STATE, "backward simulation: no generation stored");
+  BsHistory hv{h->T, h->x0, h->states, h->logw};
+  return bf_backward_sample(h->bf, hv, u, indices, trajectory);
+}
+
 extern "C" int32_t ssm_bf_history_reset(ssm_history* h) {
   if (!h) return SSM_ERR_ARG;
   ssm_ctx* ctx = h->ctx;

@@ -135,6 +135,17 @@ void history_record_push(ssm_history* h, hipStream_t s, const BfView& v);
 void history_record_commit(ssm_history* h, int T);
 // Makes h the filter's recording history (on) or stops recording into it (off).
 int bf_record_history(ssm_bf* bf, ssm_history* h, bool on);
+// Stored generations 0..T of a dense history (the filter's precision and layout).
+struct BsHistory {
+  int T;
+  const void* x0;      // time-0 state planes
+  const void* states;  // generation t at (t - 1) D Npad
+  const double* logw;  // normalised log-weights of generation t at (t - 1) Npad
+};
+// Backward simulation with the filter's transition density: idx[t], t = T .. 0, drawn with
+// uniform u[t]; traj (T + 1) x D.  Host outputs (either may be null).
+int bf_backward_sample(ssm_bf* bf, const BsHistory& hv, const double* u, int32_t* idx,
+                       double* traj);
 
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the

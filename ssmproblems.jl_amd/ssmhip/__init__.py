@@ -19,7 +19,8 @@ from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, ru
 from .serve import Server
 from .tree import ParallelParticleTree, ParallelAncestorCallback
 from .history import DenseParticleContainer, DenseAncestorCallback
-from .csmc import (ConditionalSMC, NoRefreshment, AncestorSampling, ReferenceTrajectory,
+from .csmc import (ConditionalSMC, NoRefreshment, AncestorSampling, BackwardSimulation,
+                   ReferenceTrajectory,
                    CSMCModel, CSMCState, CSMCWorkspace, csmc_sample, mcmc_step)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

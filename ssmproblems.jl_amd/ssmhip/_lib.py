@@ -154,6 +154,7 @@ def _declare(L):
         "ssm_bf_history_push": ([_vp], i32),
         "ssm_bf_history_reset": ([_vp], i32),
         "ssm_bf_history_record": ([_vp, i32], i32),
+        "ssm_bf_history_backward_sample": ([_vp, _dp, _ip, _dp], i32),
         "ssm_bf_history_info": ([_vp, _ip, _ip], i32),
         "ssm_bf_history_ancestry": ([_vp, i32, i32, _dp], i32),
         "ssm_bf_history_generation": ([_vp, i32, _dp, _dp, _ip], i32),

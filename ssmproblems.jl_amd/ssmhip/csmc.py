@@ -9,8 +9,9 @@ particle 0 is pinned to the reference (initialise and each step), its ancestor i
 filter's steps fill themselves (a homogeneous model runs the whole sweep as one
 graph-replayed call), and the returned trajectory is the path of one particle drawn from
 the final weights (_sample_trajectory, csmc.jl:111-117).  ``mcmc_step`` is the AbstractMCMC
-interface (csmc.jl:450-462).  The model must be linear-Gaussian; BackwardSimulation and the
-Rao-Blackwellised CSMC are not provided.
+interface (csmc.jl:450-462).  BackwardSimulation draws the trajectory backwards through the device
+history with the transition density (csmc.jl:392-446).  The model must be
+linear-Gaussian; the Rao-Blackwellised CSMC is not provided.
 """
 from __future__ import annotations
 
@@ -34,6 +35,12 @@ class NoRefreshment:
 class AncestorSampling:
     """CSMC-AS / PGAS: the reference's ancestor is redrawn every step (csmc.jl:21-29)."""
     mode = _lib.SSM_CSMC_ANCESTOR
+
+
+class BackwardSimulation:
+    """CSMC-BS: a plain conditional forward pass, then the trajectory drawn backwards with
+    the transition density (csmc.jl:31-41)."""
+    mode = _lib.SSM_CSMC_PLAIN
 
 
 @dataclass
@@ -136,6 +143,11 @@ def csmc_sample(rng, model: StateSpaceModel, csmc: ConditionalSMC, observations,
         for t in range(1, K + 1):
             _, inc = step(rng, model, pf, t, st, ys[t - 1])
             ll += inc
+    if isinstance(csmc.refreshment, BackwardSimulation):
+        # the last time's draw first, then t = K-1 .. 0, as csmc.jl:417-444 consumes rng
+        u = rng.random(K + 1)[::-1]
+        _, path = ws.history.backward_sample(u)
+        return ReferenceTrajectory(path[0], path[1:]), ll
     idx = st.sample_index(float(rng.random()))
     path = ws.history.get_ancestry(idx)
     return ReferenceTrajectory(path[0], path[1:]), ll

@@ -194,6 +194,14 @@ def filter(rng, model: StateSpaceModel, algo, observations, callback=None, ctx=N
         ys = np.asarray(observations, dtype=np.float64).reshape(len(observations), -1)
         if callback is None and model.is_homogeneous:
             return state, run(state, algo, ys)
+        container = getattr(callback, "container", None)
+        if getattr(callback, "records_on_device", False) and model.is_homogeneous:
+            # a dense history takes every generation inside the run's own graphs
+            container.record(True)
+            try:
+                return state, run(state, algo, ys)
+            finally:
+                container.record(False)
         state, ll = step(rng, model, algo, 1, state, ys[0], callback=callback)
         for t in range(2, ys.shape[0] + 1):
             state, inc = step(rng, model, algo, t, state, ys[t - 1], callback=callback)

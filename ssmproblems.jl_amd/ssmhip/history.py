@@ -34,6 +34,18 @@ class DenseParticleContainer:
         generation on the device; do not push those steps by hand as well."""
         self.ctx.check(self.ctx.lib.ssm_bf_history_record(self.ptr, int(bool(on))))
 
+    def backward_sample(self, u):
+        """Backward simulation over the stored generations with uniforms u[0..T] (u[t] for
+        time t): (indices (T + 1,), trajectory (T + 1, Dx))."""
+        T = len(self)
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(T + 1))
+        idx = np.zeros(T + 1, dtype=np.int32)
+        traj = np.zeros((T + 1, self.Dx))
+        self.ctx.check(self.ctx.lib.ssm_bf_history_backward_sample(
+            self.ptr, u.ctypes.data_as(_lib._dp), idx.ctypes.data_as(_lib._ip),
+            traj.ctypes.data_as(_lib._dp)))
+        return idx, traj
+
     def reset(self):
         """Empties the container and stores the filter's current particles as time 0."""
         self.ctx.check(self.ctx.lib.ssm_bf_history_reset(self.ptr))
@@ -88,7 +100,10 @@ class DenseParticleContainer:
 
 class DenseAncestorCallback:
     """callback for ``ssmhip.filter``: stores time 0 at PostInit and every generation at
-    PostUpdate (callbacks.jl:53-80)."""
+    PostUpdate (callbacks.jl:53-80).  On a homogeneous model ``filter`` keeps its single
+    graph-replayed run and the history records the generations on the device."""
+
+    records_on_device = True
 
     def __init__(self, capacity=64):
         self.capacity = capacity

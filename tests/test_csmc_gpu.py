@@ -158,7 +158,7 @@ def test_run_equals_steps_with_reference(ctx, orc, mode):
     assert np.array_equal(out[0][1][0][0], x_ref[T].astype(np.float32).astype(np.float64))
 
 
-@pytest.mark.parametrize("refresh", ["plain", "ancestor"])
+@pytest.mark.parametrize("refresh", ["plain", "ancestor", "backward"])
 def test_csmc_chain_matches_smoothing_mean(ctx, orc, refresh):
     # the reference's CSMC tests (Dx = Dy = 1, K = 10, N = 10, threshold 0.6): the chain's
     # mean of every x_t against the exact smoothing mean (the reference checks t = 2 only)
@@ -166,7 +166,8 @@ def test_csmc_chain_matches_smoothing_mean(ctx, orc, refresh):
     ys = np.random.default_rng(11).standard_normal((10, 1)) + 0.5
     mean, cov = orc.joint_posterior(m, ys)
     model = ssmhip.CSMCModel(to_ssmhip_model(m), ys)
-    refreshment = ssmhip.AncestorSampling() if refresh == "ancestor" else ssmhip.NoRefreshment()
+    refreshment = {"plain": ssmhip.NoRefreshment(), "ancestor": ssmhip.AncestorSampling(),
+                   "backward": ssmhip.BackwardSimulation()}[refresh]
     csmc = ssmhip.ConditionalSMC(ssmhip.BF(10, threshold=0.6), refreshment)
     rng = np.random.default_rng(12)
     state = None
@@ -192,6 +193,37 @@ def test_csmc_chain_matches_smoothing_mean(ctx, orc, refresh):
     ll_exact = orc.joint_gaussian_loglik(m, ys)
     assert rel_se < 0.1
     assert abs(L.max() + np.log(e.mean()) + ll_exact) < 5 * rel_se
+
+
+@pytest.mark.parametrize("Dx,Dy,N,K", [(4, 2, 20000, 12), (3, 2, 777, 9), (1, 1, 10, 10)])
+def test_backward_sample_matches_oracle(ctx, orc, Dx, Dy, N, K):
+    # CSMC-BS's backward pass over the device history (FP64) against the oracle's on the
+    # exported generations with the same uniforms: the same indices, and the trajectory is
+    # the stored states at those indices
+    rng = np.random.default_rng(40 + Dx)
+    m = config2_model() if Dx == 4 else random_lg(rng, Dx, Dy)
+    if Dx != 4:
+        m.A = 0.9 * m.A / max(abs(np.linalg.eigvals(m.A)))
+    model = to_ssmhip_model(m)
+    algo = ssmhip.BF(N, threshold=0.6, precision="f64")
+    _, _, ys = orc.simulate_lg(m, K, rng)
+    st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=9)
+    st.set_reference(rng.standard_normal((K + 1, Dx)), 0)
+    ctx.check(ctx.lib.ssm_bf_initialise(st.ptr, None))
+    h = ssmhip.DenseParticleContainer(st, capacity=4)
+    h.record(True)
+    ssmhip.run(st, algo, ys)
+    assert len(h) == K
+    xs = [h.generation(t)[0] for t in range(K + 1)]
+    lws = [h.generation(t)[1] for t in range(1, K + 1)]
+    for trial in range(3):
+        u = rng.random(K + 1)
+        idx, traj = h.backward_sample(u)
+        idx_o, traj_o = orc.backward_sample(m, xs, lws, u)
+        np.testing.assert_array_equal(idx, idx_o)
+        np.testing.assert_array_equal(traj, traj_o)
+    with pytest.raises(ssmhip.SSMError):
+        h.backward_sample(np.full(K + 1, 1.0))
 
 
 def test_reference_errors(ctx):

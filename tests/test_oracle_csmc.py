@@ -75,13 +75,13 @@ def test_pinned_step_equals_plain_step_off_the_reference(threshold):
     np.testing.assert_allclose(d, d[0], atol=1e-12)
 
 
-def _chain(m, ys, ancestor_sampling, sweeps, burn, seed, N=10):
+def _chain(m, ys, refresh, sweeps, burn, seed, N=10):
     rng = np.random.default_rng(seed)
-    traj, _ = oracle.csmc_sweep(m, ys, N, None, ancestor_sampling, 0.6, oracle.SYSTEMATIC, rng)
+    asamp, back = refresh == "ancestor", refresh == "backward"
+    traj, _ = oracle.csmc_sweep(m, ys, N, None, asamp, 0.6, oracle.SYSTEMATIC, rng, back)
     trajs, lls = [], []
     for i in range(burn + sweeps):
-        traj, ll = oracle.csmc_sweep(m, ys, N, traj, ancestor_sampling, 0.6, oracle.SYSTEMATIC,
-                                     rng)
+        traj, ll = oracle.csmc_sweep(m, ys, N, traj, asamp, 0.6, oracle.SYSTEMATIC, rng, back)
         if i >= burn:
             trajs.append(traj[:, 0])
             lls.append(ll)
@@ -95,8 +95,8 @@ def batch_se(v, nbatch=40):
     return bm.std(axis=0, ddof=1) / np.sqrt(nbatch)
 
 
-@pytest.mark.parametrize("ancestor_sampling", [False, True])
-def test_csmc_chain_matches_smoothing_mean(ancestor_sampling):
+@pytest.mark.parametrize("refresh", ["plain", "ancestor", "backward"])
+def test_csmc_chain_matches_smoothing_mean(refresh):
     # the reference's setting (Dx = Dy = 1, K = 10, N = 10, threshold 0.6) with a fixed
     # model; the reference checks x_2 only, every time is checked here (dropping the
     # transition density from the ancestor weights biases x_0, x_1, x_3 and x_9 by 7-37 SE
@@ -104,7 +104,7 @@ def test_csmc_chain_matches_smoothing_mean(ancestor_sampling):
     m = oracle.LG([0.2], [[1.0]], [[0.9]], [0.1], [[0.5]], [[1.2]], [-0.1], [[0.4]])
     ys = np.random.default_rng(11).standard_normal((10, 1)) + 0.5
     mean, cov = oracle.joint_posterior(m, ys)
-    trajs, lls = _chain(m, ys, ancestor_sampling, sweeps=6000, burn=200, seed=12)
+    trajs, lls = _chain(m, ys, refresh, sweeps=6000, burn=200, seed=12)
     se = batch_se(trajs)
     sd = np.sqrt(np.diag(cov))
     assert np.all(se < 0.05 * sd), (se, sd)  # the chain mixes well enough to judge
@@ -118,3 +118,45 @@ def test_csmc_chain_matches_smoothing_mean(ancestor_sampling):
     lr = L.max() + np.log(e.mean())
     assert rel_se < 0.1, rel_se
     assert abs(lr + ll_exact) < 5 * rel_se, (lr, -ll_exact, rel_se)
+
+
+def test_backward_sample_exact_distribution():
+    # N = 3 particles, K = 2.  Index 2 ~ w_2; index 1 | i2 ~ softmax(lw_1 + log f(x2*[i2] | x1));
+    # index 0 | i1 ~ softmax(log f(x1*[i1] | x0)).  Each conditional is checked by sweeping its
+    # uniform over a fine grid with the later draws held at the middle of their intervals
+    rng = np.random.default_rng(8)
+    m = _model(rng, 2, 1)
+    xs = [rng.standard_normal((3, 2)) for _ in range(3)]
+    lws = []
+    for _ in range(2):
+        lw = rng.standard_normal(3)
+        lws.append(lw - np.log(np.exp(lw).sum()))
+
+    def softmax(v):
+        e = np.exp(v - v.max())
+        return e / e.sum()
+
+    def logf(x_next, x):
+        return multivariate_normal.logpdf(x_next, m.A @ x + m.b, m.Q)
+
+    def mid(w, k):
+        c = np.concatenate([[0.0], np.cumsum(w)])
+        return 0.5 * (c[k] + c[k + 1])
+
+    grid = (np.arange(400) + 0.5) / 400
+    w2 = softmax(lws[1])
+    for i2 in range(3):
+        w1 = softmax(np.array([lws[0][i] + logf(xs[2][i2], xs[1][i]) for i in range(3)]))
+        got = [oracle.backward_sample(m, xs, lws, [0.5, u1, mid(w2, i2)])[0] for u1 in grid]
+        assert all(g[2] == i2 for g in got)
+        freq = np.bincount([g[1] for g in got], minlength=3) / len(grid)
+        np.testing.assert_allclose(freq, w1, atol=1.01 / len(grid))
+        for i1 in range(3):
+            w0 = softmax(np.array([logf(xs[1][i1], xs[0][i]) for i in range(3)]))
+            got = [oracle.backward_sample(m, xs, lws, [u0, mid(w1, i1), mid(w2, i2)])
+                   for u0 in grid]
+            assert all(g[0][1] == i1 and g[0][2] == i2 for g in got)
+            freq = np.bincount([g[0][0] for g in got], minlength=3) / len(grid)
+            np.testing.assert_allclose(freq, w0, atol=1.01 / len(grid))
+            for idx, traj in got[:5]:
+                np.testing.assert_array_equal(traj, [xs[t][idx[t]] for t in range(3)])
