@@ -40,32 +40,96 @@ __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
   bf_step_tile<T, DT, DYT, NOISE, REF>(p, blockIdx.x);
 }
 
-// Conditional SMC: ancestor-sampling log-weights for the reference state at the filter's
-// current step (thread 0 also hands the step to the scalars of their fold).
+// Conditional SMC: ancestor-sampling log-weights for the reference state x_ref[t] at the
+// filter's current step t (GF/src/ancestor_sampling.jl:64-130 with the generic
+// future_conditional_density): each particle's normalised log-weight (0 for the lazy
+// uniform weights) plus log N(x_ref[t]; A x_i + b, Q).  One block per tile, with the
+// tile's partials for their fold (thread 0 of tile 0 also hands the step to the fold's
+// scalars).
 template <typename T, int DT>
-__global__ void __launch_bounds__(kStepThreads)
+__global__ void __launch_bounds__(kTileThreads)
     k_as_weights(int N, int Npad, int D, const BfModelDev<T>* model, const RefModelDev<T>* rm,
                  const T* x, const T* logw, const BfScalars* sc, const T* xref, T* w_out,
-                 BfScalars* as_sc) {
+                 BfScalars* as_sc, double* partials) {
+  constexpr int MD = DT ? DT : kMaxD;
+  if (DT) D = DT;
   const int t = sc->step;
-  const int i = blockIdx.x * kStepThreads + threadIdx.x;
-  if (i == 0) {
+  const bool typeless = sc->typeless;
+  const T lse = (T)sc->lse;
+  const int tile0 = blockIdx.x * kTile;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     as_sc->step = t;
     as_sc->typeless = 0;
   }
-  bf_as_weight_item<T, DT>(N, Npad, D, model, rm, x, logw, sc, xref + (size_t)t * D, w_out, i);
+  // every load first (clamped in-bounds indices, no branch), then the arithmetic
+  T xs[MD], xv[kTileItems][MD], lw[kTileItems];
+  bool valid[kTileItems];
+#pragma unroll
+  for (int a = 0; a < MD; ++a) xs[a] = (a < D) ? xref[(size_t)t * D + a] : (T)0;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    valid[k] = i < N;
+    const int ic = valid[k] ? i : N - 1;
+#pragma unroll
+    for (int a = 0; a < MD; ++a) xv[k][a] = (a < D) ? x[(size_t)a * Npad + ic] : (T)0;
+    lw[k] = logw[ic];
+  }
+  T nl[kTileItems], lv[kTileItems];
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    const T pre = typeless ? (T)0 : lw[k] - lse;
+    const T w = pre + transition_logpdf<T, MD>(D, *model, *rm, xv[k], xs);
+    if (valid[k]) w_out[i] = w;
+    lv[k] = valid[k] ? w : (T)0;
+    nl[k] = (T)0;
+  }
+  tile_partial<T>(nl, lv, valid, true, partials, gridDim.x, blockIdx.x);
 }
 
-// CSMC-BS: backward-simulation log-weights of one stored generation (thread 0 also marks
-// the scalars of their fold as explicit weights).
+// CSMC-BS: backward-simulation log-weights of stored generation t (csmc.jl:418-437 with
+// ancestor_weight, ancestor_sampling.jl:125-130): each particle's stored normalised
+// log-weight (FP64; none for the time-0 particles, whose weights are the lazy TypelessZero)
+// plus log N(x*; A x_i + b, Q), x* the state drawn at t + 1 (particle *idx_next of plane
+// x_next); x_next null leaves the log-weights alone (the draw at the last time).  One block
+// per tile with the tile's partials (thread 0 also marks the fold's weights explicit).
 template <typename T, int DT>
-__global__ void __launch_bounds__(kStepThreads)
+__global__ void __launch_bounds__(kTileThreads)
     k_bs_weights(int N, int Npad, int D, const BfModelDev<T>* model, const RefModelDev<T>* rm,
                  const T* x, const double* logw, const T* x_next, const int32_t* idx_next,
-                 T* w_out, BfScalars* as_sc) {
-  const int i = blockIdx.x * kStepThreads + threadIdx.x;
-  if (i == 0) as_sc->typeless = 0;
-  bf_bs_weight_item<T, DT>(N, Npad, D, model, rm, x, logw, x_next, idx_next, w_out, i);
+                 T* w_out, BfScalars* as_sc, double* partials) {
+  constexpr int MD = DT ? DT : kMaxD;
+  if (DT) D = DT;
+  const int tile0 = blockIdx.x * kTile;
+  if (blockIdx.x == 0 && threadIdx.x == 0) as_sc->typeless = 0;
+  // x* (the state drawn at t + 1) and every particle's operands first, then the arithmetic
+  T xs[MD], xv[kTileItems][MD];
+  double lw[kTileItems];
+  bool valid[kTileItems];
+  const int j = x_next ? *idx_next : 0;
+#pragma unroll
+  for (int a = 0; a < MD; ++a) xs[a] = (x_next && a < D) ? x_next[(size_t)a * Npad + j] : (T)0;
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    valid[k] = i < N;
+    const int ic = valid[k] ? i : N - 1;
+#pragma unroll
+    for (int a = 0; a < MD; ++a) xv[k][a] = (a < D) ? x[(size_t)a * Npad + ic] : (T)0;
+    lw[k] = logw ? logw[ic] : 0.0;
+  }
+  T nl[kTileItems], lv[kTileItems];
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    T w = (T)lw[k];
+    if (x_next) w += transition_logpdf<T, MD>(D, *model, *rm, xv[k], xs);
+    if (valid[k]) w_out[i] = w;
+    lv[k] = valid[k] ? w : (T)0;
+    nl[k] = (T)0;
+  }
+  tile_partial<T>(nl, lv, valid, true, partials, gridDim.x, blockIdx.x);
 }
 
 // The drawn trajectory: traj[t D + d] = generation t's state d of particle idx[t].
@@ -283,21 +347,14 @@ void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
 
 // Conditional SMC with ancestor sampling: the reference particle's ancestor for this step,
 // drawn from softmax(log w_i + log N(x_ref[t]; A x_i + b, Q)) (GF/src/algorithms/
-// csmc.jl:276-292) into bf->as_idx: the weights, their tile partials, their fold and one
-// categorical draw with a Philox uniform of the ancestor stream.
+// csmc.jl:276-292) into bf->as_idx: the weights with their tile partials, their fold and
+// one categorical draw with a Philox uniform of the ancestor stream.
 template <typename T>
 int enqueue_ancestor_draw_t(ssm_bf* bf, hipStream_t s) {
-  const dim3 g((bf->N + kStepThreads - 1) / kStepThreads), b(kStepThreads);
+  const int nb = (bf->N + kTile - 1) / kTile;
+  const dim3 g(nb), b(kTileThreads);
   const BfModelDev<T>* md = (const BfModelDev<T>*)bf->model_dev;
   const RefModelDev<T>* rm = (const RefModelDev<T>*)bf->ref_model;
-  if (bf->D == 4)
-    hipLaunchKernelGGL((k_as_weights<T, 4>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
-                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
-                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc);
-  else
-    hipLaunchKernelGGL((k_as_weights<T, 0>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
-                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
-                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc);
   ResampleArgs a;
   a.kind = SSM_MULTINOMIAL;
   a.n = bf->N;
@@ -307,9 +364,14 @@ int enqueue_ancestor_draw_t(ssm_bf* bf, hipStream_t s) {
   a.seed_dev = bf->seed_dev;
   a.sc = bf->as_sc;
   resample_bind_scratch(a, bf->as_scratch, bf->N);
-  const int nb = (bf->N + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_tile_partials<T>, dim3(nb), dim3(kTileThreads), 0, s, bf->N,
-                     (const T*)bf->as_w, (const T*)nullptr, bf->as_sc, -1.0, a.partials);
+  if (bf->D == 4)
+    hipLaunchKernelGGL((k_as_weights<T, 4>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
+                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
+                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc, a.partials);
+  else
+    hipLaunchKernelGGL((k_as_weights<T, 0>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
+                       (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
+                       (const T*)bf->xref, (T*)bf->as_w, bf->as_sc, a.partials);
   const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
   // threshold +Inf: the fold treats the weights as fresh, as their partials were formed;
   // record 0 leaves the step counter at t (the Philox step of the draw)
@@ -1062,7 +1124,6 @@ int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
   hipStream_t s = ctx->stream;
   const int N = bf->N, Npad = bf->Npad, D = bf->D, Tn = hv.T;
   const size_t plane = (size_t)D * Npad;
-  const dim3 g((N + kStepThreads - 1) / kStepThreads), b(kStepThreads);
   const BfModelDev<T>* md = (const BfModelDev<T>*)bf->model_dev;
   const RefModelDev<T>* rm = (const RefModelDev<T>*)bf->ref_model;
   const T* x0 = (const T*)hv.x0;
@@ -1085,13 +1146,11 @@ int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
     const T* xn = (t == Tn) ? nullptr : st + (size_t)t * plane;  // generation t + 1
     const int32_t* in = bf->bs_idx + t + 1;  // read only when xn is set (t < Tn)
     if (D == 4)
-      hipLaunchKernelGGL((k_bs_weights<T, 4>), g, b, 0, s, N, Npad, D, md, rm, x, lw, xn, in,
-                         (T*)bf->as_w, bf->as_sc);
+      hipLaunchKernelGGL((k_bs_weights<T, 4>), dim3(nb), dim3(kTileThreads), 0, s, N, Npad, D,
+                         md, rm, x, lw, xn, in, (T*)bf->as_w, bf->as_sc, a.partials);
     else
-      hipLaunchKernelGGL((k_bs_weights<T, 0>), g, b, 0, s, N, Npad, D, md, rm, x, lw, xn, in,
-                         (T*)bf->as_w, bf->as_sc);
-    hipLaunchKernelGGL(k_tile_partials<T>, dim3(nb), dim3(kTileThreads), 0, s, N,
-                       (const T*)bf->as_w, (const T*)nullptr, bf->as_sc, -1.0, a.partials);
+      hipLaunchKernelGGL((k_bs_weights<T, 0>), dim3(nb), dim3(kTileThreads), 0, s, N, Npad, D,
+                         md, rm, x, lw, xn, in, (T*)bf->as_w, bf->as_sc, a.partials);
     launch_pf_finish<T>(s, N, nb, bf->as_sc, INFINITY, logN, ts, nullptr, nullptr, 0);
     const int rc = launch_categorical_one(ctx, s, a, u[t], kStreamAncestor, bf->bs_idx + t);
     if (rc) return rc;

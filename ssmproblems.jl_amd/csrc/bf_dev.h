@@ -351,26 +351,22 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     body(std::false_type{});
 }
 
-// Ancestor-sampling weight of particle i for the reference state x_ref (conditional SMC,
-// GF/src/ancestor_sampling.jl:64-130 with the generic future_conditional_density): its
-// normalised log-weight (0 for the lazy uniform weights) plus log N(x_ref; A x_i + b, Q).
-template <typename T, int DT>
-__device__ __forceinline__ void bf_as_weight_item(int N, int Npad, int D, const BfModelDev<T>* model,
-                                                  const RefModelDev<T>* rm, const T* x,
-                                                  const T* logw, const BfScalars* sc,
-                                                  const T* xref_t, T* w_out, int i) {
-  constexpr int MD = DT ? DT : kMaxD;
-  if (DT) D = DT;
-  if (i >= N) return;
-  const BfModelDev<T>& m = *model;
-  const T nl = sc->typeless ? (T)0 : logw[i] - (T)sc->lse;
+// log N(x*; A x + b, Q) = cst - |L_Q^{-1} (x* - A x - b)|^2 / 2 for one particle's state x
+// and a target state x* (both in registers): the transition density of ancestor sampling
+// and backward simulation.
+template <typename T, int MD>
+__device__ __forceinline__ T transition_logpdf(int D, const BfModelDev<T>& m,
+                                               const RefModelDev<T>& rm, const T x[MD],
+                                               const T xs[MD]) {
   T r[MD];
 #pragma unroll
   for (int a = 0; a < MD; ++a) {
     if (a < D) {
       T s = m.b[a];
-      for (int c = 0; c < D; ++c) s += m.A[a + c * D] * x[(size_t)c * Npad + i];
-      r[a] = xref_t[a] - s;
+#pragma unroll
+      for (int c = 0; c < MD; ++c)
+        if (c < D) s += m.A[a + c * D] * x[c];
+      r[a] = xs[a] - s;
     }
   }
   T quad = 0;
@@ -378,52 +374,12 @@ __device__ __forceinline__ void bf_as_weight_item(int N, int Npad, int D, const 
   for (int a = 0; a < MD; ++a) {
     if (a < D) {
       T v = 0;
-      for (int c = 0; c <= a; ++c) v += rm->LQi[a + c * D] * r[c];
+#pragma unroll
+      for (int c = 0; c <= a; ++c) v += rm.LQi[a + c * D] * r[c];
       quad += v * v;
     }
   }
-  w_out[i] = nl + (rm->cst - (T)0.5 * quad);
-}
-
-// Backward-simulation weight of stored particle i of generation t (CSMC-BS, csmc.jl:418-437
-// with ancestor_weight, ancestor_sampling.jl:125-130): its stored normalised log-weight
-// (FP64; 0 for the time-0 particles, whose weights are the lazy TypelessZero) plus
-// log N(x*; A x_i + b, Q), x* the state drawn at t + 1 (particle *idx_next of the plane
-// x_next).  x_next null leaves the log-weight alone (the draw at the last time).
-template <typename T, int DT>
-__device__ __forceinline__ void bf_bs_weight_item(int N, int Npad, int D,
-                                                  const BfModelDev<T>* model,
-                                                  const RefModelDev<T>* rm, const T* x,
-                                                  const double* logw, const T* x_next,
-                                                  const int32_t* idx_next, T* w_out, int i) {
-  constexpr int MD = DT ? DT : kMaxD;
-  if (DT) D = DT;
-  if (i >= N) return;
-  T nl = logw ? (T)logw[i] : (T)0;
-  if (x_next) {
-    const BfModelDev<T>& m = *model;
-    const int j = *idx_next;
-    T r[MD];
-#pragma unroll
-    for (int a = 0; a < MD; ++a) {
-      if (a < D) {
-        T s = m.b[a];
-        for (int c = 0; c < D; ++c) s += m.A[a + c * D] * x[(size_t)c * Npad + i];
-        r[a] = x_next[(size_t)a * Npad + j] - s;
-      }
-    }
-    T quad = 0;
-#pragma unroll
-    for (int a = 0; a < MD; ++a) {
-      if (a < D) {
-        T v = 0;
-        for (int c = 0; c <= a; ++c) v += rm->LQi[a + c * D] * r[c];
-        quad += v * v;
-      }
-    }
-    nl += rm->cst - (T)0.5 * quad;
-  }
-  w_out[i] = nl;
+  return rm.cst - (T)0.5 * quad;
 }
 
 // initialise: x_i = mu0 + chol(Sigma0).L z_i, log-weights 0 (particles.jl:36-50).
