@@ -22,5 +22,8 @@ from .history import DenseParticleContainer, DenseAncestorCallback
 from .csmc import (ConditionalSMC, NoRefreshment, AncestorSampling, BackwardSimulation,
                    ReferenceTrajectory,
                    CSMCModel, CSMCState, CSMCWorkspace, csmc_sample, mcmc_step)
+from .particle_gibbs import (ParticleGibbs, ParticleGibbsModel, ParticleGibbsState,
+                             ParticleGibbsTransition, ParameterisedSSM, SSMParameterLogDensity,
+                             RandomWalkMH, trajectory_logdensity, pg_step, pg_sample)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import default_context
 from .filters import BootstrapFilter, DeviceParticles, rng_to_seed, run, step
 from .history import DenseParticleContainer
-from .models import StateSpaceModel
+from .models import PackedLG, StateSpaceModel
 
 
 class NoRefreshment:
@@ -131,9 +131,9 @@ def csmc_sample(rng, model: StateSpaceModel, csmc: ConditionalSMC, observations,
         if xr.shape[0] != K + 1:
             raise ValueError(f"reference has {xr.shape[0]} states, need {K + 1}")
         st.set_reference(xr, csmc.refreshment.mode)
-    if not model.is_homogeneous:
-        from .models import PackedLG
-        st.set_params(PackedLG.from_model(model, 1))
+    packed = PackedLG.from_model(model, 1)
+    if not packed.same_as(st.packed):  # a new model (particle Gibbs' parameter draw)
+        st.set_params(packed)
     ws.ctx.check(lib.ssm_bf_initialise(st.ptr, None))
     ws.history.reset()
     if model.is_homogeneous:  # all K steps in one call, replayed from HIP graphs

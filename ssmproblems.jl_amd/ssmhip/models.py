@@ -153,6 +153,11 @@ class PackedLG:
     def ref(self):
         return C.byref(self.struct)
 
+    def same_as(self, other) -> bool:
+        """True when both hold the same dimensions and parameter values."""
+        return (other is not None and self.Dx == other.Dx and self.Dy == other.Dy and
+                all(np.array_equal(a, b) for a, b in zip(self._bufs, other._bufs)))
+
 
 # ---- hierarchical models (hierarchical.jl:4-45) -----------------------------------------
 
