@@ -700,14 +700,15 @@ def bench_csmc(args, budget_s=10.0):
     print(json.dumps(out), flush=True)
 
 
-def bench_kfll(args, Dx=8, Dy=4, T=100, budget_s=10.0):
+def bench_kfll(args, T=100, budget_s=10.0):
     """kf_loglikelihood as a batched likelihood service: B filters of one shape with
     per-step parameters (As[t], bs[t], ... of logdensity.jl:149-161), seeded stable
-    dynamics.  One bench step = one call evaluating all B likelihoods."""
+    dynamics.  One bench step = one call evaluating all B likelihoods.  The default shape,
+    Dx = Dy = 2, is the reference's own kf_loglikelihood test model size."""
     import ssmhip
     ctx = ssmhip.default_context()
     name, cus = ctx.device_info()
-    B = args.B
+    B, Dx, Dy = args.B, args.Dx, args.Dy
     log(f"device {name} CUs={cus}; B={B} Dx={Dx} Dy={Dy} T={T}")
     rng = np.random.default_rng(31)
     qr = np.linalg.qr(rng.standard_normal((B, T, Dx, Dx)))[0]
@@ -766,7 +767,7 @@ def bench_kfll(args, Dx=8, Dy=4, T=100, budget_s=10.0):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_dev,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (FP64 Cholesky of S)",
+        "dtype": "f64" if Dx <= 4 and Dy <= 4 else "f32 (FP64 Cholesky of S)",
         "data": "synthetic (seeded stable time-varying LG parameters and observations)",
         "config": {"workload": f"kf_loglikelihood, B={B}, Dx={Dx}, Dy={Dy}, T={T}, per-step "
                                f"parameters, inputs resident on the device", "B": B, "T": T},
@@ -774,7 +775,9 @@ def bench_kfll(args, Dx=8, Dy=4, T=100, budget_s=10.0):
         "loglik_max_rel_err": float(max(errs)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_kf_batch<16,16,NW,true>", "bytes_per_call": bytes_run},
+                     "kernel": ("k_kf_tiny (one filter per lane, FP64)" if Dx <= 4 and Dy <= 4
+                                else "k_kf_batch<..., true>"),
+                     "bytes_per_call": bytes_run},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -804,8 +807,10 @@ def main():
                          "batched kf_loglikelihood with per-step parameters")
     ap.add_argument("--requests", type=int, default=256, help="serve: requests per phase")
     ap.add_argument("--B", type=int, default=None,
-                    help="kf, kfll: number of filters (default 65536 for kf, 4096 for kfll)")
+                    help="kf, kfll: number of filters (default 65536)")
     ap.add_argument("--K", type=int, default=100, help="csmc: steps per sweep")
+    ap.add_argument("--Dx", type=int, default=2, help="kfll: state dimension")
+    ap.add_argument("--Dy", type=int, default=2, help="kfll: observation dimension")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -816,7 +821,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.B is None:
-        args.B = 4096 if args.workload == "kfll" else 65536
+        args.B = 65536
     if args.workload == "kf":
         return bench_kf(args)
     if args.workload == "rbpf":

@@ -317,6 +317,237 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
   }
 }
 
+// Filters with Dx <= 4 and Dy <= 4 in the kf_loglikelihood layout (the TV arrays), one
+// per lane with every matrix in registers: the recursion of k_kf_batch (Cholesky of S with
+// its log-determinant and the solves, Joseph form, Symmetric(:U) plus jitter), all in FP64.
+// Such filters are too small for matrix-core tiles or a workgroup each, where a step costs
+// about the same for Dx = 2 as for Dx = 16; at this size FP64 fits the registers (256 VGPRs,
+// no spills) and leaves only the inputs' own conditioning in the error.  Matrices are
+// zero-padded to 4, so the fixed 4-term sums equal the dx- and dy-term ones.
+constexpr int kTinyD = 4;
+constexpr int kTinyThreads = 64;
+
+__global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
+  constexpr int M = kTinyD;
+  const int f = blockIdx.x * kTinyThreads + threadIdx.x;
+  if (f >= p.count) return;
+  const int dx = p.Dx[f], dy = p.Dy[f];
+  const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
+  double mu[M], S[M][M], A[M][M], Q[M][M], H[M][M], R[M][M], b[M], c[M];
+  const double* g_mu0 = p.tv_mu0 + (int64_t)f * dx;
+  const double* g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    mu[i] = (i < dx) ? g_mu0[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
+  }
+  auto load = [&](int64_t blk) {
+    const double* gA = p.tv_A + blk * nxx;
+    const double* gQ = p.tv_Q + blk * nxx;
+    const double* gH = p.tv_H + blk * nyx;
+    const double* gR = p.tv_R + blk * nyy;
+    const double* gb = p.tv_b + blk * dx;
+    const double* gc = p.tv_c + blk * dy;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      b[i] = (i < dx) ? gb[i] : 0.0;
+      c[i] = (i < dy) ? gc[i] : 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        A[i][j] = (i < dx && j < dx) ? gA[i + j * dx] : 0.0;
+        Q[i][j] = (i < dx && j < dx) ? gQ[i + j * dx] : 0.0;
+        H[i][j] = (i < dy && j < dx) ? gH[i + j * dy] : 0.0;
+        R[i][j] = (i < dy && j < dy) ? gR[i + j * dy] : 0.0;
+      }
+    }
+  };
+  const int64_t blk0 = (int64_t)f * p.tv_blocks;
+  load(blk0);
+  const double jit = p.jitter;
+  const double* yrow = p.ys + p.y_off[f];
+  const int Tn = p.Tf ? p.Tf[f] : p.T;
+  double ll = 0.0;
+  int ok = 1;
+  for (int t = 0; t < Tn; ++t) {
+    if (p.tv_blocks > 1 && t > 0) load(blk0 + t);
+    double y[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) y[q] = (q < dy) ? yrow[(size_t)t * dy + q] : 0.0;
+    // predict: mu_hat = A mu + b, Sigma = A Sigma A' + Q
+    double mh[M], T1[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double s = b[i];
+#pragma unroll
+      for (int k = 0; k < M; ++k) s += A[i][k] * mu[k];
+      mh[i] = s;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double u = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += A[i][k] * S[k][j];
+        T1[i][j] = u;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double u = Q[i][j];
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += T1[i][k] * A[j][k];
+        S[i][j] = u;
+      }
+    // update: z = y - H mu_hat - c, P = H Sigma, S_y = P H' + R
+    double z[M], P[M][M];
+    double Sy[M][M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      double s = c[q];
+#pragma unroll
+      for (int k = 0; k < M; ++k) s += H[q][k] * mh[k];
+      z[q] = (q < dy) ? y[q] - s : 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double u = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += H[q][k] * S[k][j];
+        P[q][j] = u;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < M; ++q)
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        double u = R[q][r];
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += P[q][k] * H[r][k];
+        Sy[q][r] = (double)u;
+      }
+    // FP64 Cholesky of S_y (lower triangle read), log-determinant, L^-1 z
+    double L[M][M], ldet = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+#pragma unroll
+      for (int i = 0; i < M; ++i) L[i][j] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      if (j < dy) {
+        double d = Sy[j][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        if (!(d > 0.0)) ok = 0;
+        const double ljj = sqrt(d > 0.0 ? d : 1.0);
+        L[j][j] = ljj;
+        ldet += 2.0 * log(ljj);
+#pragma unroll
+        for (int i = j + 1; i < M; ++i) {
+          if (i < dy) {
+            double v = Sy[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+            L[i][j] = v / ljj;
+          }
+        }
+      }
+    }
+    if (!ok) break;
+    double w[M], qd = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      if (i < dy) {
+        double v = (double)z[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v -= L[i][k] * w[k];
+        w[i] = v / L[i][i];
+        qd += w[i] * w[i];
+      } else {
+        w[i] = 0.0;
+      }
+    }
+    ll += -0.5 * ((double)dy * kLog2Pi + ldet + qd);
+    // K' = S_y^-1 P (forward and back substitution per column of P), K = (K')'
+    double K[M][M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double v[M];
+#pragma unroll
+      for (int i = 0; i < M; ++i) {
+        if (i < dy) {
+          double s = (double)P[i][j];
+#pragma unroll
+          for (int k = 0; k < i; ++k) s -= L[i][k] * v[k];
+          v[i] = s / L[i][i];
+        } else {
+          v[i] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int i = M - 1; i >= 0; --i) {
+        if (i < dy) {
+          double s = v[i];
+#pragma unroll
+          for (int k = i + 1; k < M; ++k)
+            if (k < dy) s -= L[k][i] * v[k];
+          v[i] = s / L[i][i];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < M; ++q) K[j][q] = (q < dy && j < dx) ? v[q] : 0.0;
+    }
+    // mu = mu_hat + K z; Joseph: Sigma = (I - K H) Sigma (I - K H)' + K R K'
+    double Mm[M][M], KR[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      double s = mh[i];
+#pragma unroll
+      for (int q = 0; q < M; ++q) s += K[i][q] * z[q];
+      mu[i] = (i < dx) ? s : 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double u = (i == j && i < dx) ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) u -= K[i][q] * H[q][j];
+        Mm[i][j] = u;
+        double r = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) r += K[i][q] * R[q][j];
+        KR[i][j] = r;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double u = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += Mm[i][k] * S[k][j];
+        T1[i][j] = u;
+      }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+      for (int j = i; j < M; ++j) {  // the upper triangle, mirrored below (Symmetric(:U))
+        double u = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += T1[i][k] * Mm[j][k];
+#pragma unroll
+        for (int k = 0; k < M; ++k) u += KR[i][k] * K[j][k];
+        S[i][j] = u;
+      }
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) S[i][j] = S[j][i];
+      if (i < dx) S[i][i] += jit;
+    }
+  }
+  p.ll_out[f] = ok ? ll : NAN;
+  p.status[f] = ok ? SSM_OK : SSM_ERR_NOT_PD;
+}
+
 }  // namespace ssm
 using namespace ssm;
 
@@ -791,8 +1022,14 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   for (hipEvent_t& e : ctx->kfll_ev)
     if (!e) SSM_HIP_CHECK(ctx, hipEventCreate(&e));
   SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[0], s));
-  rc = launch_class_nw(ctx, s, p, cls, class_nw(cls));
-  if (rc) return rc;
+  if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane
+    hipLaunchKernelGGL(k_kf_tiny, dim3((B + kTinyThreads - 1) / kTinyThreads),
+                       dim3(kTinyThreads), 0, s, p);
+    SSM_HIP_CHECK(ctx, hipGetLastError());
+  } else {
+    rc = launch_class_nw(ctx, s, p, cls, class_nw(cls));
+    if (rc) return rc;
+  }
   SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[1], s));
   std::vector<int32_t> st(B);
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_out, d_ll, 8 * (size_t)B, hipMemcpyDeviceToHost, s));
