@@ -423,6 +423,24 @@ int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, in
                              const double* A, const double* b, const double* Q, const double* H,
                              const double* c, const double* R, const double* ys, double jitter,
                              double* ll_out, int32_t* status_out);
+/*
+ * The reverse rule of kf_loglikelihood (GF/src/integrations/kalman_rrule.jl:22-107, with the
+ * backward recursion of GF/src/algorithms/kalman_gradient.jl:103-263): the log-likelihoods
+ * as ssm_kf_loglikelihood and their gradients w.r.t. mu0, Sigma0, A, b, Q, H, c and R, each
+ * in its input's layout (P blocks; with time_varying = 0 the gradient of the one shared block
+ * sums the steps).  Not w.r.t. ys or jitter, as in the reference.  Filters up to
+ * Dx = Dy = 4 (one per lane, FP64, the forward pass caching 80 doubles per step on the
+ * device); larger ones are refused.  Gradients of a filter whose status is SSM_ERR_NOT_PD are
+ * undefined.  Any g_* may be NULL.
+ */
+int32_t ssm_kf_loglikelihood_grad(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                                  int32_t time_varying, const double* mu0, const double* Sigma0,
+                                  const double* A, const double* b, const double* Q,
+                                  const double* H, const double* c, const double* R,
+                                  const double* ys, double jitter, double* ll_out,
+                                  int32_t* status_out, double* g_mu0, double* g_Sigma0,
+                                  double* g_A, double* g_b, double* g_Q, double* g_H,
+                                  double* g_c, double* g_R);
 /* Device time of the last ssm_kf_loglikelihood's kernel on this context, from HIP events
  * (ms; the host-to-device copies of its inputs are outside). */
 int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms);

@@ -468,6 +468,81 @@ def joint_gaussian_tv(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs):
     return mu_Z, Sig_Z
 
 
+def kf_loglikelihood_grad(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=0.0):
+    """The log-likelihood and its gradients w.r.t. mu0, Sigma0 and every step's A, b, Q, H,
+    c, R: the reference's reverse rule (GF/src/integrations/kalman_rrule.jl:22-107) with the
+    backward recursion of GF/src/algorithms/kalman_gradient.jl:103-263 (Parellier et al.),
+    in FP64 numpy.  The forward pass is the Joseph-form filter with Symmetric(:U) and
+    jitter (kalman_gradient.jl:12-29).  Returns (ll, grads) with grads[name] a list over
+    steps (or one array for mu0 and Sigma0), in the log-likelihood convention."""
+    T = len(As)
+    ys = np.asarray(ys, dtype=np.float64).reshape(T, -1)
+    mu = _f64(mu0).copy()
+    S_ = np.asarray(Sigma0, dtype=np.float64).copy()
+    ll = 0.0
+    caches = []
+    for t in range(T):
+        A, b, Q = (np.asarray(v, dtype=np.float64) for v in (As[t], bs[t], Qs[t]))
+        H, c, R = (np.atleast_2d(np.asarray(v, dtype=np.float64)) if k != 1 else
+                   np.asarray(v, dtype=np.float64) for k, v in enumerate((Hs[t], cs[t], Rs[t])))
+        mu_prev, S_prev = mu.copy(), S_.copy()
+        mu_pred = A @ mu + b                                   # kalman.jl:42-49
+        S_pred = A @ S_ @ A.T + Q
+        z = ys[t] - H @ mu_pred - c                            # kalman_gradient.jl:12-29
+        Sy = H @ S_pred @ H.T + R
+        K = np.linalg.solve(Sy, H @ S_pred).T
+        IKH = np.eye(len(mu)) - K @ H
+        Sf = IKH @ S_pred @ IKH.T + K @ R @ K.T
+        if jitter:
+            Sf = Sf + jitter * np.eye(len(mu))
+        Sf = np.triu(Sf) + np.triu(Sf, 1).T
+        mu_f = mu_pred + K @ z
+        L = np.linalg.cholesky(Sy)
+        w = np.linalg.solve(L, z)
+        ll += -0.5 * (len(z) * np.log(2 * np.pi) + 2 * np.log(np.diag(L)).sum() + w @ w)
+        caches.append(dict(mu_prev=mu_prev, S_prev=S_prev, mu_pred=mu_pred, S_pred=S_pred,
+                           mu_f=mu_f, z=z, S=Sy, K=K, IKH=IKH, A=A, H=H, R=R))
+        mu, S_ = mu_f, Sf
+    # backward pass (kalman_rrule.jl:63-103), NLL convention, negated at the end
+    dmu = np.zeros_like(mu)
+    dS = np.zeros_like(S_)
+    g = {k: [None] * T for k in ("A", "b", "Q", "H", "c", "R")}
+    for t in range(T - 1, -1, -1):
+        cc = caches[t]
+        z, S, K, IKH, H, R, A = cc["z"], cc["S"], cc["K"], cc["IKH"], cc["H"], cc["R"], cc["A"]
+        Siz = np.linalg.solve(S, z)
+        Sinv = np.linalg.inv(S)
+        # gradient_c, kalman_gradient.jl:256-263
+        g["c"][t] = -Siz - K.T @ dmu
+        # gradient_H, kalman_gradient.jl:216-246
+        P_f = IKH @ cc["S_pred"]
+        dH = Sinv @ H @ cc["S_pred"] - np.outer(Siz, cc["mu_pred"]) \
+            - np.outer(Siz, Siz) @ H @ cc["S_pred"]
+        dH = dH + np.outer(Siz, dmu) @ P_f - np.outer(K.T @ dmu, cc["mu_f"])
+        dH = dH - 2 * K.T @ dS @ P_f
+        g["H"][t] = dH
+        # gradient_R, kalman_gradient.jl:160-174
+        cross = 0.5 * (np.outer(K.T @ dmu, Siz) + np.outer(Siz, K.T @ dmu))
+        g["R"][t] = K.T @ dS @ K - cross + 0.5 * (Sinv - np.outer(Siz, Siz))
+        # backward_gradient_update, kalman_gradient.jl:103-124
+        dmu_pred = IKH.T @ dmu - H.T @ Siz
+        Riz = np.linalg.solve(R, z)
+        cross2 = 0.5 * (np.outer(dmu, H.T @ Riz) + np.outer(H.T @ Riz, dmu))
+        dS_pred = IKH.T @ (dS + cross2) @ IKH \
+            + 0.5 * (H.T @ Sinv @ H - H.T @ np.outer(Siz, Siz) @ H)
+        # dynamics parameters, kalman_gradient.jl:196-208, :150-152
+        g["b"][t] = dmu_pred
+        g["A"][t] = np.outer(dmu_pred, cc["mu_prev"]) + 2 * dS_pred @ A @ cc["S_prev"]
+        g["Q"][t] = dS_pred
+        # backward_gradient_predict, kalman_gradient.jl:139-143
+        dmu = A.T @ dmu_pred
+        dS = A.T @ dS_pred @ A
+    grads = {k: [-v for v in vals] for k, vals in g.items()}
+    grads["mu0"] = -dmu
+    grads["Sigma0"] = -dS
+    return ll, grads
+
+
 def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=0.0):
     """kf_loglikelihood (GF/src/integrations/logdensity.jl:149-161): kalman_predict then the
     Joseph-form update with jitter, step parameters As[t] ..., summed increments."""

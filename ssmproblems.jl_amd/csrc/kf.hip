@@ -548,6 +548,490 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
   p.status[f] = ok ? SSM_OK : SSM_ERR_NOT_PD;
 }
 
+
+// ---- gradients of kf_loglikelihood for filters up to 4 x 4 -----------------------------
+// The reverse rule of GF/src/integrations/kalman_rrule.jl:22-107 with the backward recursion
+// of GF/src/algorithms/kalman_gradient.jl:103-263 (Parellier et al.), one filter per lane in
+// FP64.  The forward pass is k_kf_tiny's and leaves a cache per step in a filter-minor
+// array (entry e of step t of filter f at (t kGradCache + e) B + f, so the lanes of a wave
+// touch consecutive words); the backward pass walks the steps in reverse in the same lane.
+constexpr int kGradCache = 80;  // mu_prev 4, S_prev 16, mu_pred 4, S_pred 16, mu_f 4, z 4, S 16, K 16
+
+struct KfGradOut {
+  double* mu0;    // B x Dx
+  double* Sigma0; // B x Dx x Dx
+  double *A, *b, *Q, *H, *c, *R;  // the parameter layout of the inputs
+  double* cache;  // T x kGradCache x B
+  int B;
+};
+
+namespace {
+using M4 = double[4][4];
+__device__ __forceinline__ void mm4(const M4 a, const M4 b, M4 c) {  // c = a b
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += a[i][k] * b[k][j];
+      c[i][j] = s;
+    }
+}
+__device__ __forceinline__ void mtm4(const M4 a, const M4 b, M4 c) {  // c = a' b
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += a[k][i] * b[k][j];
+      c[i][j] = s;
+    }
+}
+// Lower Cholesky factor of the leading n x n block of a (identity past n); false if not PD.
+__device__ __forceinline__ bool chol4(const M4 a, int n, M4 L) {
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) L[i][j] = (i == j && i >= n) ? 1.0 : 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j < n) {
+      double d = a[j][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+      if (!(d > 0.0)) ok = false;
+      L[j][j] = sqrt(d > 0.0 ? d : 1.0);
+#pragma unroll
+      for (int i = j + 1; i < 4; ++i) {
+        if (i < n) {
+          double v = a[i][j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+          L[i][j] = v / L[j][j];
+        }
+      }
+    }
+  }
+  return ok;
+}
+// x = (L L')^-1 v
+__device__ __forceinline__ void cholsolve4(const M4 L, const double v[4], double x[4]) {
+  double y[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double s = v[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+#pragma unroll
+  for (int i = 3; i >= 0; --i) {
+    double s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 4; ++k) s -= L[k][i] * x[k];
+    x[i] = s / L[i][i];
+  }
+}
+}  // namespace
+
+__global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, KfGradOut g) {
+  const int f = blockIdx.x * kTinyThreads + threadIdx.x;
+  if (f >= p.count) return;
+  const int dx = p.Dx[f], dy = p.Dy[f];
+  const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
+  const int B = g.B;
+  auto C = [&](int t, int e) -> double& { return g.cache[((size_t)t * kGradCache + e) * B + f]; };
+  const double* g_mu0 = p.tv_mu0 + (int64_t)f * dx;
+  const double* g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
+  double mu[4], S[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    mu[i] = (i < dx) ? g_mu0[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
+  }
+  auto blk = [&](int t) -> int64_t { return (int64_t)f * p.tv_blocks + (p.tv_blocks > 1 ? t : 0); };
+  auto loadM = [&](const double* base, int64_t off, int r, int c, M4 X) {
+    const double* q = base + off;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) X[i][j] = (i < r && j < c) ? q[i + j * r] : 0.0;
+  };
+  auto loadV = [&](const double* base, int64_t off, int n, double* v) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (i < n) ? base[off + i] : 0.0;
+  };
+  const double* yrow = p.ys + p.y_off[f];
+  const int Tn = p.T;
+  double ll = 0.0;
+  int ok = 1;
+  // ---- forward (kalman_rrule.jl:27-50), caching each step ----
+  for (int t = 0; t < Tn && ok; ++t) {
+    const int64_t k = blk(t);
+    M4 A, Q, H, R;
+    double b[4], c[4];
+    loadM(p.tv_A, k * nxx, dx, dx, A);
+    loadM(p.tv_Q, k * nxx, dx, dx, Q);
+    loadM(p.tv_H, k * nyx, dy, dx, H);
+    loadM(p.tv_R, k * nyy, dy, dy, R);
+    loadV(p.tv_b, k * dx, dx, b);
+    loadV(p.tv_c, k * dy, dy, c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      C(t, i) = mu[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) C(t, 4 + 4 * i + j) = S[i][j];
+    }
+    double mp[4];
+    M4 T1, Sp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = b[i];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) s += A[i][kk] * mu[kk];
+      mp[i] = s;
+    }
+    mm4(A, S, T1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double s = Q[i][j];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) s += T1[i][kk] * A[j][kk];
+        Sp[i][j] = s;
+      }
+    double z[4];
+    M4 P, Sy;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = c[q];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) s += H[q][kk] * mp[kk];
+      z[q] = (q < dy) ? yrow[(size_t)t * dy + q] - s : 0.0;
+    }
+    mm4(H, Sp, P);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double s = R[q][r];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) s += P[q][kk] * H[r][kk];
+        Sy[q][r] = s;
+      }
+    M4 L;
+    if (!chol4(Sy, dy, L)) {
+      ok = 0;
+      break;
+    }
+    double w[4], qd = 0.0, ldet = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = z[i];
+#pragma unroll
+      for (int kk = 0; kk < i; ++kk) s -= L[i][kk] * w[kk];
+      w[i] = s / L[i][i];
+      if (i < dy) {
+        qd += w[i] * w[i];
+        ldet += 2.0 * log(L[i][i]);
+      }
+    }
+    ll += -0.5 * ((double)dy * kLog2Pi + ldet + qd);
+    M4 K;  // K = (S^-1 P)'
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double col[4], x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) col[q] = P[q][j];
+      cholsolve4(L, col, x);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) K[j][q] = (q < dy && j < dx) ? x[q] : 0.0;
+    }
+    M4 IKH, KR;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double u = (i == j && i < dx) ? 1.0 : 0.0, r = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          u -= K[i][q] * H[q][j];
+          r += K[i][q] * R[q][j];
+        }
+        IKH[i][j] = u;
+        KR[i][j] = r;
+      }
+    double mf[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = mp[i];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s += K[i][q] * z[q];
+      mf[i] = (i < dx) ? s : 0.0;
+    }
+    mm4(IKH, Sp, T1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i; j < 4; ++j) {  // upper triangle, mirrored (Symmetric(:U)), plus jitter
+        double s = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) s += T1[i][kk] * IKH[j][kk] + KR[i][kk] * K[j][kk];
+        S[i][j] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < i; ++j) S[i][j] = S[j][i];
+      if (i < dx) S[i][i] += p.jitter;
+      mu[i] = mf[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      C(t, 20 + i) = mp[i];
+      C(t, 40 + i) = mf[i];
+      C(t, 44 + i) = z[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        C(t, 24 + 4 * i + j) = Sp[i][j];
+        C(t, 48 + 4 * i + j) = Sy[i][j];
+        C(t, 64 + 4 * i + j) = K[i][j];
+      }
+    }
+  }
+  p.ll_out[f] = ok ? ll : NAN;
+  p.status[f] = ok ? SSM_OK : SSM_ERR_NOT_PD;
+  if (!ok) return;
+  // ---- backward (kalman_rrule.jl:63-103), NLL convention; outputs are negated ----
+  const bool acc = p.tv_blocks <= 1;  // one parameter set for all steps: sum the steps
+  double dmu[4] = {0.0, 0.0, 0.0, 0.0};
+  M4 dS;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dS[i][j] = 0.0;
+  auto put = [&](double* out, int64_t off, int r, int cc, const M4 X) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i < r && j < cc) {
+          double& o = out[off + i + j * r];
+          o = (acc ? o : 0.0) - X[i][j];
+        }
+  };
+  auto putv = [&](double* out, int64_t off, int n, const double* v) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < n) {
+        double& o = out[off + i];
+        o = (acc ? o : 0.0) - v[i];
+      }
+  };
+  if (acc) {  // the sums start at zero
+    const int64_t k = blk(0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i < nxx) {
+        g.A[k * nxx + i] = 0.0;
+        g.Q[k * nxx + i] = 0.0;
+      }
+      if (i < nyx) g.H[k * nyx + i] = 0.0;
+      if (i < nyy) g.R[k * nyy + i] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < dx) g.b[k * dx + i] = 0.0;
+      if (i < dy) g.c[k * dy + i] = 0.0;
+    }
+  }
+  for (int t = Tn - 1; t >= 0; --t) {
+    const int64_t k = blk(t);
+    M4 A, H, R, Sp, Sy, K, Sprev;
+    double mprev[4], mp[4], mf[4], z[4];
+    loadM(p.tv_A, k * nxx, dx, dx, A);
+    loadM(p.tv_H, k * nyx, dy, dx, H);
+    loadM(p.tv_R, k * nyy, dy, dy, R);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      mprev[i] = C(t, i);
+      mp[i] = C(t, 20 + i);
+      mf[i] = C(t, 40 + i);
+      z[i] = C(t, 44 + i);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        Sprev[i][j] = C(t, 4 + 4 * i + j);
+        Sp[i][j] = C(t, 24 + 4 * i + j);
+        Sy[i][j] = C(t, 48 + 4 * i + j);
+        K[i][j] = C(t, 64 + 4 * i + j);
+      }
+    }
+    M4 L, Sinv, IKH;
+    chol4(Sy, dy, L);
+    double Siz[4];
+    cholsolve4(L, z, Siz);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double e[4], x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) e[i] = (i == j) ? 1.0 : 0.0;
+      cholsolve4(L, e, x);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Sinv[i][j] = (i < dy && j < dy) ? x[i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double u = (i == j && i < dx) ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u -= K[i][q] * H[q][j];
+        IKH[i][j] = u;
+      }
+    double Ktd[4];  // K' dmu
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s += K[i][q] * dmu[i];
+      Ktd[q] = s;
+    }
+    // gradient_c (kalman_gradient.jl:256-263)
+    {
+      double gc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gc[q] = -Siz[q] - Ktd[q];
+      putv(g.c, k * dy, dy, gc);
+    }
+    // gradient_H (kalman_gradient.jl:216-246)
+    {
+      M4 Pf, HSp, T2, gH;
+      mm4(IKH, Sp, Pf);
+      mm4(H, Sp, HSp);
+      M4 KtdS;  // K' dS
+      mtm4(K, dS, KtdS);
+      mm4(KtdS, Pf, T2);
+      double SizTHSp[4], dmuPf[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double s = 0.0, u = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s += Siz[q] * HSp[q][j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) u += dmu[i] * Pf[i][j];
+        SizTHSp[j] = s;
+        dmuPf[j] = u;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) s += Sinv[q][r] * HSp[r][j];
+          gH[q][j] = s - Siz[q] * mp[j] - Siz[q] * SizTHSp[j] + Siz[q] * dmuPf[j] -
+                     Ktd[q] * mf[j] - 2.0 * T2[q][j];
+        }
+      put(g.H, k * nyx, dy, dx, gH);
+    }
+    // gradient_R (kalman_gradient.jl:160-174)
+    {
+      M4 KtdSK, gR;
+      M4 KtdS;
+      mtm4(K, dS, KtdS);
+      mm4(KtdS, K, KtdSK);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          gR[q][r] = KtdSK[q][r] - 0.5 * (Ktd[q] * Siz[r] + Siz[q] * Ktd[r]) +
+                     0.5 * (Sinv[q][r] - Siz[q] * Siz[r]);
+      put(g.R, k * nyy, dy, dy, gR);
+    }
+    // backward_gradient_update (kalman_gradient.jl:103-124)
+    double dmp[4];
+    M4 dSp;
+    {
+      double HtSiz[4], Riz[4], HtRiz[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double s = 0.0, u = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          s += H[q][i] * Siz[q];
+          u += IKH[q][i] * dmu[q];
+        }
+        HtSiz[i] = s;
+        dmp[i] = u - s;
+      }
+      M4 LR;
+      chol4(R, dy, LR);
+      cholsolve4(LR, z, Riz);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s += H[q][i] * (q < dy ? Riz[q] : 0.0);
+        HtRiz[i] = s;
+      }
+      M4 inner, T3, HtSinvH;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          inner[i][j] = dS[i][j] + 0.5 * (dmu[i] * HtRiz[j] + HtRiz[i] * dmu[j]);
+      mtm4(IKH, inner, T3);   // IKH' inner
+      mm4(T3, IKH, dSp);      // IKH' inner IKH
+      M4 SinvH;
+      mm4(Sinv, H, SinvH);
+      mtm4(H, SinvH, HtSinvH);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          dSp[i][j] += 0.5 * (HtSinvH[i][j] - HtSiz[i] * HtSiz[j]);
+    }
+    // dynamics parameters (kalman_gradient.jl:150-152, :196-208)
+    putv(g.b, k * dx, dx, dmp);
+    put(g.Q, k * nxx, dx, dx, dSp);
+    {
+      M4 T4, T5, gA;
+      mm4(dSp, A, T4);
+      mm4(T4, Sprev, T5);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gA[i][j] = dmp[i] * mprev[j] + 2.0 * T5[i][j];
+      put(g.A, k * nxx, dx, dx, gA);
+    }
+    // backward_gradient_predict (kalman_gradient.jl:139-143)
+    {
+      M4 T6;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s += A[q][i] * dmp[q];
+        dmu[i] = s;
+      }
+      mtm4(A, dSp, T6);
+      mm4(T6, A, dS);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i < dx) g.mu0[(int64_t)f * dx + i] = -dmu[i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (i < dx && j < dx) g.Sigma0[(int64_t)f * nxx + i + j * dx] = -dS[i][j];
+  }
+}
+
 }  // namespace ssm
 using namespace ssm;
 
@@ -925,12 +1409,13 @@ extern "C" int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* 
 // kf_loglikelihood (GF/src/integrations/logdensity.jl:149-161) for B filters of one shape:
 // the inputs are copied into one device block (grown on the context as needed) and run by
 // the size class's persistent kernel in the TV form.
-extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy,
-                                        int32_t T, int32_t time_varying, const double* mu0,
-                                        const double* Sigma0, const double* A, const double* b,
-                                        const double* Q, const double* H, const double* c,
-                                        const double* R, const double* ys, double jitter,
-                                        double* ll_out, int32_t* status_out) {
+// gout: null, or the 8 host outputs of the gradients (mu0, Sigma0, A, b, Q, H, c, R), which
+// need Dx, Dy <= 4.
+static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                    int32_t time_varying, const double* mu0, const double* Sigma0,
+                    const double* A, const double* b, const double* Q, const double* H,
+                    const double* c, const double* R, const double* ys, double jitter,
+                    double* ll_out, int32_t* status_out, double* const* gout) {
   if (!ctx) return SSM_ERR_ARG;
   if (B < 1 || T < 1 || !mu0 || !Sigma0 || !A || !b || !Q || !H || !c || !R || !ys || !ll_out)
     return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: bad arguments");
@@ -950,7 +1435,11 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   const size_t n_mu0 = (size_t)B * Dx, n_S0 = (size_t)B * nxx, n_A = (size_t)(nb * nxx),
                n_b = (size_t)(nb * Dx), n_H = (size_t)(nb * nyx), n_c = (size_t)(nb * Dy),
                n_R = (size_t)(nb * nyy), n_y = (size_t)B * T * Dy;
-  const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B;
+  // with gradients: the 8 gradient arrays (the inputs' layout) and the per-step caches
+  const size_t n_grad = gout ? n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R : 0;
+  const size_t n_cache = gout ? (size_t)T * kGradCache * B : 0;
+  const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B +
+                       n_grad + n_cache;
   const size_t bytes = 8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256;
   int rc = ensure_device(ctx, &ctx->kfll, &ctx->kfll_bytes, bytes);
   if (rc) return rc;
@@ -965,7 +1454,9 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   double* d_R = d_c + n_c;
   double* d_y = d_R + n_R;
   double* d_ll = d_y + n_y;
-  int64_t* d_yoff = (int64_t*)(d_ll + B);
+  double* d_g = d_ll + B;           // gradients, then caches (empty without gout)
+  double* d_cache = d_g + n_grad;
+  int64_t* d_yoff = (int64_t*)(d_cache + n_cache);
   int32_t* d_Dx = (int32_t*)(d_yoff + B);
   int32_t* d_Dy = d_Dx + B;
   int32_t* d_order = d_Dy + B;
@@ -1022,7 +1513,23 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   for (hipEvent_t& e : ctx->kfll_ev)
     if (!e) SSM_HIP_CHECK(ctx, hipEventCreate(&e));
   SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[0], s));
-  if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane
+  // gradient outputs in the inputs' order: mu0, Sigma0, A, b, Q, H, c, R
+  const size_t g_len[8] = {n_mu0, n_S0, n_A, n_b, n_A, n_H, n_c, n_R};
+  double* g_dev[8];
+  {
+    double* q = d_g;
+    for (int k = 0; k < 8; ++k) {
+      g_dev[k] = q;
+      q += gout ? g_len[k] : 0;
+    }
+  }
+  if (gout) {
+    KfGradOut g{g_dev[0], g_dev[1], g_dev[2], g_dev[3], g_dev[4], g_dev[5], g_dev[6],
+                g_dev[7], d_cache, B};
+    hipLaunchKernelGGL(k_kf_tiny_grad, dim3((B + kTinyThreads - 1) / kTinyThreads),
+                       dim3(kTinyThreads), 0, s, p, g);
+    SSM_HIP_CHECK(ctx, hipGetLastError());
+  } else if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane
     hipLaunchKernelGGL(k_kf_tiny, dim3((B + kTinyThreads - 1) / kTinyThreads),
                        dim3(kTinyThreads), 0, s, p);
     SSM_HIP_CHECK(ctx, hipGetLastError());
@@ -1034,6 +1541,11 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
   std::vector<int32_t> st(B);
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(ll_out, d_ll, 8 * (size_t)B, hipMemcpyDeviceToHost, s));
   SSM_HIP_CHECK(ctx, hipMemcpyAsync(st.data(), d_status, 4 * (size_t)B, hipMemcpyDeviceToHost, s));
+  if (gout)
+    for (int k = 0; k < 8; ++k)
+      if (gout[k])
+        SSM_HIP_CHECK(ctx, hipMemcpyAsync(gout[k], g_dev[k], 8 * g_len[k],
+                                          hipMemcpyDeviceToHost, s));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
   float ms = 0.f;
   SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->kfll_ev[0], ctx->kfll_ev[1]));
@@ -1044,6 +1556,32 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
       return set_error(ctx, SSM_ERR_NOT_PD,
                        "filter %d: innovation covariance S is not positive definite", f);
   return SSM_OK;
+}
+
+extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy,
+                                        int32_t T, int32_t time_varying, const double* mu0,
+                                        const double* Sigma0, const double* A, const double* b,
+                                        const double* Q, const double* H, const double* c,
+                                        const double* R, const double* ys, double jitter,
+                                        double* ll_out, int32_t* status_out) {
+  return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
+                  ll_out, status_out, nullptr);
+}
+
+extern "C" int32_t ssm_kf_loglikelihood_grad(
+    ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T, int32_t time_varying,
+    const double* mu0, const double* Sigma0, const double* A, const double* b,
+    const double* Q, const double* H, const double* c, const double* R, const double* ys,
+    double jitter, double* ll_out, int32_t* status_out, double* g_mu0, double* g_Sigma0,
+    double* g_A, double* g_b, double* g_Q, double* g_H, double* g_c, double* g_R) {
+  if (!ctx) return SSM_ERR_ARG;
+  if (Dx < 1 || Dy < 1 || Dx > kTinyD || Dy > kTinyD)
+    return set_error(ctx, SSM_ERR_ARG,
+                     "ssm_kf_loglikelihood_grad: gradients need 1 <= Dx, Dy <= %d (got %d, %d)",
+                     kTinyD, Dx, Dy);
+  double* const gout[8] = {g_mu0, g_Sigma0, g_A, g_b, g_Q, g_H, g_c, g_R};
+  return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
+                  ll_out, status_out, gout);
 }
 
 extern "C" int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms) {

@@ -174,6 +174,21 @@ def kf_loglikelihood_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=N
     R (B, T, Dy, Dy) -- As[t] of the reference -- or one set per filter without the T axis;
     a mix is broadcast along T.  Returns the B log-likelihoods (and the status codes)."""
     ctx = ctx or default_context()
+    B, Dx, Dy, T, tv, shapes, mu0c, S0, flat, yv = _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys)
+    ll = np.zeros(B)
+    st = np.zeros(B, dtype=np.int32)
+    rc = ctx.lib.ssm_kf_loglikelihood(
+        ctx.ptr, B, Dx, Dy, T, int(tv), mu0c.ctypes.data_as(_lib._dp), S0.ctypes.data_as(_lib._dp),
+        *(flat[k].ctypes.data_as(_lib._dp) for k in ("A", "b", "Q", "H", "c", "R")),
+        yv.ctypes.data_as(_lib._dp), float(jitter or 0.0), ll.ctypes.data_as(_lib._dp),
+        st.ctypes.data_as(_lib._ip))
+    if rc and not return_status:
+        ctx.check(rc)
+    return (ll, st) if return_status else ll
+
+
+def _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys):
+    """Shapes, the time-varying flag and the flat column-major inputs of the C-ABI."""
     mu0 = np.asarray(mu0, dtype=np.float64)
     B, Dx = mu0.shape
     ys = np.asarray(ys, dtype=np.float64)
@@ -193,19 +208,35 @@ def kf_loglikelihood_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=N
             raise ValueError(f"{k}: shape {a.shape}, expected {lead + shp}")
         flat[k] = (_colmajor_blocks(a, *shp, lead) if len(shp) == 2
                    else np.ascontiguousarray(a.reshape(-1)))
-    S0 = _colmajor_blocks(Sigma0, Dx, Dx, (B,))
-    yv = np.ascontiguousarray(ys.reshape(-1))
-    mu0c = np.ascontiguousarray(mu0.reshape(-1))
+    return (B, Dx, Dy, T, tv, shapes, np.ascontiguousarray(mu0.reshape(-1)),
+            _colmajor_blocks(Sigma0, Dx, Dx, (B,)), flat,
+            np.ascontiguousarray(ys.reshape(-1)))
+
+
+def kf_loglikelihood_grad_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=None):
+    """kf_loglikelihood and its reverse rule for B filters up to 4 x 4 (kalman_rrule.jl:22-107):
+    returns (ll (B,), grads) with grads["mu0"] (B, Dx), grads["Sigma0"] (B, Dx, Dx) and
+    grads[k] for k in A, b, Q, H, c, R shaped like the inputs (per step, or summed over the
+    steps for one parameter set per filter)."""
+    ctx = ctx or default_context()
+    B, Dx, Dy, T, tv, shapes, mu0c, S0, flat, yv = _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys)
+    lead = (B, T) if tv else (B,)
+    out = {"mu0": np.zeros(B * Dx), "Sigma0": np.zeros(B * Dx * Dx)}
+    for k, shp in shapes.items():
+        out[k] = np.zeros(int(np.prod(lead + shp)))
     ll = np.zeros(B)
     st = np.zeros(B, dtype=np.int32)
-    rc = ctx.lib.ssm_kf_loglikelihood(
+    ctx.check(ctx.lib.ssm_kf_loglikelihood_grad(
         ctx.ptr, B, Dx, Dy, T, int(tv), mu0c.ctypes.data_as(_lib._dp), S0.ctypes.data_as(_lib._dp),
         *(flat[k].ctypes.data_as(_lib._dp) for k in ("A", "b", "Q", "H", "c", "R")),
         yv.ctypes.data_as(_lib._dp), float(jitter or 0.0), ll.ctypes.data_as(_lib._dp),
-        st.ctypes.data_as(_lib._ip))
-    if rc and not return_status:
-        ctx.check(rc)
-    return (ll, st) if return_status else ll
+        st.ctypes.data_as(_lib._ip),
+        *(out[k].ctypes.data_as(_lib._dp) for k in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R"))))
+    unflat = lambda v, r, cc, ld: np.swapaxes(v.reshape(*ld, cc, r), -1, -2)  # column-major blocks
+    grads = {"mu0": out["mu0"].reshape(B, Dx), "Sigma0": unflat(out["Sigma0"], Dx, Dx, (B,))}
+    for k, shp in shapes.items():
+        grads[k] = unflat(out[k], *shp, lead) if len(shp) == 2 else out[k].reshape(lead + shp)
+    return ll, grads
 
 
 def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=None, ctx=None):
