@@ -703,8 +703,9 @@ def bench_csmc(args, budget_s=10.0):
 def bench_kfll(args, T=100, budget_s=10.0):
     """kf_loglikelihood as a batched likelihood service: B filters of one shape with
     per-step parameters (As[t], bs[t], ... of logdensity.jl:149-161), seeded stable
-    dynamics.  One bench step = one call evaluating all B likelihoods.  The default shape,
-    Dx = Dy = 2, is the reference's own kf_loglikelihood test model size."""
+    dynamics.  One bench step = one call evaluating all B likelihoods (with --grad, also
+    their gradients w.r.t. every parameter, the reverse rule of kalman_rrule.jl:22-107).
+    The default shape, Dx = Dy = 2, is the reference's own kf_loglikelihood test model size."""
     import ssmhip
     ctx = ssmhip.default_context()
     name, cus = ctx.device_info()
@@ -724,19 +725,32 @@ def bench_kfll(args, T=100, budget_s=10.0):
     S0 = np.broadcast_to(np.eye(Dx), (B, Dx, Dx)).copy()
     ys = rng.standard_normal((B, T, Dy))
     args_ = (mu0, S0, A, b, Q, H, c, R, ys)
+    grad = args.grad
+    if grad and (Dx > 4 or Dy > 4):
+        raise SystemExit("--grad covers filters up to Dx = Dy = 4")
+
+    def call():
+        if grad:
+            return ssmhip.kf_loglikelihood_grad_batch(*args_, ctx=ctx)
+        return ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx), None
+
     for _ in range(args.warmup):
-        ll = ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx)
+        ll, g = call()
     t0 = time.perf_counter()
     dev = []
     for _ in range(args.steps):
-        ll = ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx)
+        ll, g = call()
         ms = ssmhip._lib.C.c_double()
         ctx.check(ctx.lib.ssm_kf_loglikelihood_last_timing(ctx.ptr, ssmhip._lib.C.byref(ms)))
         dev.append(ms.value)
     el = time.perf_counter() - t0
     ms_dev = float(np.mean(dev))
-    # algorithmic HBM bytes per filter and step: its parameter block (FP64) and observation
+    # algorithmic HBM bytes per filter and step: its parameter block (FP64) and observation;
+    # with --grad also the gradient block written (the parameter block without the
+    # observation) -- the forward cache the reverse pass re-reads is not counted
     psz = (2 * Dx * Dx + Dx + Dy * Dx + Dy + Dy * Dy + Dy) * 8
+    if grad:
+        psz += psz - 8 * Dy
     bytes_run = float(B) * T * psz
     achieved = bytes_run / (ms_dev * 1e-3) / 1e9
     flops = float(kf_flops(Dx, Dy)) * T * B
@@ -746,36 +760,55 @@ def bench_kfll(args, T=100, budget_s=10.0):
     for f in range(0, B, max(1, B // 16)):
         ref = oracle.kf_loglikelihood(mu0[f], S0[f], A[f], b[f], Q[f], H[f], c[f], R[f], ys[f])
         errs.append(abs(ll[f] - ref) / abs(ref))
+    gerr = None
+    if grad:  # the gradients of a few filters against the oracle's reverse rule
+        gerr = 0.0
+        for f in range(0, B, max(1, B // 4)):
+            _, go = oracle.kf_loglikelihood_grad(mu0[f], S0[f], A[f], b[f], Q[f], H[f], c[f],
+                                                 R[f], ys[f])
+            for k in ("A", "b", "Q", "H", "c", "R"):
+                want = np.array(go[k]).reshape(g[k][f].shape)
+                gerr = max(gerr, float(np.max(np.abs(g[k][f] - want)) /
+                                       max(np.max(np.abs(want)), 1e-300)))
     log(f"{args.steps} calls in {el:.3f}s; device {ms_dev:.3f} ms per call "
         f"({B / (ms_dev * 1e-3):.3e} evals/s, {achieved:.0f} GB/s, "
         f"{flops / (ms_dev * 1e-3) / 1e12:.2f} TFLOP/s); wall incl. host packing and copies "
-        f"{el / args.steps * 1e3:.1f} ms; max rel err {max(errs):.2e}")
+        f"{el / args.steps * 1e3:.1f} ms; max rel err {max(errs):.2e}"
+        + (f", gradient max rel err {gerr:.2e}" if grad else ""))
     cpu = None
     if not args.no_cpu:
         n, t1 = 0, time.perf_counter()
+        fn = oracle.kf_loglikelihood_grad if grad else oracle.kf_loglikelihood
         while time.perf_counter() - t1 < budget_s and n < B:
-            oracle.kf_loglikelihood(mu0[n], S0[n], A[n], b[n], Q[n], H[n], c[n], R[n], ys[n])
+            fn(mu0[n], S0[n], A[n], b[n], Q[n], H[n], c[n], R[n], ys[n])
             n += 1
         secs = time.perf_counter() - t1
+        what = ("oracle kf_loglikelihood_grad (FP64 numpy forward and reverse pass)" if grad
+                else "oracle kf_loglikelihood (oracle/oracle.c predict/update per step)")
         cpu = {"value": n / secs, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
-               "sample": f"oracle kf_loglikelihood (oracle/oracle.c predict/update per step, "
-                         f"single thread) on the first {n} filters of the batch, {secs:.1f}s"}
+               "sample": f"{what}, single thread, on the first {n} filters of the batch, "
+                         f"{secs:.1f}s"}
         log(f"cpu baseline {n / secs:.1f} evals/s")
     out = {
-        "metric": "kf_loglikelihood evals/s (batched, per-step parameters)",
+        "metric": ("kf_loglikelihood gradient evals/s (batched, per-step parameters)" if grad
+                   else "kf_loglikelihood evals/s (batched, per-step parameters)"),
         "value": B / (ms_dev * 1e-3), "unit": "likelihood evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_dev,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if Dx <= 4 and Dy <= 4 else "f32 (FP64 Cholesky of S)",
         "data": "synthetic (seeded stable time-varying LG parameters and observations)",
-        "config": {"workload": f"kf_loglikelihood, B={B}, Dx={Dx}, Dy={Dy}, T={T}, per-step "
-                               f"parameters, inputs resident on the device", "B": B, "T": T},
+        "config": {"workload": f"kf_loglikelihood{' and gradient' if grad else ''}, B={B}, "
+                               f"Dx={Dx}, Dy={Dy}, T={T}, per-step parameters, inputs resident "
+                               f"on the device", "B": B, "T": T},
         "wall_ms_per_call_incl_pcie": el / args.steps * 1e3,
         "loglik_max_rel_err": float(max(errs)),
+        "grad_max_rel_err": gerr,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": ("k_kf_tiny (one filter per lane, FP64)" if Dx <= 4 and Dy <= 4
+                     "kernel": ("k_kf_tiny_grad (one filter per lane, FP64, forward cache of "
+                                "80 doubles per step)" if grad else
+                                "k_kf_tiny (one filter per lane, FP64)" if Dx <= 4 and Dy <= 4
                                 else "k_kf_batch<..., true>"),
                      "bytes_per_call": bytes_run},
         "cpu_baseline": cpu,
@@ -811,6 +844,8 @@ def main():
     ap.add_argument("--K", type=int, default=100, help="csmc: steps per sweep")
     ap.add_argument("--Dx", type=int, default=2, help="kfll: state dimension")
     ap.add_argument("--Dy", type=int, default=2, help="kfll: observation dimension")
+    ap.add_argument("--grad", action="store_true",
+                    help="kfll: the log-likelihoods and their parameter gradients")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
