@@ -323,12 +323,15 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
 // Such filters are too small for matrix-core tiles or a workgroup each, where a step costs
 // about the same for Dx = 2 as for Dx = 16; at this size FP64 fits the registers (256 VGPRs,
 // no spills) and leaves only the inputs' own conditioning in the error.  Matrices are
-// zero-padded to 4, so the fixed 4-term sums equal the dx- and dy-term ones.
+// zero-padded to M, so the fixed M-term sums equal the dx- and dy-term ones.  One instance,
+// M = 4, serves every shape: at B = 65536 (one wave per SIMD) instances sized to the filter
+// (M = 1..3) measured no faster, and at 2 x 2 slower (1.19 against 0.86 ms for T = 100), as
+// did loading the next step's parameters one step ahead.
 constexpr int kTinyD = 4;
 constexpr int kTinyThreads = 64;
 
+template <int M>
 __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
-  constexpr int M = kTinyD;
   const int f = blockIdx.x * kTinyThreads + threadIdx.x;
   if (f >= p.count) return;
   const int dx = p.Dx[f], dy = p.Dy[f];
@@ -552,52 +555,59 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
 // ---- gradients of kf_loglikelihood for filters up to 4 x 4 -----------------------------
 // The reverse rule of GF/src/integrations/kalman_rrule.jl:22-107 with the backward recursion
 // of GF/src/algorithms/kalman_gradient.jl:103-263 (Parellier et al.), one filter per lane in
-// FP64.  The forward pass is k_kf_tiny's and leaves a cache per step in a filter-minor
-// array (entry e of step t of filter f at (t kGradCache + e) B + f, so the lanes of a wave
-// touch consecutive words); the backward pass walks the steps in reverse in the same lane.
-constexpr int kGradCache = 80;  // mu_prev 4, S_prev 16, mu_pred 4, S_pred 16, mu_f 4, z 4, S 16, K 16
+// FP64, zero-padded to N = max(Dx, Dy) as k_kf_tiny.  The forward pass is k_kf_tiny's and
+// leaves a cache per step in a filter-minor array (slot e of step t of filter f at
+// (t kC + e) B + f with kC = 4 N + 4 N^2 slots a step, so the lanes of a wave touch
+// consecutive words; only the slots inside dx and dy are written and read back); the
+// backward pass walks the steps in reverse in the same lane.
+constexpr int kGradCache = 80;  // slots a step at N = 4: mu_prev, S_prev, mu_pred, S_pred, mu_f, z, S, K
 
 struct KfGradOut {
   double* mu0;    // B x Dx
   double* Sigma0; // B x Dx x Dx
   double *A, *b, *Q, *H, *c, *R;  // the parameter layout of the inputs
-  double* cache;  // T x kGradCache x B
+  double* cache;  // T x kC x B, kC = 4 N + 4 N^2 <= kGradCache
   int B;
 };
 
 namespace {
-using M4 = double[4][4];
-__device__ __forceinline__ void mm4(const M4 a, const M4 b, M4 c) {  // c = a b
+template <int N>
+__device__ __forceinline__ void mm4(const double (*a)[N], const double (*b)[N],
+                                    double (*c)[N]) {  // c = a b
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < N; ++j) {
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) s += a[i][k] * b[k][j];
+      for (int k = 0; k < N; ++k) s += a[i][k] * b[k][j];
       c[i][j] = s;
     }
 }
-__device__ __forceinline__ void mtm4(const M4 a, const M4 b, M4 c) {  // c = a' b
+template <int N>
+__device__ __forceinline__ void mtm4(const double (*a)[N], const double (*b)[N],
+                                     double (*c)[N]) {  // c = a' b
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < N; ++j) {
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) s += a[k][i] * b[k][j];
+      for (int k = 0; k < N; ++k) s += a[k][i] * b[k][j];
       c[i][j] = s;
     }
 }
-// Lower Cholesky factor of the leading n x n block of a (identity past n); false if not PD.
-__device__ __forceinline__ bool chol4(const M4 a, int n, M4 L) {
+// N x N helpers.  Lower Cholesky factor of the leading n x n block of a (identity past
+// n); false if not PD.
+template <int N>
+__device__ __forceinline__ bool chol4(const double (*a)[N], int n, double (*L)[N]) {
   bool ok = true;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) L[i][j] = (i == j && i >= n) ? 1.0 : 0.0;
+    for (int j = 0; j < N; ++j) L[i][j] = (i == j && i >= n) ? 1.0 : 0.0;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < N; ++j) {
     if (j < n) {
       double d = a[j][j];
 #pragma unroll
@@ -605,7 +615,7 @@ __device__ __forceinline__ bool chol4(const M4 a, int n, M4 L) {
       if (!(d > 0.0)) ok = false;
       L[j][j] = sqrt(d > 0.0 ? d : 1.0);
 #pragma unroll
-      for (int i = j + 1; i < 4; ++i) {
+      for (int i = j + 1; i < N; ++i) {
         if (i < n) {
           double v = a[i][j];
 #pragma unroll
@@ -618,52 +628,60 @@ __device__ __forceinline__ bool chol4(const M4 a, int n, M4 L) {
   return ok;
 }
 // x = (L L')^-1 v
-__device__ __forceinline__ void cholsolve4(const M4 L, const double v[4], double x[4]) {
-  double y[4];
+template <int N>
+__device__ __forceinline__ void cholsolve4(const double (*L)[N], const double* v, double* x) {
+  double y[N];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < N; ++i) {
     double s = v[i];
 #pragma unroll
     for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
     y[i] = s / L[i][i];
   }
 #pragma unroll
-  for (int i = 3; i >= 0; --i) {
+  for (int i = N - 1; i >= 0; --i) {
     double s = y[i];
 #pragma unroll
-    for (int k = i + 1; k < 4; ++k) s -= L[k][i] * x[k];
+    for (int k = i + 1; k < N; ++k) s -= L[k][i] * x[k];
     x[i] = s / L[i][i];
   }
 }
 }  // namespace
 
+template <int N>
 __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, KfGradOut g) {
+  using MN = double[N][N];
+  // cache slots of a step: mu_prev, S_prev, mu_pred, S_pred, mu_f, z, S, K
+  constexpr int oMu = 0, oS = N, oMp = N + N * N, oSp = 2 * N + N * N, oMf = 2 * N + 2 * N * N,
+                oZ = 3 * N + 2 * N * N, oSy = 4 * N + 2 * N * N, oK = 4 * N + 3 * N * N,
+                kC = 4 * N + 4 * N * N;
+  static_assert(kC <= kGradCache, "cache slots");
   const int f = blockIdx.x * kTinyThreads + threadIdx.x;
   if (f >= p.count) return;
   const int dx = p.Dx[f], dy = p.Dy[f];
   const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
   const int B = g.B;
-  auto C = [&](int t, int e) -> double& { return g.cache[((size_t)t * kGradCache + e) * B + f]; };
+  auto C = [&](int t, int e) -> double& { return g.cache[((size_t)t * kC + e) * B + f]; };
   const double* g_mu0 = p.tv_mu0 + (int64_t)f * dx;
   const double* g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
-  double mu[4], S[4][4];
+  double mu[N], S[N][N];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < N; ++i) {
     mu[i] = (i < dx) ? g_mu0[i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
+    for (int j = 0; j < N; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
   }
   auto blk = [&](int t) -> int64_t { return (int64_t)f * p.tv_blocks + (p.tv_blocks > 1 ? t : 0); };
-  auto loadM = [&](const double* base, int64_t off, int r, int c, M4 X) {
+  auto loadM = [&](const double* base, int64_t off, int r, int c, MN X) {
     const double* q = base + off;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) X[i][j] = (i < r && j < c) ? q[i + j * r] : 0.0;
+      for (int j = 0; j < N; ++j) X[i][j] = (i < r && j < c) ? q[i + j * r] : 0.0;
   };
   auto loadV = [&](const double* base, int64_t off, int n, double* v) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[i] = (i < n) ? base[off + i] : 0.0;
+    for (int i = 0; i < N; ++i) v[i] = (i < n) ? base[off + i] : 0.0;
   };
   const double* yrow = p.ys + p.y_off[f];
   const int Tn = p.T;
@@ -672,8 +690,8 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   // ---- forward (kalman_rrule.jl:27-50), caching each step ----
   for (int t = 0; t < Tn && ok; ++t) {
     const int64_t k = blk(t);
-    M4 A, Q, H, R;
-    double b[4], c[4];
+    MN A, Q, H, R;
+    double b[N], c[N];
     loadM(p.tv_A, k * nxx, dx, dx, A);
     loadM(p.tv_Q, k * nxx, dx, dx, Q);
     loadM(p.tv_H, k * nyx, dy, dx, H);
@@ -681,57 +699,58 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
     loadV(p.tv_b, k * dx, dx, b);
     loadV(p.tv_c, k * dy, dy, c);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      C(t, i) = mu[i];
+    for (int i = 0; i < N; ++i) {
+      if (i < dx) C(t, oMu + i) = mu[i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) C(t, 4 + 4 * i + j) = S[i][j];
+      for (int j = 0; j < N; ++j)
+        if (i < dx && j < dx) C(t, oS + N * i + j) = S[i][j];
     }
-    double mp[4];
-    M4 T1, Sp;
+    double mp[N];
+    MN T1, Sp;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
       double s = b[i];
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) s += A[i][kk] * mu[kk];
+      for (int kk = 0; kk < N; ++kk) s += A[i][kk] * mu[kk];
       mp[i] = s;
     }
-    mm4(A, S, T1);
+    mm4<N>(A, S, T1);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         double s = Q[i][j];
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) s += T1[i][kk] * A[j][kk];
+        for (int kk = 0; kk < N; ++kk) s += T1[i][kk] * A[j][kk];
         Sp[i][j] = s;
       }
-    double z[4];
-    M4 P, Sy;
+    double z[N];
+    MN P, Sy;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < N; ++q) {
       double s = c[q];
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) s += H[q][kk] * mp[kk];
+      for (int kk = 0; kk < N; ++kk) s += H[q][kk] * mp[kk];
       z[q] = (q < dy) ? yrow[(size_t)t * dy + q] - s : 0.0;
     }
-    mm4(H, Sp, P);
+    mm4<N>(H, Sp, P);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < N; ++q)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+      for (int r = 0; r < N; ++r) {
         double s = R[q][r];
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) s += P[q][kk] * H[r][kk];
+        for (int kk = 0; kk < N; ++kk) s += P[q][kk] * H[r][kk];
         Sy[q][r] = s;
       }
-    M4 L;
-    if (!chol4(Sy, dy, L)) {
+    MN L;
+    if (!chol4<N>(Sy, dy, L)) {
       ok = 0;
       break;
     }
-    double w[4], qd = 0.0, ldet = 0.0;
+    double w[N], qd = 0.0, ldet = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
       double s = z[i];
 #pragma unroll
       for (int kk = 0; kk < i; ++kk) s -= L[i][kk] * w[kk];
@@ -742,65 +761,67 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
       }
     }
     ll += -0.5 * ((double)dy * kLog2Pi + ldet + qd);
-    M4 K;  // K = (S^-1 P)'
+    MN K;  // K = (S^-1 P)'
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double col[4], x[4];
+    for (int j = 0; j < N; ++j) {
+      double col[N], x[N];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) col[q] = P[q][j];
-      cholsolve4(L, col, x);
+      for (int q = 0; q < N; ++q) col[q] = P[q][j];
+      cholsolve4<N>(L, col, x);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) K[j][q] = (q < dy && j < dx) ? x[q] : 0.0;
+      for (int q = 0; q < N; ++q) K[j][q] = (q < dy && j < dx) ? x[q] : 0.0;
     }
-    M4 IKH, KR;
+    MN IKH, KR;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         double u = (i == j && i < dx) ? 1.0 : 0.0, r = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < N; ++q) {
           u -= K[i][q] * H[q][j];
           r += K[i][q] * R[q][j];
         }
         IKH[i][j] = u;
         KR[i][j] = r;
       }
-    double mf[4];
+    double mf[N];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
       double s = mp[i];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) s += K[i][q] * z[q];
+      for (int q = 0; q < N; ++q) s += K[i][q] * z[q];
       mf[i] = (i < dx) ? s : 0.0;
     }
-    mm4(IKH, Sp, T1);
+    mm4<N>(IKH, Sp, T1);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = i; j < 4; ++j) {  // upper triangle, mirrored (Symmetric(:U)), plus jitter
+      for (int j = i; j < N; ++j) {  // upper triangle, mirrored (Symmetric(:U)), plus jitter
         double s = 0.0;
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) s += T1[i][kk] * IKH[j][kk] + KR[i][kk] * K[j][kk];
+        for (int kk = 0; kk < N; ++kk) s += T1[i][kk] * IKH[j][kk] + KR[i][kk] * K[j][kk];
         S[i][j] = s;
       }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
 #pragma unroll
       for (int j = 0; j < i; ++j) S[i][j] = S[j][i];
       if (i < dx) S[i][i] += p.jitter;
       mu[i] = mf[i];
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      C(t, 20 + i) = mp[i];
-      C(t, 40 + i) = mf[i];
-      C(t, 44 + i) = z[i];
+    for (int i = 0; i < N; ++i) {
+      if (i < dx) {
+        C(t, oMp + i) = mp[i];
+        C(t, oMf + i) = mf[i];
+      }
+      if (i < dy) C(t, oZ + i) = z[i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        C(t, 24 + 4 * i + j) = Sp[i][j];
-        C(t, 48 + 4 * i + j) = Sy[i][j];
-        C(t, 64 + 4 * i + j) = K[i][j];
+      for (int j = 0; j < N; ++j) {
+        if (i < dx && j < dx) C(t, oSp + N * i + j) = Sp[i][j];
+        if (i < dy && j < dy) C(t, oSy + N * i + j) = Sy[i][j];
+        if (i < dx && j < dy) C(t, oK + N * i + j) = K[i][j];
       }
     }
   }
@@ -809,17 +830,17 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   if (!ok) return;
   // ---- backward (kalman_rrule.jl:63-103), NLL convention; outputs are negated ----
   const bool acc = p.tv_blocks <= 1;  // one parameter set for all steps: sum the steps
-  double dmu[4] = {0.0, 0.0, 0.0, 0.0};
-  M4 dS;
+  double dmu[N] = {};
+  MN dS;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) dS[i][j] = 0.0;
-  auto put = [&](double* out, int64_t off, int r, int cc, const M4 X) {
+    for (int j = 0; j < N; ++j) dS[i][j] = 0.0;
+  auto put = [&](double* out, int64_t off, int r, int cc, const MN X) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < N; ++j)
         if (i < r && j < cc) {
           double& o = out[off + i + j * r];
           o = (acc ? o : 0.0) - X[i][j];
@@ -827,7 +848,7 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   };
   auto putv = [&](double* out, int64_t off, int n, const double* v) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
       if (i < n) {
         double& o = out[off + i];
         o = (acc ? o : 0.0) - v[i];
@@ -836,7 +857,7 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   if (acc) {  // the sums start at zero
     const int64_t k = blk(0);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < N * N; ++i) {
       if (i < nxx) {
         g.A[k * nxx + i] = 0.0;
         g.Q[k * nxx + i] = 0.0;
@@ -845,95 +866,96 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
       if (i < nyy) g.R[k * nyy + i] = 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < N; ++i) {
       if (i < dx) g.b[k * dx + i] = 0.0;
       if (i < dy) g.c[k * dy + i] = 0.0;
     }
   }
   for (int t = Tn - 1; t >= 0; --t) {
     const int64_t k = blk(t);
-    M4 A, H, R, Sp, Sy, K, Sprev;
-    double mprev[4], mp[4], mf[4], z[4];
+    MN A, H, R, Sp, Sy, K, Sprev;
+    double mprev[N], mp[N], mf[N], z[N];
     loadM(p.tv_A, k * nxx, dx, dx, A);
     loadM(p.tv_H, k * nyx, dy, dx, H);
     loadM(p.tv_R, k * nyy, dy, dy, R);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      mprev[i] = C(t, i);
-      mp[i] = C(t, 20 + i);
-      mf[i] = C(t, 40 + i);
-      z[i] = C(t, 44 + i);
+    for (int i = 0; i < N; ++i) {
+      // the padding outside dx and dy was zero when cached
+      mprev[i] = (i < dx) ? C(t, oMu + i) : 0.0;
+      mp[i] = (i < dx) ? C(t, oMp + i) : 0.0;
+      mf[i] = (i < dx) ? C(t, oMf + i) : 0.0;
+      z[i] = (i < dy) ? C(t, oZ + i) : 0.0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        Sprev[i][j] = C(t, 4 + 4 * i + j);
-        Sp[i][j] = C(t, 24 + 4 * i + j);
-        Sy[i][j] = C(t, 48 + 4 * i + j);
-        K[i][j] = C(t, 64 + 4 * i + j);
+      for (int j = 0; j < N; ++j) {
+        Sprev[i][j] = (i < dx && j < dx) ? C(t, oS + N * i + j) : 0.0;
+        Sp[i][j] = (i < dx && j < dx) ? C(t, oSp + N * i + j) : 0.0;
+        Sy[i][j] = (i < dy && j < dy) ? C(t, oSy + N * i + j) : 0.0;
+        K[i][j] = (i < dx && j < dy) ? C(t, oK + N * i + j) : 0.0;
       }
     }
-    M4 L, Sinv, IKH;
-    chol4(Sy, dy, L);
-    double Siz[4];
-    cholsolve4(L, z, Siz);
+    MN L, Sinv, IKH;
+    chol4<N>(Sy, dy, L);
+    double Siz[N];
+    cholsolve4<N>(L, z, Siz);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double e[4], x[4];
+    for (int j = 0; j < N; ++j) {
+      double e[N], x[N];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) e[i] = (i == j) ? 1.0 : 0.0;
-      cholsolve4(L, e, x);
+      for (int i = 0; i < N; ++i) e[i] = (i == j) ? 1.0 : 0.0;
+      cholsolve4<N>(L, e, x);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Sinv[i][j] = (i < dy && j < dy) ? x[i] : 0.0;
+      for (int i = 0; i < N; ++i) Sinv[i][j] = (i < dy && j < dy) ? x[i] : 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         double u = (i == j && i < dx) ? 1.0 : 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) u -= K[i][q] * H[q][j];
+        for (int q = 0; q < N; ++q) u -= K[i][q] * H[q][j];
         IKH[i][j] = u;
       }
-    double Ktd[4];  // K' dmu
+    double Ktd[N];  // K' dmu
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < N; ++q) {
       double s = 0.0;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) s += K[i][q] * dmu[i];
+      for (int i = 0; i < N; ++i) s += K[i][q] * dmu[i];
       Ktd[q] = s;
     }
     // gradient_c (kalman_gradient.jl:256-263)
     {
-      double gc[4];
+      double gc[N];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) gc[q] = -Siz[q] - Ktd[q];
+      for (int q = 0; q < N; ++q) gc[q] = -Siz[q] - Ktd[q];
       putv(g.c, k * dy, dy, gc);
     }
     // gradient_H (kalman_gradient.jl:216-246)
     {
-      M4 Pf, HSp, T2, gH;
-      mm4(IKH, Sp, Pf);
-      mm4(H, Sp, HSp);
-      M4 KtdS;  // K' dS
-      mtm4(K, dS, KtdS);
-      mm4(KtdS, Pf, T2);
-      double SizTHSp[4], dmuPf[4];
+      MN Pf, HSp, T2, gH;
+      mm4<N>(IKH, Sp, Pf);
+      mm4<N>(H, Sp, HSp);
+      MN KtdS;  // K' dS
+      mtm4<N>(K, dS, KtdS);
+      mm4<N>(KtdS, Pf, T2);
+      double SizTHSp[N], dmuPf[N];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         double s = 0.0, u = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s += Siz[q] * HSp[q][j];
+        for (int q = 0; q < N; ++q) s += Siz[q] * HSp[q][j];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) u += dmu[i] * Pf[i][j];
+        for (int i = 0; i < N; ++i) u += dmu[i] * Pf[i][j];
         SizTHSp[j] = s;
         dmuPf[j] = u;
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < N; ++q)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < N; ++j) {
           double s = 0.0;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) s += Sinv[q][r] * HSp[r][j];
+          for (int r = 0; r < N; ++r) s += Sinv[q][r] * HSp[r][j];
           gH[q][j] = s - Siz[q] * mp[j] - Siz[q] * SizTHSp[j] + Siz[q] * dmuPf[j] -
                      Ktd[q] * mf[j] - 2.0 * T2[q][j];
         }
@@ -941,93 +963,93 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
     }
     // gradient_R (kalman_gradient.jl:160-174)
     {
-      M4 KtdSK, gR;
-      M4 KtdS;
-      mtm4(K, dS, KtdS);
-      mm4(KtdS, K, KtdSK);
+      MN KtdSK, gR;
+      MN KtdS;
+      mtm4<N>(K, dS, KtdS);
+      mm4<N>(KtdS, K, KtdSK);
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < N; ++q)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < N; ++r)
           gR[q][r] = KtdSK[q][r] - 0.5 * (Ktd[q] * Siz[r] + Siz[q] * Ktd[r]) +
                      0.5 * (Sinv[q][r] - Siz[q] * Siz[r]);
       put(g.R, k * nyy, dy, dy, gR);
     }
     // backward_gradient_update (kalman_gradient.jl:103-124)
-    double dmp[4];
-    M4 dSp;
+    double dmp[N];
+    MN dSp;
     {
-      double HtSiz[4], Riz[4], HtRiz[4];
+      double HtSiz[N], Riz[N], HtRiz[N];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < N; ++i) {
         double s = 0.0, u = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < N; ++q) {
           s += H[q][i] * Siz[q];
           u += IKH[q][i] * dmu[q];
         }
         HtSiz[i] = s;
         dmp[i] = u - s;
       }
-      M4 LR;
-      chol4(R, dy, LR);
-      cholsolve4(LR, z, Riz);
+      MN LR;
+      chol4<N>(R, dy, LR);
+      cholsolve4<N>(LR, z, Riz);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < N; ++i) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s += H[q][i] * (q < dy ? Riz[q] : 0.0);
+        for (int q = 0; q < N; ++q) s += H[q][i] * (q < dy ? Riz[q] : 0.0);
         HtRiz[i] = s;
       }
-      M4 inner, T3, HtSinvH;
+      MN inner, T3, HtSinvH;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < N; ++j)
           inner[i][j] = dS[i][j] + 0.5 * (dmu[i] * HtRiz[j] + HtRiz[i] * dmu[j]);
-      mtm4(IKH, inner, T3);   // IKH' inner
-      mm4(T3, IKH, dSp);      // IKH' inner IKH
-      M4 SinvH;
-      mm4(Sinv, H, SinvH);
-      mtm4(H, SinvH, HtSinvH);
+      mtm4<N>(IKH, inner, T3);   // IKH' inner
+      mm4<N>(T3, IKH, dSp);      // IKH' inner IKH
+      MN SinvH;
+      mm4<N>(Sinv, H, SinvH);
+      mtm4<N>(H, SinvH, HtSinvH);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < N; ++j)
           dSp[i][j] += 0.5 * (HtSinvH[i][j] - HtSiz[i] * HtSiz[j]);
     }
     // dynamics parameters (kalman_gradient.jl:150-152, :196-208)
     putv(g.b, k * dx, dx, dmp);
     put(g.Q, k * nxx, dx, dx, dSp);
     {
-      M4 T4, T5, gA;
-      mm4(dSp, A, T4);
-      mm4(T4, Sprev, T5);
+      MN T4, T5, gA;
+      mm4<N>(dSp, A, T4);
+      mm4<N>(T4, Sprev, T5);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) gA[i][j] = dmp[i] * mprev[j] + 2.0 * T5[i][j];
+        for (int j = 0; j < N; ++j) gA[i][j] = dmp[i] * mprev[j] + 2.0 * T5[i][j];
       put(g.A, k * nxx, dx, dx, gA);
     }
     // backward_gradient_predict (kalman_gradient.jl:139-143)
     {
-      M4 T6;
+      MN T6;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < N; ++i) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s += A[q][i] * dmp[q];
+        for (int q = 0; q < N; ++q) s += A[q][i] * dmp[q];
         dmu[i] = s;
       }
-      mtm4(A, dSp, T6);
-      mm4(T6, A, dS);
+      mtm4<N>(A, dSp, T6);
+      mm4<N>(T6, A, dS);
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < N; ++i) {
     if (i < dx) g.mu0[(int64_t)f * dx + i] = -dmu[i];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < N; ++j)
       if (i < dx && j < dx) g.Sigma0[(int64_t)f * nxx + i + j * dx] = -dS[i][j];
   }
 }
@@ -1437,7 +1459,8 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                n_R = (size_t)(nb * nyy), n_y = (size_t)B * T * Dy;
   // with gradients: the 8 gradient arrays (the inputs' layout) and the per-step caches
   const size_t n_grad = gout ? n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R : 0;
-  const size_t n_cache = gout ? (size_t)T * kGradCache * B : 0;
+  const int n_pad = std::max(Dx, Dy);
+  const size_t n_cache = gout ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
   const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B +
                        n_grad + n_cache;
   const size_t bytes = 8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256;
@@ -1526,11 +1549,14 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
   if (gout) {
     KfGradOut g{g_dev[0], g_dev[1], g_dev[2], g_dev[3], g_dev[4], g_dev[5], g_dev[6],
                 g_dev[7], d_cache, B};
-    hipLaunchKernelGGL(k_kf_tiny_grad, dim3((B + kTinyThreads - 1) / kTinyThreads),
-                       dim3(kTinyThreads), 0, s, p, g);
+    const int n = std::max(Dx, Dy);
+    auto fn = n == 1 ? k_kf_tiny_grad<1> : n == 2 ? k_kf_tiny_grad<2>
+            : n == 3 ? k_kf_tiny_grad<3> : k_kf_tiny_grad<4>;
+    hipLaunchKernelGGL(fn, dim3((B + kTinyThreads - 1) / kTinyThreads), dim3(kTinyThreads), 0,
+                       s, p, g);
     SSM_HIP_CHECK(ctx, hipGetLastError());
   } else if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane
-    hipLaunchKernelGGL(k_kf_tiny, dim3((B + kTinyThreads - 1) / kTinyThreads),
+    hipLaunchKernelGGL(k_kf_tiny<kTinyD>, dim3((B + kTinyThreads - 1) / kTinyThreads),
                        dim3(kTinyThreads), 0, s, p);
     SSM_HIP_CHECK(ctx, hipGetLastError());
   } else {
