@@ -201,6 +201,20 @@ int32_t ssm_bf_sample_index(ssm_bf* bf, double u, int32_t* index);
  * sampling) for the calls that follow: a new sweep's randomness, as the Julia rng moves on
  * between _csmc_sample calls (csmc.jl:450-462).  Not in the reference's API. */
 int32_t ssm_bf_set_seed(ssm_bf* bf, uint64_t seed);
+/*
+ * Guided particle filter: ParticleFilter(N, proposal) (GF/src/algorithms/particles.jl:92-108)
+ * with a linear-Gaussian proposal q(x' | x, y) = N(F x + g0 + K y, P) in place of the latent
+ * dynamics.  Each step draws x' from q and adds log f(x' | x) - log q(x' | x, y) to the
+ * weight before the observation's density, as propogate does (particles.jl:175-195).  The
+ * reference's GFTest proposals are of this form: OptimalProposal (GF/src/GFTest/proposals.jl:
+ * 11-31; F = (I - K H) A, g0 = (I - K H) b - K c, the Kalman gain K and P the Joseph-form
+ * filtered covariance of Q) and OverdispersedProposal (:33-49; F = A, g0 = b, K = 0,
+ * P = k Q).  Column-major F (Dx x Dx), g0 (Dx), K (Dx x Dy, NULL for 0), P (Dx x Dx); the
+ * constants follow later ssm_bf_set_model calls' Q.  F = NULL returns to the bootstrap
+ * proposal.  Not with a reference trajectory (conditional SMC).
+ */
+int32_t ssm_bf_set_proposal(ssm_bf* bf, const double* F, const double* g0, const double* K,
+                            const double* P);
 
 /* ---- path storage of a bootstrap filter (GF/ext/CUDAExt.jl:146-298) --------------- */
 /*

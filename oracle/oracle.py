@@ -41,6 +41,13 @@ Parity pinning of each oracle function (what checks it beyond retyping its formu
   of the cumulative weights, the step against ``bf_step`` on every particle but the pinned
   one, and the reference's statistical test of both sweeps (test/algorithms/csmc.jl:5-95):
   the CSMC chain's mean of x_2 against the exact smoothing mean from the joint Gaussian.
+* ``guided_step`` / ``optimal_proposal`` / ``overdispersed_proposal`` (particles.jl:92-195,
+  GFTest/proposals.jl:11-49): the proposal N(A x + b, Q) reproduces ``bf_step`` (the C
+  restatement) in states, weights, ancestors and log-likelihood; with the optimal proposal
+  every particle's weight increment equals the closed-form predictive density
+  N(y; H (A x + b) + c, H Q H' + R), whatever its draw; with the overdispersed proposal the
+  increment log f - log q equals (D/2) log k - (k - 1) |z|^2 / 2; and the reference's
+  guided-filter test (test/algorithms/particles.jl:30-50) against the KF.
 * ``logsumexp``: parity unpinned beyond closed cases (LogExpFunctions' exact rounding is
   third-party; only tolerances are claimed).
 """
@@ -556,6 +563,75 @@ def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=0.0):
                              jitter)
         ll += inc
     return ll
+
+
+def optimal_proposal(model: LG):
+    """(F, g0, K, P) of GFTest's OptimalProposal (GF/src/GFTest/proposals.jl:11-31): the
+    predicted state N(A x + b, Q) conditioned on y by the jitter-free Joseph-form Kalman
+    update (kalman_gradient.jl:12-29), as an affine map x, y -> mean with covariance P."""
+    A, b, Q = (np.atleast_2d(model.A), np.atleast_1d(model.b), np.atleast_2d(model.Q))
+    H, c, R = (np.atleast_2d(model.H), np.atleast_1d(model.c), np.atleast_2d(model.R))
+    S = H @ Q @ H.T + R
+    K = np.linalg.solve(S, H @ Q).T                            # Q H' S^-1
+    IKH = np.eye(len(b)) - K @ H
+    P = IKH @ Q @ IKH.T + K @ R @ K.T                          # Joseph form
+    P = np.triu(P) + np.triu(P, 1).T                           # Symmetric(:U)
+    return IKH @ A, IKH @ b - K @ c, K, P
+
+
+def overdispersed_proposal(model: LG, k):
+    """(F, g0, K, P) of GFTest's OverdispersedProposal (proposals.jl:33-49): N(A x + b, k Q)."""
+    return (np.atleast_2d(model.A), np.atleast_1d(model.b), None,
+            k * np.atleast_2d(model.Q))
+
+
+def _mvn_logpdf_rows(X, M, S):
+    """log N(X[i]; M[i], S) for every row, by S's Cholesky factor."""
+    L = np.linalg.cholesky(0.5 * (S + S.T))
+    Rz = np.linalg.solve(L, (X - M).T)
+    D = S.shape[0]
+    return -0.5 * (D * np.log(2 * np.pi) + 2 * np.log(np.diag(L)).sum() + (Rz * Rz).sum(0))
+
+
+def guided_step(model: LG, state: BFState, y, threshold, kind, u, z, prop, w_override=None):
+    """One step of ParticleFilter(N, proposal) in place (particles.jl:147-162): maybe_resample
+    as ``bf_step``, then ``propogate`` (particles.jl:175-195) draws x' = F x + g0 + K y + L_P z
+    from the proposal and adds log f(x' | x) - log q(x' | x, y), then the observation's
+    log-density, and normalises the log-weights (marginalise!).  prop = (F, g0, K or None, P).  Returns (anc, ll_inc, did_resample, w_used)."""
+    N, D = state.x.shape
+    y = np.atleast_1d(_f64(y))
+    if state.typeless:
+        w = np.full(N, 1.0 / N)
+    else:
+        w = np.exp(state.logw - logsumexp(state.logw))
+    did = threshold * N >= 1.0 / np.sum(w * w)
+    if did:
+        anc = resample(kind, w if w_override is None else w_override, u)
+        state.x = state.x[anc].copy()
+        state.logw = np.zeros(N)
+        baseline = logsumexp(state.logw)
+    else:
+        anc = np.arange(N, dtype=np.int32)
+        if state.typeless:
+            baseline = np.log(N)
+            state.logw = np.zeros(N)
+        else:
+            baseline = logsumexp(state.logw)
+    state.typeless = False
+    F, g0, K, P = prop
+    A, b, Q = (np.atleast_2d(model.A), np.atleast_1d(model.b), np.atleast_2d(model.Q))
+    H, c, R = (np.atleast_2d(model.H), np.atleast_1d(model.c), np.atleast_2d(model.R))
+    g = np.atleast_1d(g0) + (0.0 if K is None else np.atleast_2d(K) @ y)
+    Mq = state.x @ np.atleast_2d(F).T + g
+    xn = Mq + _f64(z).reshape(N, D) @ np.linalg.cholesky(np.atleast_2d(P)).T
+    log_p = _mvn_logpdf_rows(xn, state.x @ A.T + b, Q)          # logdensity(dyn, x, x')
+    log_q = _mvn_logpdf_rows(xn, Mq, np.atleast_2d(P))          # logdensity(proposal, ...)
+    state.x = xn
+    state.logw = state.logw + (log_p - log_q)
+    state.logw = state.logw + _mvn_logpdf_rows(np.broadcast_to(y, (N, len(y))), xn @ H.T + c, R)
+    lse = logsumexp(state.logw)                                  # marginalise!
+    state.logw = state.logw - lse
+    return anc, float(lse - baseline), bool(did), w
 
 
 def mvn_logpdf(y, mu, S):

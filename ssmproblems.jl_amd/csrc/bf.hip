@@ -35,9 +35,9 @@
 namespace ssm {
 
 // The fused predict + update + partial-reduction kernel: one block per tile.
-template <typename T, int DT, int DYT, bool NOISE, bool REF>
+template <typename T, int DT, int DYT, bool NOISE, bool REF, bool GUIDED = false>
 __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  bf_step_tile<T, DT, DYT, NOISE, REF>(p, blockIdx.x);
+  bf_step_tile<T, DT, DYT, NOISE, REF, GUIDED>(p, blockIdx.x);
 }
 
 // Conditional SMC: ancestor-sampling log-weights for the reference state x_ref[t] at the
@@ -213,6 +213,10 @@ struct ssm_bf {
   BfScalars* as_sc = nullptr;  // the scalars of their fold
   int32_t* as_idx = nullptr;   // the drawn ancestor
   int32_t* draw_idx = nullptr; // ssm_bf_sample_index's result
+  // guided filter: the proposal N(F x + g0 + K y, P) as given (FP64, column-major) and its
+  // device constants (PropDev<T>); empty F: the bootstrap filter's latent proposal
+  std::vector<double> prop_F, prop_g0, prop_K, prop_P;
+  void* prop_dev = nullptr;
 };
 
 namespace {
@@ -234,6 +238,8 @@ int check_model(ssm_ctx* ctx, const ssm_lg_model* m) { return bf_check_model(ctx
 
 template <typename T>
 int upload_reference_t(ssm_bf* bf, int T_ref, const double* x_ref);
+template <typename T>
+int upload_proposal_t(ssm_bf* bf);
 template <typename T>
 int upload_transition_t(ssm_bf* bf);
 
@@ -268,7 +274,26 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   p.xref = (const T*)bf->xref;
   p.ref_anc = (bf->ref_mode == SSM_CSMC_ANCESTOR) ? bf->as_idx : nullptr;
   p.anc_w = bf->anc;
+  p.prop = (const PropDev<T>*)bf->prop_dev;
   const dim3 g(p.nblocks), b(kStepThreads);
+  if (!bf->prop_F.empty()) {  // guided filter (no reference pinning)
+    if (noise) {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, true>), g, b, 0, s, p);
+      else if (bf->D == 1 && bf->Dy == 1)
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, true>), g, b, 0, s, p);
+    } else {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, true>), g, b, 0, s, p);
+      else if (bf->D == 1 && bf->Dy == 1)
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, true>), g, b, 0, s, p);
+    }
+    return;
+  }
   if (bf->ref_T >= 0) {  // conditional SMC: particle 0 follows the reference
     if (noise) {
       if (bf->D == 4 && bf->Dy == 2)
@@ -507,7 +532,7 @@ extern "C" int32_t ssm_bf_destroy(ssm_bf* bf) {
                   bf->sc,        bf->partials, bf->rs_scratch, bf->w_dbg, bf->ys,
                   bf->ll_steps,  bf->rs_steps, bf->noise,    bf->u,     bf->xref,
                   bf->ref_model, bf->as_w,     bf->as_scratch, bf->as_sc, bf->as_idx,
-                  bf->draw_idx,  bf->seed_dev, bf->bs_idx,   bf->bs_traj};
+                  bf->draw_idx,  bf->seed_dev, bf->bs_idx,   bf->bs_traj, bf->prop_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (bf->ev0) (void)hipEventDestroy(bf->ev0);
@@ -524,6 +549,8 @@ extern "C" int32_t ssm_bf_set_model(ssm_bf* bf, const ssm_lg_model* model) {
     return set_error(bf->ctx, SSM_ERR_ARG, "ssm_bf_set_model: dimensions changed");
   rc = (bf->prec == SSM_F32) ? upload_model<float>(bf, model) : upload_model<double>(bf, model);
   bf->trans_ok = false;
+  if (!rc && !bf->prop_F.empty())  // the proposal's weight constants follow Q
+    rc = (bf->prec == SSM_F32) ? upload_proposal_t<float>(bf) : upload_proposal_t<double>(bf);
   if (rc || bf->ref_T < 0) return rc;
   // the reference's transition constants follow Q
   return (bf->prec == SSM_F32) ? upload_reference_t<float>(bf, bf->ref_T, bf->xref_host.data())
@@ -1062,6 +1089,8 @@ extern "C" int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_r
     return SSM_OK;
   }
   if (T < 0) return set_error(ctx, SSM_ERR_ARG, "reference: T must be >= 0 (got %d)", T);
+  if (!bf->prop_F.empty())
+    return set_error(ctx, SSM_ERR_ARG, "reference: conditional SMC runs the bootstrap proposal");
   if (mode != SSM_CSMC_PLAIN && mode != SSM_CSMC_ANCESTOR)
     return set_error(ctx, SSM_ERR_ARG, "reference: unknown refreshment %d", mode);
   int rc = ensure_csmc_buffers(bf);
@@ -1072,6 +1101,79 @@ extern "C" int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_r
   if (bf->ref_T < 0 || bf->ref_mode != mode) destroy_graphs(bf);
   bf->ref_T = T;
   bf->ref_mode = mode;
+  return SSM_OK;
+}
+
+namespace {
+// PropDev<T> from the stored proposal and the current model's Q: L_P, L_Q^-1 and
+// -1/2 (log det Q - log det P), in FP64 then rounded to T.
+template <typename T>
+int upload_proposal_t(ssm_bf* bf) {
+  ssm_ctx* ctx = bf->ctx;
+  const int D = bf->D, Dy = bf->Dy;
+  std::vector<double> LQ = bf->Q_host, LP = bf->prop_P;
+  if (!chol_lower(D, LQ)) return set_error(ctx, SSM_ERR_NOT_PD, "Q is not positive definite");
+  if (!chol_lower(D, LP))
+    return set_error(ctx, SSM_ERR_NOT_PD, "the proposal covariance P is not positive definite");
+  const std::vector<double> LQi = tri_inverse(D, LQ);
+  double ldq = 0.0, ldp = 0.0;
+  for (int q = 0; q < D; ++q) {
+    ldq += 2.0 * std::log(LQ[q + q * D]);
+    ldp += 2.0 * std::log(LP[q + q * D]);
+  }
+  if (!bf->prop_dev &&
+      hipMalloc(&bf->prop_dev, (bf->prec == SSM_F32) ? sizeof(PropDev<float>)
+                                                     : sizeof(PropDev<double>)) != hipSuccess)
+    return set_error(ctx, SSM_ERR_OOM, "proposal constants");
+  PropDev<T> h;
+  std::memset(&h, 0, sizeof h);
+  for (int e = 0; e < D * D; ++e) {
+    h.F[e] = (T)bf->prop_F[e];
+    h.LP[e] = (T)LP[e];
+    h.LQi[e] = (T)LQi[e];
+  }
+  for (int e = 0; e < D; ++e) h.g0[e] = (T)bf->prop_g0[e];
+  for (int e = 0; e < D * Dy; ++e) h.K[e] = (T)bf->prop_K[e];
+  h.dcst = (T)(-0.5 * (ldq - ldp));
+  hipStream_t s = ctx->stream;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(bf->prop_dev, &h, sizeof h, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));  // h is a host local
+  return SSM_OK;
+}
+}  // namespace
+
+extern "C" int32_t ssm_bf_set_proposal(ssm_bf* bf, const double* F, const double* g0,
+                                       const double* K, const double* P) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!F) {  // back to the bootstrap filter
+    if (!bf->prop_F.empty()) destroy_graphs(bf);
+    bf->prop_F.clear();
+    return SSM_OK;
+  }
+  if (!g0 || !P) return set_error(ctx, SSM_ERR_ARG, "proposal: F, g0 and P are required");
+  if (bf->ref_T >= 0)
+    return set_error(ctx, SSM_ERR_ARG, "proposal: conditional SMC runs the bootstrap proposal");
+  const int D = bf->D, Dy = bf->Dy;
+  const bool was = !bf->prop_F.empty();
+  std::vector<double> oF = bf->prop_F, og = bf->prop_g0, oK = bf->prop_K, oP = bf->prop_P;
+  bf->prop_F.assign(F, F + (size_t)D * D);
+  bf->prop_g0.assign(g0, g0 + D);
+  if (K)
+    bf->prop_K.assign(K, K + (size_t)D * Dy);
+  else
+    bf->prop_K.assign((size_t)D * Dy, 0.0);
+  bf->prop_P.assign(P, P + (size_t)D * D);
+  const int rc = (bf->prec == SSM_F32) ? upload_proposal_t<float>(bf) : upload_proposal_t<double>(bf);
+  if (rc) {  // keep the previous proposal
+    bf->prop_F = oF;
+    bf->prop_g0 = og;
+    bf->prop_K = oK;
+    bf->prop_P = oP;
+    return rc;
+  }
+  if (!was) destroy_graphs(bf);  // the step kernel changes
   return SSM_OK;
 }
 

@@ -39,6 +39,20 @@ struct BfModelDev {
   T cst;                   // -1/2 (Dy log 2 pi + log det R)
 };
 
+// A linear-Gaussian proposal q(x' | x, y) = N(F x + g0 + K y, P) of a guided particle
+// filter (GF/src/algorithms/particles.jl:175-195), with the constants of the weight
+// increment log f(x' | x) - log q(x' | x, y): the inverse of Q's lower Cholesky factor and
+// -1/2 (log det Q - log det P).  Column-major, leading dimension D.
+template <typename T>
+struct PropDev {
+  T F[kMaxD * kMaxD];
+  T g0[kMaxD];
+  T K[kMaxD * kMaxDy];   // D x Dy
+  T LP[kMaxD * kMaxD];   // lower Cholesky factor of P
+  T LQi[kMaxD * kMaxD];  // inverse of the lower Cholesky factor of Q
+  T dcst;
+};
+
 template <typename T>
 struct StepParams {
   int N, Npad, D, Dy, nblocks;
@@ -62,6 +76,8 @@ struct StepParams {
   const T* __restrict__ xref;
   const int32_t* ref_anc;
   int32_t* anc_w;
+  // guided filter (the GUIDED kernels): the proposal
+  const PropDev<T>* __restrict__ prop = nullptr;
 };
 
 // Per-filter constants of the reference's transition density log N(x'; A x + b, Q) for
@@ -165,6 +181,63 @@ struct ModelView<T, MD, MDY, false> {
 };
 
 // Copies the model constants to LDS (runtime-dimension kernels read them many times).
+// The proposal's constants with the step's mean offset g = g0 + K y: registers for
+// compile-time dimensions, the device struct otherwise.
+template <typename T, int MD, int MDY, bool REG>
+struct PropView;
+
+template <typename T, int MD, int MDY>
+struct PropView<T, MD, MDY, true> {
+  T f[MD * MD], lp[MD * MD], lqi[MD * MD], gg[MD];
+  T dcst;
+  __device__ __forceinline__ PropView(const PropDev<T>& m, int D, int Dy, const T* y) {
+#pragma unroll
+    for (int r = 0; r < MD; ++r) {
+      T s = (r < D) ? m.g0[r] : (T)0;
+#pragma unroll
+      for (int q = 0; q < MDY; ++q)
+        if (r < D && q < Dy) s += m.K[r + q * D] * y[q];
+      gg[r] = s;
+#pragma unroll
+      for (int c = 0; c < MD; ++c) {
+        const bool in = r < D && c < D;
+        f[r * MD + c] = in ? m.F[r + c * D] : (T)0;
+        lp[r * MD + c] = in ? m.LP[r + c * D] : (T)0;
+        lqi[r * MD + c] = in ? m.LQi[r + c * D] : (T)0;
+      }
+    }
+    dcst = m.dcst;
+  }
+  __device__ __forceinline__ T F(int r, int c) const { return f[r * MD + c]; }
+  __device__ __forceinline__ T LP(int r, int c) const { return lp[r * MD + c]; }
+  __device__ __forceinline__ T LQi(int r, int c) const { return lqi[r * MD + c]; }
+  __device__ __forceinline__ T g(int r) const { return gg[r]; }
+};
+
+template <typename T, int MD, int MDY>
+struct PropView<T, MD, MDY, false> {
+  const PropDev<T>& m;
+  int D;
+  T gg[MD];
+  T dcst;
+  __device__ __forceinline__ PropView(const PropDev<T>& mm, int D_, int Dy, const T* y)
+      : m(mm), D(D_) {
+#pragma unroll
+    for (int r = 0; r < MD; ++r) {
+      T s = (r < D) ? m.g0[r] : (T)0;
+#pragma unroll
+      for (int q = 0; q < MDY; ++q)
+        if (r < D && q < Dy) s += m.K[r + q * D] * y[q];
+      gg[r] = s;
+    }
+    dcst = m.dcst;
+  }
+  __device__ __forceinline__ T F(int r, int c) const { return m.F[r + c * D]; }
+  __device__ __forceinline__ T LP(int r, int c) const { return m.LP[r + c * D]; }
+  __device__ __forceinline__ T LQi(int r, int c) const { return m.LQi[r + c * D]; }
+  __device__ __forceinline__ T g(int r) const { return gg[r]; }
+};
+
 template <typename T>
 __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>* g,
                                                              BfModelDev<T>* s) {
@@ -189,7 +262,7 @@ __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>
 // REF (conditional SMC, GF/src/algorithms/particles.jl:59-66, resamplers.jl:51-65):
 // particle 0 takes the reference state x_ref[t] instead of a propagated one, and when the
 // step resamples its ancestor is *ref_anc (ancestor sampling) or particle 0.
-template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false>
+template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false, bool GUIDED = false>
 __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MD = DT ? DT : kMaxD;
   constexpr int MDY = DYT ? DYT : kMaxDy;
@@ -214,6 +287,13 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
   for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
   // the reference state at t and particle 0's ancestor (uniform loads)
+  // the guided filter's proposal (uniform loads, before any store)
+  struct NoProp {};
+  using PV = std::conditional_t<GUIDED, PropView<T, MD, MDY, (DT > 0)>, NoProp>;
+  const PV pv = [&]() {
+    if constexpr (GUIDED) return PV(*p.prop, D, Dy, y);
+    else return PV{};
+  }();
   T xr[MD];
   int ref_a = 0;
   if constexpr (REF) {
@@ -237,20 +317,64 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   };
   auto compute_item = [&](int k, const T* xk, const T* z, T nlk) -> T {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
-    // predict: x' = A x + b + L_Q z
     T xn[MD];
+    [[maybe_unused]] T lpq = 0;
+    if constexpr (GUIDED) {
+      // draw from the proposal: x' = F x + g + L_P z; then
+      // log f(x' | x) - log q(x' | x, y) = dcst - |L_Q^-1 (x' - A x - b)|^2 / 2 + |z|^2 / 2,
+      // the proposal's quadratic form being |L_P^-1 (x' - F x - g)|^2 = |z|^2
+      T zz = 0;
 #pragma unroll
-    for (int r = 0; r < MD; ++r) {
-      if (r < D) {
-        T s = mv.b(r);
+      for (int r = 0; r < MD; ++r) {
+        if (r < D) {
+          T s = pv.g(r);
 #pragma unroll
-        for (int c = 0; c < MD; ++c)
-          if (c < D) s += mv.A(r, c) * xk[c];
+          for (int c = 0; c < MD; ++c)
+            if (c < D) s += pv.F(r, c) * xk[c];
 #pragma unroll
-        for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
-        if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
-        xn[r] = s;
-        p.xout[(size_t)r * p.Npad + i] = s;
+          for (int c = 0; c <= r; ++c) s += pv.LP(r, c) * z[c];
+          xn[r] = s;
+          p.xout[(size_t)r * p.Npad + i] = s;
+          zz += z[r] * z[r];
+        }
+      }
+      T res[MD];
+#pragma unroll
+      for (int r = 0; r < MD; ++r) {
+        if (r < D) {
+          T s = mv.b(r);
+#pragma unroll
+          for (int c = 0; c < MD; ++c)
+            if (c < D) s += mv.A(r, c) * xk[c];
+          res[r] = xn[r] - s;
+        }
+      }
+      T qf = 0;
+#pragma unroll
+      for (int r = 0; r < MD; ++r) {
+        if (r < D) {
+          T v = 0;
+#pragma unroll
+          for (int c = 0; c <= r; ++c) v += pv.LQi(r, c) * res[c];
+          qf += v * v;
+        }
+      }
+      lpq = pv.dcst - (T)0.5 * qf + (T)0.5 * zz;
+    } else {
+      // predict: x' = A x + b + L_Q z
+#pragma unroll
+      for (int r = 0; r < MD; ++r) {
+        if (r < D) {
+          T s = mv.b(r);
+#pragma unroll
+          for (int c = 0; c < MD; ++c)
+            if (c < D) s += mv.A(r, c) * xk[c];
+#pragma unroll
+          for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
+          if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
+          xn[r] = s;
+          p.xout[(size_t)r * p.Npad + i] = s;
+        }
       }
     }
     // update: log N(y; H x' + c, R) = cst - |L_R^-1 (y - H x' - c)|^2 / 2
@@ -275,7 +399,11 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         quad += v * v;
       }
     }
-    const T l = nlk + (mv.cst - (T)0.5 * quad);
+    T l;
+    if constexpr (GUIDED)
+      l = nlk + lpq + (mv.cst - (T)0.5 * quad);
+    else
+      l = nlk + (mv.cst - (T)0.5 * quad);
     p.logw[i] = l;
     return l;
   };

@@ -14,8 +14,10 @@ from .resamplers import (Systematic, Stratified, Multinomial, ESSResampler, samp
 from .kalman import (KalmanFilter, KF, KFBatch, MvNormalState, pack_lg, kf_loglikelihood,
                      kf_loglikelihood_batch, kf_loglikelihood_grad_batch)
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
-from .filters import (BootstrapFilter, BF, DeviceParticles, initialise, step, run, filter,
-                      rng_to_seed, last_timing)
+from .filters import (BootstrapFilter, BF, ParticleFilter, PF, DeviceParticles, initialise,
+                      step, run, filter, rng_to_seed, last_timing)
+from .proposals import (AbstractProposal, LatentProposal, LinearGaussianProposal,
+                        OptimalProposal, OverdispersedProposal)
 from .serve import Server
 from .tree import ParallelParticleTree, ParallelAncestorCallback
 from .history import DenseParticleContainer, DenseAncestorCallback

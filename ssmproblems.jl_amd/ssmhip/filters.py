@@ -18,6 +18,7 @@ import numpy as np
 from . import _lib
 from ._lib import SSM_F32, SSM_F64, default_context
 from .models import PackedLG, StateSpaceModel
+from .proposals import LatentProposal
 from .resamplers import AbstractResampler, ESSResampler, Systematic, draw_uniforms
 
 # Callback triggers (GF/src/callbacks.jl:16-44).
@@ -45,6 +46,35 @@ class BootstrapFilter:
 BF = BootstrapFilter
 
 
+class ParticleFilter(BootstrapFilter):
+    """ParticleFilter(N, proposal; threshold=1.0, resampler=Systematic()) -- particles.jl:
+    92-108.  A linear-Gaussian proposal (ssmhip.proposals) makes it a guided filter; the
+    LatentProposal is the bootstrap filter."""
+
+    def __init__(self, N, proposal, threshold=1.0, resampler=None, precision="f32"):
+        super().__init__(N, threshold, resampler if resampler is not None else Systematic(),
+                         precision)
+        self.proposal = proposal
+
+    @property
+    def guided(self):
+        return not isinstance(self.proposal, LatentProposal)
+
+
+PF = ParticleFilter
+
+
+def _proposal(algo):
+    """The filter's proposal if it is not the latent dynamics."""
+    prop = getattr(algo, "proposal", None)
+    return None if prop is None or isinstance(prop, LatentProposal) else prop
+
+
+def _homogeneous(model, algo):
+    prop = _proposal(algo)
+    return model.is_homogeneous and (prop is None or prop.is_homogeneous)
+
+
 class DeviceParticles:
     """Device-resident ParticleDistribution (containers.jl:125-141) of a BF on an LG model."""
 
@@ -60,6 +90,23 @@ class DeviceParticles:
                                         C.byref(p)))
         self.ptr = p
         self.seed = seed
+        if _proposal(algo) is not None:
+            self.set_proposal(_proposal(algo).linear_form(1))
+
+    def set_proposal(self, form):
+        """Uploads a linear-Gaussian proposal (F, g0, K, P); None: the latent dynamics."""
+        if form is None:
+            self.ctx.check(self.ctx.lib.ssm_bf_set_proposal(self.ptr, None, None, None, None))
+            return
+        F, g0, K, P = form
+        cm = lambda M, r, c: np.ascontiguousarray(
+            np.asarray(M, dtype=np.float64).reshape(r, c).T).reshape(-1)
+        Fv, Pv = cm(F, self.Dx, self.Dx), cm(P, self.Dx, self.Dx)
+        gv = np.ascontiguousarray(np.asarray(g0, dtype=np.float64).reshape(self.Dx))
+        Kv = None if K is None else cm(K, self.Dx, self.Dy)
+        self.ctx.check(self.ctx.lib.ssm_bf_set_proposal(
+            self.ptr, Fv.ctypes.data_as(_lib._dp), gv.ctypes.data_as(_lib._dp),
+            None if Kv is None else Kv.ctypes.data_as(_lib._dp), Pv.ctypes.data_as(_lib._dp)))
 
     def set_params(self, packed: PackedLG):
         self.packed = packed
@@ -144,6 +191,9 @@ def step(rng, model: StateSpaceModel, algo: BootstrapFilter, t: int, state: Devi
     draws are keyed by (state.seed, t) and do not consume ``rng``."""
     if not model.is_homogeneous:
         state.set_params(PackedLG.from_model(model, t))
+    prop = _proposal(algo)
+    if prop is not None and not prop.is_homogeneous:  # calc_params of the proposal at t
+        state.set_proposal(prop.linear_form(t))
     y = np.ascontiguousarray(np.asarray(observation, dtype=np.float64).reshape(state.Dy))
     ll = C.c_double()
     did = C.c_int32()
@@ -192,10 +242,10 @@ def filter(rng, model: StateSpaceModel, algo, observations, callback=None, ctx=N
         if callback is not None:
             callback(model, algo, state, observations, PostInit)
         ys = np.asarray(observations, dtype=np.float64).reshape(len(observations), -1)
-        if callback is None and model.is_homogeneous:
+        if callback is None and _homogeneous(model, algo):
             return state, run(state, algo, ys)
         container = getattr(callback, "container", None)
-        if getattr(callback, "records_on_device", False) and model.is_homogeneous:
+        if getattr(callback, "records_on_device", False) and _homogeneous(model, algo):
             # a dense history takes every generation inside the run's own graphs
             container.record(True)
             try:
