@@ -700,6 +700,26 @@ def bench_csmc(args, budget_s=10.0):
     print(json.dumps(out), flush=True)
 
 
+def cpu_baseline_guided(p, ys, N, kind, threshold, which, budget_s=10.0):
+    """The oracle's guided step (FP64 numpy) on a bounded sample: up to 2^17 particles,
+    steps until the budget is spent.  Returns (particle-steps/s, steps, seconds, N)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    m = oracle.LG(*(p[k] for k in ("mu0", "Sigma0", "A", "b", "Q", "H", "c", "R")))
+    prop = (oracle.optimal_proposal(m) if which == "optimal"
+            else oracle.overdispersed_proposal(m, 1.5))
+    n = min(N, 1 << 17)
+    rng = np.random.default_rng(5)
+    st = oracle.BFState(oracle.bf_initialise(m, rng.standard_normal((n, 4))))
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < len(ys):
+        oracle.guided_step(m, st, ys[steps], threshold, kind, rng.random(n if kind >= 2 else 1),
+                           rng.standard_normal((n, 4)), prop)
+        steps += 1
+    secs = time.perf_counter() - t0
+    return steps * n / secs, steps, secs, n
+
+
 def bench_kfll(args, T=100, budget_s=10.0):
     """kf_loglikelihood as a batched likelihood service: B filters of one shape with
     per-step parameters (As[t], bs[t], ... of logdensity.jl:149-161), seeded stable
@@ -844,6 +864,9 @@ def main():
     ap.add_argument("--K", type=int, default=100, help="csmc: steps per sweep")
     ap.add_argument("--Dx", type=int, default=2, help="kfll: state dimension")
     ap.add_argument("--Dy", type=int, default=2, help="kfll: observation dimension")
+    ap.add_argument("--proposal", default="none", choices=["none", "optimal", "overdispersed"],
+                    help="bf: a guided filter with GFTest's OptimalProposal or "
+                         "OverdispersedProposal(k = 1.5) instead of the bootstrap filter")
     ap.add_argument("--grad", action="store_true",
                     help="kfll: the log-likelihoods and their parameter gradients")
     args = ap.parse_args()
@@ -882,7 +905,13 @@ def main():
     kind = RESAMPLERS[args.resampler]
     R = {0: ssmhip.Systematic(), 1: ssmhip.Systematic(form="gpu"), 2: ssmhip.Stratified(),
          3: ssmhip.Multinomial()}[kind]
-    algo = ssmhip.BF(args.N, threshold=args.threshold, resampler=R, precision="f32")
+    guided = args.proposal != "none"
+    if guided:  # ParticleFilter(N, proposal) with GFTest's proposals, fused into the step
+        prop = (ssmhip.OptimalProposal(model.dyn, model.obs) if args.proposal == "optimal"
+                else ssmhip.OverdispersedProposal(model.dyn, 1.5))
+        algo = ssmhip.PF(args.N, prop, threshold=args.threshold, resampler=R, precision="f32")
+    else:
+        algo = ssmhip.BF(args.N, threshold=args.threshold, resampler=R, precision="f32")
     rng = np.random.default_rng(2024)
     st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=2024)
     lib = ctx.lib
@@ -931,7 +960,14 @@ def main():
         f"finish {ms_fin * 1e3:.2f} us")
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and guided:
+        v, steps, secs, n_s = cpu_baseline_guided(p, ys, args.N, kind, args.threshold,
+                                                  args.proposal)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle guided_step (numpy, single thread) on the same model "
+                         f"and proposal with N={n_s}: {steps} steps in {secs:.1f}s"}
+        log(f"cpu baseline {v:.3e} particle-steps/s ({steps} steps)")
+    elif not args.no_cpu:
         v, steps, secs = cpu_baseline(p, ys, args.N, kind, args.threshold)
         cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": f"FP64 oracle (oracle/oracle.c, single thread, AoS) on the same model "
@@ -940,7 +976,8 @@ def main():
 
     ll_err = abs(lls[-1] - ll_kf) / abs(ll_kf)
     out = {
-        "metric": "particle-steps/s, % HBM roofline; filters/s + p50/p99 at stated loglik err",
+        "metric": ("guided-filter particle-steps/s, % HBM roofline" if guided else
+                   "particle-steps/s, % HBM roofline; filters/s + p50/p99 at stated loglik err"),
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -953,7 +990,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded LG simulation, Philox noise on device)",
         "config": {
-            "workload": f"bootstrap filter, D=4 Dy=2 LG SSM, N={args.N}, T={args.T}, "
+            "workload": (f"guided filter ({args.proposal} proposal), " if guided else
+                         "bootstrap filter, ") + f"D=4 Dy=2 LG SSM, N={args.N}, T={args.T}, "
                         f"{args.resampler} resampling, threshold {args.threshold}, FP32 SoA; "
                         "one step = one full filter() run",
             "N": args.N, "T": args.T, "Dx": 4, "Dy": 2, "resampler": args.resampler,
@@ -965,8 +1003,10 @@ def main():
         "loglik_rel_err": ll_err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(args.N, args.resampler, args.threshold),
-                     "kernel": "k_bf_step<float,4,2>", "bytes_per_launch": step_bytes,
+                     "traffic": (None if guided else
+                                 traffic_from_profile(args.N, args.resampler, args.threshold)),
+                     "kernel": ("k_bf_step<float,4,2,false,false,true> (guided)" if guided
+                                else "k_bf_step<float,4,2>"), "bytes_per_launch": step_bytes,
                      "us_per_launch": ms_st * 1e3,
                      "us_per_launch_in_sequence": ms[1] * 1e3,
                      "resample_us_per_launch": ms_rs * 1e3, "finish_us_per_launch": ms_fin * 1e3,
