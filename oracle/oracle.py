@@ -604,7 +604,7 @@ def guided_step(model: LG, state: BFState, y, threshold, kind, u, z, prop, w_ove
         w = np.full(N, 1.0 / N)
     else:
         w = np.exp(state.logw - logsumexp(state.logw))
-    did = threshold * N >= 1.0 / np.sum(w * w)
+    did = threshold * N >= 1.0 / np.cumsum(w * w)[-1]  # summed left to right, as bf_step
     if did:
         anc = resample(kind, w if w_override is None else w_override, u)
         state.x = state.x[anc].copy()
