@@ -455,6 +455,22 @@ int32_t ssm_kf_loglikelihood_grad(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t D
                                   int32_t* status_out, double* g_mu0, double* g_Sigma0,
                                   double* g_A, double* g_b, double* g_Q, double* g_H,
                                   double* g_c, double* g_R);
+/*
+ * Square-root Kalman filter (GF/src/algorithms/srkf.jl:93-176, SRKalmanFilter(; jitter)):
+ * filter(model, SRKF(), ys) for B filters of one shape in the ssm_kf_loglikelihood layout,
+ * the covariance carried as its upper Cholesky factor U (Sigma = U' U) through QR
+ * factorisations of [U_Q; U A'] (predict) and [[U_R, 0], [U H', U]] (update).  Writes the
+ * log-likelihoods, the final filtered means mu_out (B x Dx) and factors U_out (B blocks of
+ * Dx x Dx, column-major, upper triangular with a positive diagonal).  jitter is added to
+ * the factor's diagonal after each update, as srkf_update does (0: none).  Filters up to
+ * Dx = Dy = 4, one per lane in FP64.  Outputs of a filter whose status is SSM_ERR_NOT_PD
+ * (Sigma0, Q or R not positive definite) are undefined.  mu_out and U_out may be NULL.
+ */
+int32_t ssm_srkf_filter(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                        int32_t time_varying, const double* mu0, const double* Sigma0,
+                        const double* A, const double* b, const double* Q, const double* H,
+                        const double* c, const double* R, const double* ys, double jitter,
+                        double* ll_out, int32_t* status_out, double* mu_out, double* U_out);
 /* Device time of the last ssm_kf_loglikelihood's kernel on this context, from HIP events
  * (ms; the host-to-device copies of its inputs are outside). */
 int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms);

@@ -48,6 +48,10 @@ Parity pinning of each oracle function (what checks it beyond retyping its formu
   N(y; H (A x + b) + c, H Q H' + R), whatever its draw; with the overdispersed proposal the
   increment log f - log q equals (D/2) log k - (k - 1) |z|^2 / 2; and the reference's
   guided-filter test (test/algorithms/particles.jl:30-50) against the KF.
+* ``srkf_filter`` (srkf.jl:93-176): log-likelihood, filtered mean and U' U against the
+  Joseph-form KF (``kf_step``), which is the same filter in exact arithmetic, over several
+  shapes with per-step parameters; U upper triangular with a positive diagonal; with
+  jitter j at T = 1, U - j I equals the Cholesky factor of the KF's filtered covariance.
 * ``logsumexp``: parity unpinned beyond closed cases (LogExpFunctions' exact rounding is
   third-party; only tolerances are claimed).
 """
@@ -632,6 +636,43 @@ def guided_step(model: LG, state: BFState, y, threshold, kind, u, z, prop, w_ove
     lse = logsumexp(state.logw)                                  # marginalise!
     state.logw = state.logw - lse
     return anc, float(lse - baseline), bool(did), w
+
+
+def _qr_r_positive(M):
+    """R of a QR factorisation with its rows made to have a nonnegative diagonal
+    (_correct_cholesky_sign, srkf.jl:101)."""
+    R = np.linalg.qr(M, mode="r")
+    return np.sign(np.diag(R))[:, None] * R
+
+
+def srkf_filter(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=None):
+    """filter(model, SRKF(jitter), ys) with per-step parameters (srkf.jl:93-176) in FP64
+    numpy.  Sigma = U' U with U upper triangular.  Returns (ll, mu, U) after the last step."""
+    T = len(As)
+    ys = np.asarray(ys, dtype=np.float64).reshape(T, -1)
+    mu = _f64(mu0).copy()
+    U = np.linalg.cholesky(np.atleast_2d(_f64(Sigma0))).T
+    ll = 0.0
+    for t in range(T):
+        A, Q = np.atleast_2d(_f64(As[t])), np.atleast_2d(_f64(Qs[t]))
+        H, R = np.atleast_2d(_f64(Hs[t])), np.atleast_2d(_f64(Rs[t]))
+        b, c = np.atleast_1d(_f64(bs[t])), np.atleast_1d(_f64(cs[t]))
+        Dx, Dy = A.shape[0], H.shape[0]
+        mu = A @ mu + b                                             # srkf_predict
+        U = _qr_r_positive(np.vstack([np.linalg.cholesky(Q).T, U @ A.T]))[:Dx]
+        M = np.zeros((Dy + Dx, Dy + Dx))                            # _srkf_form_update_matrix
+        M[:Dy, :Dy] = np.linalg.cholesky(R).T
+        M[Dy:, :Dy] = U @ H.T
+        M[Dy:, Dy:] = U
+        Rm = _qr_r_positive(M)
+        U_S, PHt, U = Rm[:Dy, :Dy], Rm[:Dy, Dy:].T, Rm[Dy:, Dy:]
+        z = ys[t] - H @ mu - c
+        w = np.linalg.solve(U_S.T, z)
+        mu = mu + PHt @ w
+        ll += -0.5 * (w @ w + 2 * np.log(np.diag(U_S)).sum() + Dy * np.log(2 * np.pi))
+        if jitter is not None:
+            U = U + jitter * np.eye(Dx)
+    return ll, mu, U
 
 
 def mvn_logpdf(y, mu, S):

@@ -1054,6 +1054,183 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   }
 }
 
+// ---- square-root Kalman filter for filters up to 4 x 4 -------------------------------
+// GF/src/algorithms/srkf.jl:93-176, one filter per lane in FP64, matrices padded to N.  The
+// covariance is carried as its upper Cholesky factor U (Sigma = U' U).  Predict: the R of
+// a QR factorisation of [U_Q; U A'] (2N x N), rows made to have a positive diagonal.
+// Update: the R of [[U_R, 0], [U H', U]] (2N x 2N), sign-corrected, gives U_S (the factor
+// of S), X = U_S'^-1 H P and the filtered factor; w = U_S'^-1 z, mu += X' w and
+// ll += -(w'w + 2 sum log diag U_S + Dy log 2 pi) / 2.  Jitter is added to the factor's
+// diagonal, as srkf_update does.  The QR is by Givens rotations below the diagonal, column
+// by column; padding rows and columns stay zero through every rotation, so the blocks sit
+// at fixed offsets (N) whatever Dx and Dy.
+namespace {
+template <int M, int C>
+__device__ __forceinline__ void givens_r(double (&X)[M][C], int ncols) {
+#pragma unroll
+  for (int j = 0; j < C; ++j) {
+    if (j >= ncols) break;
+#pragma unroll
+    for (int i = j + 1; i < M; ++i) {
+      const double a = X[j][j], bb = X[i][j];
+      if (bb != 0.0) {
+        const double r = sqrt(a * a + bb * bb);
+        const double cs = a / r, sn = bb / r;
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          if (k >= j) {
+            const double u = X[j][k], v = X[i][k];
+            X[j][k] = cs * u + sn * v;
+            X[i][k] = cs * v - sn * u;
+          }
+        }
+      }
+    }
+  }
+  // _correct_cholesky_sign: rows scaled by the sign of their diagonal entry (0 stays 0)
+#pragma unroll
+  for (int i = 0; i < M && i < C; ++i) {
+    const double d = X[i][i];
+    const double sg = d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : 0.0);
+#pragma unroll
+    for (int k = 0; k < C; ++k) X[i][k] *= sg;
+  }
+}
+}  // namespace
+
+template <int N>
+__global__ void __launch_bounds__(kTinyThreads) k_srkf_tiny(KfBatchParams p, double* mu_out,
+                                                            double* U_out) {
+  const int f = blockIdx.x * kTinyThreads + threadIdx.x;
+  if (f >= p.count) return;
+  const int dx = p.Dx[f], dy = p.Dy[f];
+  const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
+  double mu[N], U[N][N];
+  int ok = 1;
+  {  // U = cholesky(Sigma0).U
+    double S0[N][N], L[N][N];
+    const double* g = p.tv_Sigma0 + (int64_t)f * nxx;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      mu[i] = (i < dx) ? p.tv_mu0[(int64_t)f * dx + i] : 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) S0[i][j] = (i < dx && j < dx) ? g[i + j * dx] : 0.0;
+    }
+    ok &= chol4<N>(S0, dx, L);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) U[i][j] = (i < dx && j < dx) ? L[j][i] : 0.0;
+  }
+  auto loadM = [&](const double* base, int64_t off, int r, int c, double (&X)[N][N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) X[i][j] = (i < r && j < c) ? base[off + i + j * r] : 0.0;
+  };
+  const double* yrow = p.ys + p.y_off[f];
+  double ll = 0.0;
+  for (int t = 0; t < p.T && ok; ++t) {
+    const int64_t k = (int64_t)f * p.tv_blocks + (p.tv_blocks > 1 ? t : 0);
+    double A[N][N], Q[N][N], H[N][N], R[N][N], L[N][N];
+    loadM(p.tv_A, k * nxx, dx, dx, A);
+    loadM(p.tv_Q, k * nxx, dx, dx, Q);
+    loadM(p.tv_H, k * nyx, dy, dx, H);
+    loadM(p.tv_R, k * nyy, dy, dy, R);
+    // predict (srkf_predict): mu = A mu + b; U = R of qr([U_Q; U A'])
+    {
+      double m2[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double s = (i < dx) ? p.tv_b[k * dx + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) s += A[i][j] * mu[j];
+        m2[i] = s;
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) mu[i] = m2[i];
+      ok &= chol4<N>(Q, dx, L);
+      double Mp[2 * N][N];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          Mp[i][j] = (i < dx && j < dx) ? L[j][i] : 0.0;  // U_Q
+          double s = 0.0;                                   // (U A')[i][j]
+#pragma unroll
+          for (int q = 0; q < N; ++q) s += U[i][q] * A[j][q];
+          Mp[N + i][j] = s;
+        }
+      givens_r<2 * N, N>(Mp, dx);
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = 0; j < N; ++j) U[i][j] = (i < dx && j < dx) ? Mp[i][j] : 0.0;
+    }
+    // update (srkf_update): R of qr([[U_R, 0], [U H', U]])
+    ok &= chol4<N>(R, dy, L);
+    double Mu[2 * N][2 * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        Mu[i][j] = (i < dy && j < dy) ? L[j][i] : 0.0;  // U_R
+        Mu[i][N + j] = 0.0;
+        double s = 0.0;                                   // (U H')[i][j]
+#pragma unroll
+        for (int q = 0; q < N; ++q) s += U[i][q] * H[j][q];
+        Mu[N + i][j] = s;
+        Mu[N + i][N + j] = U[i][j];
+      }
+    givens_r<2 * N, 2 * N>(Mu, 2 * N);
+    double z[N], w[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double s = (q < dy) ? p.tv_c[k * dy + q] : 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += H[q][j] * mu[j];
+      z[q] = (q < dy) ? yrow[(size_t)t * dy + q] - s : 0.0;
+    }
+    double qd = 0.0, ldet = 0.0;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {  // w = U_S' \ z (U_S' lower triangular)
+      if (q < dy) {
+        double s = z[q];
+#pragma unroll
+        for (int r = 0; r < q; ++r) s -= Mu[r][q] * w[r];
+        const double d = Mu[q][q];
+        if (!(d > 0.0)) ok = 0;
+        w[q] = s / d;
+        qd += w[q] * w[q];
+        ldet += 2.0 * log(d > 0.0 ? d : 1.0);
+      } else {
+        w[q] = 0.0;
+      }
+    }
+    ll += -0.5 * (qd + ldet + (double)dy * kLog2Pi);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {  // mu += (P H' U_S^-1) w = X' w
+      double s = mu[i];
+#pragma unroll
+      for (int q = 0; q < N; ++q) s += Mu[q][N + i] * w[q];
+      mu[i] = (i < dx) ? s : 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        U[i][j] = (i < dx && j < dx) ? Mu[N + i][N + j] + ((i == j) ? p.jitter : 0.0) : 0.0;
+    }
+  }
+  p.ll_out[f] = ok ? ll : NAN;
+  p.status[f] = ok ? SSM_OK : SSM_ERR_NOT_PD;
+  if (!ok) return;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i < dx) mu_out[(int64_t)f * dx + i] = mu[i];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (i < dx && j < dx) U_out[(int64_t)f * nxx + i + j * dx] = U[i][j];
+  }
+}
+
 }  // namespace ssm
 using namespace ssm;
 
@@ -1437,7 +1614,8 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                     int32_t time_varying, const double* mu0, const double* Sigma0,
                     const double* A, const double* b, const double* Q, const double* H,
                     const double* c, const double* R, const double* ys, double jitter,
-                    double* ll_out, int32_t* status_out, double* const* gout) {
+                    double* ll_out, int32_t* status_out, double* const* gout,
+                    bool srkf = false) {
   if (!ctx) return SSM_ERR_ARG;
   if (B < 1 || T < 1 || !mu0 || !Sigma0 || !A || !b || !Q || !H || !c || !R || !ys || !ll_out)
     return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: bad arguments");
@@ -1457,10 +1635,12 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
   const size_t n_mu0 = (size_t)B * Dx, n_S0 = (size_t)B * nxx, n_A = (size_t)(nb * nxx),
                n_b = (size_t)(nb * Dx), n_H = (size_t)(nb * nyx), n_c = (size_t)(nb * Dy),
                n_R = (size_t)(nb * nyy), n_y = (size_t)B * T * Dy;
-  // with gradients: the 8 gradient arrays (the inputs' layout) and the per-step caches
-  const size_t n_grad = gout ? n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R : 0;
+  // with gradients: the 8 gradient arrays (the inputs' layout) and the per-step caches;
+  // the square-root KF: the final means and factors in the first two
+  const size_t n_grad = srkf ? n_mu0 + n_S0 : gout ? n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R : 0;
   const int n_pad = std::max(Dx, Dy);
-  const size_t n_cache = gout ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
+  const size_t n_cache =
+      (gout && !srkf) ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
   const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B +
                        n_grad + n_cache;
   const size_t bytes = 8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256;
@@ -1537,7 +1717,8 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
     if (!e) SSM_HIP_CHECK(ctx, hipEventCreate(&e));
   SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[0], s));
   // gradient outputs in the inputs' order: mu0, Sigma0, A, b, Q, H, c, R
-  const size_t g_len[8] = {n_mu0, n_S0, n_A, n_b, n_A, n_H, n_c, n_R};
+  const size_t g_len[8] = {n_mu0, n_S0, srkf ? 0 : n_A, srkf ? 0 : n_b, srkf ? 0 : n_A,
+                           srkf ? 0 : n_H, srkf ? 0 : n_c, srkf ? 0 : n_R};
   double* g_dev[8];
   {
     double* q = d_g;
@@ -1546,7 +1727,14 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
       q += gout ? g_len[k] : 0;
     }
   }
-  if (gout) {
+  if (srkf) {
+    const int n = std::max(Dx, Dy);
+    auto fn = n == 1 ? k_srkf_tiny<1> : n == 2 ? k_srkf_tiny<2> : n == 3 ? k_srkf_tiny<3>
+                                                                      : k_srkf_tiny<4>;
+    hipLaunchKernelGGL(fn, dim3((B + kTinyThreads - 1) / kTinyThreads), dim3(kTinyThreads), 0,
+                       s, p, g_dev[0], g_dev[1]);
+    SSM_HIP_CHECK(ctx, hipGetLastError());
+  } else if (gout) {
     KfGradOut g{g_dev[0], g_dev[1], g_dev[2], g_dev[3], g_dev[4], g_dev[5], g_dev[6],
                 g_dev[7], d_cache, B};
     const int n = std::max(Dx, Dy);
@@ -1608,6 +1796,23 @@ extern "C" int32_t ssm_kf_loglikelihood_grad(
   double* const gout[8] = {g_mu0, g_Sigma0, g_A, g_b, g_Q, g_H, g_c, g_R};
   return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
                   ll_out, status_out, gout);
+}
+
+extern "C" int32_t ssm_srkf_filter(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                                   int32_t time_varying, const double* mu0,
+                                   const double* Sigma0, const double* A, const double* b,
+                                   const double* Q, const double* H, const double* c,
+                                   const double* R, const double* ys, double jitter,
+                                   double* ll_out, int32_t* status_out, double* mu_out,
+                                   double* U_out) {
+  if (!ctx) return SSM_ERR_ARG;
+  if (Dx < 1 || Dy < 1 || Dx > kTinyD || Dy > kTinyD)
+    return set_error(ctx, SSM_ERR_ARG,
+                     "ssm_srkf_filter: filters need 1 <= Dx, Dy <= %d (got %d, %d)", kTinyD,
+                     Dx, Dy);
+  double* const out[8] = {mu_out, U_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
+                  ll_out, status_out, out, true);
 }
 
 extern "C" int32_t ssm_kf_loglikelihood_last_timing(ssm_ctx* ctx, double* ms) {

@@ -12,7 +12,8 @@ from .models import (StateSpaceModel, HomogeneousGaussianPrior,
 from .resamplers import (Systematic, Stratified, Multinomial, ESSResampler, sample_ancestors,
                          offspring_to_ancestors, ancestors_to_offspring)
 from .kalman import (KalmanFilter, KF, KFBatch, MvNormalState, pack_lg, kf_loglikelihood,
-                     kf_loglikelihood_batch, kf_loglikelihood_grad_batch)
+                     kf_loglikelihood_batch, kf_loglikelihood_grad_batch, SRKalmanFilter,
+                     SRKF, srkf_filter_batch)
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, ParticleFilter, PF, DeviceParticles, initialise,
                       step, run, filter, rng_to_seed, last_timing)

@@ -138,6 +138,8 @@ def _declare(L):
                                   _dp, _dp, _dp, dbl, _dp, _ip], i32),
         "ssm_kf_loglikelihood_last_timing": ([_vp, _dp], i32),
         "ssm_bf_set_proposal": ([_vp, _dp, _dp, _dp, _dp], i32),
+        "ssm_srkf_filter": ([_vp, i32, i32, i32, i32, i32] + [_dp] * 9 + [dbl, _dp, _ip, _dp, _dp],
+                            i32),
         "ssm_kf_loglikelihood_grad": ([_vp, i32, i32, i32, i32, i32] + [_dp] * 9 +
                                       [dbl, _dp, _ip] + [_dp] * 8, i32),
         "ssm_rbpf_create": ([_vp, C.POINTER(RBModelC), i32, u64, C.POINTER(_vp)], i32),

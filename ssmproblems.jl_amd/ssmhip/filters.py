@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import SSM_F32, SSM_F64, default_context
-from .models import PackedLG, StateSpaceModel
+from .models import PackedLG, StateSpaceModel, lg_params
 from .proposals import LatentProposal
 from .resamplers import AbstractResampler, ESSResampler, Systematic, draw_uniforms
 
@@ -257,7 +257,23 @@ def filter(rng, model: StateSpaceModel, algo, observations, callback=None, ctx=N
             state, inc = step(rng, model, algo, t, state, ys[t - 1], callback=callback)
             ll += inc
         return state, ll
-    from .kalman import KalmanFilter, kalman_filter
+    from .kalman import KalmanFilter, MvNormalState, SRKalmanFilter, kalman_filter
+    from .kalman import srkf_filter_batch
+    if isinstance(algo, SRKalmanFilter):  # srkf.jl:93-176; calc_params at every step
+        if not model.is_linear_gaussian:
+            raise TypeError("the device SRKF needs a linear-Gaussian model")
+        ys = np.asarray(observations, dtype=np.float64).reshape(len(observations), -1)
+        T = ys.shape[0]
+        steps = [lg_params(model, t) for t in range(1, T + 1)]
+        mu0, Sigma0 = steps[0][0], np.atleast_2d(steps[0][1])
+        Dx, Dy = mu0.shape[0], ys.shape[1]
+        shp = (None, None, (Dx, Dx), (Dx,), (Dx, Dx), (Dy, Dx), (Dy,), (Dy, Dy))
+        per = lambda k: np.stack([np.reshape(q[k], shp[k]) for q in steps])[None]
+        ll, mu, U = srkf_filter_batch(mu0[None], Sigma0[None], per(2), per(3), per(4), per(5),
+                                      per(6), per(7), ys[None], jitter=algo.jitter, ctx=ctx)
+        st = MvNormalState(mu[0], U[0].T @ U[0])
+        st.U = U[0]
+        return st, float(ll[0])
     if isinstance(algo, KalmanFilter):
         if not model.is_homogeneous:
             raise TypeError("the device KalmanFilter path needs a homogeneous LG model")

@@ -239,6 +239,36 @@ def kf_loglikelihood_grad_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, 
     return ll, grads
 
 
+class SRKalmanFilter:
+    """SRKalmanFilter(; jitter=nothing) -- GF/src/algorithms/srkf.jl:93-176: the covariance
+    carried as its upper Cholesky factor, propagated by QR factorisations."""
+
+    def __init__(self, jitter=None):
+        self.jitter = jitter
+
+
+def SRKF():
+    return SRKalmanFilter()
+
+
+def srkf_filter_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=None):
+    """filter(model, SRKF(), ys) for B filters of one shape up to 4 x 4 in one device call,
+    arguments as kf_loglikelihood_batch.  Returns (ll (B,), mu (B, Dx), U (B, Dx, Dx)), U the
+    final filtered covariance's upper Cholesky factor (Sigma = U' U)."""
+    ctx = ctx or default_context()
+    B, Dx, Dy, T, tv, shapes, mu0c, S0, flat, yv = _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys)
+    ll = np.zeros(B)
+    st = np.zeros(B, dtype=np.int32)
+    mu = np.zeros(B * Dx)
+    U = np.zeros(B * Dx * Dx)
+    ctx.check(ctx.lib.ssm_srkf_filter(
+        ctx.ptr, B, Dx, Dy, T, int(tv), mu0c.ctypes.data_as(_lib._dp), S0.ctypes.data_as(_lib._dp),
+        *(flat[k].ctypes.data_as(_lib._dp) for k in ("A", "b", "Q", "H", "c", "R")),
+        yv.ctypes.data_as(_lib._dp), float(jitter or 0.0), ll.ctypes.data_as(_lib._dp),
+        st.ctypes.data_as(_lib._ip), mu.ctypes.data_as(_lib._dp), U.ctypes.data_as(_lib._dp)))
+    return ll, mu.reshape(B, Dx), np.swapaxes(U.reshape(B, Dx, Dx), -1, -2)
+
+
 def kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter=None, ctx=None):
     """kf_loglikelihood(mu0, Sigma0, As, bs, Qs, Hs, cs, Rs, ys, jitter) of one filter: the
     sequences hold one parameter per step (logdensity.jl:149-161)."""
