@@ -745,13 +745,15 @@ def bench_kfll(args, T=100, budget_s=10.0):
     S0 = np.broadcast_to(np.eye(Dx), (B, Dx, Dx)).copy()
     ys = rng.standard_normal((B, T, Dy))
     args_ = (mu0, S0, A, b, Q, H, c, R, ys)
-    grad = args.grad
-    if grad and (Dx > 4 or Dy > 4):
-        raise SystemExit("--grad covers filters up to Dx = Dy = 4")
+    grad, srkf = args.grad, args.srkf
+    if (grad or srkf) and (Dx > 4 or Dy > 4):
+        raise SystemExit("--grad and --srkf cover filters up to Dx = Dy = 4")
 
     def call():
         if grad:
             return ssmhip.kf_loglikelihood_grad_batch(*args_, ctx=ctx)
+        if srkf:  # filter(model, SRKF(), ys) per filter: the square-root KF kernel
+            return ssmhip.srkf_filter_batch(*args_, ctx=ctx)[0], None
         return ssmhip.kf_loglikelihood_batch(*args_, ctx=ctx), None
 
     for _ in range(args.warmup):
@@ -778,7 +780,9 @@ def bench_kfll(args, T=100, budget_s=10.0):
     import oracle
     errs = []
     for f in range(0, B, max(1, B // 16)):
-        ref = oracle.kf_loglikelihood(mu0[f], S0[f], A[f], b[f], Q[f], H[f], c[f], R[f], ys[f])
+        ref = (oracle.srkf_filter if srkf else oracle.kf_loglikelihood)(
+            mu0[f], S0[f], A[f], b[f], Q[f], H[f], c[f], R[f], ys[f])
+        ref = ref[0] if srkf else ref
         errs.append(abs(ll[f] - ref) / abs(ref))
     gerr = None
     if grad:  # the gradients of a few filters against the oracle's reverse rule
@@ -798,12 +802,14 @@ def bench_kfll(args, T=100, budget_s=10.0):
     cpu = None
     if not args.no_cpu:
         n, t1 = 0, time.perf_counter()
-        fn = oracle.kf_loglikelihood_grad if grad else oracle.kf_loglikelihood
+        fn = (oracle.kf_loglikelihood_grad if grad else oracle.srkf_filter if srkf
+              else oracle.kf_loglikelihood)
         while time.perf_counter() - t1 < budget_s and n < B:
             fn(mu0[n], S0[n], A[n], b[n], Q[n], H[n], c[n], R[n], ys[n])
             n += 1
         secs = time.perf_counter() - t1
         what = ("oracle kf_loglikelihood_grad (FP64 numpy forward and reverse pass)" if grad
+                else "oracle srkf_filter (FP64 numpy QR per step)" if srkf
                 else "oracle kf_loglikelihood (oracle/oracle.c predict/update per step)")
         cpu = {"value": n / secs, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
                "sample": f"{what}, single thread, on the first {n} filters of the batch, "
@@ -811,6 +817,7 @@ def bench_kfll(args, T=100, budget_s=10.0):
         log(f"cpu baseline {n / secs:.1f} evals/s")
     out = {
         "metric": ("kf_loglikelihood gradient evals/s (batched, per-step parameters)" if grad
+                   else "square-root KF filter evals/s (batched, per-step parameters)" if srkf
                    else "kf_loglikelihood evals/s (batched, per-step parameters)"),
         "value": B / (ms_dev * 1e-3), "unit": "likelihood evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -818,7 +825,8 @@ def bench_kfll(args, T=100, budget_s=10.0):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if Dx <= 4 and Dy <= 4 else "f32 (FP64 Cholesky of S)",
         "data": "synthetic (seeded stable time-varying LG parameters and observations)",
-        "config": {"workload": f"kf_loglikelihood{' and gradient' if grad else ''}, B={B}, "
+        "config": {"workload": ("filter(model, SRKF(), ys)" if srkf else
+                                f"kf_loglikelihood{' and gradient' if grad else ''}") + f", B={B}, "
                                f"Dx={Dx}, Dy={Dy}, T={T}, per-step parameters, inputs resident "
                                f"on the device", "B": B, "T": T},
         "wall_ms_per_call_incl_pcie": el / args.steps * 1e3,
@@ -828,6 +836,7 @@ def bench_kfll(args, T=100, budget_s=10.0):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": ("k_kf_tiny_grad (one filter per lane, FP64, forward cache of "
                                 "80 doubles per step)" if grad else
+                                "k_srkf_tiny (one filter per lane, FP64, Givens QR)" if srkf else
                                 "k_kf_tiny (one filter per lane, FP64)" if Dx <= 4 and Dy <= 4
                                 else "k_kf_batch<..., true>"),
                      "bytes_per_call": bytes_run},
@@ -867,6 +876,8 @@ def main():
     ap.add_argument("--proposal", default="none", choices=["none", "optimal", "overdispersed"],
                     help="bf: a guided filter with GFTest's OptimalProposal or "
                          "OverdispersedProposal(k = 1.5) instead of the bootstrap filter")
+    ap.add_argument("--srkf", action="store_true",
+                    help="kfll: the square-root Kalman filter (final means and factors)")
     ap.add_argument("--grad", action="store_true",
                     help="kfll: the log-likelihoods and their parameter gradients")
     args = ap.parse_args()
