@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import default_context
-from .filters import BootstrapFilter, DeviceParticles, rng_to_seed, run, step
+from .filters import BootstrapFilter, DeviceParticles, _proposal, rng_to_seed, run, step
 from .history import DenseParticleContainer
 from .models import PackedLG, StateSpaceModel
 
@@ -86,6 +86,8 @@ class CSMCWorkspace:
     def __init__(self, model: StateSpaceModel, pf: BootstrapFilter, K: int, ctx=None):
         if not model.is_linear_gaussian:
             raise TypeError("device conditional SMC needs a linear-Gaussian model")
+        if _proposal(pf) is not None:
+            raise TypeError("device conditional SMC runs the bootstrap proposal")
         self.ctx = ctx or default_context()
         self.state = DeviceParticles(self.ctx, model, pf, 0)
         self.ctx.check(self.ctx.lib.ssm_bf_initialise(self.state.ptr, None))

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import RBModelC, default_context
-from .filters import BootstrapFilter, rng_to_seed
+from .filters import BootstrapFilter, _proposal, rng_to_seed
 from .kalman import KalmanFilter
 
 
@@ -107,6 +107,8 @@ class DeviceRBParticles:
 def rbpf_filter(rng, model: RBModel, algo: RBPF, observations, ctx=None, seed=None):
     """filter(rng, hier_model, RBPF(BF(N), KF()), ys) -> (state, log_evidence)."""
     ctx = ctx or default_context()
+    if _proposal(algo.pf) is not None:
+        raise TypeError("the device RBPF runs the bootstrap proposal for the outer state")
     st = DeviceRBParticles(ctx, model, algo.pf.N, rng_to_seed(rng) if seed is None else seed)
     st.initialise()
     ll = st.run(observations, algo.pf.threshold, algo.pf.resampler.kind)

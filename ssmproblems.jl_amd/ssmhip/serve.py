@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import RequestC, default_context
-from .filters import BootstrapFilter, rng_to_seed
+from .filters import BootstrapFilter, _proposal, rng_to_seed
 from .kalman import KalmanFilter
 from .models import PackedLG, StateSpaceModel
 
@@ -62,6 +62,9 @@ class Server:
         if isinstance(algo, KalmanFilter):
             return self._submit(REQ_KF, packed, observations)
         if isinstance(algo, BootstrapFilter):
+            if _proposal(algo) is not None:
+                raise _lib.ArgumentError(_lib.SSM_ERR_ARG,
+                                         "the server runs the bootstrap proposal only")
             if algo.precision != "f32":
                 raise _lib.ArgumentError(_lib.SSM_ERR_ARG, "the server runs FP32 particles")
             s = rng_to_seed(rng) if seed is None else int(seed)

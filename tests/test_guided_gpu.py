@@ -160,3 +160,14 @@ def test_errors(ctx, orc):
     st.set_reference(np.zeros((3, 4)))
     with pytest.raises(ssmhip.ArgumentError):
         st.set_proposal((m.A, m.b, None, m.Q))
+    # paths that run only the bootstrap proposal refuse a guided filter instead of
+    # dropping its proposal
+    pf = ssmhip.PF(64, ssmhip.OverdispersedProposal(model.dyn, 1.5))
+    with ssmhip.Server(hbm_budget_bytes=64 << 20, max_active=4, ctx=ctx) as sv:
+        with pytest.raises(ssmhip.ArgumentError):
+            sv.submit(np.random.default_rng(0), model, pf, np.zeros((3, 2)))
+    with pytest.raises(TypeError):
+        ssmhip.rbpf_filter(np.random.default_rng(0), None, ssmhip.RBPF(pf), np.zeros((3, 1)),
+                           ctx=ctx)
+    with pytest.raises(TypeError):
+        ssmhip.CSMCWorkspace(model, pf, 3, ctx=ctx)
