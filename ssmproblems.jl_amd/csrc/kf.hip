@@ -25,6 +25,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "device_common.h"
@@ -65,6 +66,7 @@ struct KfBatchParams {
   const double* __restrict__ tv_c = nullptr;
   const double* __restrict__ tv_R = nullptr;
   int tv_blocks = 1;
+  int fm = 0;  // the tiny kernels' inputs staged filter-minor (kf_at, kf_yat, kf_0at)
 };
 
 constexpr double kLog2Pi = 1.8378770664093454835606594728112;
@@ -330,6 +332,22 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
 constexpr int kTinyD = 4;
 constexpr int kTinyThreads = 64;
 
+// Indices into the tiny kernels' inputs: the caller's layout (parameter block f T + t, or f
+// with one set a filter; mu0 and Sigma0 filter-major; ys at y_off[f]), or with p.fm the
+// filter-minor staging of the same values, element e of step t of filter f at (t E + e) B + f,
+// so the 64 lanes of a wave read 64 consecutive words.
+__device__ __forceinline__ int64_t kf_at(const KfBatchParams& p, int f, int t, int64_t E, int e) {
+  const int64_t tb = p.tv_blocks > 1 ? t : 0;
+  return p.fm ? (tb * E + e) * (int64_t)p.count + f : ((int64_t)f * p.tv_blocks + tb) * E + e;
+}
+__device__ __forceinline__ int64_t kf_yat(const KfBatchParams& p, int f, int t, int dy, int q) {
+  return p.fm ? ((int64_t)t * dy + q) * (int64_t)p.count + f
+              : p.y_off[f] + (int64_t)t * dy + q;
+}
+__device__ __forceinline__ int64_t kf_0at(const KfBatchParams& p, int f, int64_t E, int e) {
+  return p.fm ? (int64_t)e * p.count + f : (int64_t)f * E + e;
+}
+
 template <int M>
 __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
   const int f = blockIdx.x * kTinyThreads + threadIdx.x;
@@ -337,46 +355,37 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
   const int dx = p.Dx[f], dy = p.Dy[f];
   const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
   double mu[M], S[M][M], A[M][M], Q[M][M], H[M][M], R[M][M], b[M], c[M];
-  const double* g_mu0 = p.tv_mu0 + (int64_t)f * dx;
-  const double* g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
 #pragma unroll
   for (int i = 0; i < M; ++i) {
-    mu[i] = (i < dx) ? g_mu0[i] : 0.0;
+    mu[i] = (i < dx) ? p.tv_mu0[kf_0at(p, f, dx, i)] : 0.0;
 #pragma unroll
-    for (int j = 0; j < M; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
+    for (int j = 0; j < M; ++j)
+      S[i][j] = (i < dx && j < dx) ? p.tv_Sigma0[kf_0at(p, f, nxx, i + j * dx)] : 0.0;
   }
-  auto load = [&](int64_t blk) {
-    const double* gA = p.tv_A + blk * nxx;
-    const double* gQ = p.tv_Q + blk * nxx;
-    const double* gH = p.tv_H + blk * nyx;
-    const double* gR = p.tv_R + blk * nyy;
-    const double* gb = p.tv_b + blk * dx;
-    const double* gc = p.tv_c + blk * dy;
+  auto load = [&](int t) {
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-      b[i] = (i < dx) ? gb[i] : 0.0;
-      c[i] = (i < dy) ? gc[i] : 0.0;
+      b[i] = (i < dx) ? p.tv_b[kf_at(p, f, t, dx, i)] : 0.0;
+      c[i] = (i < dy) ? p.tv_c[kf_at(p, f, t, dy, i)] : 0.0;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        A[i][j] = (i < dx && j < dx) ? gA[i + j * dx] : 0.0;
-        Q[i][j] = (i < dx && j < dx) ? gQ[i + j * dx] : 0.0;
-        H[i][j] = (i < dy && j < dx) ? gH[i + j * dy] : 0.0;
-        R[i][j] = (i < dy && j < dy) ? gR[i + j * dy] : 0.0;
+        A[i][j] = (i < dx && j < dx) ? p.tv_A[kf_at(p, f, t, nxx, i + j * dx)] : 0.0;
+        Q[i][j] = (i < dx && j < dx) ? p.tv_Q[kf_at(p, f, t, nxx, i + j * dx)] : 0.0;
+        H[i][j] = (i < dy && j < dx) ? p.tv_H[kf_at(p, f, t, nyx, i + j * dy)] : 0.0;
+        R[i][j] = (i < dy && j < dy) ? p.tv_R[kf_at(p, f, t, nyy, i + j * dy)] : 0.0;
       }
     }
   };
-  const int64_t blk0 = (int64_t)f * p.tv_blocks;
-  load(blk0);
+  load(0);
   const double jit = p.jitter;
-  const double* yrow = p.ys + p.y_off[f];
   const int Tn = p.Tf ? p.Tf[f] : p.T;
   double ll = 0.0;
   int ok = 1;
   for (int t = 0; t < Tn; ++t) {
-    if (p.tv_blocks > 1 && t > 0) load(blk0 + t);
+    if (p.tv_blocks > 1 && t > 0) load(t);
     double y[M];
 #pragma unroll
-    for (int q = 0; q < M; ++q) y[q] = (q < dy) ? yrow[(size_t)t * dy + q] : 0.0;
+    for (int q = 0; q < M; ++q) y[q] = (q < dy) ? p.ys[kf_yat(p, f, t, dy, q)] : 0.0;
     // predict: mu_hat = A mu + b, Sigma = A Sigma A' + Q
     double mh[M], T1[M][M];
 #pragma unroll
@@ -662,28 +671,25 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
   const int B = g.B;
   auto C = [&](int t, int e) -> double& { return g.cache[((size_t)t * kC + e) * B + f]; };
-  const double* g_mu0 = p.tv_mu0 + (int64_t)f * dx;
-  const double* g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
   double mu[N], S[N][N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    mu[i] = (i < dx) ? g_mu0[i] : 0.0;
+    mu[i] = (i < dx) ? p.tv_mu0[kf_0at(p, f, dx, i)] : 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) S[i][j] = (i < dx && j < dx) ? g_S0[i + j * dx] : 0.0;
+    for (int j = 0; j < N; ++j)
+      S[i][j] = (i < dx && j < dx) ? p.tv_Sigma0[kf_0at(p, f, nxx, i + j * dx)] : 0.0;
   }
   auto blk = [&](int t) -> int64_t { return (int64_t)f * p.tv_blocks + (p.tv_blocks > 1 ? t : 0); };
-  auto loadM = [&](const double* base, int64_t off, int r, int c, MN X) {
-    const double* q = base + off;
+  auto loadM = [&](const double* base, int t, int64_t E, int r, int c, MN X) {
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) X[i][j] = (i < r && j < c) ? q[i + j * r] : 0.0;
+      for (int j = 0; j < N; ++j) X[i][j] = (i < r && j < c) ? base[kf_at(p, f, t, E, i + j * r)] : 0.0;
   };
-  auto loadV = [&](const double* base, int64_t off, int n, double* v) {
+  auto loadV = [&](const double* base, int t, int n, double* v) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = (i < n) ? base[off + i] : 0.0;
+    for (int i = 0; i < N; ++i) v[i] = (i < n) ? base[kf_at(p, f, t, n, i)] : 0.0;
   };
-  const double* yrow = p.ys + p.y_off[f];
   const int Tn = p.T;
   double ll = 0.0;
   int ok = 1;
@@ -692,12 +698,12 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
     const int64_t k = blk(t);
     MN A, Q, H, R;
     double b[N], c[N];
-    loadM(p.tv_A, k * nxx, dx, dx, A);
-    loadM(p.tv_Q, k * nxx, dx, dx, Q);
-    loadM(p.tv_H, k * nyx, dy, dx, H);
-    loadM(p.tv_R, k * nyy, dy, dy, R);
-    loadV(p.tv_b, k * dx, dx, b);
-    loadV(p.tv_c, k * dy, dy, c);
+    loadM(p.tv_A, t, nxx, dx, dx, A);
+    loadM(p.tv_Q, t, nxx, dx, dx, Q);
+    loadM(p.tv_H, t, nyx, dy, dx, H);
+    loadM(p.tv_R, t, nyy, dy, dy, R);
+    loadV(p.tv_b, t, dx, b);
+    loadV(p.tv_c, t, dy, c);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       if (i < dx) C(t, oMu + i) = mu[i];
@@ -731,7 +737,7 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
       double s = c[q];
 #pragma unroll
       for (int kk = 0; kk < N; ++kk) s += H[q][kk] * mp[kk];
-      z[q] = (q < dy) ? yrow[(size_t)t * dy + q] - s : 0.0;
+      z[q] = (q < dy) ? p.ys[kf_yat(p, f, t, dy, q)] - s : 0.0;
     }
     mm4<N>(H, Sp, P);
 #pragma unroll
@@ -875,9 +881,9 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
     const int64_t k = blk(t);
     MN A, H, R, Sp, Sy, K, Sprev;
     double mprev[N], mp[N], mf[N], z[N];
-    loadM(p.tv_A, k * nxx, dx, dx, A);
-    loadM(p.tv_H, k * nyx, dy, dx, H);
-    loadM(p.tv_R, k * nyy, dy, dy, R);
+    loadM(p.tv_A, t, nxx, dx, dx, A);
+    loadM(p.tv_H, t, nyx, dy, dx, H);
+    loadM(p.tv_R, t, nyy, dy, dy, R);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       // the padding outside dx and dy was zero when cached
@@ -1109,12 +1115,12 @@ __global__ void __launch_bounds__(kTinyThreads) k_srkf_tiny(KfBatchParams p, dou
   int ok = 1;
   {  // U = cholesky(Sigma0).U
     double S0[N][N], L[N][N];
-    const double* g = p.tv_Sigma0 + (int64_t)f * nxx;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      mu[i] = (i < dx) ? p.tv_mu0[(int64_t)f * dx + i] : 0.0;
+      mu[i] = (i < dx) ? p.tv_mu0[kf_0at(p, f, dx, i)] : 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) S0[i][j] = (i < dx && j < dx) ? g[i + j * dx] : 0.0;
+      for (int j = 0; j < N; ++j)
+        S0[i][j] = (i < dx && j < dx) ? p.tv_Sigma0[kf_0at(p, f, nxx, i + j * dx)] : 0.0;
     }
     ok &= chol4<N>(S0, dx, L);
 #pragma unroll
@@ -1122,27 +1128,26 @@ __global__ void __launch_bounds__(kTinyThreads) k_srkf_tiny(KfBatchParams p, dou
 #pragma unroll
       for (int j = 0; j < N; ++j) U[i][j] = (i < dx && j < dx) ? L[j][i] : 0.0;
   }
-  auto loadM = [&](const double* base, int64_t off, int r, int c, double (&X)[N][N]) {
+  auto loadM = [&](const double* base, int t, int64_t E, int r, int c, double (&X)[N][N]) {
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) X[i][j] = (i < r && j < c) ? base[off + i + j * r] : 0.0;
+      for (int j = 0; j < N; ++j)
+        X[i][j] = (i < r && j < c) ? base[kf_at(p, f, t, E, i + j * r)] : 0.0;
   };
-  const double* yrow = p.ys + p.y_off[f];
   double ll = 0.0;
   for (int t = 0; t < p.T && ok; ++t) {
-    const int64_t k = (int64_t)f * p.tv_blocks + (p.tv_blocks > 1 ? t : 0);
     double A[N][N], Q[N][N], H[N][N], R[N][N], L[N][N];
-    loadM(p.tv_A, k * nxx, dx, dx, A);
-    loadM(p.tv_Q, k * nxx, dx, dx, Q);
-    loadM(p.tv_H, k * nyx, dy, dx, H);
-    loadM(p.tv_R, k * nyy, dy, dy, R);
+    loadM(p.tv_A, t, nxx, dx, dx, A);
+    loadM(p.tv_Q, t, nxx, dx, dx, Q);
+    loadM(p.tv_H, t, nyx, dy, dx, H);
+    loadM(p.tv_R, t, nyy, dy, dy, R);
     // predict (srkf_predict): mu = A mu + b; U = R of qr([U_Q; U A'])
     {
       double m2[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double s = (i < dx) ? p.tv_b[k * dx + i] : 0.0;
+        double s = (i < dx) ? p.tv_b[kf_at(p, f, t, dx, i)] : 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) s += A[i][j] * mu[j];
         m2[i] = s;
@@ -1186,10 +1191,10 @@ __global__ void __launch_bounds__(kTinyThreads) k_srkf_tiny(KfBatchParams p, dou
     double z[N], w[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-      double s = (q < dy) ? p.tv_c[k * dy + q] : 0.0;
+      double s = (q < dy) ? p.tv_c[kf_at(p, f, t, dy, q)] : 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) s += H[q][j] * mu[j];
-      z[q] = (q < dy) ? yrow[(size_t)t * dy + q] - s : 0.0;
+      z[q] = (q < dy) ? p.ys[kf_yat(p, f, t, dy, q)] - s : 0.0;
     }
     double qd = 0.0, ldet = 0.0;
 #pragma unroll
@@ -1674,13 +1679,49 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
     dims[2 * (size_t)B + f] = f;
   }
   hipStream_t s = ctx->stream;
+  // The one-filter-per-lane kernels read their inputs filter-minor: transposed here, on the
+  // host, in blocks of 64 filters (each block's reads stay within 64 source rows).
+  const bool fm = srkf || gout || (Dx <= kTinyD && Dy <= kTinyD);
+  const int64_t nbf = time_varying ? T : 1;  // parameter blocks per filter
+  std::vector<double> stage;
+  auto staged = [&](const double* src, int64_t per_filter) -> const double* {
+    if (!fm) return src;
+    const size_t at = stage.size();
+    stage.resize(at + (size_t)B * per_filter);
+    double* dst = stage.data() + at;
+    // threads take disjoint runs of 64-filter blocks, so their writes never share a line
+    const int nblk = (B + 63) / 64;
+    const int nth = std::max(1, std::min({8, nblk, (int)std::thread::hardware_concurrency()}));
+    auto work = [&](int w) {
+      for (int k = w; k < nblk; k += nth) {
+        const int f0 = 64 * k, f1 = std::min(B, f0 + 64);
+        for (int64_t r = 0; r < per_filter; ++r)
+          for (int f = f0; f < f1; ++f) dst[r * B + f] = src[(int64_t)f * per_filter + r];
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (std::thread& th : pool) th.join();
+    return dst;
+  };
+  if (fm) stage.reserve(n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y);
+  const double* h_mu0 = staged(mu0, Dx);
+  const double* h_S0 = staged(Sigma0, nxx);
+  const double* h_A = staged(A, nbf * nxx);
+  const double* h_b = staged(b, nbf * Dx);
+  const double* h_Q = staged(Q, nbf * nxx);
+  const double* h_H = staged(H, nbf * nyx);
+  const double* h_c = staged(c, nbf * Dy);
+  const double* h_R = staged(R, nbf * nyy);
+  const double* h_y = staged(ys, (int64_t)T * Dy);
   struct Up {
     void* dst;
     const void* src;
     size_t bytes;
-  } ups[] = {{d_mu0, mu0, 8 * n_mu0}, {d_S0, Sigma0, 8 * n_S0}, {d_A, A, 8 * n_A},
-             {d_b, b, 8 * n_b},       {d_Q, Q, 8 * n_A},         {d_H, H, 8 * n_H},
-             {d_c, c, 8 * n_c},       {d_R, R, 8 * n_R},         {d_y, ys, 8 * n_y},
+  } ups[] = {{d_mu0, h_mu0, 8 * n_mu0}, {d_S0, h_S0, 8 * n_S0}, {d_A, h_A, 8 * n_A},
+             {d_b, h_b, 8 * n_b},       {d_Q, h_Q, 8 * n_A},    {d_H, h_H, 8 * n_H},
+             {d_c, h_c, 8 * n_c},       {d_R, h_R, 8 * n_R},    {d_y, h_y, 8 * n_y},
              {d_yoff, yoff.data(), 8 * (size_t)B}, {d_Dx, dims.data(), 12 * (size_t)B}};
   for (const Up& u : ups)
     SSM_HIP_CHECK(ctx, hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, s));
@@ -1713,6 +1754,7 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
   p.tv_c = d_c;
   p.tv_R = d_R;
   p.tv_blocks = time_varying ? T : 1;
+  p.fm = fm ? 1 : 0;
   for (hipEvent_t& e : ctx->kfll_ev)
     if (!e) SSM_HIP_CHECK(ctx, hipEventCreate(&e));
   SSM_HIP_CHECK(ctx, hipEventRecord(ctx->kfll_ev[0], s));
