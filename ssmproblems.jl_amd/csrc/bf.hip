@@ -246,7 +246,7 @@ int upload_transition_t(ssm_bf* bf);
 TileScratch tile_scratch(ssm_bf* bf) {
   ResampleArgs a;
   resample_bind_scratch(a, bf->rs_scratch, bf->N);
-  return TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix};
+  return TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
 }
 
 template <typename T>
@@ -397,7 +397,7 @@ int enqueue_ancestor_draw_t(ssm_bf* bf, hipStream_t s) {
     hipLaunchKernelGGL((k_as_weights<T, 0>), g, b, 0, s, bf->N, bf->Npad, bf->D, md, rm,
                        (const T*)bf->x[bf->cur], (const T*)bf->logw, bf->sc,
                        (const T*)bf->xref, (T*)bf->as_w, bf->as_sc, a.partials);
-  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
   // threshold +Inf: the fold treats the weights as fresh, as their partials were formed;
   // record 0 leaves the step counter at t (the Philox step of the draw)
   launch_pf_finish<T>(s, bf->N, nb, bf->as_sc, INFINITY, std::log((double)bf->N), ts, nullptr,
@@ -1241,7 +1241,7 @@ int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
   a.seed_dev = bf->seed_dev;
   a.sc = bf->as_sc;
   resample_bind_scratch(a, bf->as_scratch, N);
-  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
   for (int t = Tn; t >= 0; --t) {
     const T* x = (t == 0) ? x0 : st + (size_t)(t - 1) * plane;
     const double* lw = (t == 0) ? nullptr : hv.logw + (size_t)(t - 1) * Npad;

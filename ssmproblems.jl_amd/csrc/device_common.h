@@ -201,6 +201,30 @@ __device__ __forceinline__ float wave_scan_max_dpp(float v) {
   v = fmaxf(v, dpp_f<kDppRowBcast31, 0xC>(ninf, v));
   return v;
 }
+// Inclusive max scan of int32 values over the 64 lanes.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ int wave_scan_max_i32(int v) {
+  constexpr int lo = -2147483647 - 1;
+  v = max(v, dpp_i<kDppRowShr1>(lo, v));
+  v = max(v, dpp_i<kDppRowShr2>(lo, v));
+  v = max(v, dpp_i<kDppRowShr4>(lo, v));
+  v = max(v, dpp_i<kDppRowShr8>(lo, v));
+  v = max(v, dpp_i<kDppRowBcast15, 0xA>(lo, v));
+  v = max(v, dpp_i<kDppRowBcast31, 0xC>(lo, v));
+  return v;
+}
+
+// Orders one wave's LDS traffic: DS instructions of one wave complete in issue order, so a
+// compiler-level fence at wavefront scope is enough between dependent phases (writes before
+// the call are seen by reads after it from any lane of the same wave).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Lane 63's value (the scan total) broadcast to the wave.
 __device__ __forceinline__ double lane63(double v) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),

@@ -87,7 +87,9 @@ struct ResampleArgs {
   double* blk_m = nullptr;
   double* blk_c = nullptr;
   double* prefix = nullptr;
-  double* Wbuf = nullptr;            // multinomial prefix sums (n)
+  double* groups = nullptr;          // finish group records (more than 1024 tiles)
+  double* Wbuf = nullptr;            // multinomial cumulative weights (tile-padded)
+  double* Wsub = nullptr;            // multinomial: every 16th of them
   int32_t* counts_out = nullptr;     // optional: offspring count of each particle
                                      // (systematic and stratified; ignored otherwise)
 };

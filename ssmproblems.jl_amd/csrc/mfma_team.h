@@ -70,13 +70,6 @@ __device__ __forceinline__ void gemm16(float* C, int ldc, int M, int N, float al
   }
 }
 
-// Orders this wave's LDS traffic: DS instructions of one wave complete in issue order, so
-// a compiler-level fence at wavefront scope is enough between dependent phases.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
                           __builtin_amdgcn_readlane(__double2loint(v), lane));

@@ -34,7 +34,11 @@ struct TileScratch {
   double* blk_m;     // nb: m1_b (exactly representable in T)
   double* blk_c;     // nb: c_b
   double* prefix;    // nb + 1: exclusive tile prefixes of the weights, prefix[nb] = total
+  double* groups;    // kGroupRec per group of kFinishGroup tiles (finishes of > 1024 tiles)
 };
+
+constexpr int kFinishGroup = 1024;  // tiles one finish block folds
+constexpr int kGroupRec = 8;        // doubles per group record
 
 // Accurate exponential of the weights (the resampler recomputes it bit for bit).
 __device__ __forceinline__ float exp_w(float v) { return expf(v); }
@@ -360,216 +364,278 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
   }
 }
 
-template <typename T, int CH, int NT>
-__global__ void __launch_bounds__(NT)
-    k_pf_finish(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
-                double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                int record) {
-  pf_finish_block<T, CH, NT>(N, nb, sc_in, sc, threshold_n, logN, ts, ll_steps, rs_steps,
-                             record);
-}
 
-// The finish for many tiles (nb > 1024), in rounds: in round r thread t handles tile
-// r NT + t (NT = 512: 256 registers per lane), so every load of a wave is one coalesced run (a thread-owned chunk of
-// contiguous tiles would make each lane touch its own cache line, and a single block's
-// loads then run at one line per cycle).  Maxima and sums accumulate per thread over the
-// rounds and reduce across the block in a fixed order.  The exclusive prefixes of
-// c_b tau_b are a block scan per round plus the running carry, made nondecreasing by a
-// block max-scan (a parallel scan of nonnegative values is not monotone in floating
-// point).  The loads of R rounds are issued together.
-template <int R>
-__device__ __forceinline__ void load_rounds(const double* __restrict__ a, int r0, int nb, int NT,
-                                            double v[R]) {
-#pragma unroll
-  for (int u = 0; u < R; ++u) v[u] = a[min((r0 + u) * NT + (int)threadIdx.x, nb - 1)];
-}
-
-// Scans across an NT-thread block in thread order (fixed association), with the per-wave
-// totals exchanged through lds[NT / 64]: the exclusive sum, and the inclusive max.
+// ---- the finish with one tile per thread ------------------------------------------
+// Block-wide fixed-order reductions over NT threads (NT / 64 waves): the wave values go
+// through lds[NT / 64] and every thread sums (or maximises) them in wave order.
 template <int NT>
-__device__ __forceinline__ double block_scan_sum(double v, double* lds, double* total) {
+__device__ __forceinline__ double block_sum_fixed(double v, double* lds) {
+  constexpr int NW = NT / 64;
+  v = wave_sum_dpp(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = lds[0];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) s += lds[k];
+  return s;
+}
+
+// Exclusive prefix of one value per thread across NT threads, nondecreasing in thread order
+// by construction: the in-wave exclusive prefixes x are made monotone by an exact running
+// max, and the wave offsets are off_{w+1} = max(off_w + tot_w, off_w + xmax_w), so no value
+// of wave w exceeds the first one of wave w + 1.  *total = off_NW, at least every prefix.
+// One barrier; lds_tot and lds_xmax hold NT / 64 doubles each.
+template <int NT>
+__device__ __forceinline__ double block_prefix_monotone(double v, double* lds_tot,
+                                                        double* lds_xmax, double* total) {
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double incl = wave_scan_dpp(v);
-  double excl = wave_shfl_up(incl, 1);
-  if (lane == 0) excl = 0.0;
-  if (lane == 63) lds[wv] = incl;
-  __syncthreads();
-  double before = 0.0, tot = 0.0;
-  for (int k = 0; k < NW; ++k) {
-    if (k < wv) before += lds[k];
-    tot += lds[k];
+  double x = wave_shfl_up(incl, 1);
+  if (lane == 0) x = 0.0;
+  x = wave_scan_max_dpp(x);
+  if (lane == 63) {
+    lds_tot[wv] = incl;
+    lds_xmax[wv] = x;
   }
   __syncthreads();
-  *total = tot;
-  return before + excl;
-}
-template <int NT>
-__device__ __forceinline__ double block_scan_max(double v, double* lds, double* total) {
-  constexpr int NW = NT / 64;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double incl = wave_scan_max_dpp(v);
-  if (lane == 63) lds[wv] = incl;
-  __syncthreads();
-  double tot = -INFINITY;
+  double off = 0.0, mine = 0.0;
+#pragma unroll
   for (int k = 0; k < NW; ++k) {
-    if (k < wv) incl = fmax(incl, lds[k]);
-    tot = fmax(tot, lds[k]);
+    if (k == wv) mine = off;
+    off = fmax(off + lds_tot[k], off + lds_xmax[k]);
   }
-  __syncthreads();
-  *total = tot;
-  return incl;
+  *total = off;
+  return mine + x;
 }
 
+// The finish of nb <= NT tiles by one block of NT threads, tile b on thread b: the maxima,
+// the scaled sums (exp(m1_b - M1) in T) and the prefixes of c_b tau_b in three barriers.
 template <typename T, int NT>
-__device__ __forceinline__ void pf_finish_stream(int N, int nb, const BfScalars* sc_in,
-                                                 BfScalars* sc, double threshold_n,
-                                                 double logN, const TileScratch& ts,
-                                                 double* ll_steps, int32_t* rs_steps,
-                                                 int record) {
-  constexpr int NW = NT / 64, G = 4;
-  __shared__ double red[3][NW];
-  __shared__ double lds[NW];
+__global__ void __launch_bounds__(NT)
+    k_pf_finish_tiles(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
+                      double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
+                      int record) {
+  constexpr int NW = NT / 64;
+  __shared__ double red[5][NW];
   const bool rs = threshold_n >= sc_in->ess;
   const bool fresh = rs || sc_in->typeless;
   const double* P = ts.partials;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rounds = (nb + NT - 1) / NT;
-  // pass 1: maxima of the tile maxima (exact, any order)
-  double M1 = -INFINITY, M0 = -INFINITY;
-  for (int r0 = 0; r0 < rounds; r0 += G) {
-    double m1[G], m0[G];
-    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
-    load_rounds<G>(P, r0, nb, NT, m0);
-#pragma unroll
-    for (int u = 0; u < G; ++u) {
-      const bool in = (r0 + u) * NT + tid < nb;
-      M1 = fmax(M1, in ? m1[u] : -INFINITY);
-      M0 = fmax(M0, (in && !fresh) ? m0[u] : -INFINITY);
-    }
-  }
-  M1 = wave_max_dpp(M1);
-  M0 = wave_max_dpp(M0);
-  if (lane == 0) {
-    red[0][wv] = M1;
-    red[1][wv] = M0;
+  const int b = threadIdx.x;
+  const bool in = b < nb;
+  const int bc = in ? b : nb - 1;
+  double m0 = P[bc], s0 = P[nb + bc], m1 = P[2 * nb + bc], tau = P[3 * nb + bc],
+         s2 = P[4 * nb + bc];
+  m0 = (in && !fresh) ? m0 : -INFINITY;
+  s0 = (in && !fresh) ? s0 : 0.0;
+  m1 = in ? m1 : -INFINITY;
+  tau = in ? tau : 0.0;
+  s2 = in ? s2 : 0.0;
+  // maxima (exact, any order)
+  double M1 = wave_max_dpp(m1), M0 = wave_max_dpp(m0);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = M1;
+    red[1][threadIdx.x >> 6] = M0;
   }
   __syncthreads();
   M1 = red[0][0];
   M0 = red[1][0];
+#pragma unroll
   for (int k = 1; k < NW; ++k) {
     M1 = fmax(M1, red[0][k]);
     M0 = fmax(M0, red[1][k]);
   }
+  const double c = (m1 != -INFINITY) ? (double)exp_w((T)(m1 - M1)) : 0.0;
+  const double e0 = (!fresh && m0 != -INFINITY) ? s0 * (double)exp_w((T)(m0 - M0)) : 0.0;
+  double S = wave_sum_dpp(tau * c), S2 = wave_sum_dpp(s2 * c * c), S0 = wave_sum_dpp(e0);
+  if ((threadIdx.x & 63) == 0) {
+    red[2][threadIdx.x >> 6] = S;
+    red[3][threadIdx.x >> 6] = S2;
+    red[4][threadIdx.x >> 6] = S0;
+  }
   __syncthreads();
-  // the scale of tile b: exp(m1_b - M1), in T, 0 for an empty tile
-  auto scale = [&](double m1) {
-    return (m1 != -INFINITY) ? (double)exp_w((T)(m1 - M1)) : 0.0;
-  };
-  // pass 2: scaled sums, per thread over the rounds, then the block in wave order
-  double S = 0.0, S2 = 0.0, S0 = 0.0;
-  for (int r0 = 0; r0 < rounds; r0 += G) {
-    double m1[G], tau[G], s2[G], m0[G], s0[G];
-    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
-    load_rounds<G>(P + 3 * nb, r0, nb, NT, tau);
-    load_rounds<G>(P + 4 * nb, r0, nb, NT, s2);
-    load_rounds<G>(P, r0, nb, NT, m0);
-    load_rounds<G>(P + nb, r0, nb, NT, s0);
+  S = red[2][0];
+  S2 = red[3][0];
+  S0 = red[4][0];
 #pragma unroll
-    for (int u = 0; u < G; ++u) {
-      if ((r0 + u) * NT + tid < nb) {
-        const double c = scale(m1[u]);
-        S += tau[u] * c;
-        S2 += s2[u] * c * c;
-        if (!fresh && m0[u] != -INFINITY) S0 += s0[u] * (double)exp_w((T)(m0[u] - M0));
-      }
-    }
-  }
-  S = wave_sum_dpp(S);
-  S2 = wave_sum_dpp(S2);
-  S0 = wave_sum_dpp(S0);
-  if (lane == 0) {
-    red[0][wv] = S;
-    red[1][wv] = S2;
-    red[2][wv] = S0;
-  }
-  __syncthreads();
-  S = red[0][0];
-  S2 = red[1][0];
-  S0 = red[2][0];
   for (int k = 1; k < NW; ++k) {
-    S += red[0][k];
-    S2 += red[1][k];
-    S0 += red[2][k];
+    S += red[2][k];
+    S2 += red[3][k];
+    S0 += red[4][k];
   }
   const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
-  const double invS = dead ? 0.0 : 1.0 / S;
-  // pass 3: per round, scales, exclusive prefixes of c_b tau_b and the outputs
-  double carry = 0.0, cmax = -INFINITY;
-  for (int r0 = 0; r0 < rounds; r0 += G) {
-    double m1[G], tau[G];
-    load_rounds<G>(P + 2 * nb, r0, nb, NT, m1);
-    load_rounds<G>(P + 3 * nb, r0, nb, NT, tau);
-#pragma unroll
-    for (int u = 0; u < G; ++u) {
-      if (r0 + u < rounds) {  // block-uniform
-        const int b = (r0 + u) * NT + tid;
-        const bool in = b < nb;
-        const double c = in ? scale(m1[u]) * invS : 0.0;
-        const double v = c * (in ? tau[u] : 0.0);
-        double rtot, rmax;
-        const double excl = block_scan_sum<NT>(v, lds, &rtot);
-        // carry + exclusive prefix, then nondecreasing across the block and the rounds
-        const double pre = block_scan_max<NT>(fmax(carry + excl, cmax), lds, &rmax);
-        if (in) {
-          ts.prefix[b] = pre;
-          ts.blk_c[b] = c;
-          ts.blk_m[b] = m1[u];
-        }
-        carry += rtot;
-        cmax = fmax(cmax, rmax);
-      }
-    }
+  const double cb = c * (dead ? 0.0 : 1.0 / S);
+  double tot;
+  const double pre = block_prefix_monotone<NT>(cb * tau, red[0], red[1], &tot);
+  if (in) {
+    ts.prefix[b] = pre;
+    ts.blk_c[b] = cb;
+    ts.blk_m[b] = m1;
   }
-  if (tid == 0) {
-    ts.prefix[nb] = fmax(carry, cmax);
+  if (threadIdx.x == 0) {
+    ts.prefix[nb] = tot;
     TileFold<1> F;
     F.M1 = M1;
     F.S = S;
     F.S2 = S2;
     F.M0 = M0;
     F.S0 = S0;
-    F.tot = fmax(carry, cmax);
+    F.tot = tot;
     F.dead = dead;
     *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
   }
 }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT)
-    k_pf_finish_stream(int N, int nb, const BfScalars* sc_in, BfScalars* sc,
-                       double threshold_n, double logN, TileScratch ts, double* ll_steps,
-                       int32_t* rs_steps, int record) {
-  pf_finish_stream<T, NT>(N, nb, sc_in, sc, threshold_n, logN, ts, ll_steps, rs_steps,
-                          record);
+// The finish of more than 1024 tiles, in two kernels over groups of kFinishGroup tiles.
+// Group pass (one block per group): the group maxima M1_g, M0_g; per tile d_b =
+// exp(m1_b - M1_g) in T; the group sums A_g = sum tau_b d_b, B_g = sum s2_b d_b^2,
+// C_g = sum s0_b exp(m0_b - M0_g); and the in-group monotone prefixes r_b of d_b tau_b with
+// their closing value R_g (r_b goes to prefix[b] and d_b to blk_c[b] until the second pass).
+template <typename T>
+__global__ void __launch_bounds__(kFinishGroup)
+    k_pf_finish_groups(int nb, const BfScalars* sc_in, double threshold_n, TileScratch ts) {
+  constexpr int NT = kFinishGroup, NW = NT / 64;
+  __shared__ double red[5][NW];
+  const bool rs = threshold_n >= sc_in->ess;
+  const bool fresh = rs || sc_in->typeless;
+  const double* P = ts.partials;
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const bool in = b < nb;
+  const int bc = in ? b : nb - 1;
+  double m0 = P[bc], s0 = P[nb + bc], m1 = P[2 * nb + bc], tau = P[3 * nb + bc],
+         s2 = P[4 * nb + bc];
+  m0 = (in && !fresh) ? m0 : -INFINITY;
+  s0 = (in && !fresh) ? s0 : 0.0;
+  m1 = in ? m1 : -INFINITY;
+  tau = in ? tau : 0.0;
+  s2 = in ? s2 : 0.0;
+  double M1 = wave_max_dpp(m1), M0 = wave_max_dpp(m0);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = M1;
+    red[1][threadIdx.x >> 6] = M0;
+  }
+  __syncthreads();
+  M1 = red[0][0];
+  M0 = red[1][0];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) {
+    M1 = fmax(M1, red[0][k]);
+    M0 = fmax(M0, red[1][k]);
+  }
+  const double d = (m1 != -INFINITY) ? (double)exp_w((T)(m1 - M1)) : 0.0;
+  const double e0 = (!fresh && m0 != -INFINITY) ? s0 * (double)exp_w((T)(m0 - M0)) : 0.0;
+  double A = wave_sum_dpp(tau * d), B = wave_sum_dpp(s2 * d * d), C = wave_sum_dpp(e0);
+  if ((threadIdx.x & 63) == 0) {
+    red[2][threadIdx.x >> 6] = A;
+    red[3][threadIdx.x >> 6] = B;
+    red[4][threadIdx.x >> 6] = C;
+  }
+  __syncthreads();
+  A = red[2][0];
+  B = red[3][0];
+  C = red[4][0];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) {
+    A += red[2][k];
+    B += red[3][k];
+    C += red[4][k];
+  }
+  double R;
+  const double r = block_prefix_monotone<NT>(d * tau, red[0], red[1], &R);
+  if (in) {
+    ts.prefix[b] = r;
+    ts.blk_c[b] = d;
+  }
+  if (threadIdx.x == 0) {
+    double* g = ts.groups + (size_t)blockIdx.x * kGroupRec;
+    g[0] = M1;
+    g[1] = M0;
+    g[2] = A;
+    g[3] = B;
+    g[4] = C;
+    g[5] = R;
+  }
 }
 
-// Enqueues the finish kernel with a register chunk that fits nb tiles.
+// Second pass (one block per group): every block folds the G group records in group order
+// -- M1 = max M1_g, S = sum A_g exp(M1_g - M1), S2 = sum B_g exp(2 (M1_g - M1)),
+// S0 = sum C_g exp(M0_g - M0) in FP64 -- then writes its tiles' scales c_b = d_b f_g and
+// prefixes off_g + f_g r_b, with f_g = exp(M1_g - M1) / S and off_{g+1} = off_g + f_g R_g
+// (f_g r_b <= f_g R_g, so the prefixes stay nondecreasing across groups).  Block 0 writes the
+// total and the step's scalars.
+template <typename T>
+__global__ void __launch_bounds__(kFinishGroup)
+    k_pf_finish_apply(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
+                      double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
+                      int record) {
+  const int G = (nb + kFinishGroup - 1) / kFinishGroup;
+  const bool rs = threshold_n >= sc_in->ess;
+  const bool fresh = rs || sc_in->typeless;
+  const double* gr = ts.groups;
+  double M1 = -INFINITY, M0 = -INFINITY;
+  for (int g = 0; g < G; ++g) {
+    M1 = fmax(M1, gr[g * kGroupRec + 0]);
+    M0 = fmax(M0, gr[g * kGroupRec + 1]);
+  }
+  double S = 0.0, S2 = 0.0, S0 = 0.0;
+  for (int g = 0; g < G; ++g) {
+    const double m1g = gr[g * kGroupRec + 0], m0g = gr[g * kGroupRec + 1];
+    const double e = (m1g != -INFINITY) ? exp(m1g - M1) : 0.0;
+    S += gr[g * kGroupRec + 2] * e;
+    S2 += gr[g * kGroupRec + 3] * e * e;
+    if (!fresh && m0g != -INFINITY) S0 += gr[g * kGroupRec + 4] * exp(m0g - M0);
+  }
+  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
+  const double invS = dead ? 0.0 : 1.0 / S;
+  double off = 0.0, off_mine = 0.0, f_mine = 0.0;
+  for (int g = 0; g < G; ++g) {
+    const double m1g = gr[g * kGroupRec + 0];
+    const double f = ((m1g != -INFINITY) ? exp(m1g - M1) : 0.0) * invS;
+    if (g == (int)blockIdx.x) {
+      off_mine = off;
+      f_mine = f;
+    }
+    off = off + f * gr[g * kGroupRec + 5];
+  }
+  const int b = blockIdx.x * kFinishGroup + threadIdx.x;
+  if (b < nb) {
+    ts.blk_c[b] = ts.blk_c[b] * f_mine;
+    ts.prefix[b] = off_mine + f_mine * ts.prefix[b];
+    ts.blk_m[b] = ts.partials[2 * nb + b];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ts.prefix[nb] = off;
+    TileFold<1> F;
+    F.M1 = M1;
+    F.S = S;
+    F.S2 = S2;
+    F.M0 = M0;
+    F.S0 = S0;
+    F.tot = off;
+    F.dead = dead;
+    *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
+  }
+}
+
+// Enqueues the finish: one block with a thread per tile up to 1024 tiles, the two group
+// passes beyond (ts.groups must hold a record per group).
 template <typename T>
 inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double threshold_n,
                              double logN, const TileScratch& ts, double* ll_steps,
                              int32_t* rs_steps, int record) {
-#define SSM_FINISH(CH, NT)                                                                  \
-  hipLaunchKernelGGL((k_pf_finish<T, CH, NT>), dim3(1), dim3(NT), 0, s, N, nb, sc, sc,     \
-                     threshold_n, logN, ts, ll_steps, rs_steps, record)
-  if (nb <= 256)
-    SSM_FINISH(1, 256);
-  else if (nb <= 1024)
-    SSM_FINISH(4, 256);
-  else  // many tiles: rounds of 512 coalesced tiles
-    hipLaunchKernelGGL((k_pf_finish_stream<T, 512>), dim3(1), dim3(512), 0, s, N, nb, sc, sc,
+  if (nb <= 256) {
+    hipLaunchKernelGGL((k_pf_finish_tiles<T, 256>), dim3(1), dim3(256), 0, s, N, nb, sc, sc,
                        threshold_n, logN, ts, ll_steps, rs_steps, record);
-#undef SSM_FINISH
+  } else if (nb <= 1024) {
+    hipLaunchKernelGGL((k_pf_finish_tiles<T, 1024>), dim3(1), dim3(1024), 0, s, N, nb, sc, sc,
+                       threshold_n, logN, ts, ll_steps, rs_steps, record);
+  } else {
+    const int G = (nb + kFinishGroup - 1) / kFinishGroup;
+    hipLaunchKernelGGL((k_pf_finish_groups<T>), dim3(G), dim3(kFinishGroup), 0, s, nb, sc,
+                       threshold_n, ts);
+    hipLaunchKernelGGL((k_pf_finish_apply<T>), dim3(G), dim3(kFinishGroup), 0, s, N, nb, sc,
+                       sc, threshold_n, logN, ts, ll_steps, rs_steps, record);
+  }
 }
 
 }  // namespace ssm

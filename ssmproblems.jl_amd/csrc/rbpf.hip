@@ -365,7 +365,7 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   resample_bind_scratch(a, rb->rs_scratch, rb->N);
   int rc = launch_resample(rb->ctx, s, a);
   if (rc) return rc;
-  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix};
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
   RbStepParams p;
   p.N = rb->N;
   p.nblocks = rb->nblocks;

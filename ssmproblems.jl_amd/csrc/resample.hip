@@ -28,7 +28,9 @@ namespace ssm {
 size_t resample_scratch_bytes(int n) {
   const size_t nb = (size_t)(n + kTile - 1) / kTile;
   auto r = [](size_t b) { return (b + 255) / 256 * 256; };
-  return r(nb * kPartial * 8) + r(nb * 8) + r(nb * 8) + r((nb + 1) * 8) + r((size_t)n * 8);
+  const size_t ng = (nb + kFinishGroup - 1) / kFinishGroup;
+  return r(nb * kPartial * 8) + r(nb * 8) + r(nb * 8) + r((nb + 1) * 8) + r(ng * kGroupRec * 8) +
+         r(nb * kTile * 8) + r(nb * kSubPerTile * 8);
 }
 
 void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
@@ -43,7 +45,11 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   p += r(nb * 8);
   a.prefix = (double*)p;
   p += r((nb + 1) * 8);
+  a.groups = (double*)p;
+  p += r((nb + kFinishGroup - 1) / kFinishGroup * kGroupRec * 8);
   a.Wbuf = (double*)p;
+  p += r(nb * kTile * 8);
+  a.Wsub = (double*)p;
 }
 
 // Explicit weights, kernel 1: FP64 tile sums (thread-serial over its striped items, then
@@ -122,8 +128,9 @@ void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
     hipLaunchKernelGGL((k_scan_offspring<MODE>), g, blk, 0, stream, rp);
 }
 
+// Multinomial queries: one block per 1024 queries.
 __global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
-  multinomial_search_item(p, blockIdx.x * kRsThreads + threadIdx.x);
+  multinomial_search_block(p, blockIdx.x);
 }
 
 // The kernels' parameter block for one resample call.
@@ -146,6 +153,7 @@ static RsParams rs_params_of(const ResampleArgs& a) {
   rp.tile_sum = a.partials;
   rp.prefix = a.prefix;
   rp.Wbuf = a.Wbuf;
+  rp.Wsub = a.Wsub;
   rp.counts_out = (a.kind == SSM_MULTINOMIAL) ? nullptr : a.counts_out;
   return rp;
 }
@@ -200,8 +208,7 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
     default: return set_error(ctx, SSM_ERR_ARG, "resample: bad weight mode");
   }
   if (a.kind == SSM_MULTINOMIAL)
-    hipLaunchKernelGGL(k_multinomial_search, dim3((a.n + kRsThreads - 1) / kRsThreads),
-                       dim3(kRsThreads), 0, stream, rp);
+    hipLaunchKernelGGL(k_multinomial_search, g, blk, 0, stream, rp);
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
 }

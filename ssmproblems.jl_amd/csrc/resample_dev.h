@@ -32,7 +32,8 @@ struct RsParams {
   const double* blk_c;
   double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
   double* prefix;
-  double* Wbuf;
+  double* Wbuf;  // multinomial: cumulative weights, tile-padded (nb * kTile)
+  double* Wsub;  // multinomial: every 16th cumulative weight (nb * kTile / 16)
   int32_t* counts_out;  // optional offspring count per particle (systematic, stratified)
 };
 
@@ -119,15 +120,19 @@ __device__ __forceinline__ double block_incl_max_d(double v, double* lds, double
 
 // Cumulative offspring count for an inclusive prefix W.
 // SYSTEMATIC: O = #{s in [0,n) : fl(u0 + s) < fl(n W)} (strict, resamplers.jl:231).
+// The estimate ceil(fl(n W - u0)) is off by at most one: the rounding of n W - u0 and of
+// u0 + s both stay below 2^-28 for n < 2^31 (|u0| < 1), so the two can only disagree when
+// n W - u0 lies that close to an integer K, and then the estimate and the count are both K
+// or K + 1.  One exact test in each direction therefore settles it, branch-free.
 __device__ __forceinline__ int offspring_systematic(double W, double u0, int n) {
   const double vs = W * (double)n;
   double est = ceil(vs - u0);
-  if (!(est >= 0.0)) est = 0.0;  // also catches NaN
-  if (est > (double)n) est = (double)n;
-  int m = (int)est;
-  while (m > 0 && !(u0 + (double)(m - 1) < vs)) --m;
-  while (m < n && (u0 + (double)m < vs)) ++m;
-  return m;
+  est = (est >= 0.0) ? est : 0.0;  // also catches NaN
+  est = fmin(est, (double)n);
+  const int m = (int)est;
+  const bool down = (m > 0) && !(u0 + (double)(m - 1) < vs);
+  const bool up = (m < n) && (u0 + (double)m < vs);
+  return m + (int)up - (int)down;
 }
 
 __device__ __forceinline__ int clamp_floor(double v, int n) {
@@ -156,44 +161,14 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
   return clamp_floor(r + uk, n);
 }
 
-// Exclusive FP64 scan of one value per thread across the block (thread order, fixed
-// order); *tot = block total.
-__device__ __forceinline__ double block_excl_scan_d(double v, double* lds, double* tot) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  const double incl = wave_scan_dpp(v);
-  double ex = wave_shfl_up(incl, 1);
-  if (lane == 0) ex = 0.0;
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = incl;
-  __syncthreads();
-  for (int k = 0; k < w; ++k) ex += lds[k];
-  *tot = ((lds[0] + lds[1]) + lds[2]) + lds[3];
-  return ex;
-}
+// Output slots one wave fills per round of its ancestor fill (4 per lane).
+constexpr int kFillWin = kWave * kTileItems;
+// Multinomial search index: the cumulative weight at every 16th position of a tile.
+constexpr int kSubGroup = 16;
+constexpr int kSubPerTile = kTile / kSubGroup;
+// Entries of the LDS-resident top level of the multinomial search.
+constexpr int kCoarse = 1024;
 
-// Exclusive max-scan of one value per thread across the block (exact).
-__device__ __forceinline__ double block_excl_max_d(double v, double* lds) {
-  const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-  v = wave_scan_max_dpp(v);
-  double ex = wave_shfl_up(v, 1);
-  if (lane == 0) ex = -INFINITY;
-  __syncthreads();
-  if (lane == kWave - 1) lds[w] = v;
-  __syncthreads();
-  for (int k = 0; k < w; ++k) ex = fmax(ex, lds[k]);
-  return ex;
-}
-
-// Slots a thread writes itself; threads with more hand their range to the whole block.
-constexpr int kDirectSlots = 32;
-
-// Main kernel: tile rescan -> cumulative offspring counts -> ancestor fill.  Thread t owns
-// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial
-// scan plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by
-// its own thread (neighbouring threads write neighbouring ranges); a thread whose range
-// exceeds kDirectSlots hands it to the whole block, which keeps degenerate weights (one
-// particle taking every slot) balanced.  For MULTINOMIAL the kernel writes the
-// nondecreasing prefix sums W to Wbuf instead.
 // What a tile's resample needs besides its weights: the decision, the lazy-uniform flag,
 // the tile's prefix P, the next tile's prefix, the total, the tile's weight scale (m_b, c_b)
 // and the Philox step of the uniforms.
@@ -203,35 +178,61 @@ struct TileIn {
   uint32_t step;
 };
 
-// Loads the raw weights of tile b (clamped in-bounds indices).
+// Loads the raw weights of tile b for thread t's positions 4t..4t+3 (clamped in-bounds
+// indices; a full tile is read with one vector load per thread).
 template <int MODE>
 __device__ __forceinline__ void load_tile_raw(const RsParams& p, int b,
                                               RawW<MODE> raw[kTileItems]) {
+  using V = typename RawW<MODE>::type;
   const int n = p.n, tile0 = b * kTile;
+  const int j0 = tile0 + kTileItems * threadIdx.x;
+  const V* w = (const V*)p.w;
+  if (tile0 + kTile <= n && ((uintptr_t)w & 15) == 0) {
+    if constexpr (sizeof(V) == 4) {
+      const float4 v = *reinterpret_cast<const float4*>(w + j0);
+      raw[0].v = v.x;
+      raw[1].v = v.y;
+      raw[2].v = v.z;
+      raw[3].v = v.w;
+    } else {
+      const double2 a = *reinterpret_cast<const double2*>(w + j0);
+      const double2 c = *reinterpret_cast<const double2*>(w + j0 + 2);
+      raw[0].v = a.x;
+      raw[1].v = a.y;
+      raw[2].v = c.x;
+      raw[3].v = c.y;
+    }
+  } else {
 #pragma unroll
-  for (int k = 0; k < kTileItems; ++k) {
-    const int j = tile0 + kTileItems * threadIdx.x + k;
-    raw[k] = rs_raw<MODE>(p, j < n ? j : n - 1);
+    for (int k = 0; k < kTileItems; ++k) raw[k].v = w[min(j0 + k, n - 1)];
   }
 }
 
-// The cumulative weights of tile b's particles, W[k] for position kTileItems t + k: the
-// tile's prefix P plus the in-tile sums (a thread-serial scan of kTileItems consecutive
-// weights, then one block scan of the thread sums), made nondecreasing by an exact running
-// max.  Positions past n add zero.  Also returns the tile's prefix, the next tile's prefix
-// and the total (for the lazy TypelessZero weights: tile0/n, the next tile's start over n
-// or 1, and 1).  Optionally records each consumed FP64 weight in w_debug64.
+// The cumulative weights of tile b's particles, W[k] for position 4t + k: the tile's prefix
+// P plus the in-tile sums, nondecreasing in position by construction.  Thread t sums its 4
+// consecutive weights serially (acc_k); a wave scan of the thread sums gives each thread's
+// in-wave start x_t, and y_k = x_t + acc_k is the in-wave cumulative weight, made
+// nondecreasing across lanes by an exact running max.  Lane 63 of each wave publishes the
+// wave total and the wave's largest y; after the one barrier every thread forms the wave
+// offsets off_{v+1} = max(off_v + tot_v, off_v + ymax_v), so no value of wave v exceeds the
+// first value of wave v + 1, and W = P + (off_w + y).  Positions past n add zero.  Also
+// returns the tile's prefix, the next tile's prefix, the total (for the lazy TypelessZero
+// weights: tile0/n, the next tile's start over n or 1, and 1) and *Wedge, the W of the last
+// position of the previous wave (of the tile's first wave: P).  Optionally records each
+// consumed FP64 weight in w_debug64.
 template <int MODE>
 __device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
                                                  const RawW<MODE> raw[kTileItems],
                                                  const TileIn& in, double* w_debug64,
                                                  double W[kTileItems], double* P_out,
-                                                 double* Pnext_out, double* Wn_out) {
-  __shared__ double lds[kRsThreads / kWave];
+                                                 double* Pnext_out, double* Wn_out,
+                                                 double* Wedge_out) {
+  constexpr int NW = kRsThreads / kWave;
+  __shared__ double lds_tot[NW], lds_ymax[NW];
   const int n = p.n;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   double P, Pnext, Wn;
   if (in.typeless) {
     P = (double)tile0 / (double)n;
@@ -242,7 +243,8 @@ __device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
     Pnext = in.Pnext;
     Wn = in.Wn;
   }
-  double acc = 0.0;
+  double acc[kTileItems];
+  double run = 0.0;
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) {
     const int j = tile0 + kTileItems * t + k;
@@ -250,60 +252,136 @@ __device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
     double wj = rs_weight_of<MODE>(p, raw[k], in.m_b, in.c_b, in.typeless);
     wj = (j < n) ? wj : 0.0;
     if (w_debug64 && j < n) w_debug64[j] = wj;
-    acc += wj;
-    W[k] = acc;
+    run += wj;
+    acc[k] = run;
   }
-  double tot;
-  const double ex = block_excl_scan_d(acc, lds, &tot);
-  (void)tot;
+  const double incl = wave_scan_dpp(run);
+  double x = wave_shfl_up(incl, 1);
+  if (lane == 0) x = 0.0;
+  double y[kTileItems];
 #pragma unroll
-  for (int k = 0; k < kTileItems; ++k) W[k] = P + (ex + W[k]);
-  // nondecreasing in particle order (exact running max)
+  for (int k = 0; k < kTileItems; ++k) y[k] = x + acc[k];  // nondecreasing in k
+  const double ym_incl = wave_scan_max_dpp(y[kTileItems - 1]);
+  double ym_prev = wave_shfl_up(ym_incl, 1);
+  if (lane == 0) ym_prev = 0.0;
+  if (lane == kWave - 1) {
+    lds_tot[w] = incl;
+    lds_ymax[w] = ym_incl;
+  }
+  __syncthreads();
+  double off = 0.0, edge = P;
+  for (int v = 0; v < w; ++v) {
+    edge = P + (off + lds_ymax[v]);
+    off = fmax(off + lds_tot[v], off + lds_ymax[v]);
+  }
 #pragma unroll
-  for (int k = 1; k < kTileItems; ++k) W[k] = fmax(W[k], W[k - 1]);
-  const double carry = block_excl_max_d(W[kTileItems - 1], lds);
-#pragma unroll
-  for (int k = 0; k < kTileItems; ++k) W[k] = fmax(W[k], carry);
+  for (int k = 0; k < kTileItems; ++k) W[k] = P + (off + fmax(y[k], ym_prev));
   *P_out = P;
   *Pnext_out = Pnext;
   *Wn_out = Wn;
+  *Wedge_out = edge;
+}
+
+// One wave's ancestor fill.  The wave's particles own the contiguous output slots
+// [wstart, wend); particle k of this lane owns [start[k], O[k]).  Slots are written in
+// windows of kFillWin: every particle with offspring marks the slot where its range starts
+// with its position in the wave; lane l then takes the window's slots 4l..4l+3, and a
+// running max over them (serial in the lane, then an exclusive max scan across the lanes,
+// then the carry of the previous window) assigns every slot the last particle that started
+// at or before it.  The results go back through LDS so each store instruction writes 64
+// consecutive slots.  A particle whose range covers a whole window is written directly
+// (ow holds every position's range end), which keeps degenerate weights balanced.  anc0 is
+// the particle index of the wave's position 0.
+__device__ __forceinline__ void wave_fill(int* marks, const int* ow,
+                                          const int start[kTileItems],
+                                          const int O[kTileItems], int wstart, int wend,
+                                          int anc0, int32_t* __restrict__ anc) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int4* const mine = reinterpret_cast<int4*>(marks) + lane;
+  int base = wstart, carry = -1;
+  while (base < wend) {
+    if (carry >= 0) {
+      const int e = __builtin_amdgcn_readfirstlane(min(ow[carry], wend));
+      if (e - base >= kFillWin) {
+        const int v = anc0 + carry;
+        for (int s = base + lane; s < e; s += kWave) anc[s] = v;
+        base = e;
+        continue;
+      }
+    }
+    *mine = make_int4(-1, -1, -1, -1);
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) {
+      const int s = start[k] - base;
+      if (O[k] > start[k] && s >= 0 && s < kFillWin) marks[s] = kTileItems * lane + k;
+    }
+    wave_lds_sync();
+    int4 m = *mine;
+    m.y = max(m.y, m.x);
+    m.z = max(m.z, m.y);
+    m.w = max(m.w, m.z);
+    int before = wave_shfl_up(wave_scan_max_i32(m.w), 1);
+    before = max(lane == 0 ? -1 : before, carry);
+    m.x = max(m.x, before);
+    m.y = max(m.y, before);
+    m.z = max(m.z, before);
+    m.w = max(m.w, before);
+    carry = __builtin_amdgcn_readlane(m.w, kWave - 1);
+    wave_lds_sync();
+    *mine = m;
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < kTileItems; ++i) {
+      const int s = base + i * kWave + lane;
+      const int a = marks[i * kWave + lane];
+      if (s < wend) anc[s] = anc0 + a;
+    }
+    wave_lds_sync();
+    base += kFillWin;
+  }
 }
 
 // Tile rescan -> cumulative offspring counts -> ancestor fill, for one tile.  KIND/PLAIN as
 // offspring_of; PLAIN also means no debug weights and no offspring counts.  Thread t owns
-// the 4 consecutive particles 4t..4t+3 of the tile, so the tile scan is a thread-serial scan
-// plus one block scan.  Every particle's output slots [O_{j-1}, O_j) are written by its own
-// thread (neighbouring threads write neighbouring ranges); a thread whose range exceeds
-// kDirectSlots hands it to the whole block, which keeps degenerate weights (one particle
-// taking every slot) balanced.  For MULTINOMIAL it writes the nondecreasing prefix sums W to
-// Wbuf instead.  All threads of the block call it; it returns at once when !in.rs.
+// the 4 consecutive particles 4t..4t+3 of the tile.  Each particle's cumulative offspring
+// count O comes from its cumulative weight; the wave's output range starts at the count of
+// the previous wave's last position (recomputed from its published cumulative weight, so
+// no second barrier), and each wave writes its own range (wave_fill).  For MULTINOMIAL it
+// writes the nondecreasing cumulative weights of every position of the tile to Wbuf
+// (positions past n hold the tile's closing prefix) and every 16th of them to Wsub.
+// All threads of the block call it; it returns at once when !in.rs.
 template <int MODE, int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
                                                        const RawW<MODE> raw[kTileItems],
                                                        const TileIn& in) {
   if (!in.rs) return;
+  constexpr int NW = kRsThreads / kWave;
   const int kind = KIND >= 0 ? KIND : p.kind;
   double* const w_debug64 = PLAIN ? nullptr : p.w_debug64;
   int32_t* const counts_out = PLAIN ? nullptr : p.counts_out;
   const double* const uvec = PLAIN ? nullptr : p.u;
-  __shared__ int olast[kRsThreads];
-  __shared__ int heavy[kRsThreads][kTileItems + 2];  // {start, O_0..O_3, owner}
-  __shared__ int nheavy;
+  __shared__ __align__(16) int o_lds[kTile];
+  __shared__ __align__(16) int marks[NW][kFillWin];
   const int n = p.n;
   const int nb = (n + kTile - 1) / kTile;
   const int tile0 = b * kTile;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, w = t / kWave, lane = t & (kWave - 1);
   const int cnt = min(kTile, n - tile0);
-  double W[kTileItems], P, Pnext, Wn;
-  if (t == 0) nheavy = 0;
-  tile_cum_weights<MODE>(p, b, raw, in, w_debug64, W, &P, &Pnext, &Wn);
+  double W[kTileItems], P, Pnext, Wn, Wedge;
+  tile_cum_weights<MODE>(p, b, raw, in, w_debug64, W, &P, &Pnext, &Wn, &Wedge);
   const uint32_t step = in.step;
   if (kind == SSM_MULTINOMIAL) {
+    double v[kTileItems];
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
       const int pos = kTileItems * t + k;
-      if (pos < cnt) p.Wbuf[tile0 + pos] = (pos == cnt - 1) ? Pnext : fmin(W[k], Pnext);
+      v[k] = (pos >= cnt - 1) ? Pnext : fmin(W[k], Pnext);
     }
+    double2* dst = reinterpret_cast<double2*>(p.Wbuf + (size_t)tile0 + kTileItems * t);
+    dst[0] = make_double2(v[0], v[1]);
+    dst[1] = make_double2(v[2], v[3]);
+    if ((t & 3) == 3) p.Wsub[(size_t)b * kSubPerTile + (t >> 2)] = v[kTileItems - 1];
     return;
   }
   const double u0 = (kind == SSM_STRATIFIED)
@@ -311,51 +389,32 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
                         : (uvec ? uvec[0] : resample_uniform(rs_seed(p), step, 0u));
   const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step);
   const int o_end = (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step);
+  // the tile's last particle closes its range; positions past n own none
+  auto count_at = [&](int pos, double Wv) {
+    return (pos >= cnt - 1) ? o_end
+                            : min(max(offspring_of<KIND, PLAIN>(p, Wv, Wn, u0, step), o_prev),
+                                  o_end);
+  };
   int O[kTileItems];
 #pragma unroll
-  for (int k = 0; k < kTileItems; ++k) {
-    const int pos = kTileItems * t + k;
-    if (pos >= cnt - 1)
-      O[k] = o_end;  // the tile's last particle closes its range; positions past n own none
-    else
-      O[k] = min(max(offspring_of<KIND, PLAIN>(p, W[k], Wn, u0, step), o_prev), o_end);
-  }
-  olast[t] = O[kTileItems - 1];
-  __syncthreads();
-  const int start = (t == 0) ? o_prev : olast[t - 1];
-  if (counts_out) {  // particle k owns the output slots [O[k-1], O[k])
+  for (int k = 0; k < kTileItems; ++k) O[k] = count_at(kTileItems * t + k, W[k]);
+  const int wstart = __builtin_amdgcn_readfirstlane(
+      (w == 0) ? o_prev : count_at(kFillWin * w - 1, Wedge));
+  int s0 = wave_shfl_up(O[kTileItems - 1], 1);
+  if (lane == 0) s0 = wstart;
+  const int start[kTileItems] = {s0, O[0], O[1], O[2]};
+  if (counts_out) {  // particle k owns the output slots [start[k], O[k])
 #pragma unroll
     for (int k = 0; k < kTileItems; ++k) {
       const int pos = kTileItems * t + k;
-      if (pos < cnt) counts_out[tile0 + pos] = O[k] - (k == 0 ? start : O[k - 1]);
+      if (pos < cnt) counts_out[tile0 + pos] = O[k] - start[k];
     }
   }
-  if (O[kTileItems - 1] - start <= kDirectSlots) {
-    int s0 = start;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) {
-      for (int s = s0; s < O[k]; ++s) p.anc[s] = tile0 + kTileItems * t + k;
-      s0 = max(s0, O[k]);
-    }
-  } else {
-    const int h = atomicAdd(&nheavy, 1);
-    heavy[h][0] = start;
-#pragma unroll
-    for (int k = 0; k < kTileItems; ++k) heavy[h][k + 1] = O[k];
-    heavy[h][kTileItems + 1] = t;
-  }
-  __syncthreads();
-  // heavy ranges: the whole block strides over each one
-  const int nh = nheavy;
-  for (int h = 0; h < nh; ++h) {
-    const int hs = heavy[h][0], he = heavy[h][kTileItems];
-    const int owner = heavy[h][kTileItems + 1];
-    for (int s = hs + t; s < he; s += kRsThreads) {
-      int k = 0;
-      while (k < kTileItems - 1 && heavy[h][k + 1] <= s) ++k;
-      p.anc[s] = tile0 + kTileItems * owner + k;
-    }
-  }
+  int* ow = o_lds + w * kFillWin;
+  *reinterpret_cast<int4*>(ow + kTileItems * lane) = make_int4(O[0], O[1], O[2], O[3]);
+  wave_lds_sync();
+  const int wend = __builtin_amdgcn_readlane(O[kTileItems - 1], kWave - 1);
+  wave_fill(marks[w], ow, start, O, wstart, wend, tile0 + w * kFillWin, p.anc);
 }
 
 // The resample of tile b with its inputs from memory: the decision and flags from the
@@ -380,25 +439,61 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   scan_offspring_tile_in<MODE, KIND, PLAIN>(p, b, raw, in);
 }
 
-// Multinomial: a_k = min{j : W_j >= u_k * W_n} by binary search over Wbuf.
-__device__ __forceinline__ void multinomial_search_item(const RsParams& p, int k) {
+// The cumulative weight that closes tile b: the last entry of its Wbuf positions.
+__device__ __forceinline__ double tile_close(const RsParams& p, int b, int nb, bool typeless,
+                                             double Wn) {
+  if (b >= nb - 1) return Wn;
+  return typeless ? (double)((b + 1) * kTile) / (double)p.n : p.prefix[b + 1];
+}
+
+// Multinomial: a_k = min{j : W_j >= u_k * W_n} for the queries k of block `blk`
+// (blk * kTile + 256 i + t, i < 4), with W the nondecreasing cumulative weights the scan
+// wrote to Wbuf.  Three branch-free lower-bound searches: the tile (over the tiles' closing
+// prefixes, whose top level -- every 2^sh-th close, at most kCoarse of them -- sits in
+// LDS), the group of 16 within the tile (Wsub), and the position within the group (Wbuf).
+// Each level's last entry is at least the query, so the result equals a search of the
+// whole Wbuf.  All threads of the block call it.
+__device__ __forceinline__ void multinomial_search_block(const RsParams& p, int blk) {
+  __shared__ double coarse[kCoarse];
   if (!rs_active(p)) return;
-  if (k >= p.n) return;
+  const int n = p.n, nb = (n + kTile - 1) / kTile, t = threadIdx.x;
   const uint32_t step = rs_step(p);
-  const int nb = (p.n + kTile - 1) / kTile;
   const bool typeless = (p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64) && p.sc->typeless;
   const double Wn = typeless ? 1.0 : p.prefix[nb];
-  const double uk = p.u ? p.u[k] : resample_uniform(rs_seed(p), step, (uint32_t)k);
-  const double q = uk * Wn;
-  int lo = 0, hi = p.n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (p.Wbuf[mid] < q)
-      lo = mid + 1;
-    else
-      hi = mid;
+  int sh = 0;
+  while (((nb + (1 << sh) - 1) >> sh) > kCoarse) ++sh;
+  const int ns = (nb + (1 << sh) - 1) >> sh;
+  for (int i = t; i < ns; i += kRsThreads)
+    coarse[i] = tile_close(p, min((i + 1) << sh, nb) - 1, nb, typeless, Wn);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kTileItems; ++i) {
+    const int k = blk * kTile + i * kRsThreads + t;
+    if (k < n) {
+      const double uk = p.u ? p.u[k] : resample_uniform(rs_seed(p), step, (uint32_t)k);
+      const double q = uk * Wn;
+      int c = 0;
+#pragma unroll
+      for (int h = kCoarse / 2; h >= 1; h >>= 1)
+        if (c + h <= ns - 1 && coarse[c + h - 1] < q) c += h;
+      const int g0 = c << sh, gl = min(g0 + (1 << sh), nb) - g0;
+      int bt = 0;
+      for (int h = (1 << sh) >> 1; h >= 1; h >>= 1)
+        if (bt + h <= gl - 1 && tile_close(p, g0 + bt + h - 1, nb, typeless, Wn) < q) bt += h;
+      const int bs = g0 + bt;
+      const double* ws = p.Wsub + (size_t)bs * kSubPerTile;
+      int g = 0;
+#pragma unroll
+      for (int h = kSubPerTile / 2; h >= 1; h >>= 1)
+        if (ws[g + h - 1] < q) g += h;
+      const double* wb = p.Wbuf + (size_t)bs * kTile + kSubGroup * g;
+      int e = 0;
+#pragma unroll
+      for (int h = kSubGroup / 2; h >= 1; h >>= 1)
+        if (wb[e + h - 1] < q) e += h;
+      p.anc[k] = min(bs * kTile + kSubGroup * g + e, n - 1);
+    }
   }
-  p.anc[k] = min(lo, p.n - 1);
 }
 
 }  // namespace ssm
@@ -455,8 +550,8 @@ __device__ __forceinline__ void categorical_one_block(const RsParams& p, double 
   in.m_b = kLog ? p.blk_m[b] : 0.0;
   in.c_b = kLog ? p.blk_c[b] : 0.0;
   in.step = step;
-  double W[kTileItems], P, Pnext, Wt;
-  tile_cum_weights<MODE>(p, b, raw, in, nullptr, W, &P, &Pnext, &Wt);
+  double W[kTileItems], P, Pnext, Wt, Wedge;
+  tile_cum_weights<MODE>(p, b, raw, in, nullptr, W, &P, &Pnext, &Wt, &Wedge);
   const int tile0 = b * kTile, cnt = min(kTile, n - tile0);
   // the first position of the tile whose cumulative weight reaches q (the tile's last
   // particle closes at Pnext, as in the multinomial resampler's W array)

@@ -57,6 +57,7 @@ struct BfbFilter {
   BfScalars* sc;
   TileScratch ts;
   double* Wbuf;
+  double* Wsub;
 };
 
 __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
@@ -77,6 +78,7 @@ __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
   p.tile_sum = nullptr;
   p.prefix = F.ts.prefix;
   p.Wbuf = F.Wbuf;
+  p.Wsub = F.Wsub;
   p.counts_out = nullptr;
   return p;
 }
@@ -93,9 +95,7 @@ __global__ void __launch_bounds__(kTileThreads)
   const BfbFilter& F = fs[owner[blockIdx.x]];
   if (F.kind != SSM_MULTINOMIAL) return;
   const RsParams p = bfb_rs_params(F);
-  const int k0 = (blockIdx.x - F.tile_base) * kTile + threadIdx.x;
-#pragma unroll
-  for (int j = 0; j < kTileItems; ++j) multinomial_search_item(p, k0 + j * kTileThreads);
+  multinomial_search_block(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
@@ -342,8 +342,9 @@ int bf_setup(ssm_server* sv, SvReq* r) {
   F.sc = (BfScalars*)(base + L.sc);
   ResampleArgs a;
   resample_bind_scratch(a, base + L.rs, r->N);
-  F.ts = TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix};
+  F.ts = TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
   F.Wbuf = a.Wbuf;
+  F.Wsub = a.Wsub;
   BfScalars h;
   std::memset(&h, 0, sizeof h);
   h.s1 = (double)r->N;
