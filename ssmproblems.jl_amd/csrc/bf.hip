@@ -355,8 +355,8 @@ void enqueue_move(ssm_bf* bf, hipStream_t s, double threshold, const void* noise
   bf->cur ^= 1;
 }
 
-// Enqueues the one-block finish kernel (record = 1: a filter step; 0: re-derive the
-// weight normalisers of imported log-weights without recording a step).
+// Enqueues the finish kernel (record = 1: a filter step; 0: re-derive the weight
+// normalisers of imported log-weights without recording a step).
 void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
   const int nb = (bf->N + kTile - 1) / kTile;
   const double threshold_n = threshold * (double)bf->N;

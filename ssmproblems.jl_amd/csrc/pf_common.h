@@ -1,5 +1,5 @@
-// pf_common.h -- tiles, per-tile weight partials and the one-block finish kernel shared by
-// the particle filters and the resampler.
+// pf_common.h -- tiles, per-tile weight partials and the finish kernels shared by the
+// particle filters and the resampler.
 //
 // Particles are processed in tiles of kTile = 1024 (256 threads x 4, lane-striped: item k
 // of thread t is particle tile0 + 256 k + t, so every load and store of a wave is one
@@ -9,7 +9,8 @@
 //   m1_b        max of the updated log-weights l (in the filter's type T);
 //   tau_b       sum_i e_i in FP64 with e_i = exp(l_i - m1_b) evaluated in T;
 //   s2_b        sum_i e_i^2 in FP64.
-// The finish kernel (one block) folds them in a fixed order into
+// The finish kernel (one block up to 1024 tiles, two group passes beyond) folds them in a
+// fixed order into
 //   lse = M + log S (M = max m1_b, S = sum tau_b exp(m1_b - M)),  ll = lse - baseline
 //   (marginalise!, GF/src/containers.jl:160-173),  ESS = S^2 / sum s2_b exp(2(m1_b - M))
 //   (GF/src/resamplers.jl:179-183),
@@ -40,8 +41,13 @@ struct TileScratch {
 constexpr int kFinishGroup = 1024;  // tiles one finish block folds
 constexpr int kGroupRec = 8;        // doubles per group record
 
-// Accurate exponential of the weights (the resampler recomputes it bit for bit).
-__device__ __forceinline__ float exp_w(float v) { return expf(v); }
+// Exponential of the weights, e = 2^(v log2 e) on the hardware exp2 (v <= 0 here, so the
+// relative error stays below 6e-6 even at v = -88; at v = 0 it is exactly 1).  The step
+// kernel's partials, the finish's scales and the resampler's rescan all call this one
+// function, so the weights the resampler scans are exactly the ones the sums describe.
+__device__ __forceinline__ float exp_w(float v) {
+  return __builtin_amdgcn_exp2f(v * 1.44269504088896341f);
+}
 __device__ __forceinline__ double exp_w(double v) { return exp(v); }
 
 // Writes tile b's partial from each thread's 4 items.  nl: pre-update normalised
@@ -365,19 +371,169 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
 }
 
 
-// ---- the finish with one tile per thread ------------------------------------------
-// Block-wide fixed-order reductions over NT threads (NT / 64 waves): the wave values go
-// through lds[NT / 64] and every thread sums (or maximises) them in wave order.
-template <int NT>
-__device__ __forceinline__ double block_sum_fixed(double v, double* lds) {
-  constexpr int NW = NT / 64;
-  v = wave_sum_dpp(v);
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = lds[0];
+// ---- the fold of at most 1024 tiles by one 256-thread block ------------------------
+// Thread t owns the tiles t + 256 q (q < 4): row q of the tiles is one coalesced load per
+// field.  (Folding inside every block of the next step's resampler instead, which saves
+// this kernel's launch, was measured 4x slower at 1024 tiles: the fold's ~700 VALU
+// instructions per wave, repeated in every block, cost more than the launch.)
+constexpr int kFoldThreads = 256;
+constexpr int kFoldTiles = 4;
+constexpr int kFoldMax = kFoldThreads * kFoldTiles;
+
+struct Fold4 {
+  double m1[kFoldTiles], c[kFoldTiles], pre[kFoldTiles];  // tile t + 256 q
+  double M1, S, S2, M0, S0, tot;
+  bool dead;
+};
+
+// Folds the partials of nb <= kFoldMax tiles: maxima, the scaled sums (exp(m1_b - M1) in T)
+// and the exclusive prefixes of c_b tau_b, nondecreasing in tile order.  Row q's prefixes
+// are the row offset plus the in-row prefix; the in-row exclusive prefixes x are made
+// monotone across lanes by an exact running max, the waves of a row are chained by
+// off_{w+1} = max(off_w + tot_w, off_w + xmax_w), and the rows by the same rule on the row
+// totals, so no prefix exceeds the next tile's.  tot is the last offset, at least every
+// prefix.  The baseline partials are only read when the folded step did not start fresh.
+// Three barriers; the LDS of the maxima is reused for the prefix exchange.
+template <typename T>
+__device__ __forceinline__ void fold_step4(int nb, const double* __restrict__ P, bool fresh,
+                                           Fold4& F) {
+  constexpr int NW = kFoldThreads / 64, R = kFoldTiles;
+  __shared__ double red[5][NW];
+  __shared__ double rows[2][R][NW];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double m0[R], s0[R], tau[R], s2[R];
 #pragma unroll
-  for (int k = 1; k < NW; ++k) s += lds[k];
-  return s;
+  for (int q = 0; q < R; ++q) {
+    const int bc = min(q * kFoldThreads + t, nb - 1);
+    F.m1[q] = P[2 * nb + bc];
+    tau[q] = P[3 * nb + bc];
+    s2[q] = P[4 * nb + bc];
+  }
+  if (!fresh) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int bc = min(q * kFoldThreads + t, nb - 1);
+      m0[q] = P[bc];
+      s0[q] = P[nb + bc];
+    }
+  }
+  double M1 = -INFINITY, M0 = -INFINITY;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool in = q * kFoldThreads + t < nb;
+    m0[q] = (in && !fresh) ? m0[q] : -INFINITY;
+    s0[q] = (in && !fresh) ? s0[q] : 0.0;
+    F.m1[q] = in ? F.m1[q] : -INFINITY;
+    tau[q] = in ? tau[q] : 0.0;
+    s2[q] = in ? s2[q] : 0.0;
+    M1 = fmax(M1, F.m1[q]);
+    M0 = fmax(M0, m0[q]);
+  }
+  M1 = wave_max_dpp(M1);
+  M0 = wave_max_dpp(M0);
+  if (lane == 0) {
+    red[0][wv] = M1;
+    red[1][wv] = M0;
+  }
+  __syncthreads();
+  M1 = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+  M0 = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+  double S = 0.0, S2 = 0.0, S0 = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    F.c[q] = (F.m1[q] != -INFINITY) ? (double)exp_w((T)(F.m1[q] - M1)) : 0.0;
+    S += tau[q] * F.c[q];
+    S2 += s2[q] * F.c[q] * F.c[q];
+    if (!fresh && m0[q] != -INFINITY) S0 += s0[q] * (double)exp_w((T)(m0[q] - M0));
+  }
+  S = wave_sum_dpp(S);
+  S2 = wave_sum_dpp(S2);
+  S0 = wave_sum_dpp(S0);
+  if (lane == 0) {
+    red[2][wv] = S;
+    red[3][wv] = S2;
+    red[4][wv] = S0;
+  }
+  __syncthreads();
+  S = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+  S2 = ((red[3][0] + red[3][1]) + red[3][2]) + red[3][3];
+  S0 = ((red[4][0] + red[4][1]) + red[4][2]) + red[4][3];
+  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
+  const double invS = dead ? 0.0 : 1.0 / S;
+  double x[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    F.c[q] *= invS;
+    const double v = F.c[q] * tau[q];
+    const double incl = wave_scan_dpp(v);
+    double xe = wave_shfl_up(incl, 1);
+    if (lane == 0) xe = 0.0;
+    x[q] = wave_scan_max_dpp(xe);
+    if (lane == 63) {
+      rows[0][q][wv] = incl;
+      rows[1][q][wv] = x[q];
+    }
+  }
+  __syncthreads();
+  double off = 0.0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    double mine = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      if (k == wv) mine = off;
+      off = fmax(off + rows[0][q][k], off + rows[1][q][k]);
+    }
+    F.pre[q] = mine + x[q];
+  }
+  F.M1 = M1;
+  F.S = S;
+  F.S2 = S2;
+  F.M0 = M0;
+  F.S0 = S0;
+  F.tot = off;
+  F.dead = dead;
+}
+
+// The folded step as the scalars need it.
+__device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
+  TileFold<1> G;
+  G.M1 = F.M1;
+  G.S = F.S;
+  G.S2 = F.S2;
+  G.M0 = F.M0;
+  G.S0 = F.S0;
+  G.tot = F.tot;
+  G.dead = F.dead;
+  return G;
+}
+
+// The finish of nb <= 1024 tiles (one block): the fold, every tile's scale, maximum and
+// prefix, the total and the new scalars (sc_in may equal sc: it is read before the write).
+template <typename T>
+__global__ void __launch_bounds__(kFoldThreads)
+    k_pf_finish_fold(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
+                     double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
+                     int record) {
+  const BfScalars in = *sc_in;
+  const bool rs = threshold_n >= in.ess;
+  const bool fresh = rs || in.typeless;
+  Fold4 F;
+  fold_step4<T>(nb, ts.partials, fresh, F);
+#pragma unroll
+  for (int q = 0; q < kFoldTiles; ++q) {
+    const int b = q * kFoldThreads + threadIdx.x;
+    if (b < nb) {
+      ts.prefix[b] = F.pre[q];
+      ts.blk_c[b] = F.c[q];
+      ts.blk_m[b] = F.m1[q];
+    }
+  }
+  if (threadIdx.x == 0) {
+    ts.prefix[nb] = F.tot;
+    *sc = fold_scalars<T, 1>(fold_summary(F), in, N, logN, rs, fresh, ll_steps, rs_steps,
+                             record);
+  }
 }
 
 // Exclusive prefix of one value per thread across NT threads, nondecreasing in thread order
@@ -407,83 +563,6 @@ __device__ __forceinline__ double block_prefix_monotone(double v, double* lds_to
   }
   *total = off;
   return mine + x;
-}
-
-// The finish of nb <= NT tiles by one block of NT threads, tile b on thread b: the maxima,
-// the scaled sums (exp(m1_b - M1) in T) and the prefixes of c_b tau_b in three barriers.
-template <typename T, int NT>
-__global__ void __launch_bounds__(NT)
-    k_pf_finish_tiles(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
-                      double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                      int record) {
-  constexpr int NW = NT / 64;
-  __shared__ double red[5][NW];
-  const bool rs = threshold_n >= sc_in->ess;
-  const bool fresh = rs || sc_in->typeless;
-  const double* P = ts.partials;
-  const int b = threadIdx.x;
-  const bool in = b < nb;
-  const int bc = in ? b : nb - 1;
-  double m0 = P[bc], s0 = P[nb + bc], m1 = P[2 * nb + bc], tau = P[3 * nb + bc],
-         s2 = P[4 * nb + bc];
-  m0 = (in && !fresh) ? m0 : -INFINITY;
-  s0 = (in && !fresh) ? s0 : 0.0;
-  m1 = in ? m1 : -INFINITY;
-  tau = in ? tau : 0.0;
-  s2 = in ? s2 : 0.0;
-  // maxima (exact, any order)
-  double M1 = wave_max_dpp(m1), M0 = wave_max_dpp(m0);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = M1;
-    red[1][threadIdx.x >> 6] = M0;
-  }
-  __syncthreads();
-  M1 = red[0][0];
-  M0 = red[1][0];
-#pragma unroll
-  for (int k = 1; k < NW; ++k) {
-    M1 = fmax(M1, red[0][k]);
-    M0 = fmax(M0, red[1][k]);
-  }
-  const double c = (m1 != -INFINITY) ? (double)exp_w((T)(m1 - M1)) : 0.0;
-  const double e0 = (!fresh && m0 != -INFINITY) ? s0 * (double)exp_w((T)(m0 - M0)) : 0.0;
-  double S = wave_sum_dpp(tau * c), S2 = wave_sum_dpp(s2 * c * c), S0 = wave_sum_dpp(e0);
-  if ((threadIdx.x & 63) == 0) {
-    red[2][threadIdx.x >> 6] = S;
-    red[3][threadIdx.x >> 6] = S2;
-    red[4][threadIdx.x >> 6] = S0;
-  }
-  __syncthreads();
-  S = red[2][0];
-  S2 = red[3][0];
-  S0 = red[4][0];
-#pragma unroll
-  for (int k = 1; k < NW; ++k) {
-    S += red[2][k];
-    S2 += red[3][k];
-    S0 += red[4][k];
-  }
-  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
-  const double cb = c * (dead ? 0.0 : 1.0 / S);
-  double tot;
-  const double pre = block_prefix_monotone<NT>(cb * tau, red[0], red[1], &tot);
-  if (in) {
-    ts.prefix[b] = pre;
-    ts.blk_c[b] = cb;
-    ts.blk_m[b] = m1;
-  }
-  if (threadIdx.x == 0) {
-    ts.prefix[nb] = tot;
-    TileFold<1> F;
-    F.M1 = M1;
-    F.S = S;
-    F.S2 = S2;
-    F.M0 = M0;
-    F.S0 = S0;
-    F.tot = tot;
-    F.dead = dead;
-    *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
-  }
 }
 
 // The finish of more than 1024 tiles, in two kernels over groups of kFinishGroup tiles.
@@ -617,17 +696,14 @@ __global__ void __launch_bounds__(kFinishGroup)
   }
 }
 
-// Enqueues the finish: one block with a thread per tile up to 1024 tiles, the two group
-// passes beyond (ts.groups must hold a record per group).
+// Enqueues the finish: one block up to 1024 tiles, the two group passes beyond (ts.groups
+// must hold a record per group).
 template <typename T>
 inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double threshold_n,
                              double logN, const TileScratch& ts, double* ll_steps,
                              int32_t* rs_steps, int record) {
-  if (nb <= 256) {
-    hipLaunchKernelGGL((k_pf_finish_tiles<T, 256>), dim3(1), dim3(256), 0, s, N, nb, sc, sc,
-                       threshold_n, logN, ts, ll_steps, rs_steps, record);
-  } else if (nb <= 1024) {
-    hipLaunchKernelGGL((k_pf_finish_tiles<T, 1024>), dim3(1), dim3(1024), 0, s, N, nb, sc, sc,
+  if (nb <= kFoldMax) {
+    hipLaunchKernelGGL((k_pf_finish_fold<T>), dim3(1), dim3(kFoldThreads), 0, s, N, nb, sc, sc,
                        threshold_n, logN, ts, ll_steps, rs_steps, record);
   } else {
     const int G = (nb + kFinishGroup - 1) / kFinishGroup;

@@ -13,14 +13,16 @@
 // from a particle filter's finish kernel (its tile partials already hold each tile's
 // weight sum) or, for explicit weight vectors, from a tile-sum kernel plus a one-block
 // scan.  The main kernel rescans one tile in FP64 on top of its prefix (thread-serial over
-// 4 consecutive particles, then one block scan), turns the prefix sums into cumulative
-// offspring counts O_j, and writes each particle's output slots [O_{j-1}, O_j) from the
-// thread that owns it; ranges longer than 32 slots are written by the whole block, so
-// degenerate weights stay balanced.
-// The FP64 prefix sums never reach HBM except for Multinomial, whose queries are unordered
-// and need a global binary search.  Neighbouring tiles compute their shared boundary count
-// from the same stored prefix, and every prefix sequence is made nondecreasing with an exact
-// running max, so the tiles' output ranges partition [0, n) exactly.
+// 4 consecutive particles, a wave scan and one barrier), turns the cumulative weights into
+// cumulative offspring counts O_j, and each wave writes its own particles' contiguous range
+// of output slots through LDS so that every store instruction covers 64 consecutive slots;
+// a particle whose range covers a whole fill window is written directly, so degenerate
+// weights stay balanced.
+// The FP64 cumulative weights never reach HBM except for Multinomial, whose queries are
+// unordered and search them (three levels: tile, group of 16, position).  Neighbouring tiles
+// compute their shared boundary count from the same stored prefix, and every cumulative
+// sequence is nondecreasing by construction, so the tiles' output ranges partition [0, n)
+// exactly.
 #include "resample_dev.h"
 
 namespace ssm {

@@ -138,10 +138,13 @@ def test_philox_noise_matches_oracle(ctx, orc):
 
 @pytest.mark.parametrize("kind,N,prec", [(0, 20000, "f32"), (2, 20000, "f32"),
                                          (3, 20000, "f32"), (0, 300000, "f32"),
-                                         (2, 5000, "f64"), (3, 1 << 20, "f32")])
+                                         (2, 5000, "f64"), (3, 1 << 20, "f32"),
+                                         (1, 70001, "f32"), (0, 1 << 20, "f32"),
+                                         (0, (1 << 20) + 1, "f32"), (3, 3000, "f64")])
 def test_run_equals_steps(ctx, kind, N, prec):
-    # The graph-replayed run and the step-by-step path execute the same kernels on the
-    # same counters: identical log-likelihood increments, bit for bit.
+    # The graph-replayed run (up to 1024 tiles: the finish folded into the next step's
+    # resampler) and the step-by-step path (resampler, step kernel, finish) run the same
+    # instructions on the same counters: identical log-likelihood increments, bit for bit.
     m = config2_model()
     model = to_ssmhip_model(m)
     T = 37
