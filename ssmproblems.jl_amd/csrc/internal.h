@@ -89,7 +89,8 @@ struct ResampleArgs {
   double* prefix = nullptr;
   double* groups = nullptr;          // finish group records (more than 1024 tiles)
   double* Wbuf = nullptr;            // multinomial cumulative weights (tile-padded)
-  double* Wsub = nullptr;            // multinomial: every 16th of them
+  double* Wmid = nullptr;            // multinomial: every 4th of them
+  double* Wroot = nullptr;           // multinomial: every 64th of them
   int32_t* counts_out = nullptr;     // optional: offspring count of each particle
                                      // (systematic and stratified; ignored otherwise)
 };

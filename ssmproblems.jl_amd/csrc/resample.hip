@@ -32,7 +32,7 @@ size_t resample_scratch_bytes(int n) {
   auto r = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t ng = (nb + kFinishGroup - 1) / kFinishGroup;
   return r(nb * kPartial * 8) + r(nb * 8) + r(nb * 8) + r((nb + 1) * 8) + r(ng * kGroupRec * 8) +
-         r(nb * kTile * 8) + r(nb * kSubPerTile * 8);
+         r(nb * kTile * 8) + r(nb * kMidPerTile * 8) + r(nb * kRootPerTile * 8);
 }
 
 void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
@@ -51,7 +51,9 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   p += r((nb + kFinishGroup - 1) / kFinishGroup * kGroupRec * 8);
   a.Wbuf = (double*)p;
   p += r(nb * kTile * 8);
-  a.Wsub = (double*)p;
+  a.Wmid = (double*)p;
+  p += r(nb * kMidPerTile * 8);
+  a.Wroot = (double*)p;
 }
 
 // Explicit weights, kernel 1: FP64 tile sums (thread-serial over its striped items, then
@@ -155,7 +157,8 @@ static RsParams rs_params_of(const ResampleArgs& a) {
   rp.tile_sum = a.partials;
   rp.prefix = a.prefix;
   rp.Wbuf = a.Wbuf;
-  rp.Wsub = a.Wsub;
+  rp.Wmid = a.Wmid;
+  rp.Wroot = a.Wroot;
   rp.counts_out = (a.kind == SSM_MULTINOMIAL) ? nullptr : a.counts_out;
   return rp;
 }

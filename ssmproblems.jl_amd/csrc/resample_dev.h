@@ -32,8 +32,9 @@ struct RsParams {
   const double* blk_c;
   double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
   double* prefix;
-  double* Wbuf;  // multinomial: cumulative weights, tile-padded (nb * kTile)
-  double* Wsub;  // multinomial: every 16th cumulative weight (nb * kTile / 16)
+  double* Wbuf;   // multinomial: cumulative weights, tile-padded (nb * kTile)
+  double* Wmid;   // multinomial: every 4th of them (nb * kTile / 4)
+  double* Wroot;  // multinomial: every 64th of them (nb * kTile / 64)
   int32_t* counts_out;  // optional offspring count per particle (systematic, stratified)
 };
 
@@ -163,9 +164,10 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
 
 // Output slots one wave fills per round of its ancestor fill (4 per lane).
 constexpr int kFillWin = kWave * kTileItems;
-// Multinomial search index: the cumulative weight at every 16th position of a tile.
-constexpr int kSubGroup = 16;
-constexpr int kSubPerTile = kTile / kSubGroup;
+// Multinomial search index: per tile, the cumulative weight closing every group of 64
+// positions (16 per tile, one 128-byte node) and every group of 4 (16 per 64-group).
+constexpr int kRootGroup = 64, kRootPerTile = kTile / kRootGroup;
+constexpr int kMidGroup = 4, kMidPerTile = kTile / kMidGroup;
 // Entries of the LDS-resident top level of the multinomial search.
 constexpr int kCoarse = 1024;
 
@@ -349,7 +351,8 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
 // the previous wave's last position (recomputed from its published cumulative weight, so
 // no second barrier), and each wave writes its own range (wave_fill).  For MULTINOMIAL it
 // writes the nondecreasing cumulative weights of every position of the tile to Wbuf
-// (positions past n hold the tile's closing prefix) and every 16th of them to Wsub.
+// (positions past n hold the tile's closing prefix), every 4th of them to Wmid and every
+// 64th to Wroot.
 // All threads of the block call it; it returns at once when !in.rs.
 template <int MODE, int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
@@ -381,7 +384,8 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
     double2* dst = reinterpret_cast<double2*>(p.Wbuf + (size_t)tile0 + kTileItems * t);
     dst[0] = make_double2(v[0], v[1]);
     dst[1] = make_double2(v[2], v[3]);
-    if ((t & 3) == 3) p.Wsub[(size_t)b * kSubPerTile + (t >> 2)] = v[kTileItems - 1];
+    p.Wmid[(size_t)b * kMidPerTile + t] = v[kTileItems - 1];
+    if ((t & 15) == 15) p.Wroot[(size_t)b * kRootPerTile + (t >> 4)] = v[kTileItems - 1];
     return;
   }
   const double u0 = (kind == SSM_STRATIFIED)
@@ -446,13 +450,31 @@ __device__ __forceinline__ double tile_close(const RsParams& p, int b, int nb, b
   return typeless ? (double)((b + 1) * kTile) / (double)p.n : p.prefix[b + 1];
 }
 
+// The number of the first L - 1 of L nondecreasing values below q (the lower bound of q when
+// the last value is at least q): one vector load of the node, no dependent steps.
+template <int L>
+__device__ __forceinline__ int node_rank(const double* __restrict__ node, double q) {
+  const double2* v2 = reinterpret_cast<const double2*>(node);
+  double2 v[L / 2];
+#pragma unroll
+  for (int i = 0; i < L / 2; ++i) v[i] = v2[i];
+  int r = 0;
+#pragma unroll
+  for (int i = 0; i < L / 2; ++i) {
+    r += (v[i].x < q) ? 1 : 0;
+    if (2 * i + 1 < L - 1) r += (v[i].y < q) ? 1 : 0;
+  }
+  return r;
+}
+
 // Multinomial: a_k = min{j : W_j >= u_k * W_n} for the queries k of block `blk`
 // (blk * kTile + 256 i + t, i < 4), with W the nondecreasing cumulative weights the scan
-// wrote to Wbuf.  Three branch-free lower-bound searches: the tile (over the tiles' closing
-// prefixes, whose top level -- every 2^sh-th close, at most kCoarse of them -- sits in
-// LDS), the group of 16 within the tile (Wsub), and the position within the group (Wbuf).
-// Each level's last entry is at least the query, so the result equals a search of the
-// whole Wbuf.  All threads of the block call it.
+// wrote to Wbuf.  The tile comes from a branch-free lower-bound search over the tiles'
+// closing prefixes (its top level -- every 2^sh-th close, at most kCoarse of them -- sits
+// in LDS); within the tile a three-level tree of 16-way nodes (Wroot: the closes of the 64-
+// position groups, Wmid: of the 4-position groups, then Wbuf) costs three dependent vector
+// loads of one cache line or less.  Each node's last entry is at least the query, so the
+// result equals a search of the whole Wbuf.  All threads of the block call it.
 __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int blk) {
   __shared__ double coarse[kCoarse];
   if (!rs_active(p)) return;
@@ -466,33 +488,44 @@ __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int 
   for (int i = t; i < ns; i += kRsThreads)
     coarse[i] = tile_close(p, min((i + 1) << sh, nb) - 1, nb, typeless, Wn);
   __syncthreads();
+  // two queries at a time: their node loads are in flight together
 #pragma unroll
-  for (int i = 0; i < kTileItems; ++i) {
-    const int k = blk * kTile + i * kRsThreads + t;
-    if (k < n) {
-      const double uk = p.u ? p.u[k] : resample_uniform(rs_seed(p), step, (uint32_t)k);
-      const double q = uk * Wn;
+  for (int i0 = 0; i0 < kTileItems; i0 += 2) {
+    int k[2], bs[2];
+    double q[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      k[j] = blk * kTile + (i0 + j) * kRsThreads + t;
+      const int kc = min(k[j], n - 1);
+      const double uk = p.u ? p.u[kc] : resample_uniform(rs_seed(p), step, (uint32_t)kc);
+      q[j] = uk * Wn;
       int c = 0;
 #pragma unroll
       for (int h = kCoarse / 2; h >= 1; h >>= 1)
-        if (c + h <= ns - 1 && coarse[c + h - 1] < q) c += h;
+        if (c + h <= ns - 1 && coarse[c + h - 1] < q[j]) c += h;
       const int g0 = c << sh, gl = min(g0 + (1 << sh), nb) - g0;
       int bt = 0;
       for (int h = (1 << sh) >> 1; h >= 1; h >>= 1)
-        if (bt + h <= gl - 1 && tile_close(p, g0 + bt + h - 1, nb, typeless, Wn) < q) bt += h;
-      const int bs = g0 + bt;
-      const double* ws = p.Wsub + (size_t)bs * kSubPerTile;
-      int g = 0;
-#pragma unroll
-      for (int h = kSubPerTile / 2; h >= 1; h >>= 1)
-        if (ws[g + h - 1] < q) g += h;
-      const double* wb = p.Wbuf + (size_t)bs * kTile + kSubGroup * g;
-      int e = 0;
-#pragma unroll
-      for (int h = kSubGroup / 2; h >= 1; h >>= 1)
-        if (wb[e + h - 1] < q) e += h;
-      p.anc[k] = min(bs * kTile + kSubGroup * g + e, n - 1);
+        if (bt + h <= gl - 1 && tile_close(p, g0 + bt + h - 1, nb, typeless, Wn) < q[j])
+          bt += h;
+      bs[j] = g0 + bt;
     }
+    int g1[2], g2[2], e[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      g1[j] = node_rank<kRootPerTile>(p.Wroot + (size_t)bs[j] * kRootPerTile, q[j]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      g2[j] = node_rank<kRootGroup / kMidGroup>(
+          p.Wmid + (size_t)bs[j] * kMidPerTile + g1[j] * (kRootGroup / kMidGroup), q[j]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      e[j] = node_rank<kMidGroup>(
+          p.Wbuf + (size_t)bs[j] * kTile + g1[j] * kRootGroup + g2[j] * kMidGroup, q[j]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      if (k[j] < n)
+        p.anc[k[j]] = min(bs[j] * kTile + g1[j] * kRootGroup + g2[j] * kMidGroup + e[j], n - 1);
   }
 }
 

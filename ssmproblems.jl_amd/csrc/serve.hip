@@ -57,7 +57,8 @@ struct BfbFilter {
   BfScalars* sc;
   TileScratch ts;
   double* Wbuf;
-  double* Wsub;
+  double* Wmid;
+  double* Wroot;
 };
 
 __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
@@ -78,7 +79,8 @@ __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
   p.tile_sum = nullptr;
   p.prefix = F.ts.prefix;
   p.Wbuf = F.Wbuf;
-  p.Wsub = F.Wsub;
+  p.Wmid = F.Wmid;
+  p.Wroot = F.Wroot;
   p.counts_out = nullptr;
   return p;
 }
@@ -344,7 +346,8 @@ int bf_setup(ssm_server* sv, SvReq* r) {
   resample_bind_scratch(a, base + L.rs, r->N);
   F.ts = TileScratch{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
   F.Wbuf = a.Wbuf;
-  F.Wsub = a.Wsub;
+  F.Wmid = a.Wmid;
+  F.Wroot = a.Wroot;
   BfScalars h;
   std::memset(&h, 0, sizeof h);
   h.s1 = (double)r->N;
