@@ -111,6 +111,18 @@ int32_t ssm_memcpy_d2h(ssm_ctx* ctx, void* dst_host, const void* src_dev, uint64
 int32_t ssm_sample_ancestors(ssm_ctx* ctx, int32_t kind, int32_t n, const void* d_w,
                              int32_t w_dtype, const double* d_u, uint64_t seed, uint32_t step,
                              int32_t* d_ancestors);
+/*
+ * ssm_sample_ancestors without any device allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as ssm_sample_ancestors for
+ * n <= 1048576 (larger n returns SSM_ERR_ARG); the scratch is a fixed ~10 MiB device array
+ * per GPU, held until the process ends and shared by every call on that GPU, so calls that
+ * overlap on one GPU (several contexts or host threads at once) race.  It calls no hipMalloc,
+ * hipFree or allocating library routine.
+ */
+int32_t ssm_sample_ancestors_noalloc(ssm_ctx* ctx, int32_t kind, int32_t n, const void* d_w,
+                                     int32_t w_dtype, const double* d_u, uint64_t seed,
+                                     uint32_t step, int32_t* d_ancestors);
 /* Same, with host buffers (copied in and out on the context stream). */
 int32_t ssm_sample_ancestors_host(ssm_ctx* ctx, int32_t kind, int32_t n, const double* w,
                                   const double* u, int32_t* ancestors);

@@ -48,6 +48,12 @@ extern "C" int32_t ssm_ctx_create(int32_t device, uint64_t hbm_budget_bytes, ssm
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  // the no-allocation resampler's scratch address on this device (the device is current)
+  if (noalloc_setup(c) != SSM_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SSM_ERR_HIP;
+  }
   *out = c;
   return SSM_OK;
 }

@@ -95,6 +95,8 @@ struct ResampleArgs {
                                      // (systematic and stratified; ignored otherwise)
 };
 
+// Takes this device's address of the no-allocation resampler's scratch (ctx's device current).
+int noalloc_setup(ssm_ctx* ctx);
 // Scratch bytes the resampler needs for n weights.
 size_t resample_scratch_bytes(int n);
 // Carves ResampleArgs scratch pointers out of one device block of resample_scratch_bytes(n).

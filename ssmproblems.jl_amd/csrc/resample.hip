@@ -23,17 +23,23 @@
 // compute their shared boundary count from the same stored prefix, and every cumulative
 // sequence is nondecreasing by construction, so the tiles' output ranges partition [0, n)
 // exactly.
+#include <mutex>
+
 #include "resample_dev.h"
 
 namespace ssm {
 
-size_t resample_scratch_bytes(int n) {
-  const size_t nb = (size_t)(n + kTile - 1) / kTile;
-  auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+constexpr size_t scratch_bytes_for(size_t n) {
+  const size_t nb = (n + kTile - 1) / kTile;
   const size_t ng = (nb + kFinishGroup - 1) / kFinishGroup;
-  return r(nb * kPartial * 8) + r(nb * 8) + r(nb * 8) + r((nb + 1) * 8) + r(ng * kGroupRec * 8) +
-         r(nb * kTile * 8) + r(nb * kMidPerTile * 8) + r(nb * kRootPerTile * 8);
+  return round256(nb * kPartial * 8) + round256(nb * 8) + round256(nb * 8) +
+         round256((nb + 1) * 8) + round256(ng * kGroupRec * 8) + round256(nb * kTile * 8) +
+         round256(nb * kMidPerTile * 8) + round256(nb * kRootPerTile * 8);
 }
+
+size_t resample_scratch_bytes(int n) { return scratch_bytes_for((size_t)n); }
 
 void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   const size_t nb = (size_t)(n + kTile - 1) / kTile;
@@ -249,9 +255,73 @@ __global__ void __launch_bounds__(kRsThreads)
   if (a >= 0 && a < n) atomicAdd(&off[a], 1);
 }
 
+// ---- scratch of the entry point that allocates nothing ----------------------------------
+// ssm_sample_ancestors_noalloc's scratch is this file-scope device array: one copy per GPU,
+// held until the process ends, shared by every call on that GPU.  Host code never passes
+// the array's name (a host-side stand-in) to a kernel: each device's address comes from
+// hipGetSymbolAddress, taken once per device when a context is created on it.
+constexpr int kNoAllocMaxN = 1 << 20;
+constexpr size_t kNoAllocScratchBytes = scratch_bytes_for(kNoAllocMaxN);
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_setup(ssm_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_error(ctx, SSM_ERR_ARG, "device %d beyond the %d supported", ctx->device,
+                     kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[ctx->device]) return SSM_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // device is current
+  if (e != hipSuccess || !ptr)
+    return set_error(ctx, SSM_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                     hipGetErrorString(e));
+  g_noalloc_ptr[ctx->device] = ptr;
+  return SSM_OK;
+}
+
 }  // namespace ssm
 
 using namespace ssm;
+
+extern "C" int32_t ssm_sample_ancestors_noalloc(ssm_ctx* ctx, int32_t kind, int32_t n,
+                                                const void* d_w, int32_t w_dtype,
+                                                const double* d_u, uint64_t seed,
+                                                uint32_t step, int32_t* d_ancestors) {
+  if (!ctx) return SSM_ERR_ARG;
+  if (n < 1 || !d_w || !d_ancestors)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_sample_ancestors_noalloc: bad arguments (n=%d)", n);
+  if (n > kNoAllocMaxN)
+    return set_error(ctx, SSM_ERR_ARG,
+                     "ssm_sample_ancestors_noalloc: n=%d exceeds its maximum of %d weights "
+                     "(use ssm_sample_ancestors)", n, kNoAllocMaxN);
+  if (w_dtype != SSM_F32 && w_dtype != SSM_F64)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_sample_ancestors_noalloc: bad w_dtype %d", w_dtype);
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  void* ptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    ptr = g_noalloc_ptr[ctx->device];
+  }
+  if (!ptr)
+    return set_error(ctx, SSM_ERR_STATE, "ssm_sample_ancestors_noalloc: no scratch address");
+  // every scratch word the kernels read is written earlier in the same call (tile sums,
+  // prefixes, cumulative weights), so nothing needs clearing between calls
+  ResampleArgs a;
+  a.kind = kind;
+  a.n = n;
+  a.weight_mode = (w_dtype == SSM_F64) ? kWeightsF64 : kWeightsF32;
+  a.w = d_w;
+  a.u = d_u;
+  a.seed = seed;
+  a.step_host = step;
+  a.anc = d_ancestors;
+  resample_bind_scratch(a, ptr, n);
+  return launch_resample(ctx, ctx->stream, a);
+}
 
 extern "C" int32_t ssm_sample_ancestors(ssm_ctx* ctx, int32_t kind, int32_t n, const void* d_w,
                                         int32_t w_dtype, const double* d_u, uint64_t seed,

@@ -112,6 +112,7 @@ def _declare(L):
         "ssm_memcpy_h2d": ([_vp, _vp, _vp, u64], i32),
         "ssm_memcpy_d2h": ([_vp, _vp, _vp, u64], i32),
         "ssm_sample_ancestors": ([_vp, i32, i32, _vp, i32, _vp, u64, u32, _vp], i32),
+        "ssm_sample_ancestors_noalloc": ([_vp, i32, i32, _vp, i32, _vp, u64, u32, _vp], i32),
         "ssm_sample_ancestors_host": ([_vp, i32, i32, _dp, _dp, _ip], i32),
         "ssm_offspring_to_ancestors": ([_vp, i32, _vp, _vp], i32),
         "ssm_ancestors_to_offspring": ([_vp, i32, _vp, _vp], i32),
