@@ -296,6 +296,15 @@ int32_t ssm_bf_history_record(ssm_history* h, int32_t on);
  */
 int32_t ssm_bf_history_backward_sample(ssm_history* h, const double* u, int32_t* indices,
                                        double* trajectory);
+/*
+ * The same backward pass for time-varying dynamics: time t < T is weighted with the
+ * transition of step t + 1, as csmc.jl:424 calls ancestor_weight(..., t + 1, ...).
+ * dyn holds T models, dyn[t] = calc_params(dyn, t + 1) (only Dx, A, b and Q are read; Dx
+ * must be the filter's).  SSM_ERR_NOT_PD names a step whose Q is not positive definite.
+ */
+int32_t ssm_bf_history_backward_sample_tv(ssm_history* h, const ssm_lg_model* dyn,
+                                          const double* u, int32_t* indices,
+                                          double* trajectory);
 int32_t ssm_bf_history_info(ssm_history* h, int32_t* T, int32_t* capacity);
 int32_t ssm_bf_history_ancestry(ssm_history* h, int32_t i0, int32_t count, double* paths);
 int32_t ssm_bf_history_generation(ssm_history* h, int32_t t, double* x, double* logw,
@@ -442,13 +451,27 @@ int32_t ssm_kf_batch_destroy(ssm_kf_batch* kb);
  *   every step).
  * ll_out (B) receives each log-likelihood (NaN where S lost positive definiteness; its
  * status_out entry, optional, is then SSM_ERR_NOT_PD and the call returns it).
- * 1 <= Dx <= 64, 1 <= Dy <= 32.  The inputs are staged in a device block the context keeps.
+ * 1 <= Dx <= 64, 1 <= Dy <= 32.  The inputs are staged in a device block the context keeps
+ * (for filters up to 4 x 4, copied in the caller's layout and transposed on the device to
+ * the filter-minor layout the one-filter-per-lane kernel reads).
  */
 int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                              int32_t time_varying, const double* mu0, const double* Sigma0,
                              const double* A, const double* b, const double* Q, const double* H,
                              const double* c, const double* R, const double* ys, double jitter,
                              double* ll_out, int32_t* status_out);
+/*
+ * ssm_kf_loglikelihood with its inputs already in device memory (the same layouts; a
+ * ROCArray host passes pointer(x)): no host-to-device copy of the parameters or
+ * observations, so a sampler that keeps them on the device pays only the kernels.
+ * ll_out and status_out are host arrays, as above.
+ */
+int32_t ssm_kf_loglikelihood_device(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
+                                    int32_t time_varying, const double* d_mu0,
+                                    const double* d_Sigma0, const double* d_A,
+                                    const double* d_b, const double* d_Q, const double* d_H,
+                                    const double* d_c, const double* d_R, const double* d_ys,
+                                    double jitter, double* ll_out, int32_t* status_out);
 /*
  * The reverse rule of kf_loglikelihood (GF/src/integrations/kalman_rrule.jl:22-107, with the
  * backward recursion of GF/src/algorithms/kalman_gradient.jl:103-263): the log-likelihoods

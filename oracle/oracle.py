@@ -352,18 +352,21 @@ def categorical(logw, u):
     return int(resample(MULTINOMIAL, w, np.full(lw.shape[0], float(u)))[0])
 
 
-def backward_sample(model: LG, xs, logws, u):
+def backward_sample(model: LG, xs, logws, u, dynamics=None):
     """CSMC-BS's backward pass (csmc.jl:415-444): xs[t] (N, Dx) for t = 0..K, logws[t-1] the
     normalised log-weights of generation t, u[t] the uniform of time t.  Index K ~ the last
     weights; index t ~ softmax(log w_t + log N(x*_{t+1}; A x_t + b, Q)), the time-0 weights
-    uniform.  Returns (indices (K+1,), trajectory (K+1, Dx))."""
+    uniform.  dynamics (time-varying models): dynamics[t] is the model of step t + 1, whose
+    transition weights time t (ancestor_weight(..., t + 1, ...) at csmc.jl:424); None uses
+    `model` at every time.  Returns (indices (K+1,), trajectory (K+1, Dx))."""
     K = len(xs) - 1
     idx = np.zeros(K + 1, dtype=np.int64)
     idx[K] = categorical(logws[K - 1], u[K])
     for t in range(K - 1, -1, -1):
         st = BFState(np.asarray(xs[t], dtype=np.float64),
                      None if t == 0 else logws[t - 1], typeless=(t == 0))
-        idx[t] = categorical(as_logweights(model, st, np.asarray(xs[t + 1])[idx[t + 1]]), u[t])
+        m_t = model if dynamics is None else dynamics[t]
+        idx[t] = categorical(as_logweights(m_t, st, np.asarray(xs[t + 1])[idx[t + 1]]), u[t])
     traj = np.array([np.asarray(xs[t])[idx[t]] for t in range(K + 1)])
     return idx, traj
 

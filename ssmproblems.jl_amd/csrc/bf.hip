@@ -1220,14 +1220,16 @@ namespace ssm {
 
 namespace {
 
+// md_tv / rm_tv (time-varying dynamics): Tn entries each, entry t the transition of step
+// t + 1 that weights time t (null: the filter's current homogeneous model).
 template <typename T>
-int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
+int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u,
+                      const BfModelDev<T>* md_tv = nullptr,
+                      const RefModelDev<T>* rm_tv = nullptr) {
   ssm_ctx* ctx = bf->ctx;
   hipStream_t s = ctx->stream;
   const int N = bf->N, Npad = bf->Npad, D = bf->D, Tn = hv.T;
   const size_t plane = (size_t)D * Npad;
-  const BfModelDev<T>* md = (const BfModelDev<T>*)bf->model_dev;
-  const RefModelDev<T>* rm = (const RefModelDev<T>*)bf->ref_model;
   const T* x0 = (const T*)hv.x0;
   const T* st = (const T*)hv.states;
   const int nb = (N + kTile - 1) / kTile;
@@ -1247,6 +1249,11 @@ int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
     const double* lw = (t == 0) ? nullptr : hv.logw + (size_t)(t - 1) * Npad;
     const T* xn = (t == Tn) ? nullptr : st + (size_t)t * plane;  // generation t + 1
     const int32_t* in = bf->bs_idx + t + 1;  // read only when xn is set (t < Tn)
+    // the transition x_t -> x_{t+1} (unused at t = Tn)
+    const BfModelDev<T>* md =
+        (md_tv && t < Tn) ? md_tv + t : (const BfModelDev<T>*)bf->model_dev;
+    const RefModelDev<T>* rm =
+        (rm_tv && t < Tn) ? rm_tv + t : (const RefModelDev<T>*)bf->ref_model;
     if (D == 4)
       hipLaunchKernelGGL((k_bs_weights<T, 4>), dim3(nb), dim3(kTileThreads), 0, s, N, Npad, D,
                          md, rm, x, lw, xn, in, (T*)bf->as_w, bf->as_sc, a.partials);
@@ -1266,8 +1273,44 @@ int backward_sample_t(ssm_bf* bf, const BsHistory& hv, const double* u) {
 
 }  // namespace
 
+// The per-step transition constants of time-varying dynamics (dyn[t] the parameters of step
+// t + 1, t < T), uploaded into one device block for a backward pass.
+template <typename T>
+int upload_transitions_tv(ssm_bf* bf, int Tn, const ssm_lg_model* dyn, void** out) {
+  ssm_ctx* ctx = bf->ctx;
+  const int D = bf->D;
+  std::vector<BfModelDev<T>> md(Tn);
+  std::vector<RefModelDev<T>> rm(Tn);
+  for (int t = 0; t < Tn; ++t) {
+    const ssm_lg_model& m = dyn[t];
+    if (m.Dx != D || !m.A || !m.b || !m.Q)
+      return set_error(ctx, SSM_ERR_ARG, "backward simulation: dynamics %d: bad dimensions", t);
+    std::memset(&md[t], 0, sizeof(BfModelDev<T>));
+    for (int e = 0; e < D * D; ++e) md[t].A[e] = (T)m.A[e];
+    for (int e = 0; e < D; ++e) md[t].b[e] = (T)m.b[e];
+    std::vector<double> LQ(m.Q, m.Q + (size_t)D * D);
+    if (!chol_lower(D, LQ))
+      return set_error(ctx, SSM_ERR_NOT_PD, "backward simulation: Q of step %d is not positive "
+                       "definite", t + 1);
+    const std::vector<double> LQi = tri_inverse(D, LQ);
+    double logdet = 0.0;
+    for (int q = 0; q < D; ++q) logdet += 2.0 * std::log(LQ[q + q * D]);
+    std::memset(&rm[t], 0, sizeof(RefModelDev<T>));
+    for (int e = 0; e < D * D; ++e) rm[t].LQi[e] = (T)LQi[e];
+    rm[t].cst = (T)(-0.5 * ((double)D * 1.8378770664093454835606594728112 + logdet));
+  }
+  const size_t bm = sizeof(BfModelDev<T>) * Tn, br = sizeof(RefModelDev<T>) * Tn;
+  if (hipMalloc(out, bm + br) != hipSuccess)
+    return set_error(ctx, SSM_ERR_OOM, "backward simulation: %d transitions", Tn);
+  hipStream_t s = ctx->stream;
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(*out, md.data(), bm, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync((char*)*out + bm, rm.data(), br, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));  // md and rm are host locals
+  return SSM_OK;
+}
+
 int bf_backward_sample(ssm_bf* bf, const BsHistory& hv, const double* u, int32_t* idx,
-                       double* traj) {
+                       double* traj, const ssm_lg_model* dyn) {
   ssm_ctx* ctx = bf->ctx;
   for (int t = 0; t <= hv.T; ++t)
     if (!(u[t] >= 0.0 && u[t] < 1.0))
@@ -1291,10 +1334,27 @@ int bf_backward_sample(ssm_bf* bf, const BsHistory& hv, const double* u, int32_t
       return set_error(ctx, SSM_ERR_OOM, "backward simulation buffers");
     bf->bs_cap = cap;
   }
-  rc = (bf->prec == SSM_F32) ? backward_sample_t<float>(bf, hv, u)
-                             : backward_sample_t<double>(bf, hv, u);
-  if (rc) return rc;
+  void* tv = nullptr;
+  if (dyn) {
+    rc = (bf->prec == SSM_F32) ? upload_transitions_tv<float>(bf, hv.T, dyn, &tv)
+                               : upload_transitions_tv<double>(bf, hv.T, dyn, &tv);
+    if (rc) return rc;
+  }
+  if (bf->prec == SSM_F32) {
+    const BfModelDev<float>* md = (const BfModelDev<float>*)tv;
+    rc = backward_sample_t<float>(
+        bf, hv, u, md, tv ? (const RefModelDev<float>*)(md + hv.T) : nullptr);
+  } else {
+    const BfModelDev<double>* md = (const BfModelDev<double>*)tv;
+    rc = backward_sample_t<double>(
+        bf, hv, u, md, tv ? (const RefModelDev<double>*)(md + hv.T) : nullptr);
+  }
   hipStream_t s = ctx->stream;
+  if (tv) {
+    (void)hipStreamSynchronize(s);  // the kernels read tv
+    (void)hipFree(tv);
+  }
+  if (rc) return rc;
   if (idx)
     SSM_HIP_CHECK(ctx, hipMemcpyAsync(idx, bf->bs_idx, sizeof(int32_t) * (hv.T + 1),
                                       hipMemcpyDeviceToHost, s));

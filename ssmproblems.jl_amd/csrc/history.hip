@@ -227,7 +227,18 @@ extern "C" int32_t ssm_bf_history_backward_sample(ssm_history* h, const double* 
   if (!h->bf) return set_error(ctx, SSM_ERR_STATE, "particle history: its filter was destroyed");
   if (h->T < 1) return set_error(ctx, SSM_ERR_STATE, "backward simulation: no generation stored");
   BsHistory hv{h->T, h->x0, h->states, h->logw};
-  return bf_backward_sample(h->bf, hv, u, indices, trajectory);
+  return bf_backward_sample(h->bf, hv, u, indices, trajectory, nullptr);
+}
+
+extern "C" int32_t ssm_bf_history_backward_sample_tv(ssm_history* h, const ssm_lg_model* dyn,
+                                                     const double* u, int32_t* indices,
+                                                     double* trajectory) {
+  if (!h || !u || !dyn) return SSM_ERR_ARG;
+  ssm_ctx* ctx = h->ctx;
+  if (!h->bf) return set_error(ctx, SSM_ERR_STATE, "particle history: its filter was destroyed");
+  if (h->T < 1) return set_error(ctx, SSM_ERR_STATE, "backward simulation: no generation stored");
+  BsHistory hv{h->T, h->x0, h->states, h->logw};
+  return bf_backward_sample(h->bf, hv, u, indices, trajectory, dyn);
 }
 
 extern "C" int32_t ssm_bf_history_reset(ssm_history* h) {

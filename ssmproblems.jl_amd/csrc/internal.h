@@ -149,8 +149,10 @@ struct BsHistory {
 };
 // Backward simulation with the filter's transition density: idx[t], t = T .. 0, drawn with
 // uniform u[t]; traj (T + 1) x D.  Host outputs (either may be null).
+// dyn: null (the filter's current dynamics) or T entries, dyn[t] the parameters of step
+// t + 1 (only Dx, A, b and Q are read).
 int bf_backward_sample(ssm_bf* bf, const BsHistory& hv, const double* u, int32_t* idx,
-                       double* traj);
+                       double* traj, const ssm_lg_model* dyn);
 
 // ---- batched Kalman filters for the server (kf.hip) ---------------------------------
 // Device arrays of one group of independent filters with per-filter horizons Tf; the

@@ -25,7 +25,6 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
-#include <thread>
 #include <vector>
 
 #include "device_common.h"
@@ -1610,6 +1609,29 @@ extern "C" int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* 
   return rc;
 }
 
+// The filter-minor copy of a caller's array: src is B rows of P doubles (one row per
+// filter), dst is P rows of B.  32 x 32 tiles through LDS (the +1 column avoids bank
+// conflicts), so both the reads and the writes are coalesced runs.
+__global__ void __launch_bounds__(256) k_transpose_fm(const double* __restrict__ src,
+                                                      double* __restrict__ dst, int B,
+                                                      int64_t P) {
+  __shared__ double tile[32][33];
+  const int64_t c0 = (int64_t)blockIdx.x * 32;
+  const int f0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const int f = f0 + j;
+    const int64_t c = c0 + tx;
+    if (f < B && c < P) tile[j][tx] = src[(int64_t)f * P + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t c = c0 + j;
+    const int f = f0 + tx;
+    if (f < B && c < P) dst[c * B + f] = tile[tx][j];
+  }
+}
+
 // kf_loglikelihood (GF/src/integrations/logdensity.jl:149-161) for B filters of one shape:
 // the inputs are copied into one device block (grown on the context as needed) and run by
 // the size class's persistent kernel in the TV form.
@@ -1620,7 +1642,7 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                     const double* A, const double* b, const double* Q, const double* H,
                     const double* c, const double* R, const double* ys, double jitter,
                     double* ll_out, int32_t* status_out, double* const* gout,
-                    bool srkf = false) {
+                    bool srkf = false, bool inputs_on_device = false) {
   if (!ctx) return SSM_ERR_ARG;
   if (B < 1 || T < 1 || !mu0 || !Sigma0 || !A || !b || !Q || !H || !c || !R || !ys || !ll_out)
     return set_error(ctx, SSM_ERR_ARG, "ssm_kf_loglikelihood: bad arguments");
@@ -1648,8 +1670,13 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
       (gout && !srkf) ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
   const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B +
                        n_grad + n_cache;
-  const size_t bytes = 8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256;
-  int rc = ensure_device(ctx, &ctx->kfll, &ctx->kfll_bytes, bytes);
+  const size_t bytes =
+      (8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256 + 255) / 256 * 256;
+  // the filter-minor kernels' inputs arrive in the caller's layout behind the main block
+  const size_t n_raw = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y;
+  const bool fm_in = srkf || gout || (Dx <= kTinyD && Dy <= kTinyD);
+  const size_t raw_bytes = (fm_in && !inputs_on_device) ? 8 * n_raw : 0;
+  int rc = ensure_device(ctx, &ctx->kfll, &ctx->kfll_bytes, bytes + raw_bytes);
   if (rc) return rc;
   double* d = (double*)ctx->kfll;
   double* d_mu0 = d;
@@ -1679,52 +1706,46 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
     dims[2 * (size_t)B + f] = f;
   }
   hipStream_t s = ctx->stream;
-  // The one-filter-per-lane kernels read their inputs filter-minor: transposed here, on the
-  // host, in blocks of 64 filters (each block's reads stay within 64 source rows).
+  // Any failure after the first asynchronous copy waits for the stream before returning, so
+  // no copy is left running into or out of host memory this call owns or was handed.
+  struct StreamGuard {
+    hipStream_t s;
+    bool armed = true;
+    ~StreamGuard() {
+      if (armed) (void)hipStreamSynchronize(s);
+    }
+  } guard{s};
+  // The one-filter-per-lane kernels read their inputs filter-minor: the caller's layout is
+  // uploaded unchanged behind the main block and transposed on the device.
   const bool fm = srkf || gout || (Dx <= kTinyD && Dy <= kTinyD);
   const int64_t nbf = time_varying ? T : 1;  // parameter blocks per filter
-  std::vector<double> stage;
-  auto staged = [&](const double* src, int64_t per_filter) -> const double* {
-    if (!fm) return src;
-    const size_t at = stage.size();
-    stage.resize(at + (size_t)B * per_filter);
-    double* dst = stage.data() + at;
-    // threads take disjoint runs of 64-filter blocks, so their writes never share a line
-    const int nblk = (B + 63) / 64;
-    const int nth = std::max(1, std::min({8, nblk, (int)std::thread::hardware_concurrency()}));
-    auto work = [&](int w) {
-      for (int k = w; k < nblk; k += nth) {
-        const int f0 = 64 * k, f1 = std::min(B, f0 + 64);
-        for (int64_t r = 0; r < per_filter; ++r)
-          for (int f = f0; f < f1; ++f) dst[r * B + f] = src[(int64_t)f * per_filter + r];
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
-    work(0);
-    for (std::thread& th : pool) th.join();
-    return dst;
-  };
-  if (fm) stage.reserve(n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y);
-  const double* h_mu0 = staged(mu0, Dx);
-  const double* h_S0 = staged(Sigma0, nxx);
-  const double* h_A = staged(A, nbf * nxx);
-  const double* h_b = staged(b, nbf * Dx);
-  const double* h_Q = staged(Q, nbf * nxx);
-  const double* h_H = staged(H, nbf * nyx);
-  const double* h_c = staged(c, nbf * Dy);
-  const double* h_R = staged(R, nbf * nyy);
-  const double* h_y = staged(ys, (int64_t)T * Dy);
+  double* raw = (double*)((char*)ctx->kfll + bytes);
   struct Up {
-    void* dst;
-    const void* src;
-    size_t bytes;
-  } ups[] = {{d_mu0, h_mu0, 8 * n_mu0}, {d_S0, h_S0, 8 * n_S0}, {d_A, h_A, 8 * n_A},
-             {d_b, h_b, 8 * n_b},       {d_Q, h_Q, 8 * n_A},    {d_H, h_H, 8 * n_H},
-             {d_c, h_c, 8 * n_c},       {d_R, h_R, 8 * n_R},    {d_y, h_y, 8 * n_y},
-             {d_yoff, yoff.data(), 8 * (size_t)B}, {d_Dx, dims.data(), 12 * (size_t)B}};
-  for (const Up& u : ups)
-    SSM_HIP_CHECK(ctx, hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, s));
+    double* dst;
+    const double* src;
+    int64_t per_filter;  // doubles per filter (the row length of the caller's layout)
+  } ups[] = {{d_mu0, mu0, Dx},          {d_S0, Sigma0, nxx},      {d_A, A, nbf * nxx},
+             {d_b, b, nbf * Dx},         {d_Q, Q, nbf * nxx},      {d_H, H, nbf * nyx},
+             {d_c, c, nbf * Dy},         {d_R, R, nbf * nyy},      {d_y, ys, (int64_t)T * Dy}};
+  const hipMemcpyKind in_kind = inputs_on_device ? hipMemcpyDeviceToDevice
+                                                 : hipMemcpyHostToDevice;
+  for (const Up& u : ups) {
+    const size_t nbytes = 8 * (size_t)B * (size_t)u.per_filter;
+    if (!fm) {
+      SSM_HIP_CHECK(ctx, hipMemcpyAsync(u.dst, u.src, nbytes, in_kind, s));
+      continue;
+    }
+    const double* src = u.src;  // device inputs are transposed where they lie
+    if (!inputs_on_device) {
+      SSM_HIP_CHECK(ctx, hipMemcpyAsync(raw, u.src, nbytes, hipMemcpyHostToDevice, s));
+      src = raw;
+      raw += (size_t)B * u.per_filter;
+    }
+    const dim3 g((unsigned)((u.per_filter + 31) / 32), (unsigned)((B + 31) / 32));
+    hipLaunchKernelGGL(k_transpose_fm, g, dim3(256), 0, s, src, u.dst, B, u.per_filter);
+  }
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(d_yoff, yoff.data(), 8 * (size_t)B, hipMemcpyHostToDevice, s));
+  SSM_HIP_CHECK(ctx, hipMemcpyAsync(d_Dx, dims.data(), 12 * (size_t)B, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, d_counter, 16);
   KfBatchParams p;
   p.T = T;
@@ -1785,9 +1806,12 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
     hipLaunchKernelGGL(fn, dim3((B + kTinyThreads - 1) / kTinyThreads), dim3(kTinyThreads), 0,
                        s, p, g);
     SSM_HIP_CHECK(ctx, hipGetLastError());
-  } else if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane
-    hipLaunchKernelGGL(k_kf_tiny<kTinyD>, dim3((B + kTinyThreads - 1) / kTinyThreads),
-                       dim3(kTinyThreads), 0, s, p);
+  } else if (Dx <= kTinyD && Dy <= kTinyD) {  // one filter per lane, padded to max(Dx, Dy)
+    const int n = std::max(Dx, Dy);
+    auto fn = n == 1 ? k_kf_tiny<1> : n == 2 ? k_kf_tiny<2> : n == 3 ? k_kf_tiny<3>
+                                                                  : k_kf_tiny<kTinyD>;
+    hipLaunchKernelGGL(fn, dim3((B + kTinyThreads - 1) / kTinyThreads), dim3(kTinyThreads), 0,
+                       s, p);
     SSM_HIP_CHECK(ctx, hipGetLastError());
   } else {
     rc = launch_class_nw(ctx, s, p, cls, class_nw(cls));
@@ -1803,6 +1827,7 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
         SSM_HIP_CHECK(ctx, hipMemcpyAsync(gout[k], g_dev[k], 8 * g_len[k],
                                           hipMemcpyDeviceToHost, s));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(s));
+  guard.armed = false;
   float ms = 0.f;
   SSM_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->kfll_ev[0], ctx->kfll_ev[1]));
   ctx->kfll_ms = ms;
@@ -1822,6 +1847,15 @@ extern "C" int32_t ssm_kf_loglikelihood(ssm_ctx* ctx, int32_t B, int32_t Dx, int
                                         double* ll_out, int32_t* status_out) {
   return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
                   ll_out, status_out, nullptr);
+}
+
+extern "C" int32_t ssm_kf_loglikelihood_device(
+    ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T, int32_t time_varying,
+    const double* d_mu0, const double* d_Sigma0, const double* d_A, const double* d_b,
+    const double* d_Q, const double* d_H, const double* d_c, const double* d_R,
+    const double* d_ys, double jitter, double* ll_out, int32_t* status_out) {
+  return kfll_run(ctx, B, Dx, Dy, T, time_varying, d_mu0, d_Sigma0, d_A, d_b, d_Q, d_H, d_c,
+                  d_R, d_ys, jitter, ll_out, status_out, nullptr, false, true);
 }
 
 extern "C" int32_t ssm_kf_loglikelihood_grad(

@@ -13,7 +13,7 @@ from .resamplers import (Systematic, Stratified, Multinomial, ESSResampler, samp
                          offspring_to_ancestors, ancestors_to_offspring)
 from .kalman import (KalmanFilter, KF, KFBatch, MvNormalState, pack_lg, kf_loglikelihood,
                      kf_loglikelihood_batch, kf_loglikelihood_grad_batch, SRKalmanFilter,
-                     SRKF, srkf_filter_batch)
+                     SRKF, srkf_filter_batch, DeviceKFInputs)
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, ParticleFilter, PF, DeviceParticles, initialise,
                       step, run, filter, rng_to_seed, last_timing)

@@ -137,6 +137,8 @@ def _declare(L):
         "ssm_kf_batch_destroy": ([_vp], i32),
         "ssm_kf_loglikelihood": ([_vp, i32, i32, i32, i32, i32, _dp, _dp, _dp, _dp, _dp, _dp,
                                   _dp, _dp, _dp, dbl, _dp, _ip], i32),
+        "ssm_kf_loglikelihood_device": ([_vp, i32, i32, i32, i32, i32] + [_vp] * 9 +
+                                        [dbl, _dp, _ip], i32),
         "ssm_kf_loglikelihood_last_timing": ([_vp, _dp], i32),
         "ssm_bf_set_proposal": ([_vp, _dp, _dp, _dp, _dp], i32),
         "ssm_srkf_filter": ([_vp, i32, i32, i32, i32, i32] + [_dp] * 9 + [dbl, _dp, _ip, _dp, _dp],
@@ -161,6 +163,7 @@ def _declare(L):
         "ssm_bf_history_reset": ([_vp], i32),
         "ssm_bf_history_record": ([_vp, i32], i32),
         "ssm_bf_history_backward_sample": ([_vp, _dp, _ip, _dp], i32),
+        "ssm_bf_history_backward_sample_tv": ([_vp, C.POINTER(LGModelC), _dp, _ip, _dp], i32),
         "ssm_bf_history_info": ([_vp, _ip, _ip], i32),
         "ssm_bf_history_ancestry": ([_vp, i32, i32, _dp], i32),
         "ssm_bf_history_generation": ([_vp, i32, _dp, _dp, _ip], i32),

@@ -148,7 +148,10 @@ def csmc_sample(rng, model: StateSpaceModel, csmc: ConditionalSMC, observations,
     if isinstance(csmc.refreshment, BackwardSimulation):
         # the last time's draw first, then t = K-1 .. 0, as csmc.jl:417-444 consumes rng
         u = rng.random(K + 1)[::-1]
-        _, path = ws.history.backward_sample(u)
+        # time t is weighted with the dynamics of step t + 1 (csmc.jl:424)
+        dyn = None if model.is_homogeneous else [PackedLG.from_model(model, t + 1)
+                                                 for t in range(K)]
+        _, path = ws.history.backward_sample(u, dyn)
         return ReferenceTrajectory(path[0], path[1:]), ll
     idx = st.sample_index(float(rng.random()))
     path = ws.history.get_ancestry(idx)

@@ -34,15 +34,25 @@ class DenseParticleContainer:
         generation on the device; do not push those steps by hand as well."""
         self.ctx.check(self.ctx.lib.ssm_bf_history_record(self.ptr, int(bool(on))))
 
-    def backward_sample(self, u):
+    def backward_sample(self, u, dynamics=None):
         """Backward simulation over the stored generations with uniforms u[0..T] (u[t] for
-        time t): (indices (T + 1,), trajectory (T + 1, Dx))."""
+        time t): (indices (T + 1,), trajectory (T + 1, Dx)).  dynamics: None (the filter's
+        homogeneous model) or T PackedLG models, dynamics[t] the parameters of step t + 1
+        (csmc.jl:424 weights time t with the transition of step t + 1)."""
         T = len(self)
         u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(T + 1))
         idx = np.zeros(T + 1, dtype=np.int32)
         traj = np.zeros((T + 1, self.Dx))
-        self.ctx.check(self.ctx.lib.ssm_bf_history_backward_sample(
-            self.ptr, u.ctypes.data_as(_lib._dp), idx.ctypes.data_as(_lib._ip),
+        if dynamics is None:
+            self.ctx.check(self.ctx.lib.ssm_bf_history_backward_sample(
+                self.ptr, u.ctypes.data_as(_lib._dp), idx.ctypes.data_as(_lib._ip),
+                traj.ctypes.data_as(_lib._dp)))
+            return idx, traj
+        if len(dynamics) != T:
+            raise ValueError(f"{len(dynamics)} dynamics for {T} stored steps")
+        arr = (_lib.LGModelC * T)(*[d.struct for d in dynamics])
+        self.ctx.check(self.ctx.lib.ssm_bf_history_backward_sample_tv(
+            self.ptr, arr, u.ctypes.data_as(_lib._dp), idx.ctypes.data_as(_lib._ip),
             traj.ctypes.data_as(_lib._dp)))
         return idx, traj
 

@@ -187,6 +187,34 @@ def kf_loglikelihood_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=N
     return (ll, st) if return_status else ll
 
 
+class DeviceKFInputs:
+    """The inputs of kf_loglikelihood_batch uploaded once to device memory (the same flat
+    layouts), for a sampler that keeps them resident: ``loglikelihood()`` calls
+    ssm_kf_loglikelihood_device, which copies nothing from the host."""
+
+    def __init__(self, mu0, Sigma0, A, b, Q, H, c, R, ys, ctx=None):
+        self.ctx = ctx or default_context()
+        (self.B, self.Dx, self.Dy, self.T, self.tv, _, mu0c, S0, flat,
+         yv) = _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys)
+        arrs = [mu0c, S0] + [flat[k] for k in ("A", "b", "Q", "H", "c", "R")] + [yv]
+        self.bufs = []
+        for a in arrs:
+            d = _lib.DeviceBuffer(self.ctx, max(8, a.nbytes))
+            d.upload(a)
+            self.bufs.append(d)
+
+    def loglikelihood(self, jitter=None, return_status=False):
+        ll = np.zeros(self.B)
+        st = np.zeros(self.B, dtype=np.int32)
+        rc = self.ctx.lib.ssm_kf_loglikelihood_device(
+            self.ctx.ptr, self.B, self.Dx, self.Dy, self.T, int(self.tv),
+            *(d.ptr for d in self.bufs), float(jitter or 0.0), ll.ctypes.data_as(_lib._dp),
+            st.ctypes.data_as(_lib._ip))
+        if rc and not return_status:
+            self.ctx.check(rc)
+        return (ll, st) if return_status else ll
+
+
 def _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys):
     """Shapes, the time-varying flag and the flat column-major inputs of the C-ABI."""
     mu0 = np.asarray(mu0, dtype=np.float64)

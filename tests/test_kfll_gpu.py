@@ -89,3 +89,19 @@ def test_not_positive_definite_reports_status(ctx, Dx, Dy):
     assert np.all(st[np.arange(8) != 3] == 0) and np.all(np.isfinite(np.delete(ll, 3)))
     with pytest.raises(ssmhip.PosDefException):
         ssmhip.kf_loglikelihood_batch(*arrs, ys, ctx=ctx)
+
+
+@pytest.mark.parametrize("Dx,Dy,T,B,tv", [(2, 2, 30, 1000, True), (3, 1, 10, 70, False),
+                                          (5, 3, 9, 33, True)])
+def test_device_resident_inputs_equal_host_inputs(ctx, Dx, Dy, T, B, tv):
+    # ssm_kf_loglikelihood_device on inputs uploaded once equals the host-array entry point
+    # bit for bit (the same kernels on the same filter-minor or caller-layout data), call
+    # after call
+    rng = np.random.default_rng(5 * Dx + T)
+    pars, ys, arrs = _batch(rng, B, Dx, Dy, T)
+    if not tv:  # one parameter set per filter: drop the step axis
+        arrs = arrs[:2] + [a[:, 0] for a in arrs[2:]]
+    want = ssmhip.kf_loglikelihood_batch(*arrs, ys, ctx=ctx)
+    dev = ssmhip.DeviceKFInputs(*arrs, ys, ctx=ctx)
+    for _ in range(2):
+        assert np.array_equal(dev.loglikelihood(), want)

@@ -160,3 +160,29 @@ def test_backward_sample_exact_distribution():
             np.testing.assert_allclose(freq, w0, atol=1.01 / len(grid))
             for idx, traj in got[:5]:
                 np.testing.assert_array_equal(traj, [xs[t][idx[t]] for t in range(3)])
+
+
+def test_backward_sample_weights_time_t_with_the_dynamics_of_step_t_plus_1():
+    # csmc.jl:424 weights time t with ancestor_weight(..., t + 1, ...).  Two particles at
+    # time 0 (x = 0 and x = 10), both at 10 afterwards.  Step 1's dynamics (A = 1, tiny Q)
+    # only reach x* = 10 from x = 10; step 2's (A = 0) treat both particles alike.  With the
+    # right step the time-0 draw is particle 1 whatever the uniform; with step 2's dynamics
+    # (or the homogeneous model below) u = 0.01 would pick particle 0.
+    def lg(a):
+        return oracle.LG([0.0], [[1.0]], [[a]], [0.0], [[1e-2]], [[1.0]], [0.0], [[1.0]])
+    xs = [np.array([[0.0], [10.0]]), np.array([[10.0], [10.0]]), np.array([[10.0], [10.0]])]
+    lws = [np.log([0.5, 0.5]), np.log([0.5, 0.5])]
+    u = np.array([0.01, 0.3, 0.7])
+    idx, _ = oracle.backward_sample(lg(0.0), xs, lws, u, dynamics=[lg(1.0), lg(0.0)])
+    assert idx[0] == 1
+    idx_h, _ = oracle.backward_sample(lg(0.0), xs, lws, u)
+    assert idx_h[0] == 0
+    # constant dynamics reduce to the homogeneous pass
+    rng = np.random.default_rng(3)
+    m = lg(0.9)
+    xs = [rng.standard_normal((50, 1)) for _ in range(5)]
+    lws = [np.log(rng.dirichlet(np.ones(50))) for _ in range(4)]
+    u = rng.random(5)
+    a, ta = oracle.backward_sample(m, xs, lws, u)
+    b, tb = oracle.backward_sample(m, xs, lws, u, dynamics=[m] * 4)
+    assert np.array_equal(a, b) and np.array_equal(ta, tb)
