@@ -49,6 +49,11 @@ extern "C" {
 #define SSM_CSMC_PLAIN 0     /* NoRefreshment: the reference keeps ancestor 0 */
 #define SSM_CSMC_ANCESTOR 1  /* AncestorSampling: its ancestor is drawn each step */
 
+/* Auxiliary particle filter look-ahead (GF/src/algorithms/particles.jl:247-393). */
+#define SSM_APF_NONE 0 /* the bootstrap filter                                           */
+#define SSM_APF_MEAN 1 /* RepresentativeStateLookAhead(MeanPredictive()); for Gaussian     */
+                       /* transitions also ModePredictive (the mode is the mean)           */
+
 /* ---- arithmetic types ------------------------------------------------------------- */
 #define SSM_F32 0
 #define SSM_F64 1
@@ -187,6 +192,21 @@ int32_t ssm_bf_last_timing(ssm_bf* bf, double* ms_total, double* ms_per_step);
  */
 int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys, double threshold,
                        int32_t resampler, int32_t reps, double* ms_kernels, double* ll_total);
+
+/*
+ * AuxiliaryParticleFilter(BF, MeanPredictive()) (GF/src/algorithms/particles.jl:316-344)
+ * on this filter: mode SSM_APF_MEAN makes every later step (ssm_bf_step, ssm_bf_run) first
+ * score each particle with log eta_i = log N(y_t; H (A x_i + b) + c, R), the observation
+ * density at its transition mean (compute_logeta, :282-298), then resample with the
+ * AuxiliaryResampler (GF/src/resamplers.jl:92-143): the ESS of softmax(log w + log eta)
+ * decides, ancestors are drawn from those weights, a resampled particle starts from
+ * -log eta of its ancestor, and the step's increment carries the first-stage correction
+ * LSE(log w + log eta) - LSE(log w) of construct_new_state.  Steps that do not resample are
+ * the bootstrap filter's.  ssm_bf_export's w is the look-ahead weights the last resample
+ * used.  SSM_APF_NONE returns to the bootstrap filter.  Not with a reference trajectory or
+ * a proposal (SSM_ERR_ARG); ssm_bf_profile is refused while it is on.
+ */
+int32_t ssm_bf_set_auxiliary(ssm_bf* bf, int32_t mode);
 
 /* ---- conditional SMC on the bootstrap filter (GF/src/algorithms/csmc.jl) ---------- */
 /*

@@ -522,6 +522,125 @@ int orc_bf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* 
 }
 
 /*
+ * One auxiliary particle filter step with the transition mean as representative state
+ * (AuxiliaryParticleFilter with MeanPredictive, GF/src/algorithms/particles.jl:248-305 and
+ * step at :316-344; AuxiliaryResampler, GF/src/resamplers.jl:92-143):
+ *   log eta_i = logpdf(MvNormal(H (A x_i + b) + c, R), y)      (compute_logeta)
+ *   w~ = softmax(logw + log eta); ESS = 1 / sum w~^2; resample iff threshold N >= ESS
+ *   resample: a ~ w~, x_i <- x_{a_i}, logw_i <- -log eta_{a_i} (resample_ancestor), and
+ *     ll_baseline = -((LSE(log eta + logw) - LSE(logw)) + (LSE(-log eta[a]) - log N))
+ *     (construct_new_state, LSE(logw) of the lazy TypelessZero weights being log N);
+ *   otherwise preserve_sample: particles and baseline kept.
+ * Then predict adds LSE(logw) to the baseline, update adds the observation log-density and
+ * marginalise! returns LSE(logw) - baseline (particles.jl:52-90, containers.jl:160-173).
+ * u, z, w_override, w_out (the auxiliary weights) as in orc_bf_step.
+ */
+int orc_apf_step(const orc_lg_model* m, int64_t N, double* x, double* logw, int* typeless,
+                 double threshold, int kind, const double* u, const double* z, const double* y,
+                 const double* w_override, int32_t* anc, double* ll_inc, int* did_resample,
+                 double* w_out) {
+  const int D = m->Dx, Dy = m->Dy;
+  double* eta = (double*)malloc(sizeof(double) * (size_t)N);
+  double* aux = (double*)malloc(sizeof(double) * (size_t)N);
+  double* w = (double*)malloc(sizeof(double) * (size_t)N);
+  double* xn = (double*)malloc(sizeof(double) * (size_t)(N * D));
+  double* LQ = (double*)malloc(sizeof(double) * (size_t)(D * D));
+  double* LR = (double*)malloc(sizeof(double) * (size_t)(Dy * Dy));
+  double* xm = (double*)malloc(sizeof(double) * (size_t)D);
+  double* r = (double*)malloc(sizeof(double) * (size_t)Dy);
+  double* work = (double*)malloc(sizeof(double) * (size_t)Dy);
+  int rc = ORC_OK;
+  memcpy(LQ, m->Q, sizeof(double) * (size_t)(D * D));
+  memcpy(LR, m->R, sizeof(double) * (size_t)(Dy * Dy));
+  if (chol_lower(D, LQ) || chol_lower(Dy, LR)) {
+    rc = ORC_ERR_NOT_PD;
+    goto done;
+  }
+  /* look-ahead scores at the transition means */
+  for (int64_t i = 0; i < N; ++i) {
+    for (int rr = 0; rr < D; ++rr) {
+      double s = m->b[rr];
+      for (int k = 0; k < D; ++k) s += m->A[rr + k * D] * x[i * D + k];
+      xm[rr] = s;
+    }
+    for (int q = 0; q < Dy; ++q) {
+      double s = m->c[q];
+      for (int k = 0; k < D; ++k) s += m->H[q + k * Dy] * xm[k];
+      r[q] = y[q] - s;
+    }
+    eta[i] = mvn_logpdf_chol(Dy, LR, r, work);
+    aux[i] = (*typeless ? 0.0 : logw[i]) + eta[i];
+  }
+  softmax(N, aux, w);
+  {
+    double s2 = 0.0;
+    for (int64_t i = 0; i < N; ++i) s2 += w[i] * w[i];
+    *did_resample = (threshold * (double)N >= 1.0 / s2) ? 1 : 0;
+  }
+  if (w_out) memcpy(w_out, w, sizeof(double) * (size_t)N);
+  double baseline;
+  if (*did_resample) {
+    const double lse1 = orc_logsumexp(N, aux);
+    const double lse2 = *typeless ? log((double)N) : orc_logsumexp(N, logw);
+    rc = orc_resample(kind, N, w_override ? w_override : w, u, anc);
+    if (rc) goto done;
+    for (int64_t i = 0; i < N; ++i) {
+      const int64_t a = anc[i];
+      for (int d = 0; d < D; ++d) xn[i * D + d] = x[a * D + d];
+      logw[i] = -eta[a];
+    }
+    memcpy(x, xn, sizeof(double) * (size_t)(N * D));
+    *typeless = 0;
+    const double lse3 = orc_logsumexp(N, logw);
+    const double beta = -((lse1 - lse2) + (lse3 - log((double)N)));
+    baseline = lse3 + beta; /* predict: LSE(logw) + ll_baseline */
+  } else {
+    for (int64_t i = 0; i < N; ++i) anc[i] = (int32_t)i;
+    if (*typeless) {
+      baseline = log((double)N);
+      for (int64_t i = 0; i < N; ++i) logw[i] = 0.0;
+      *typeless = 0;
+    } else {
+      baseline = orc_logsumexp(N, logw);
+    }
+  }
+  /* predict and update as the bootstrap filter */
+  for (int64_t i = 0; i < N; ++i)
+    for (int rr = 0; rr < D; ++rr) {
+      double s = m->b[rr];
+      for (int k = 0; k < D; ++k) s += m->A[rr + k * D] * x[i * D + k];
+      double e = 0.0;
+      for (int k = 0; k <= rr; ++k) e += LQ[rr + k * D] * z[i * D + k];
+      xn[i * D + rr] = s + e;
+    }
+  memcpy(x, xn, sizeof(double) * (size_t)(N * D));
+  for (int64_t i = 0; i < N; ++i) {
+    for (int q = 0; q < Dy; ++q) {
+      double s = m->c[q];
+      for (int k = 0; k < D; ++k) s += m->H[q + k * Dy] * x[i * D + k];
+      r[q] = y[q] - s;
+    }
+    logw[i] += mvn_logpdf_chol(Dy, LR, r, work);
+  }
+  {
+    const double lse = orc_logsumexp(N, logw);
+    *ll_inc = lse - baseline;
+    for (int64_t i = 0; i < N; ++i) logw[i] -= lse;
+  }
+done:
+  free(eta);
+  free(aux);
+  free(w);
+  free(xn);
+  free(LQ);
+  free(LR);
+  free(xm);
+  free(r);
+  free(work);
+  return rc;
+}
+
+/*
  * One conditional-SMC step (csmc.jl:239-306): the BootstrapFilter step above with the
  * reference pinned to particle 1 (index 0).  When the step resamples, particle 0's
  * ancestor is ref_anc (0 for NoRefreshment's _set_ref_index!, resamplers.jl:51-65, or

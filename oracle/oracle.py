@@ -123,6 +123,10 @@ def _declare(L):
                               C.c_double, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
                               C.POINTER(C.c_int), _dp]
     L.orc_bf_step.restype = C.c_int
+    L.orc_apf_step.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp, C.POINTER(C.c_int),
+                               C.c_double, C.c_int, _dp, _dp, _dp, _dp, _ip, _dp,
+                               C.POINTER(C.c_int), _dp]
+    L.orc_apf_step.restype = C.c_int
     L.orc_csmc_step.argtypes = [C.POINTER(LGModel), C.c_int64, _dp, _dp, C.POINTER(C.c_int),
                                 C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int32, _ip,
                                 _dp, C.POINTER(C.c_int), _dp]
@@ -301,6 +305,30 @@ def bf_step(model: LG, state: BFState, y, threshold, kind, u, z, w_override=None
                            C.byref(ll), C.byref(did), _d(w_out))
     if rc:
         raise ValueError(f"orc_bf_step rc={rc}")
+    state.typeless = bool(typ.value)
+    return anc, ll.value, bool(did.value), w_out
+
+
+def apf_step(model: LG, state: BFState, y, threshold, kind, u, z, w_override=None):
+    """One auxiliary-particle-filter step (MeanPredictive look-ahead) in place
+    (orc_apf_step: particles.jl:248-344, resamplers.jl:92-143); returns
+    (anc, ll_inc, did_resample, auxiliary weights used)."""
+    N = state.x.shape[0]
+    anc = np.zeros(N, dtype=np.int32)
+    ll = C.c_double()
+    did = C.c_int()
+    typ = C.c_int(int(state.typeless))
+    w_out = np.zeros(N)
+    m, keep = model.cstruct()
+    u = _f64(np.atleast_1d(u))
+    z = _f64(z)
+    y = _f64(np.atleast_1d(y))
+    wo = None if w_override is None else _f64(w_override)
+    rc = lib().orc_apf_step(C.byref(m), N, _d(state.x), _d(state.logw), C.byref(typ),
+                            float(threshold), kind, _d(u), _d(z), _d(y), _d(wo), _i(anc),
+                            C.byref(ll), C.byref(did), _d(w_out))
+    if rc:
+        raise ValueError(f"orc_apf_step rc={rc}")
     state.typeless = bool(typ.value)
     return anc, ll.value, bool(did.value), w_out
 

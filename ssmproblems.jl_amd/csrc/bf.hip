@@ -88,6 +88,62 @@ __global__ void __launch_bounds__(kTileThreads)
   tile_partial<T>(nl, lv, valid, true, partials, gridDim.x, blockIdx.x);
 }
 
+// Auxiliary particle filter, first stage (GF/src/algorithms/particles.jl:316-337 with
+// AuxiliaryResampler, GF/src/resamplers.jl:104-111): the look-ahead weights
+// log w_i + log eta_i of the current particles (log w the normalised log-weights, 0 for the
+// lazy uniform ones; eta the MeanPredictive score apf_log_eta at the step's observation),
+// stored for the resampler, with their tile partials for the fold (thread 0 of tile 0 also
+// hands the step to the fold's scalars).
+template <typename T, int DT, int DYT>
+__global__ void __launch_bounds__(kTileThreads)
+    k_apf_weights(StepParams<T> p, T* aux_w, BfScalars* aux_sc, double* aux_partials) {
+  constexpr int MD = DT ? DT : kMaxD;
+  constexpr int MDY = DYT ? DYT : kMaxDy;
+  const int D = DT ? DT : p.D;
+  const int Dy = DYT ? DYT : p.Dy;
+  const BfScalars* sc = p.sc;
+  const int t = sc->step;
+  const bool typeless = sc->typeless;
+  const T lse = (T)sc->lse;
+  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
+  const BfModelDev<T>* msrc = p.model;
+  if constexpr (DT == 0) {
+    __shared__ BfModelDev<T> lds_model;
+    msrc = &model_to_lds<T>(p.model, &lds_model);
+  }
+  const ModelView<T, MD, MDY, (DT > 0)> mv(*msrc, D, Dy);
+  T y[MDY];
+#pragma unroll
+  for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    aux_sc->step = t;
+    aux_sc->typeless = 0;
+  }
+  const int tile0 = blockIdx.x * kTile;
+  T xv[kTileItems][MD], lw[kTileItems];
+  bool valid[kTileItems];
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    valid[k] = i < p.N;
+    const int ic = valid[k] ? i : p.N - 1;
+#pragma unroll
+    for (int d = 0; d < MD; ++d) xv[k][d] = (d < D) ? p.xin[(size_t)d * p.Npad + ic] : (T)0;
+    lw[k] = p.logw[ic];
+  }
+  T nl[kTileItems], lv[kTileItems];
+#pragma unroll
+  for (int k = 0; k < kTileItems; ++k) {
+    const int i = tile0 + k * kTileThreads + threadIdx.x;
+    const T pre = typeless ? (T)0 : lw[k] - lse;
+    const T w = pre + apf_log_eta<T, MD, MDY>(mv, D, Dy, xv[k], y);
+    if (valid[k]) aux_w[i] = w;
+    lv[k] = valid[k] ? w : (T)0;
+    nl[k] = (T)0;
+  }
+  tile_partial<T>(nl, lv, valid, true, aux_partials, gridDim.x, blockIdx.x);
+}
+
 // CSMC-BS: backward-simulation log-weights of stored generation t (csmc.jl:418-437 with
 // ancestor_weight, ancestor_sampling.jl:125-130): each particle's stored normalised
 // log-weight (FP64; none for the time-0 particles, whose weights are the lazy TypelessZero)
@@ -217,6 +273,9 @@ struct ssm_bf {
   // device constants (PropDev<T>); empty F: the bootstrap filter's latent proposal
   std::vector<double> prop_F, prop_g0, prop_K, prop_P;
   void* prop_dev = nullptr;
+  // auxiliary particle filter (MeanPredictive look-ahead): its weights, tile scratch and
+  // fold share the ancestor-sampling buffers (as_w, as_scratch, as_sc)
+  bool apf = false;
 };
 
 namespace {
@@ -275,6 +334,7 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   p.ref_anc = (bf->ref_mode == SSM_CSMC_ANCESTOR) ? bf->as_idx : nullptr;
   p.anc_w = bf->anc;
   p.prop = (const PropDev<T>*)bf->prop_dev;
+  p.aux = bf->apf ? bf->as_sc : nullptr;
   const dim3 g(p.nblocks), b(kStepThreads);
   if (!bf->prop_F.empty()) {  // guided filter (no reference pinning)
     if (noise) {
@@ -332,17 +392,51 @@ int enqueue_resample(ssm_bf* bf, hipStream_t s, double threshold, int kind, cons
   a.kind = kind;
   a.n = bf->N;
   a.weight_mode = (bf->prec == SSM_F32) ? kWeightsLogF32 : kWeightsLogF64;
-  a.w = bf->logw;
+  // the auxiliary filter resamples on its look-ahead weights and their fold
+  a.w = bf->apf ? bf->as_w : bf->logw;
   a.u = u_dev;
   a.seed = bf->seed;
   a.seed_dev = bf->seed_dev;
-  a.sc = bf->sc;
+  a.sc = bf->apf ? bf->as_sc : bf->sc;
   a.threshold_n = threshold * (double)bf->N;
   a.anc = bf->anc;
   a.w_debug64 = debug_weights ? bf->w_dbg : nullptr;
   a.counts_out = counts_out;
-  resample_bind_scratch(a, bf->rs_scratch, bf->N);
+  resample_bind_scratch(a, bf->apf ? bf->as_scratch : bf->rs_scratch, bf->N);
   return launch_resample(bf->ctx, s, a);
+}
+
+// Auxiliary particle filter: the look-ahead weights of the current particles and their fold
+// (threshold +Inf: the fold's weights are fresh; record 0 keeps the step counter at t).
+template <typename T>
+void enqueue_apf_lookahead_t(ssm_bf* bf, hipStream_t s) {
+  const int nb = (bf->N + kTile - 1) / kTile;
+  StepParams<T> p;
+  p.N = bf->N;
+  p.Npad = bf->Npad;
+  p.D = bf->D;
+  p.Dy = bf->Dy;
+  p.nblocks = nb;
+  p.xin = (const T*)bf->x[bf->cur];
+  p.logw = (T*)bf->logw;
+  p.model = (const BfModelDev<T>*)bf->model_dev;
+  p.ys = bf->ys;
+  p.sc = bf->sc;
+  ResampleArgs a;
+  resample_bind_scratch(a, bf->as_scratch, bf->N);
+  const dim3 g(nb), b(kTileThreads);
+  if (bf->D == 4 && bf->Dy == 2)
+    hipLaunchKernelGGL((k_apf_weights<T, 4, 2>), g, b, 0, s, p, (T*)bf->as_w, bf->as_sc,
+                       a.partials);
+  else if (bf->D == 1 && bf->Dy == 1)
+    hipLaunchKernelGGL((k_apf_weights<T, 1, 1>), g, b, 0, s, p, (T*)bf->as_w, bf->as_sc,
+                       a.partials);
+  else
+    hipLaunchKernelGGL((k_apf_weights<T, 0, 0>), g, b, 0, s, p, (T*)bf->as_w, bf->as_sc,
+                       a.partials);
+  const TileScratch ts{a.partials, a.blk_m, a.blk_c, a.prefix, a.groups};
+  launch_pf_finish<T>(s, bf->N, nb, bf->as_sc, INFINITY, std::log((double)bf->N), ts, nullptr,
+                      nullptr, 0);
 }
 
 // Enqueues the fused predict/update kernel of one step and flips the state buffers.
@@ -362,12 +456,13 @@ void enqueue_finish(ssm_bf* bf, hipStream_t s, double threshold, int record) {
   const double threshold_n = threshold * (double)bf->N;
   const double logN = std::log((double)bf->N);
   const TileScratch ts = tile_scratch(bf);
+  const BfScalars* aux = (bf->apf && record) ? bf->as_sc : nullptr;
   if (bf->prec == SSM_F32)
     launch_pf_finish<float>(s, bf->N, nb, bf->sc, threshold_n, logN, ts, bf->ll_steps,
-                            bf->rs_steps, record);
+                            bf->rs_steps, record, aux);
   else
     launch_pf_finish<double>(s, bf->N, nb, bf->sc, threshold_n, logN, ts, bf->ll_steps,
-                             bf->rs_steps, record);
+                             bf->rs_steps, record, aux);
 }
 
 // Conditional SMC with ancestor sampling: the reference particle's ancestor for this step,
@@ -414,6 +509,12 @@ int enqueue_step(ssm_bf* bf, hipStream_t s, double threshold, int kind, const do
     const int rc = (bf->prec == SSM_F32) ? enqueue_ancestor_draw_t<float>(bf, s)
                                          : enqueue_ancestor_draw_t<double>(bf, s);
     if (rc) return rc;
+  }
+  if (bf->apf) {  // the first stage: look-ahead weights and their fold
+    if (bf->prec == SSM_F32)
+      enqueue_apf_lookahead_t<float>(bf, s);
+    else
+      enqueue_apf_lookahead_t<double>(bf, s);
   }
   const int rc = enqueue_resample(bf, s, threshold, kind, u_dev, debug_weights, counts_out);
   if (rc) return rc;
@@ -867,6 +968,8 @@ extern "C" int32_t ssm_bf_import(ssm_bf* bf, const double* x, const double* logw
 extern "C" int32_t ssm_bf_profile(ssm_bf* bf, int32_t t0, int32_t T, const double* ys,
                                   double threshold, int32_t resampler, int32_t reps,
                                   double* ms_kernels, double* ll_total) {
+  if (bf && bf->apf)
+    return set_error(bf->ctx, SSM_ERR_ARG, "ssm_bf_profile: times the bootstrap filter only");
   if (!bf) return SSM_ERR_ARG;
   ssm_ctx* ctx = bf->ctx;
   if (!bf->initialised) return set_error(ctx, SSM_ERR_STATE, "ssm_bf_profile before initialise");
@@ -1080,6 +1183,9 @@ int ensure_csmc_buffers(ssm_bf* bf) {
 
 extern "C" int32_t ssm_bf_set_reference(ssm_bf* bf, int32_t T, const double* x_ref,
                                         int32_t mode) {
+  if (bf && bf->apf && x_ref)
+    return set_error(bf->ctx, SSM_ERR_ARG,
+                     "ssm_bf_set_reference: not with the auxiliary particle filter");
   if (!bf) return SSM_ERR_ARG;
   ssm_ctx* ctx = bf->ctx;
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -1144,6 +1250,9 @@ int upload_proposal_t(ssm_bf* bf) {
 
 extern "C" int32_t ssm_bf_set_proposal(ssm_bf* bf, const double* F, const double* g0,
                                        const double* K, const double* P) {
+  if (bf && bf->apf && F)
+    return set_error(bf->ctx, SSM_ERR_ARG,
+                     "ssm_bf_set_proposal: not with the auxiliary particle filter");
   if (!bf) return SSM_ERR_ARG;
   ssm_ctx* ctx = bf->ctx;
   SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -1186,6 +1295,26 @@ extern "C" int32_t ssm_bf_set_seed(ssm_bf* bf, uint64_t seed) {
                                     ctx->stream));
   SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   bf->seed = seed;
+  return SSM_OK;
+}
+
+extern "C" int32_t ssm_bf_set_auxiliary(ssm_bf* bf, int32_t mode) {
+  if (!bf) return SSM_ERR_ARG;
+  ssm_ctx* ctx = bf->ctx;
+  if (mode != SSM_APF_NONE && mode != SSM_APF_MEAN)
+    return set_error(ctx, SSM_ERR_ARG, "ssm_bf_set_auxiliary: unknown look-ahead %d", mode);
+  if (mode == SSM_APF_MEAN && (bf->ref_T >= 0 || !bf->prop_F.empty()))
+    return set_error(ctx, SSM_ERR_ARG,
+                     "ssm_bf_set_auxiliary: the auxiliary filter takes neither a reference "
+                     "trajectory nor a proposal");
+  SSM_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (mode == SSM_APF_MEAN) {
+    const int rc = ensure_csmc_buffers(bf);
+    if (rc) return rc;
+  }
+  SSM_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  destroy_graphs(bf);  // the captured steps differ
+  bf->apf = mode == SSM_APF_MEAN;
   return SSM_OK;
 }
 

@@ -78,6 +78,10 @@ struct StepParams {
   int32_t* anc_w;
   // guided filter (the GUIDED kernels): the proposal
   const PropDev<T>* __restrict__ prop = nullptr;
+  // auxiliary particle filter: the fold of the look-ahead weights (null: the bootstrap
+  // filter).  Its ESS decides the resample, and a resampled particle starts from the
+  // log-weight -log eta of its ancestor.
+  const BfScalars* aux = nullptr;
 };
 
 // Per-filter constants of the reference's transition density log N(x'; A x + b, Q) for
@@ -180,6 +184,44 @@ struct ModelView<T, MD, MDY, false> {
   __device__ __forceinline__ T LRi(int q, int c) const { return m.LRi[q + c * Dy]; }
 };
 
+// The auxiliary particle filter's look-ahead score of a state x (RepresentativeStateLookAhead
+// with MeanPredictive, GF/src/algorithms/particles.jl:282-298 and :346-352): the observation
+// log-density of y at the transition mean, log N(y; H (A x + b) + c, R).  The look-ahead
+// kernel and the step kernel call this one function, so a resampled particle's -log eta
+// is the score its ancestor was drawn with, bit for bit.
+template <typename T, int MD, int MDY, class MV>
+__device__ __forceinline__ T apf_log_eta(const MV& mv, int D, int Dy, const T* x, const T* y) {
+  T xm[MD];
+#pragma unroll
+  for (int r = 0; r < MD; ++r) {
+    T s = (r < D) ? mv.b(r) : (T)0;
+#pragma unroll
+    for (int c = 0; c < MD; ++c)
+      if (r < D && c < D) s += mv.A(r, c) * x[c];
+    xm[r] = s;
+  }
+  T res[MDY];
+#pragma unroll
+  for (int q = 0; q < MDY; ++q) {
+    T s = (q < Dy) ? mv.c(q) : (T)0;
+#pragma unroll
+    for (int c = 0; c < MD; ++c)
+      if (q < Dy && c < D) s += mv.H(q, c) * xm[c];
+    res[q] = (q < Dy) ? y[q] - s : (T)0;
+  }
+  T quad = 0;
+#pragma unroll
+  for (int q = 0; q < MDY; ++q) {
+    if (q < Dy) {
+      T v = 0;
+#pragma unroll
+      for (int c = 0; c <= q; ++c) v += mv.LRi(q, c) * res[c];
+      quad += v * v;
+    }
+  }
+  return mv.cst - (T)0.5 * quad;
+}
+
 // Copies the model constants to LDS (runtime-dimension kernels read them many times).
 // The proposal's constants with the step's mean offset g = g0 + K y: registers for
 // compile-time dimensions, the device struct otherwise.
@@ -269,8 +311,10 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   const int D = DT ? DT : p.D;
   const int Dy = DYT ? DYT : p.Dy;
   BfScalars* sc = p.sc;
-  const bool rs = p.threshold_n >= sc->ess;
-  const bool fresh = rs || sc->typeless;  // normalised pre-update log-weights are all 0
+  // the auxiliary filter decides on its look-ahead weights' ESS
+  const bool apf = p.aux != nullptr;
+  const bool rs = p.threshold_n >= (apf ? p.aux->ess : sc->ess);
+  const bool fresh = rs || sc->typeless;  // pre-update log-weights 0 (or -log eta: apf)
   const T lse = (T)sc->lse;
   const int t = sc->step;
   const uint64_t seed = p.seed_dev ? *p.seed_dev : p.seed;
@@ -445,6 +489,9 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
       else
         nl[k] = lw[k] - lse;
     }
+    // a particle the auxiliary filter resampled starts from -log eta of its ancestor
+    // (resample_ancestor, GF/src/resamplers.jl:140-143); `apf && rs` is block-uniform
+    [[maybe_unused]] const bool apf_rs = FRESH && apf && rs;
     if constexpr (DT > 0) {
       // the normals while the ancestor (or log-weight) loads are in flight, then every
       // gather, then the arithmetic
@@ -458,6 +505,13 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
         for (int d = 0; d < MD; ++d) x[k][d] = p.xin[(size_t)d * p.Npad + src];
       }
+      if constexpr (FRESH && !GUIDED && !REF) {
+        if (apf_rs) {
+#pragma unroll
+          for (int k = 0; k < kStepItems; ++k)
+            nl[k] = -apf_log_eta<T, MD, MDY>(mv, D, Dy, x[k], y);
+        }
+      }
 #pragma unroll
       for (int k = 0; k < kStepItems; ++k) lnew[k] = compute_item(k, x[k], z[k], nl[k]);
     } else {
@@ -468,6 +522,9 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         for (int d = 0; d < MD; ++d)
           if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
         normals(k, z);
+        if constexpr (FRESH && !GUIDED && !REF) {
+          if (apf_rs) nl[k] = -apf_log_eta<T, MD, MDY>(mv, D, Dy, xk, y);
+        }
         lnew[k] = compute_item(k, xk, z, nl[k]);
       }
     }

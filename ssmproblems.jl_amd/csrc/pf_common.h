@@ -309,8 +309,10 @@ template <typename T, int CH>
 __device__ __forceinline__ BfScalars fold_scalars(const TileFold<CH>& F, const BfScalars& sc_in,
                                                   int N, double logN, bool rs, bool fresh,
                                                   double* ll_steps, int32_t* rs_steps,
-                                                  int record) {
-  const double baseline = fresh ? logN : F.M0 + log(F.S0);
+                                                  int record, double base = NAN) {
+  // base: the predict baseline when the caller knows it (the auxiliary filter's resampled
+  // steps), otherwise log N after a fresh start or the log-sum-exp of the kept weights
+  const double baseline = !isnan(base) ? base : fresh ? logN : F.M0 + log(F.S0);
   const int t = sc_in.step;
   const int idx = t - 1 - sc_in.ybase;
   BfScalars s = sc_in;
@@ -495,6 +497,26 @@ __device__ __forceinline__ void fold_step4(int nb, const double* __restrict__ P,
   F.dead = dead;
 }
 
+// The decision a folded step ran with and, for the auxiliary particle filter, its predict
+// baseline.  aux (null for the plain filters) holds the fold of the look-ahead weights
+// log w_i + log eta_i: the step resampled iff threshold N >= their ESS, and after a resample
+// the baseline is log N - (LSE(log w + log eta) - LSE(log w)) (construct_new_state,
+// GF/src/resamplers.jl:126-138, plus the LSE(-log eta[a]) that predict adds and the
+// correction subtracts), LSE(log w) being log N for the lazy uniform weights and 0 for
+// normalised ones.
+struct StepDecision {
+  bool rs, fresh;
+  double base;
+};
+__device__ __forceinline__ StepDecision step_decision(const BfScalars& in, const BfScalars* aux,
+                                                     double threshold_n, double logN) {
+  StepDecision d;
+  d.rs = threshold_n >= (aux ? aux->ess : in.ess);
+  d.fresh = d.rs || in.typeless;
+  d.base = (aux && d.rs) ? logN - (aux->lse - (in.typeless ? logN : 0.0)) : NAN;
+  return d;
+}
+
 // The folded step as the scalars need it.
 __device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
   TileFold<1> G;
@@ -514,10 +536,10 @@ template <typename T>
 __global__ void __launch_bounds__(kFoldThreads)
     k_pf_finish_fold(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
                      double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                     int record) {
+                     int record, const BfScalars* aux) {
   const BfScalars in = *sc_in;
-  const bool rs = threshold_n >= in.ess;
-  const bool fresh = rs || in.typeless;
+  const StepDecision dec = step_decision(in, aux, threshold_n, logN);
+  const bool rs = dec.rs, fresh = dec.fresh;
   Fold4 F;
   fold_step4<T>(nb, ts.partials, fresh, F);
 #pragma unroll
@@ -532,7 +554,7 @@ __global__ void __launch_bounds__(kFoldThreads)
   if (threadIdx.x == 0) {
     ts.prefix[nb] = F.tot;
     *sc = fold_scalars<T, 1>(fold_summary(F), in, N, logN, rs, fresh, ll_steps, rs_steps,
-                             record);
+                             record, dec.base);
   }
 }
 
@@ -572,11 +594,11 @@ __device__ __forceinline__ double block_prefix_monotone(double v, double* lds_to
 // their closing value R_g (r_b goes to prefix[b] and d_b to blk_c[b] until the second pass).
 template <typename T>
 __global__ void __launch_bounds__(kFinishGroup)
-    k_pf_finish_groups(int nb, const BfScalars* sc_in, double threshold_n, TileScratch ts) {
+    k_pf_finish_groups(int nb, const BfScalars* sc_in, double threshold_n, TileScratch ts,
+                       const BfScalars* aux) {
   constexpr int NT = kFinishGroup, NW = NT / 64;
   __shared__ double red[5][NW];
-  const bool rs = threshold_n >= sc_in->ess;
-  const bool fresh = rs || sc_in->typeless;
+  const bool fresh = step_decision(*sc_in, aux, threshold_n, 0.0).fresh;
   const double* P = ts.partials;
   const int b = blockIdx.x * NT + threadIdx.x;
   const bool in = b < nb;
@@ -646,10 +668,10 @@ template <typename T>
 __global__ void __launch_bounds__(kFinishGroup)
     k_pf_finish_apply(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
                       double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                      int record) {
+                      int record, const BfScalars* aux) {
   const int G = (nb + kFinishGroup - 1) / kFinishGroup;
-  const bool rs = threshold_n >= sc_in->ess;
-  const bool fresh = rs || sc_in->typeless;
+  const StepDecision dec = step_decision(*sc_in, aux, threshold_n, logN);
+  const bool rs = dec.rs, fresh = dec.fresh;
   const double* gr = ts.groups;
   double M1 = -INFINITY, M0 = -INFINITY;
   for (int g = 0; g < G; ++g) {
@@ -692,25 +714,27 @@ __global__ void __launch_bounds__(kFinishGroup)
     F.S0 = S0;
     F.tot = off;
     F.dead = dead;
-    *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
+    *sc = fold_scalars<T, 1>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record,
+                             dec.base);
   }
 }
 
 // Enqueues the finish: one block up to 1024 tiles, the two group passes beyond (ts.groups
-// must hold a record per group).
+// must hold a record per group).  aux: the auxiliary filter's look-ahead fold (null for the
+// plain filters).
 template <typename T>
 inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double threshold_n,
                              double logN, const TileScratch& ts, double* ll_steps,
-                             int32_t* rs_steps, int record) {
+                             int32_t* rs_steps, int record, const BfScalars* aux = nullptr) {
   if (nb <= kFoldMax) {
     hipLaunchKernelGGL((k_pf_finish_fold<T>), dim3(1), dim3(kFoldThreads), 0, s, N, nb, sc, sc,
-                       threshold_n, logN, ts, ll_steps, rs_steps, record);
+                       threshold_n, logN, ts, ll_steps, rs_steps, record, aux);
   } else {
     const int G = (nb + kFinishGroup - 1) / kFinishGroup;
     hipLaunchKernelGGL((k_pf_finish_groups<T>), dim3(G), dim3(kFinishGroup), 0, s, nb, sc,
-                       threshold_n, ts);
+                       threshold_n, ts, aux);
     hipLaunchKernelGGL((k_pf_finish_apply<T>), dim3(G), dim3(kFinishGroup), 0, s, N, nb, sc,
-                       sc, threshold_n, logN, ts, ll_steps, rs_steps, record);
+                       sc, threshold_n, logN, ts, ll_steps, rs_steps, record, aux);
   }
 }
 

@@ -605,8 +605,10 @@ void kf_lane(ssm_server* sv) {
     ok = ok && kf_group_launch(ctx, s, g) == SSM_OK;
     ok = ok &&
          hipMemcpyAsync(ll.data(), g.ll, 8 * B, hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipMemcpyAsync(st.data(), g.status, 4 * B, hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipStreamSynchronize(s) == hipSuccess;
+         hipMemcpyAsync(st.data(), g.status, 4 * B, hipMemcpyDeviceToHost, s) == hipSuccess;
+    // wait on every path: the copies above read and write the host vectors of this scope,
+    // and the arena block must be idle before it is released
+    ok = (hipStreamSynchronize(s) == hipSuccess) && ok;
     std::lock_guard<std::mutex> lk(sv->m);
     sv->arena.release(off, bytes);
     ++sv->kf_groups;

@@ -16,7 +16,8 @@ from .kalman import (KalmanFilter, KF, KFBatch, MvNormalState, pack_lg, kf_logli
                      SRKF, srkf_filter_batch, DeviceKFInputs)
 from .rbpf import RBPF, RBModel, DeviceRBParticles, rbpf_filter
 from .filters import (BootstrapFilter, BF, ParticleFilter, PF, DeviceParticles, initialise,
-                      step, run, filter, rng_to_seed, last_timing)
+                      step, run, filter, rng_to_seed, last_timing, AuxiliaryParticleFilter,
+                      APF, MeanPredictive, ModePredictive, PredictiveStatistic)
 from .proposals import (AbstractProposal, LatentProposal, LinearGaussianProposal,
                         OptimalProposal, OverdispersedProposal)
 from .serve import Server

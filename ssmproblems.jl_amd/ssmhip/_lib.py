@@ -25,6 +25,7 @@ SSM_ERR_NO_DEVICE = -7
 
 SSM_SYSTEMATIC, SSM_SYSTEMATIC_MURRAY, SSM_STRATIFIED, SSM_MULTINOMIAL = 0, 1, 2, 3
 SSM_CSMC_PLAIN, SSM_CSMC_ANCESTOR = 0, 1
+SSM_APF_NONE, SSM_APF_MEAN = 0, 1
 SSM_F32, SSM_F64 = 0, 1
 
 
@@ -129,6 +130,7 @@ def _declare(L):
         "ssm_bf_set_reference": ([_vp, i32, _dp, i32], i32),
         "ssm_bf_sample_index": ([_vp, dbl, _ip], i32),
         "ssm_bf_set_seed": ([_vp, u64], i32),
+        "ssm_bf_set_auxiliary": ([_vp, i32], i32),
         "ssm_kf_batch_create": ([_vp, i32, _ip, _ip, _dp, C.POINTER(C.c_int64), i32, _dp,
                                  C.POINTER(C.c_int64), C.POINTER(_vp)], i32),
         "ssm_kf_batch_run": ([_vp, _dp, _ip, _dp, _dp], i32),

@@ -64,6 +64,40 @@ class ParticleFilter(BootstrapFilter):
 PF = ParticleFilter
 
 
+class PredictiveStatistic:
+    """The representative state of a look-ahead score (particles.jl:264-266)."""
+
+
+class MeanPredictive(PredictiveStatistic):
+    """The transition mean (particles.jl:346-352)."""
+
+
+class ModePredictive(PredictiveStatistic):
+    """The transition mode (particles.jl:354-360): the mean for Gaussian transitions."""
+
+
+class AuxiliaryParticleFilter(BootstrapFilter):
+    """AuxiliaryParticleFilter(pf, MeanPredictive()) -- particles.jl:272-281, :316-344 with
+    RepresentativeStateLookAhead: each step first scores the particles with the observation
+    density at their transition mean and resamples with the AuxiliaryResampler
+    (resamplers.jl:92-143).  pf is a BootstrapFilter; its N, threshold, resampler and
+    precision are the filter's."""
+
+    def __init__(self, pf: BootstrapFilter, pp: PredictiveStatistic = None):
+        if _proposal(pf) is not None:
+            raise TypeError("the device auxiliary filter wraps a bootstrap filter")
+        pp = MeanPredictive() if pp is None else pp
+        if not isinstance(pp, (MeanPredictive, ModePredictive)):
+            raise TypeError(f"look-ahead {type(pp).__name__} is not supported on the device "
+                            "(MeanPredictive, ModePredictive)")
+        super().__init__(pf.N, pf.threshold, pf.resampler, pf.precision)
+        self.pf = pf
+        self.weight_strategy = pp
+
+
+APF = AuxiliaryParticleFilter
+
+
 def _proposal(algo):
     """The filter's proposal if it is not the latent dynamics."""
     prop = getattr(algo, "proposal", None)
@@ -92,6 +126,8 @@ class DeviceParticles:
         self.seed = seed
         if _proposal(algo) is not None:
             self.set_proposal(_proposal(algo).linear_form(1))
+        if isinstance(algo, AuxiliaryParticleFilter):
+            ctx.check(ctx.lib.ssm_bf_set_auxiliary(self.ptr, _lib.SSM_APF_MEAN))
 
     def set_proposal(self, form):
         """Uploads a linear-Gaussian proposal (F, g0, K, P); None: the latent dynamics."""
