@@ -91,6 +91,41 @@ def cpu_baseline(p, ys, N, kind, threshold, budget_s=12.0):
     return N * steps / el, steps, el
 
 
+def cpu_info():
+    """(logical CPUs, model name) of this host, from os and /proc/cpuinfo."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, name
+
+
+def cpu_baseline_all_cores(p, ys, N, threshold, budget_s=10.0):
+    """The all-cores CPU port of the same filter (oracle/bf_omp.c: FP64 SoA, OpenMP over
+    particles, systematic resampling by blocks) on this host's CPU share (OMP_NUM_THREADS,
+    else every logical CPU): runs growing step counts until one takes ~budget_s / 2.
+    Returns (particle-steps/s, threads, steps, seconds)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    m = oracle.LG(p["mu0"], p["Sigma0"], p["A"], p["b"], p["Q"], p["H"], p["c"], p["R"])
+    nproc, _ = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc) or nproc)
+    x0 = oracle.bf_initialise(m, np.random.default_rng(5).standard_normal((N, 4)))
+    T = 2
+    while True:
+        t0 = time.perf_counter()
+        oracle.bfomp_run(m, x0, ys[:T], threshold, 17, threads)
+        el = time.perf_counter() - t0
+        if el > budget_s / 2 or T >= len(ys):
+            return N * T / el, threads, T, el
+        T = min(len(ys), max(T + 1, int(T * min(8.0, budget_s / 2 / max(el, 1e-3)))))
+
+
 def copy_calibration(nbytes=256 << 20, reps=5):
     """A plain device-to-device copy of known size: the reference point that scales the
     FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes nbytes)."""
@@ -858,6 +893,42 @@ def bench_kfll(args, T=100, budget_s=10.0):
     print(json.dumps(out), flush=True)
 
 
+def report_apf(args, value, ms_per_step, lls, ll_kf):
+    """The auxiliary particle filter's line: whole-filter throughput and the whole step's
+    HBM fraction at 72 B per particle-step when every step resamples (look-ahead: state 16 +
+    log-weight 4 read, look-ahead weight 4 written; resampler: 4 read, ancestor 4 written;
+    step kernel 40)."""
+    step_s = ms_per_step * 1e-3 / args.T
+    bpps = 72.0 if args.threshold >= 1.0 else None
+    whole = None
+    if bpps is not None:
+        wa = bpps * args.N / step_s / 1e9
+        whole = {"bytes_per_particle_step": bpps, "achieved": wa, "unit": "GB/s",
+                 "frac": wa / HBM_PEAK_GBS}
+    out = {
+        "metric": "auxiliary-filter particle-steps/s, % HBM roofline", "value": value,
+        "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded LG simulation, Philox noise on device)",
+        "config": {"workload": f"auxiliary particle filter (MeanPredictive), D=4 Dy=2 LG SSM, "
+                               f"N={args.N}, T={args.T}, {args.resampler} resampling, "
+                               f"threshold {args.threshold}, FP32 SoA; one step = one filter() run",
+                   "N": args.N, "T": args.T},
+        "loglik": lls[-1], "loglik_kalman": ll_kf,
+        "loglik_rel_err": abs(lls[-1] - ll_kf) / abs(ll_kf),
+        "roofline": {"bound": "hbm", "achieved": whole["achieved"] if whole else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": whole["frac"] if whole else None, "traffic": None,
+                     "kernel": "whole step (k_apf_weights, fold, k_scan_offspring, k_bf_step, "
+                               "finish)", "step_total_us": step_s * 1e6},
+        "cpu_baseline": None,
+    }
+    log(f"APF: {value:.4e} particle-steps/s, {step_s * 1e6:.2f} us per step, "
+        f"ll={lls[-1]:.4f} kf={ll_kf:.4f}")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -889,6 +960,8 @@ def main():
     ap.add_argument("--proposal", default="none", choices=["none", "optimal", "overdispersed"],
                     help="bf: a guided filter with GFTest's OptimalProposal or "
                          "OverdispersedProposal(k = 1.5) instead of the bootstrap filter")
+    ap.add_argument("--apf", action="store_true",
+                    help="bf: the auxiliary particle filter (MeanPredictive look-ahead)")
     ap.add_argument("--srkf", action="store_true",
                     help="kfll: the square-root Kalman filter (final means and factors)")
     ap.add_argument("--grad", action="store_true",
@@ -936,6 +1009,8 @@ def main():
         algo = ssmhip.PF(args.N, prop, threshold=args.threshold, resampler=R, precision="f32")
     else:
         algo = ssmhip.BF(args.N, threshold=args.threshold, resampler=R, precision="f32")
+    if args.apf:  # AuxiliaryParticleFilter(BF, MeanPredictive())
+        algo = ssmhip.APF(algo, ssmhip.MeanPredictive())
     rng = np.random.default_rng(2024)
     st = ssmhip.initialise(rng, model, algo, ctx=ctx, seed=2024)
     lib = ctx.lib
@@ -961,6 +1036,8 @@ def main():
     # Per-kernel timing with HIP events (plain launches, same kernels and stream).
     import ctypes as C
     ctx.check(lib.ssm_bf_initialise(st.ptr, None))
+    if args.apf:
+        return report_apf(args, value, ms_per_step, lls, ll_kf)
     ms = (C.c_double * 6)()
     llp = C.c_double()
     TP = min(args.profile_steps, args.T)
@@ -978,10 +1055,32 @@ def main():
     # (log-weight read 4, ancestor write 4).
     step_bytes = 40.0 * args.N
     resample_bytes = 8.0 * args.N
+    nb = (args.N + 1023) // 1024
+    finish_bytes = 64.0 * nb  # per tile: the 5 FP64 partials read, 3 FP64 outputs written
     achieved = step_bytes / (ms_st * 1e-3) / 1e9
     log(f"step kernel {ms_st * 1e3:.2f} us ({achieved:.0f} GB/s); resampler "
         f"{ms_rs * 1e3:.2f} us ({resample_bytes / (ms_rs * 1e-3) / 1e9:.0f} GB/s); "
         f"finish {ms_fin * 1e3:.2f} us")
+    # the dominant kernel by measured share of the step (per-launch durations); the roofline
+    # below is that kernel's, with the whole step's and the resampler's beside it
+    kern = {"k_bf_step": (step_bytes, ms_st), "k_scan_offspring": (resample_bytes, ms_rs),
+            "k_pf_finish": (finish_bytes, ms_fin)}
+    tot_ms = ms_st + ms_rs + ms_fin
+    shares = {k: v[1] / tot_ms for k, v in kern.items()}
+    dominant = max(shares, key=shares.get)
+    dom_bytes, dom_ms = kern[dominant]
+    dom_achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_total_s = ms_per_step * 1e-3 / args.T
+    # whole step: 40 B per particle, plus 28 B on steps that resample (every step at a
+    # threshold >= 1); at lower thresholds the resampled fraction is not reported here
+    whole_bpps = 68.0 if args.threshold >= 1.0 else None
+    whole = None
+    if whole_bpps is not None:
+        wa = whole_bpps * args.N / step_total_s / 1e9
+        whole = {"bytes_per_particle_step": whole_bpps, "achieved": wa, "unit": "GB/s",
+                 "frac": wa / HBM_PEAK_GBS, "us_per_step": step_total_s * 1e6}
+        log(f"whole step {step_total_s * 1e6:.2f} us: {wa:.0f} GB/s = "
+            f"{100 * wa / HBM_PEAK_GBS:.1f}% of HBM peak (68 B per particle-step)")
 
     cpu = None
     if not args.no_cpu and guided:
@@ -992,11 +1091,25 @@ def main():
                          f"and proposal with N={n_s}: {steps} steps in {secs:.1f}s"}
         log(f"cpu baseline {v:.3e} particle-steps/s ({steps} steps)")
     elif not args.no_cpu:
-        v, steps, secs = cpu_baseline(p, ys, args.N, kind, args.threshold)
-        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle (oracle/oracle.c, single thread, AoS) on the same model "
-                         f"with N={args.N}: {steps} steps in {secs:.1f}s incl. noise draws"}
-        log(f"cpu baseline {v:.3e} particle-steps/s ({steps} steps)")
+        v1, steps1, secs1 = cpu_baseline(p, ys, args.N, kind, args.threshold, budget_s=8.0)
+        log(f"cpu baseline, faithful (1 thread): {v1:.3e} particle-steps/s ({steps1} steps)")
+        nproc, model_name = cpu_info()
+        faithful = {"value": v1, "cores": 1,
+                    "sample": f"FP64 oracle (oracle/oracle.c, single thread, AoS, the "
+                              f"reference's loop order) with N={args.N}: {steps1} steps in "
+                              f"{secs1:.1f}s incl. noise draws"}
+        if kind == 0:  # the all-cores port implements the systematic resampler
+            v, thr, steps, secs = cpu_baseline_all_cores(p, ys, args.N, args.threshold)
+            cpu = {"value": v, "unit": "particle-steps/s", "cores": thr, "kind": "port",
+                   "sample": f"all-cores FP64 port (oracle/bf_omp.c, OpenMP over {thr} threads, "
+                             f"SoA, Philox noise in the loop) with N={args.N}: {steps} steps in "
+                             f"{secs:.1f}s", "nproc": nproc, "cpu_model": model_name,
+                   "faithful_single_thread": faithful}
+            log(f"cpu baseline, all cores ({thr} threads of {nproc}, {model_name}): "
+                f"{v:.3e} particle-steps/s ({steps} steps)")
+        else:
+            cpu = dict(faithful, unit="particle-steps/s", kind="port", nproc=nproc,
+                       cpu_model=model_name)
 
     ll_err = abs(lls[-1] - ll_kf) / abs(ll_kf)
     out = {
@@ -1025,16 +1138,26 @@ def main():
         "loglik": lls[-1],
         "loglik_kalman": ll_kf,
         "loglik_rel_err": ll_err,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (None if guided else
+        "roofline": {"bound": "hbm", "achieved": dom_achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom_achieved / HBM_PEAK_GBS,
+                     "traffic": (None if guided or dominant != "k_bf_step" else
                                  traffic_from_profile(args.N, args.resampler, args.threshold)),
-                     "kernel": ("k_bf_step<float,4,2,false,false,true> (guided)" if guided
-                                else "k_bf_step<float,4,2>"), "bytes_per_launch": step_bytes,
-                     "us_per_launch": ms_st * 1e3,
-                     "us_per_launch_in_sequence": ms[1] * 1e3,
-                     "resample_us_per_launch": ms_rs * 1e3, "finish_us_per_launch": ms_fin * 1e3,
-                     "step_total_us": ms_per_step * 1e3 / args.T},
+                     "kernel": (("k_bf_step<float,4,2,false,false,true> (guided)" if guided
+                                 else "k_bf_step<float,4,2>") if dominant == "k_bf_step"
+                                else dominant),
+                     "dominant_by_share": dominant,
+                     "kernel_share": shares,
+                     "bytes_per_launch": dom_bytes, "us_per_launch": dom_ms * 1e3,
+                     "step_kernel": {"bytes_per_launch": step_bytes, "us_per_launch": ms_st * 1e3,
+                                     "achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+                                     "us_per_launch_in_sequence": ms[1] * 1e3},
+                     "resampler": {"bytes_per_launch": resample_bytes,
+                                   "us_per_launch": ms_rs * 1e3,
+                                   "achieved": resample_bytes / (ms_rs * 1e-3) / 1e9,
+                                   "frac": resample_bytes / (ms_rs * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "finish_us_per_launch": ms_fin * 1e3,
+                     "whole_step": whole,
+                     "step_total_us": step_total_s * 1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

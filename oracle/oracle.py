@@ -855,3 +855,50 @@ def dense_ancestry(xs, ancs):
         if t > 0:
             idx = np.asarray(ancs[t - 1])[idx]
     return paths, distinct
+
+
+# ---- the all-cores CPU baseline (oracle/bf_omp.c) ----------------------------------------
+class _BfOmpModel(C.Structure):
+    _fields_ = [("D", C.c_int), ("Dy", C.c_int), ("A", C.c_double * 256), ("b", C.c_double * 16),
+                ("LQ", C.c_double * 256), ("H", C.c_double * 256), ("c", C.c_double * 16),
+                ("LRi", C.c_double * 256), ("cst", C.c_double)]
+
+
+_omp = None
+
+
+def _omp_lib():
+    global _omp
+    if _omp is None:
+        _omp = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbfomp.so"))
+        _omp.bfomp_run.argtypes = [C.POINTER(_BfOmpModel), C.c_int64, C.c_int, _dp, C.c_double,
+                                   C.c_uint64, C.c_int, _dp, _dp, C.c_int, _dp]
+        _omp.bfomp_run.restype = C.c_int
+    return _omp
+
+
+def bfomp_run(model: LG, x0, ys, threshold, seed, nthreads):
+    """T systematic BF steps from the states x0 (N, Dx) with OpenMP over nthreads threads,
+    FP64 SoA (bf_omp.c, the all-cores CPU baseline).  Returns (ll, resampled steps, x)."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    N, D = x0.shape
+    ys = np.ascontiguousarray(np.asarray(ys, dtype=np.float64).reshape(len(ys), -1))
+    Dy = ys.shape[1]
+    m = _BfOmpModel()
+    m.D, m.Dy = D, Dy
+    fill = lambda dst, M: [dst.__setitem__(k, float(v)) for k, v in
+                           enumerate(np.asarray(M, dtype=np.float64).flatten(order="F"))]
+    fill(m.A, model.A)
+    fill(m.b, model.b)
+    fill(m.LQ, np.linalg.cholesky(model.Q))
+    fill(m.H, model.H)
+    fill(m.c, model.c)
+    LR = np.linalg.cholesky(model.R)
+    fill(m.LRi, np.linalg.inv(LR))
+    m.cst = -0.5 * (Dy * np.log(2 * np.pi) + 2 * np.log(np.diag(LR)).sum())
+    x = np.ascontiguousarray(x0.T.copy())  # SoA planes
+    lw = np.zeros(N)
+    ll = C.c_double()
+    n_rs = _omp_lib().bfomp_run(C.byref(m), N, ys.shape[0], _d(ys), float(threshold),
+                                int(seed), int(nthreads), _d(x), _d(lw), 1, C.byref(ll))
+    return ll.value, n_rs, x.T.copy()

@@ -275,3 +275,21 @@ def test_rbpf_degenerate_outer_equals_kf(orc):
                                math.exp(x) * R0, ys[t])
         ll_kf += l
     assert ll == pytest.approx(ll_kf, rel=1e-12)
+
+
+def test_all_cores_baseline_tracks_the_kalman_filter(orc):
+    # The OpenMP CPU baseline (bf_omp.c) is the same bootstrap filter: on the 1-D config-1
+    # model its log-likelihood averaged over seeds is within 3 standard errors of the exact
+    # KF value, and it resamples at threshold 1.0 on every step.
+    m = config1_model()
+    rng = np.random.default_rng(99)
+    _, _, ys = orc.simulate_lg(m, 50, rng)
+    _, _, ll_kf, _ = orc.kf_filter(m, ys)
+    lls = []
+    for rep in range(10):
+        x0 = orc.bf_initialise(m, np.random.default_rng(rep).standard_normal((4096, 1)))
+        ll, n_rs, x = orc.bfomp_run(m, x0, ys, 1.0, 1000 + rep, 4)
+        assert n_rs == 50 and np.all(np.isfinite(x))
+        lls.append(ll)
+    se = np.std(lls, ddof=1) / np.sqrt(len(lls))
+    assert abs(np.mean(lls) - ll_kf) < 3 * se + 0.05
