@@ -498,9 +498,12 @@ int32_t ssm_kf_loglikelihood_device(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t
  * as ssm_kf_loglikelihood and their gradients w.r.t. mu0, Sigma0, A, b, Q, H, c and R, each
  * in its input's layout (P blocks; with time_varying = 0 the gradient of the one shared block
  * sums the steps).  Not w.r.t. ys or jitter, as in the reference.  Filters up to
- * Dx = Dy = 4 (one per lane, FP64, the forward pass caching 80 doubles per step on the
- * device); larger ones are refused.  Gradients of a filter whose status is SSM_ERR_NOT_PD are
- * undefined.  Any g_* may be NULL.
+ * Dx = 64, Dy = 32, in FP64: up to Dx = Dy = 4 one per lane, the forward pass caching 80
+ * doubles per step on the device; beyond, one workgroup per filter that keeps the filtered
+ * state of every step ((T + 1)(Dx + Dx^2) doubles a filter, filters in chunks of at most
+ * 2 GiB of device workspace) and recomputes each step's update in the backward pass.  The
+ * log-likelihoods are those of this FP64 forward pass.  Gradients of a filter whose status
+ * is SSM_ERR_NOT_PD are undefined.  Any g_* may be NULL.
  */
 int32_t ssm_kf_loglikelihood_grad(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                                   int32_t time_varying, const double* mu0, const double* Sigma0,
@@ -518,8 +521,10 @@ int32_t ssm_kf_loglikelihood_grad(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t D
  * log-likelihoods, the final filtered means mu_out (B x Dx) and factors U_out (B blocks of
  * Dx x Dx, column-major, upper triangular with a positive diagonal).  jitter is added to
  * the factor's diagonal after each update, as srkf_update does (0: none).  Filters up to
- * Dx = Dy = 4, one per lane in FP64.  Outputs of a filter whose status is SSM_ERR_NOT_PD
- * (Sigma0, Q or R not positive definite) are undefined.  mu_out and U_out may be NULL.
+ * Dx = 64, Dy = 32 in FP64: up to Dx = Dy = 4 one per lane (Givens rotations), beyond one
+ * workgroup per filter (Householder QR).  Outputs of a filter whose status is
+ * SSM_ERR_NOT_PD (Sigma0, Q or R not positive definite) are undefined.  mu_out and U_out
+ * may be NULL.
  */
 int32_t ssm_srkf_filter(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
                         int32_t time_varying, const double* mu0, const double* Sigma0,

@@ -64,6 +64,7 @@ extern "C" int32_t ssm_ctx_destroy(ssm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->rs_scratch) (void)hipFree(ctx->rs_scratch);
   if (ctx->kfll) (void)hipFree(ctx->kfll);
+  if (ctx->kfwg) (void)hipFree(ctx->kfwg);
   for (hipEvent_t e : ctx->kfll_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->rs_w) (void)hipFree(ctx->rs_w);

@@ -27,6 +27,9 @@ struct ssm_ctx {
   size_t kfll_bytes = 0;
   hipEvent_t kfll_ev[2] = {nullptr, nullptr};  // around its kernel
   double kfll_ms = 0.0;
+  // Lazily grown workspace of the workgroup gradient and square-root filter kernels.
+  void* kfwg = nullptr;
+  size_t kfwg_bytes = 0;
 };
 
 namespace ssm {

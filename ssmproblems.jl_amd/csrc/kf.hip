@@ -29,46 +29,10 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "kf_common.h"
 #include "mfma_team.h"
 
 namespace ssm {
-
-struct KfBatchParams {
-  int T;
-  const int32_t* __restrict__ Tf;  // optional per-filter horizons (else T for all)
-  int count;                      // filters in this class
-  const int32_t* __restrict__ order;  // filter ids of this class, costliest first
-  uint32_t* counter;              // work-queue head (zeroed before launch)
-  const int32_t* __restrict__ Dx;
-  const int32_t* __restrict__ Dy;
-  const double* __restrict__ params;
-  const int64_t* __restrict__ param_off;
-  const double* __restrict__ ys;
-  const int64_t* __restrict__ y_off;
-  double* __restrict__ ll_out;
-  int32_t* __restrict__ status;
-  double* __restrict__ mu_out;     // optional final state (packed, offsets below)
-  double* __restrict__ Sigma_out;
-  const int64_t* __restrict__ mu_off;
-  const int64_t* __restrict__ cov_off;
-  // added to the diagonal of every filtered covariance (KalmanFilter's jitter; 0 = none)
-  double jitter = 0.0;
-  // kf_loglikelihood's layout (the TV kernels; uniform Dx, Dy): filter f starts from
-  // mu0 + f Dx and Sigma0 + f Dx^2, and step t (0-based) uses parameter block
-  // f * blocks + (blocks > 1 ? t : 0) of each array (column-major matrices)
-  const double* __restrict__ tv_mu0 = nullptr;
-  const double* __restrict__ tv_Sigma0 = nullptr;
-  const double* __restrict__ tv_A = nullptr;
-  const double* __restrict__ tv_b = nullptr;
-  const double* __restrict__ tv_Q = nullptr;
-  const double* __restrict__ tv_H = nullptr;
-  const double* __restrict__ tv_c = nullptr;
-  const double* __restrict__ tv_R = nullptr;
-  int tv_blocks = 1;
-  int fm = 0;  // the tiny kernels' inputs staged filter-minor (kf_at, kf_yat, kf_0at)
-};
-
-constexpr double kLog2Pi = 1.8378770664093454835606594728112;
 
 template <int DXP, int DYP>
 struct KfLds {
@@ -331,22 +295,6 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
 constexpr int kTinyD = 4;
 constexpr int kTinyThreads = 64;
 
-// Indices into the tiny kernels' inputs: the caller's layout (parameter block f T + t, or f
-// with one set a filter; mu0 and Sigma0 filter-major; ys at y_off[f]), or with p.fm the
-// filter-minor staging of the same values, element e of step t of filter f at (t E + e) B + f,
-// so the 64 lanes of a wave read 64 consecutive words.
-__device__ __forceinline__ int64_t kf_at(const KfBatchParams& p, int f, int t, int64_t E, int e) {
-  const int64_t tb = p.tv_blocks > 1 ? t : 0;
-  return p.fm ? (tb * E + e) * (int64_t)p.count + f : ((int64_t)f * p.tv_blocks + tb) * E + e;
-}
-__device__ __forceinline__ int64_t kf_yat(const KfBatchParams& p, int f, int t, int dy, int q) {
-  return p.fm ? ((int64_t)t * dy + q) * (int64_t)p.count + f
-              : p.y_off[f] + (int64_t)t * dy + q;
-}
-__device__ __forceinline__ int64_t kf_0at(const KfBatchParams& p, int f, int64_t E, int e) {
-  return p.fm ? (int64_t)e * p.count + f : (int64_t)f * E + e;
-}
-
 template <int M>
 __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
   const int f = blockIdx.x * kTinyThreads + threadIdx.x;
@@ -569,14 +517,6 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny(KfBatchParams p) {
 // consecutive words; only the slots inside dx and dy are written and read back); the
 // backward pass walks the steps in reverse in the same lane.
 constexpr int kGradCache = 80;  // slots a step at N = 4: mu_prev, S_prev, mu_pred, S_pred, mu_f, z, S, K
-
-struct KfGradOut {
-  double* mu0;    // B x Dx
-  double* Sigma0; // B x Dx x Dx
-  double *A, *b, *Q, *H, *c, *R;  // the parameter layout of the inputs
-  double* cache;  // T x kC x B, kC = 4 N + 4 N^2 <= kGradCache
-  int B;
-};
 
 namespace {
 template <int N>
@@ -1662,22 +1602,33 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
   const size_t n_mu0 = (size_t)B * Dx, n_S0 = (size_t)B * nxx, n_A = (size_t)(nb * nxx),
                n_b = (size_t)(nb * Dx), n_H = (size_t)(nb * nyx), n_c = (size_t)(nb * Dy),
                n_R = (size_t)(nb * nyy), n_y = (size_t)B * T * Dy;
-  // with gradients: the 8 gradient arrays (the inputs' layout) and the per-step caches;
-  // the square-root KF: the final means and factors in the first two
+  // with gradients: the 8 gradient arrays (the inputs' layout) and the lane kernel's
+  // per-step caches; the square-root KF: the final means and factors in the first two
   const size_t n_grad = srkf ? n_mu0 + n_S0 : gout ? n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R : 0;
   const int n_pad = std::max(Dx, Dy);
+  // filters up to 4 x 4 run one per lane (inputs staged filter-minor); the reverse rule and
+  // the square-root filter of larger ones run a workgroup each on the caller's layout
+  const bool tiny = Dx <= kTinyD && Dy <= kTinyD;
+  const bool wg = (srkf || gout) && !tiny;
   const size_t n_cache =
-      (gout && !srkf) ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
+      (gout && !srkf && tiny) ? (size_t)T * (4 * n_pad + 4 * n_pad * n_pad) * B : 0;
   const size_t n_dbl = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y + (size_t)B +
                        n_grad + n_cache;
   const size_t bytes =
       (8 * n_dbl + 8 * (size_t)B /* y_off */ + 4 * 4 * (size_t)B + 256 + 255) / 256 * 256;
   // the filter-minor kernels' inputs arrive in the caller's layout behind the main block
   const size_t n_raw = n_mu0 + n_S0 + 2 * n_A + n_b + n_H + n_c + n_R + n_y;
-  const bool fm_in = srkf || gout || (Dx <= kTinyD && Dy <= kTinyD);
-  const size_t raw_bytes = (fm_in && !inputs_on_device) ? 8 * n_raw : 0;
+  const size_t raw_bytes = (tiny && !inputs_on_device) ? 8 * n_raw : 0;
   int rc = ensure_device(ctx, &ctx->kfll, &ctx->kfll_bytes, bytes + raw_bytes);
   if (rc) return rc;
+  // the workgroup kernels' workspace: filters in chunks of at most 2 GiB of it
+  const int64_t wg_per = !wg ? 0 : srkf ? srkf_wg_doubles(Dx, Dy) : kf_grad_wg_doubles(Dx, Dy, T);
+  const int wg_chunk =
+      !wg ? 0 : (int)std::min<int64_t>(B, std::max<int64_t>(1, (int64_t(1) << 28) / wg_per));
+  if (wg) {
+    rc = ensure_device(ctx, &ctx->kfwg, &ctx->kfwg_bytes, 8 * (size_t)wg_per * wg_chunk);
+    if (rc) return rc;
+  }
   double* d = (double*)ctx->kfll;
   double* d_mu0 = d;
   double* d_S0 = d_mu0 + n_mu0;
@@ -1717,7 +1668,7 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
   } guard{s};
   // The one-filter-per-lane kernels read their inputs filter-minor: the caller's layout is
   // uploaded unchanged behind the main block and transposed on the device.
-  const bool fm = srkf || gout || (Dx <= kTinyD && Dy <= kTinyD);
+  const bool fm = tiny;
   const int64_t nbf = time_varying ? T : 1;  // parameter blocks per filter
   double* raw = (double*)((char*)ctx->kfll + bytes);
   struct Up {
@@ -1790,7 +1741,13 @@ static int kfll_run(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t Dy, int32_t T,
       q += gout ? g_len[k] : 0;
     }
   }
-  if (srkf) {
+  if (wg) {
+    const KfGradOut g{g_dev[0], g_dev[1], g_dev[2], g_dev[3], g_dev[4], g_dev[5], g_dev[6],
+                      g_dev[7], nullptr, B};
+    rc = launch_kf_wg(ctx, s, p, srkf ? nullptr : &g, g_dev[0], g_dev[1], (double*)ctx->kfwg,
+                      wg_per, wg_chunk);
+    if (rc) return rc;
+  } else if (srkf) {
     const int n = std::max(Dx, Dy);
     auto fn = n == 1 ? k_srkf_tiny<1> : n == 2 ? k_srkf_tiny<2> : n == 3 ? k_srkf_tiny<3>
                                                                       : k_srkf_tiny<4>;
@@ -1865,10 +1822,10 @@ extern "C" int32_t ssm_kf_loglikelihood_grad(
     double jitter, double* ll_out, int32_t* status_out, double* g_mu0, double* g_Sigma0,
     double* g_A, double* g_b, double* g_Q, double* g_H, double* g_c, double* g_R) {
   if (!ctx) return SSM_ERR_ARG;
-  if (Dx < 1 || Dy < 1 || Dx > kTinyD || Dy > kTinyD)
+  if (Dx < 1 || Dy < 1 || Dx > kCfg[kClasses - 1].dx_max || Dy > kCfg[kClasses - 1].dy_max)
     return set_error(ctx, SSM_ERR_ARG,
-                     "ssm_kf_loglikelihood_grad: gradients need 1 <= Dx, Dy <= %d (got %d, %d)",
-                     kTinyD, Dx, Dy);
+                     "ssm_kf_loglikelihood_grad: gradients need 1 <= Dx <= %d, 1 <= Dy <= %d "
+                     "(got %d, %d)", kCfg[kClasses - 1].dx_max, kCfg[kClasses - 1].dy_max, Dx, Dy);
   double* const gout[8] = {g_mu0, g_Sigma0, g_A, g_b, g_Q, g_H, g_c, g_R};
   return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
                   ll_out, status_out, gout);
@@ -1882,10 +1839,10 @@ extern "C" int32_t ssm_srkf_filter(ssm_ctx* ctx, int32_t B, int32_t Dx, int32_t 
                                    double* ll_out, int32_t* status_out, double* mu_out,
                                    double* U_out) {
   if (!ctx) return SSM_ERR_ARG;
-  if (Dx < 1 || Dy < 1 || Dx > kTinyD || Dy > kTinyD)
+  if (Dx < 1 || Dy < 1 || Dx > kCfg[kClasses - 1].dx_max || Dy > kCfg[kClasses - 1].dy_max)
     return set_error(ctx, SSM_ERR_ARG,
-                     "ssm_srkf_filter: filters need 1 <= Dx, Dy <= %d (got %d, %d)", kTinyD,
-                     Dx, Dy);
+                     "ssm_srkf_filter: filters need 1 <= Dx <= %d, 1 <= Dy <= %d (got %d, %d)",
+                     kCfg[kClasses - 1].dx_max, kCfg[kClasses - 1].dy_max, Dx, Dy);
   double* const out[8] = {mu_out, U_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   return kfll_run(ctx, B, Dx, Dy, T, time_varying, mu0, Sigma0, A, b, Q, H, c, R, ys, jitter,
                   ll_out, status_out, out, true);

@@ -242,8 +242,8 @@ def _kfll_args(mu0, Sigma0, A, b, Q, H, c, R, ys):
 
 
 def kf_loglikelihood_grad_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=None):
-    """kf_loglikelihood and its reverse rule for B filters up to 4 x 4 (kalman_rrule.jl:22-107):
-    returns (ll (B,), grads) with grads["mu0"] (B, Dx), grads["Sigma0"] (B, Dx, Dx) and
+    """kf_loglikelihood and its reverse rule (kalman_rrule.jl:22-107) for B filters up to
+    Dx = 64, Dy = 32, in FP64: returns (ll (B,), grads) with grads["mu0"] (B, Dx), grads["Sigma0"] (B, Dx, Dx) and
     grads[k] for k in A, b, Q, H, c, R shaped like the inputs (per step, or summed over the
     steps for one parameter set per filter)."""
     ctx = ctx or default_context()
@@ -280,7 +280,7 @@ def SRKF():
 
 
 def srkf_filter_batch(mu0, Sigma0, A, b, Q, H, c, R, ys, jitter=None, ctx=None):
-    """filter(model, SRKF(), ys) for B filters of one shape up to 4 x 4 in one device call,
+    """filter(model, SRKF(), ys) for B filters of one shape up to Dx = 64, Dy = 32 in one call,
     arguments as kf_loglikelihood_batch.  Returns (ll (B,), mu (B, Dx), U (B, Dx, Dx)), U the
     final filtered covariance's upper Cholesky factor (Sigma = U' U)."""
     ctx = ctx or default_context()

@@ -1,6 +1,9 @@
 """Device square-root Kalman filter (GF/src/algorithms/srkf.jl:93-176) against the FP64
 oracle's restatement, itself pinned to the Joseph-form KF: both FP64, so log-likelihoods,
-final means and factors within 1e-9 relative (1e-11 absolute)."""
+final means and factors within 1e-9 relative (1e-11 absolute) up to 4 x 4 (one filter per
+lane, Givens rotations); beyond (a workgroup per filter, Householder QR, up to Dx = 64,
+Dy = 32) within 1e-8 relative plus 1e-10 of the array's largest entry.  Without jitter the
+device SRKF also equals the oracle's Joseph-form KF on the same filters."""
 import numpy as np
 import pytest
 
@@ -50,8 +53,43 @@ def test_filter_srkf_equals_kalman(ctx, orc):
     np.testing.assert_allclose(st.Sigma, S_k, rtol=1e-9, atol=1e-11)
 
 
+def _joseph_kf(orc, pars, ys):
+    mu0, S0, As, bs, Qs, Hs, cs, Rs = pars
+    mu, S = np.array(mu0, dtype=float), np.array(S0, dtype=float)
+    ll = 0.0
+    for t in range(len(As)):
+        mu, S, inc = orc.kf_step(len(mu), len(cs[t]), mu, S, As[t], bs[t], Qs[t], Hs[t], cs[t],
+                                 Rs[t], ys[t])
+        ll += inc
+    return ll, mu, S
+
+
+@pytest.mark.parametrize("Dx,Dy,T,B,jit", [(5, 2, 6, 9, None), (8, 4, 6, 7, None),
+                                           (16, 8, 5, 4, 0.01), (33, 17, 4, 3, None),
+                                           (64, 32, 3, 2, None), (3, 7, 5, 4, None)])
+def test_workgroup_srkf_matches_oracle_and_kalman(ctx, orc, Dx, Dy, T, B, jit):
+    rng = np.random.default_rng(3 * Dx + Dy)
+    pars, ys, arrs = _batch(rng, B, Dx, Dy, T)
+    ll, mu, U = ssmhip.srkf_filter_batch(*arrs, ys, jitter=jit, ctx=ctx)
+    assert np.allclose(U, np.triu(U))
+    for f in range(B):
+        ll_o, mu_o, U_o = orc.srkf_filter(*pars[f], ys[f], jitter=jit)
+        assert ll[f] == pytest.approx(ll_o, rel=1e-10)
+        np.testing.assert_allclose(mu[f], mu_o, rtol=1e-8, atol=1e-10 * np.abs(mu_o).max())
+        np.testing.assert_allclose(U[f], U_o, rtol=1e-8, atol=1e-10 * np.abs(U_o).max())
+        if jit is None:  # the Joseph-form KF on the same filter
+            ll_k, mu_k, S_k = _joseph_kf(orc, pars[f], ys[f])
+            assert ll[f] == pytest.approx(ll_k, rel=1e-9)
+            np.testing.assert_allclose(mu[f], mu_k, rtol=1e-7, atol=1e-9 * np.abs(mu_k).max())
+            np.testing.assert_allclose(U[f].T @ U[f], S_k, rtol=1e-7,
+                                       atol=1e-9 * np.abs(S_k).max())
+
+
 def test_srkf_refuses_large_filters(ctx):
     rng = np.random.default_rng(4)
-    _, ys, arrs = _batch(rng, 4, 5, 2, 3)
+    _, ys, arrs = _batch(rng, 2, 65, 2, 2)
+    with pytest.raises(ssmhip.ArgumentError):
+        ssmhip.srkf_filter_batch(*arrs, ys, ctx=ctx)
+    _, ys, arrs = _batch(rng, 2, 4, 33, 2)
     with pytest.raises(ssmhip.ArgumentError):
         ssmhip.srkf_filter_batch(*arrs, ys, ctx=ctx)
