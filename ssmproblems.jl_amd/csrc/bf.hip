@@ -35,9 +35,9 @@
 namespace ssm {
 
 // The fused predict + update + partial-reduction kernel: one block per tile.
-template <typename T, int DT, int DYT, bool NOISE, bool REF, bool GUIDED = false>
+template <typename T, int DT, int DYT, bool NOISE, bool REF, bool GUIDED = false, bool APF = false>
 __global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  bf_step_tile<T, DT, DYT, NOISE, REF, GUIDED>(p, blockIdx.x);
+  bf_step_tile<T, DT, DYT, NOISE, REF, GUIDED, APF>(p, blockIdx.x);
 }
 
 // Conditional SMC: ancestor-sampling log-weights for the reference state x_ref[t] at the
@@ -351,6 +351,24 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
         hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, true>), g, b, 0, s, p);
       else
         hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, true>), g, b, 0, s, p);
+    }
+    return;
+  }
+  if (bf->apf) {  // auxiliary particle filter (no reference, no proposal)
+    if (noise) {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, false, true>), g, b, 0, s, p);
+      else if (bf->D == 1 && bf->Dy == 1)
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, false, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, false, true>), g, b, 0, s, p);
+    } else {
+      if (bf->D == 4 && bf->Dy == 2)
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, false, true>), g, b, 0, s, p);
+      else if (bf->D == 1 && bf->Dy == 1)
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, false, true>), g, b, 0, s, p);
+      else
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, false, true>), g, b, 0, s, p);
     }
     return;
   }

@@ -304,16 +304,16 @@ __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>
 // REF (conditional SMC, GF/src/algorithms/particles.jl:59-66, resamplers.jl:51-65):
 // particle 0 takes the reference state x_ref[t] instead of a propagated one, and when the
 // step resamples its ancestor is *ref_anc (ancestor sampling) or particle 0.
-template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false, bool GUIDED = false>
+template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false, bool GUIDED = false,
+          bool APF = false>
 __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MD = DT ? DT : kMaxD;
   constexpr int MDY = DYT ? DYT : kMaxDy;
   const int D = DT ? DT : p.D;
   const int Dy = DYT ? DYT : p.Dy;
   BfScalars* sc = p.sc;
-  // the auxiliary filter decides on its look-ahead weights' ESS
-  const bool apf = p.aux != nullptr;
-  const bool rs = p.threshold_n >= (apf ? p.aux->ess : sc->ess);
+  // the auxiliary filter (APF: p.aux set) decides on its look-ahead weights' ESS
+  const bool rs = p.threshold_n >= (APF ? p.aux->ess : sc->ess);
   const bool fresh = rs || sc->typeless;  // pre-update log-weights 0 (or -log eta: apf)
   const T lse = (T)sc->lse;
   const int t = sc->step;
@@ -491,7 +491,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     }
     // a particle the auxiliary filter resampled starts from -log eta of its ancestor
     // (resample_ancestor, GF/src/resamplers.jl:140-143); `apf && rs` is block-uniform
-    [[maybe_unused]] const bool apf_rs = FRESH && apf && rs;
+    [[maybe_unused]] const bool apf_rs = FRESH && APF && rs;
     if constexpr (DT > 0) {
       // the normals while the ancestor (or log-weight) loads are in flight, then every
       // gather, then the arithmetic
@@ -505,7 +505,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
         for (int d = 0; d < MD; ++d) x[k][d] = p.xin[(size_t)d * p.Npad + src];
       }
-      if constexpr (FRESH && !GUIDED && !REF) {
+      if constexpr (FRESH && APF) {
         if (apf_rs) {
 #pragma unroll
           for (int k = 0; k < kStepItems; ++k)
@@ -522,7 +522,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         for (int d = 0; d < MD; ++d)
           if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
         normals(k, z);
-        if constexpr (FRESH && !GUIDED && !REF) {
+        if constexpr (FRESH && APF) {
           if (apf_rs) nl[k] = -apf_log_eta<T, MD, MDY>(mv, D, Dy, xk, y);
         }
         lnew[k] = compute_item(k, xk, z, nl[k]);
