@@ -420,7 +420,9 @@ int32_t ssm_serve_cancel(ssm_server* sv, int64_t id);
 /* Forgets a finished request (its result can no longer be polled). */
 int32_t ssm_serve_release(ssm_server* sv, int64_t id);
 /* out[0..n): completed, failed, cancelled, batched BF steps, Kalman groups, arena bytes in
- * use, peak arena bytes, peak concurrent BF filters, arena capacity, queued requests. */
+ * use, peak arena bytes, peak concurrent BF filters, arena capacity, queued requests, BF
+ * chunks run, BF chunk graphs captured, ms spent capturing them, ms from each chunk's
+ * launch to its retirement (summed). */
 int32_t ssm_serve_stats(ssm_server* sv, double* out, int32_t n);
 /* Cancels what is queued or running, joins the workers and frees the arena. */
 int32_t ssm_serve_destroy(ssm_server* sv);

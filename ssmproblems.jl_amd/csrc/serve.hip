@@ -17,9 +17,13 @@
 // filter (a tile-to-filter map routes each workgroup), so the launch count per step does
 // not grow with the number of filters.  Each workgroup runs exactly the per-tile code of
 // the single-filter kernels (bf_dev.h, resample_dev.h, pf_common.h), so a request's
-// result does not depend on what else is in the batch.  The lane runs chunks of steps up
-// to the next completion (at most kMaxChunk), retires finished filters, and admits new
-// ones between chunks.
+// result does not depend on what else is in the batch.  The lane runs chunks of steps
+// replayed from HIP graphs: a chunk lasts until the soonest-finishing filter is done but
+// at least kMinChunk steps (a filter past its last step is skipped on the device, so it
+// idles for the rest of the chunk) and at most kMaxChunk; the graph of a chunk reads the
+// descriptor tables, so it is captured once per (tiles, filters, multinomial, steps) and
+// replayed for every active set of that shape.  Finished filters retire and new ones are
+// admitted between chunks.
 //
 // Kalman lane.  Waiting Kalman requests are packed into one group (ragged horizons) and
 // run by the persistent size-class kernels of kf.hip on the lane's own stream, so the
@@ -33,6 +37,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -43,10 +48,13 @@ namespace ssm {
 
 constexpr int kServeMaxN = 1 << 18;  // one finish block per filter (<= 256 tiles)
 constexpr int kMaxChunk = 64;        // batched steps between admission points
+constexpr int kMinChunk = 16;        // ... at least
+constexpr int kMaxGraphs = 64;       // captured chunks kept by the lane
 
 // One active bootstrap filter as the batched kernels see it.
 struct BfbFilter {
   int N, Npad, D, Dy, nb, kind, tile_base;
+  int t_last;  // the filter's last step: past it every batched kernel skips the filter
   double threshold_n, logN;
   uint64_t seed;
   float* x[2];  // state planes; step t reads x[(t-1)&1] and writes x[t&1]
@@ -85,25 +93,35 @@ __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
   return p;
 }
 
+// The batched kernels run over a grid padded to a power of two of tiles (owner -1) and of
+// filters (index >= *nf), so one captured chunk serves every active set of that size.
 __global__ void __launch_bounds__(kTileThreads)
     k_bfb_resample(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ owner) {
-  const BfbFilter& F = fs[owner[blockIdx.x]];
+  const int o = owner[blockIdx.x];
+  if (o < 0) return;
+  const BfbFilter& F = fs[o];
+  if (F.sc->step > F.t_last) return;  // finished inside this chunk
   const RsParams p = bfb_rs_params(F);
   scan_offspring_tile<kWeightsLogF32, -1, true>(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
     k_bfb_multinomial(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ owner) {
-  const BfbFilter& F = fs[owner[blockIdx.x]];
-  if (F.kind != SSM_MULTINOMIAL) return;
+  const int o = owner[blockIdx.x];
+  if (o < 0) return;
+  const BfbFilter& F = fs[o];
+  if (F.kind != SSM_MULTINOMIAL || F.sc->step > F.t_last) return;
   const RsParams p = bfb_rs_params(F);
   multinomial_search_block(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
     k_bfb_step(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ owner) {
-  const BfbFilter& F = fs[owner[blockIdx.x]];
+  const int o = owner[blockIdx.x];
+  if (o < 0) return;
+  const BfbFilter& F = fs[o];
   const int t = F.sc->step;
+  if (t > F.t_last) return;
   StepParams<float> p;
   p.N = F.N;
   p.Npad = F.Npad;
@@ -132,8 +150,12 @@ __global__ void __launch_bounds__(kTileThreads)
     bf_step_tile<float, 0, 0>(p, tile);
 }
 
-__global__ void __launch_bounds__(kTileThreads) k_bfb_finish(const BfbFilter* __restrict__ fs) {
+__global__ void __launch_bounds__(kTileThreads)
+    k_bfb_finish(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ nf) {
+  if ((int)blockIdx.x >= *nf) return;
   const BfbFilter& F = fs[blockIdx.x];
+  // every thread reads the step before the block's barriers, thread 0 writes it after
+  if (F.sc->step > F.t_last) return;
   pf_finish_block<float, 1, kTileThreads>(F.N, F.nb, F.sc, F.sc, F.threshold_n, F.logN, F.ts,
                                           nullptr, nullptr, 1);
 }
@@ -303,16 +325,29 @@ struct ssm_server {
   // statistics
   int64_t n_done = 0, n_failed = 0, n_cancelled = 0, bf_steps = 0, kf_groups = 0;
   int peak_active = 0;
+  int64_t bf_chunks = 0, bf_captures = 0;
+  double bf_capture_ms = 0.0, bf_chunk_ms = 0.0;
   // bootstrap-filter lane device tables (descriptors and tile owners) and host staging
   BfbFilter* d_desc = nullptr;
   int32_t* d_owner = nullptr;
+  int32_t* d_nf = nullptr;  // active filters (the finish grid is padded beyond them)
   BfbFilter* h_desc = nullptr;
   int32_t* h_owner = nullptr;
   double* h_ll = nullptr;
   int owner_cap = 0;
+  // captured chunks of batched steps, keyed by (padded tiles, padded filters, multinomial,
+  // steps): the kernels read the descriptor tables, so one graph serves every active set
+  // that pads to the same shape
+  std::map<std::tuple<int, int, bool, int>, hipGraphExec_t> graphs;
 };
 
 namespace {
+
+int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -332,6 +367,7 @@ int bf_setup(ssm_server* sv, SvReq* r) {
   F.nb = r->nb;
   F.kind = r->resampler;
   F.tile_base = 0;
+  F.t_last = r->T;
   F.threshold_n = r->threshold * (double)r->N;
   F.logN = std::log((double)r->N);
   F.seed = r->seed;
@@ -389,7 +425,7 @@ void bf_lane(ssm_server* sv) {
   hipStream_t s = ctx->stream;
   std::vector<SvReq*> active;
   bool tables_dirty = false;
-  int total_tiles = 0;
+  int total_tiles = 0, tiles_pad = 0;
   bool any_multinomial = false;
   while (true) {
     std::vector<SvReq*> admitted;
@@ -446,30 +482,73 @@ void bf_lane(ssm_server* sv) {
         total_tiles += r->nb;
         any_multinomial = any_multinomial || r->resampler == SSM_MULTINOMIAL;
       }
+      tiles_pad = pow2_at_least(total_tiles);
+      for (int b = total_tiles; b < tiles_pad; ++b) sv->h_owner[b] = -1;
+      sv->h_owner[sv->owner_cap] = (int32_t)active.size();  // staged copy of *d_nf
       rc = hipMemcpyAsync(sv->d_desc, sv->h_desc, sizeof(BfbFilter) * active.size(),
                           hipMemcpyHostToDevice, s) == hipSuccess &&
-                   hipMemcpyAsync(sv->d_owner, sv->h_owner, sizeof(int32_t) * total_tiles,
+                   hipMemcpyAsync(sv->d_owner, sv->h_owner, sizeof(int32_t) * tiles_pad,
+                                  hipMemcpyHostToDevice, s) == hipSuccess &&
+                   hipMemcpyAsync(sv->d_nf, sv->h_owner + sv->owner_cap, sizeof(int32_t),
                                   hipMemcpyHostToDevice, s) == hipSuccess
                ? SSM_OK
                : SSM_ERR_HIP;
       tables_dirty = false;
     }
     int k = 0;
+    double chunk_capture_ms = 0.0;
+    Clock::time_point chunk_start = Clock::now();
     if (rc == SSM_OK && !active.empty()) {
-      k = kMaxChunk;
-      for (SvReq* r : active) k = std::min(k, r->T - r->done);
-      k = std::max(k, 1);
-      const int nf = (int)active.size();
-      for (int step = 0; step < k; ++step) {
-        hipLaunchKernelGGL(k_bfb_resample, dim3(total_tiles), dim3(kTileThreads), 0, s,
-                           sv->d_desc, sv->d_owner);
-        if (any_multinomial)
-          hipLaunchKernelGGL(k_bfb_multinomial, dim3(total_tiles), dim3(kTileThreads), 0, s,
-                             sv->d_desc, sv->d_owner);
-        hipLaunchKernelGGL(k_bfb_step, dim3(total_tiles), dim3(kTileThreads), 0, s, sv->d_desc,
-                           sv->d_owner);
-        hipLaunchKernelGGL(k_bfb_finish, dim3(nf), dim3(kTileThreads), 0, s, sv->d_desc);
+      // The chunk runs until the soonest-finishing filter is done, but at least kMinChunk
+      // steps (filters finishing earlier skip the rest on the device) and at most
+      // kMaxChunk or what the longest needs.
+      int kmin = kMaxChunk, kmax = 1;
+      for (SvReq* r : active) {
+        kmin = std::min(kmin, r->T - r->done);
+        kmax = std::max(kmax, r->T - r->done);
       }
+      // chunk lengths 16, 32 or 64 (a filter past its last step idles to the chunk's end)
+      k = kmin >= kMaxChunk ? kMaxChunk : kmin >= 2 * kMinChunk ? 2 * kMinChunk : kMinChunk;
+      const int nf = (int)active.size();
+      const int nf_pad = pow2_at_least(nf);
+      const auto key = std::make_tuple(tiles_pad, nf_pad, any_multinomial, k);
+      auto it = sv->graphs.find(key);
+      hipGraphExec_t exec = nullptr;
+      const auto tc0 = Clock::now();
+      bool captured = false;
+      if (it != sv->graphs.end()) {
+        exec = it->second;
+      } else {
+        captured = true;
+        if ((int)sv->graphs.size() >= kMaxGraphs) {
+          for (auto& kv : sv->graphs) (void)hipGraphExecDestroy(kv.second);
+          sv->graphs.clear();
+        }
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        for (int step = 0; ok && step < k; ++step) {
+          hipLaunchKernelGGL(k_bfb_resample, dim3(tiles_pad), dim3(kTileThreads), 0, s,
+                             sv->d_desc, sv->d_owner);
+          if (any_multinomial)
+            hipLaunchKernelGGL(k_bfb_multinomial, dim3(tiles_pad), dim3(kTileThreads), 0, s,
+                               sv->d_desc, sv->d_owner);
+          hipLaunchKernelGGL(k_bfb_step, dim3(tiles_pad), dim3(kTileThreads), 0, s,
+                             sv->d_desc, sv->d_owner);
+          hipLaunchKernelGGL(k_bfb_finish, dim3(nf_pad), dim3(kTileThreads), 0, s, sv->d_desc,
+                             sv->d_nf);
+        }
+        ok = (hipStreamEndCapture(s, &g) == hipSuccess) && ok;
+        ok = ok && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (ok)
+          sv->graphs[key] = exec;
+        else
+          rc = SSM_ERR_HIP;
+      }
+      const auto tc1 = Clock::now();
+      if (rc == SSM_OK && hipGraphLaunch(exec, s) != hipSuccess) rc = SSM_ERR_HIP;
+      chunk_capture_ms = captured ? ms_between(tc0, tc1) : 0.0;
+      chunk_start = tc1;
       for (int f = 0; f < nf; ++f) {
         SvReq* r = active[f];
         if (r->done + k >= r->T && rc == SSM_OK)
@@ -482,14 +561,23 @@ void bf_lane(ssm_server* sv) {
         rc = SSM_ERR_HIP;
     }
     // retire finished, cancelled or (on a device error) every active filter
+    const double chunk_ms = ms_between(chunk_start, Clock::now());
     std::lock_guard<std::mutex> lk(sv->m);
     sv->bf_steps += k;
+    if (k > 0) {
+      ++sv->bf_chunks;
+      sv->bf_chunk_ms += chunk_ms;
+      if (chunk_capture_ms > 0.0) {
+        ++sv->bf_captures;
+        sv->bf_capture_ms += chunk_capture_ms;
+      }
+    }
     sv->peak_active = std::max(sv->peak_active, (int)active.size());
     if (rc != SSM_OK) sv->lane_error = "bootstrap-filter lane: HIP failure";
     std::vector<SvReq*> keep;
     for (size_t f = 0; f < active.size(); ++f) {
       SvReq* r = active[f];
-      r->done += k;
+      r->done = std::min(r->T, r->done + k);
       if (rc != SSM_OK) {
         sv->arena.release(r->off, r->bytes);
         finish_req(sv, r, SSM_SERVE_FAILED, rc, 0.0);
@@ -628,8 +716,11 @@ void kf_lane(ssm_server* sv) {
 }
 
 void server_free(ssm_server* sv) {
+  for (auto& kv : sv->graphs) (void)hipGraphExecDestroy(kv.second);
+  sv->graphs.clear();
   if (sv->d_desc) (void)hipFree(sv->d_desc);
   if (sv->d_owner) (void)hipFree(sv->d_owner);
+  if (sv->d_nf) (void)hipFree(sv->d_nf);
   if (sv->h_desc) (void)hipHostFree(sv->h_desc);
   if (sv->h_owner) (void)hipHostFree(sv->h_owner);
   if (sv->h_ll) (void)hipHostFree(sv->h_ll);
@@ -654,15 +745,16 @@ extern "C" int32_t ssm_serve_create(ssm_ctx* ctx, uint64_t hbm_budget_bytes, int
   sv->user_ctx = ctx;
   sv->device = ctx->device;
   sv->max_active = max_active;
-  sv->owner_cap = max_active * (kServeMaxN / kTile);
+  sv->owner_cap = pow2_at_least(max_active * (kServeMaxN / kTile));  // padded grids fit
   int rc = ssm_ctx_create(ctx->device, 0, &sv->bf_ctx);
   if (rc == SSM_OK) rc = ssm_ctx_create(ctx->device, 0, &sv->kf_ctx);
   const size_t cap = (size_t)hbm_budget_bytes / 256 * 256;
   bool ok = rc == SSM_OK && hipMalloc(&sv->arena_base, cap) == hipSuccess &&
             hipMalloc(&sv->d_desc, sizeof(BfbFilter) * max_active) == hipSuccess &&
             hipMalloc(&sv->d_owner, sizeof(int32_t) * sv->owner_cap) == hipSuccess &&
+            hipMalloc(&sv->d_nf, sizeof(int32_t) * 64) == hipSuccess &&
             hipHostMalloc(&sv->h_desc, sizeof(BfbFilter) * max_active) == hipSuccess &&
-            hipHostMalloc(&sv->h_owner, sizeof(int32_t) * sv->owner_cap) == hipSuccess &&
+            hipHostMalloc(&sv->h_owner, sizeof(int32_t) * (sv->owner_cap + 1)) == hipSuccess &&
             hipHostMalloc(&sv->h_ll, sizeof(double) * max_active) == hipSuccess;
   if (!ok) {
     server_free(sv);
@@ -810,7 +902,9 @@ extern "C" int32_t ssm_serve_stats(ssm_server* sv, double* out, int32_t n) {
                       (double)sv->n_cancelled,  (double)sv->bf_steps,
                       (double)sv->kf_groups,    (double)sv->arena.used(),
                       (double)sv->arena.peak(), (double)sv->peak_active,
-                      (double)sv->arena.cap(),  (double)(sv->bf_queue.size() + sv->kf_queue.size())};
+                      (double)sv->arena.cap(),  (double)(sv->bf_queue.size() + sv->kf_queue.size()),
+                      (double)sv->bf_chunks,    (double)sv->bf_captures,
+                      sv->bf_capture_ms,        sv->bf_chunk_ms};
   const int k = std::min<int>(n, (int)(sizeof v / sizeof v[0]));
   for (int i = 0; i < k; ++i) out[i] = v[i];
   return SSM_OK;

@@ -24,7 +24,8 @@ from .models import PackedLG, StateSpaceModel
 QUEUED, RUNNING, DONE, FAILED, CANCELLED = 0, 1, 2, 3, 4
 REQ_BF, REQ_KF = 0, 1
 STATS_FIELDS = ("completed", "failed", "cancelled", "bf_batched_steps", "kf_groups",
-                "bytes_in_use", "peak_bytes", "peak_active_bf", "arena_bytes", "queued")
+                "bytes_in_use", "peak_bytes", "peak_active_bf", "arena_bytes", "queued",
+                "bf_chunks", "bf_graph_captures", "bf_capture_ms", "bf_chunk_ms")
 
 
 @dataclass
