@@ -467,16 +467,49 @@ __device__ __forceinline__ int node_rank(const double* __restrict__ node, double
   return r;
 }
 
+// Sum of v over each aligned team of 8 lanes, in every lane of the team: quad exchanges
+// (xor 1, xor 2), then the other quad of the half-row (row_half_mirror).
+__device__ __forceinline__ int team8_sum(int v) {
+  v += dpp_i<0xB1>(0, v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_i<0x4E>(0, v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_i<0x141>(0, v);  // row_half_mirror
+  return v;
+}
+
+// node_rank for a team of 8 lanes: lane tl holds entries 2 tl and 2 tl + 1 of the node (one
+// 16-byte load; the team's loads together are the node's line), counts those of the first
+// L - 1 below q, and the team sums the counts.
+template <int L>
+__device__ __forceinline__ int team_node_rank(const double* __restrict__ node, double q,
+                                              int tl) {
+  static_assert(L == 16 || L == 4, "16- and 4-entry nodes");
+  int c = 0;
+  if (2 * tl < L) {
+    const double2 v = reinterpret_cast<const double2*>(node)[tl];
+    c = ((v.x < q) ? 1 : 0) + ((2 * tl + 1 < L - 1 && v.y < q) ? 1 : 0);
+  }
+  return team8_sum(c);
+}
+
 // Multinomial: a_k = min{j : W_j >= u_k * W_n} for the queries k of block `blk`
-// (blk * kTile + 256 i + t, i < 4), with W the nondecreasing cumulative weights the scan
-// wrote to Wbuf.  The tile comes from a branch-free lower-bound search over the tiles'
-// closing prefixes (its top level -- every 2^sh-th close, at most kCoarse of them -- sits
-// in LDS); within the tile a three-level tree of 16-way nodes (Wroot: the closes of the 64-
-// position groups, Wmid: of the 4-position groups, then Wbuf) costs three dependent vector
-// loads of one cache line or less.  Each node's last entry is at least the query, so the
-// result equals a search of the whole Wbuf.  All threads of the block call it.
+// (blk * kTile + j, j < 1024), with W the nondecreasing cumulative weights the scan wrote
+// to Wbuf.  Each lane first finds the tiles of 4 queries by a branch-free lower-bound
+// search over the tiles' closing prefixes (its top level -- every 2^sh-th close, at most
+// kCoarse of them -- sits in LDS) and leaves (q, tile) in LDS for its wave.  Within the
+// tile a three-level tree of 16-way nodes (Wroot: the closes of the 64-position groups,
+// Wmid: of the 4-position groups, then Wbuf) needs three dependent node loads; each node's
+// last entry is at least the query, so the result equals a search of the whole Wbuf.  The
+// queries are random, so in a one-lane-per-query search every lane of a load touches its
+// own cache line and the texture units' per-line lookups bound the kernel; here a team of
+// 8 lanes walks the tree for one query, each lane loading 16 bytes of the node, so a
+// wave's load instruction touches 8 lines instead of 64.  A wave's 256 queries go in
+// rounds of kTeamBatch per team, their loads in flight together.  All threads of the
+// block call it.
+constexpr int kTeamBatch = 8;
 __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int blk) {
   __shared__ double coarse[kCoarse];
+  __shared__ double qs[kTile];
+  __shared__ int tiles[kTile];
   if (!rs_active(p)) return;
   const int n = p.n, nb = (n + kTile - 1) / kTile, t = threadIdx.x;
   const uint32_t step = rs_step(p);
@@ -488,44 +521,54 @@ __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int 
   for (int i = t; i < ns; i += kRsThreads)
     coarse[i] = tile_close(p, min((i + 1) << sh, nb) - 1, nb, typeless, Wn);
   __syncthreads();
-  // two queries at a time: their node loads are in flight together
+  const int w = t / kWave, lane = t & (kWave - 1), tm = lane >> 3, tl = lane & 7;
+  constexpr int kPerWave = kTile / (kRsThreads / kWave), kTeams = kWave / 8;
+  // the tiles of this wave's queries j = 256 w + 64 i + lane, two in flight at a time
 #pragma unroll
   for (int i0 = 0; i0 < kTileItems; i0 += 2) {
-    int k[2], bs[2];
-    double q[2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      k[j] = blk * kTile + (i0 + j) * kRsThreads + t;
-      const int kc = min(k[j], n - 1);
+    for (int i = i0; i < i0 + 2; ++i) {
+      const int j = w * kPerWave + i * kWave + lane;
+      const int kc = min(blk * kTile + j, n - 1);
       const double uk = p.u ? p.u[kc] : resample_uniform(rs_seed(p), step, (uint32_t)kc);
-      q[j] = uk * Wn;
+      const double q = uk * Wn;
       int c = 0;
 #pragma unroll
       for (int h = kCoarse / 2; h >= 1; h >>= 1)
-        if (c + h <= ns - 1 && coarse[c + h - 1] < q[j]) c += h;
+        if (c + h <= ns - 1 && coarse[c + h - 1] < q) c += h;
       const int g0 = c << sh, gl = min(g0 + (1 << sh), nb) - g0;
       int bt = 0;
       for (int h = (1 << sh) >> 1; h >= 1; h >>= 1)
-        if (bt + h <= gl - 1 && tile_close(p, g0 + bt + h - 1, nb, typeless, Wn) < q[j])
-          bt += h;
-      bs[j] = g0 + bt;
+        if (bt + h <= gl - 1 && tile_close(p, g0 + bt + h - 1, nb, typeless, Wn) < q) bt += h;
+      qs[j] = q;
+      tiles[j] = g0 + bt;
     }
-    int g1[2], g2[2], e[2];
+  }
+  wave_lds_sync();  // each wave reads back only its own queries
+  for (int r0 = 0; r0 < kPerWave / kTeams; r0 += kTeamBatch) {
+    int jj[kTeamBatch], bs[kTeamBatch], g1[kTeamBatch], g2[kTeamBatch];
+    double q[kTeamBatch];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      g1[j] = node_rank<kRootPerTile>(p.Wroot + (size_t)bs[j] * kRootPerTile, q[j]);
+    for (int b = 0; b < kTeamBatch; ++b) {
+      jj[b] = w * kPerWave + (r0 + b) * kTeams + tm;
+      q[b] = qs[jj[b]];
+      bs[b] = tiles[jj[b]];
+    }
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      g2[j] = node_rank<kRootGroup / kMidGroup>(
-          p.Wmid + (size_t)bs[j] * kMidPerTile + g1[j] * (kRootGroup / kMidGroup), q[j]);
+    for (int b = 0; b < kTeamBatch; ++b)
+      g1[b] = team_node_rank<kRootPerTile>(p.Wroot + (size_t)bs[b] * kRootPerTile, q[b], tl);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      e[j] = node_rank<kMidGroup>(
-          p.Wbuf + (size_t)bs[j] * kTile + g1[j] * kRootGroup + g2[j] * kMidGroup, q[j]);
+    for (int b = 0; b < kTeamBatch; ++b)
+      g2[b] = team_node_rank<kRootGroup / kMidGroup>(
+          p.Wmid + (size_t)bs[b] * kMidPerTile + g1[b] * (kRootGroup / kMidGroup), q[b], tl);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-      if (k[j] < n)
-        p.anc[k[j]] = min(bs[j] * kTile + g1[j] * kRootGroup + g2[j] * kMidGroup + e[j], n - 1);
+    for (int b = 0; b < kTeamBatch; ++b) {
+      const int e = team_node_rank<kMidGroup>(
+          p.Wbuf + (size_t)bs[b] * kTile + g1[b] * kRootGroup + g2[b] * kMidGroup, q[b], tl);
+      const int k = blk * kTile + jj[b];
+      if (tl == 0 && k < n)
+        p.anc[k] = min(bs[b] * kTile + g1[b] * kRootGroup + g2[b] * kMidGroup + e, n - 1);
+    }
   }
 }
 
