@@ -456,7 +456,7 @@ int32_t ssm_kf_batch_last_timing(ssm_kf_batch* kb, double* ms);
  * filters for T_tune steps, and keeps the fastest for every later run in the process (the
  * server's Kalman groups included).  nw_out (3): the chosen waves per workgroup of the
  * classes Dx <= 16, <= 32, <= 64; ms_out (9, optional): the time of each candidate
- * (NaN for an empty class).
+ * (NaN for an empty class or an unused candidate slot).
  */
 int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* nw_out, double* ms_out);
 int32_t ssm_kf_batch_destroy(ssm_kf_batch* kb);

@@ -11,16 +11,19 @@
 //
 // Design.  Filters are sorted by cost (Dx^3) and split into three size classes; each class
 // runs one persistent kernel whose workgroups pull filters from an atomic work queue and
-// keep one filter's whole state (Sigma, A, Q, H, R and temporaries) in LDS for all T
-// steps, so HBM carries only the parameters once and the observations.  Dimensions are
-// padded to multiples of 16 and every dense product runs on the matrix cores as
-// v_mfma_f32_16x16x4f32 tiles spread over the workgroup's waves (FP32 in, FP32
-// accumulate).  The innovation covariance S is factorised in FP64 (Cholesky, log-det and
-// the solves for K and S^-1 z) by one wave.
+// keep one filter's state (Sigma, H, R and the step's temporaries) in LDS and its A and Q
+// in registers for all T steps, so HBM carries only the parameters once and the
+// observations.  Dimensions are padded to multiples of 16 and every dense product runs on
+// the matrix cores as v_mfma_f32_16x16x4f32 tiles spread over the workgroup's waves (FP32
+// in, FP32 accumulate).  The innovation covariance S is inverted in FP64 by one wave
+// (Gauss-Jordan; its pivots give the log-determinant), and K = P' S^-1 is a matrix-core
+// product.  The Joseph form is evaluated as T2 + K U with T2 = Sigma_p (I - K H)' and
+// U = R K' - H T2, the same expression associated so that no Dx x Dx product with I - K H
+// is formed; only its upper tiles are computed, as Symmetric(:U) keeps the upper triangle.
 //   class    Dx      Dy     waves/WG  LDS/WG
-//   small    <= 16   <= 16  1         ~13 KB
-//   mid      <= 32   <= 16  2         ~33 KB
-//   large    <= 64   <= 32  4         ~127 KB
+//   small    <= 16   <= 16  1         ~9 KB
+//   mid      <= 32   <= 16  2         ~19 KB
+//   large    <= 64   <= 32  4         ~74 KB (two workgroups a CU)
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -34,86 +37,203 @@
 
 namespace ssm {
 
+// LDS matrices are row-major with LD columns and no padding; the 16-byte column group
+// c / 4 of row r is XOR-ed with r mod min(LD / 4, 16).  A lane reading four consecutive
+// columns of its row (one ds_read_b128) and the transposed write of an MFMA output tile
+// then spread a wave's rows over distinct banks.
+template <int LD>
+__device__ __forceinline__ int swz(int r, int c) {
+  constexpr int M = (LD / 4 > 16 ? 16 : LD / 4) - 1;
+  return r * LD + (c ^ ((r & M) << 2));
+}
+
 template <int DXP, int DYP>
 struct KfLds {
-  static constexpr int LX = DXP + 1, LY = DYP + 1;
-  static constexpr int kScal = 5 * DXP * LX + 3 * DYP * LX + DYP * LY + DXP * LY + 3 * DXP +
-                               3 * DYP;  // float offset of the scalar block
-  static_assert(kScal % 2 == 0, "scalar block must be 8-byte aligned");
-  static constexpr int kFloats = kScal + 16;
-  // FP64 block: the Cholesky factor (ld DYP + 1), the pivot column (2 DYP) and 1 / L_jj
-  static constexpr size_t kBytes =
-      ((size_t)kFloats * 4 + 7) / 8 * 8 + ((size_t)DYP * (DYP + 1) + 3 * DYP) * 8;
+  static constexpr int kB0 = 0;                // Sigma, then Sigma_p^T, then the new Sigma
+  static constexpr int kB1 = kB0 + DXP * DXP;  // A Sigma, then T2^T
+  static constexpr int kH = kB1 + DXP * DXP;   // H (DYP x DXP)
+  static constexpr int kP = kH + DYP * DXP;    // P = H Sigma_p (DYP x DXP)
+  static constexpr int kPT = kP + DYP * DXP;   // P^T, then U^T (DXP x DYP)
+  static constexpr int kK = kPT + DXP * DYP;   // K (DXP x DYP)
+  static constexpr int kR = kK + DXP * DYP;    // R (DYP x DYP)
+  static constexpr int kS = kR + DYP * DYP;    // S, then S^-1
+  static constexpr int kMu = kS + DYP * DYP;   // vectors: mu, mu_hat, b (DXP); c, z, y (DYP)
+  static constexpr int kMuh = kMu + DXP, kBv = kMuh + DXP, kCv = kBv + DXP, kZ = kCv + DYP,
+                       kY = kZ + DYP;
+  static constexpr int kScal = kY + DYP;       // 16 words: queue index, status
+  static constexpr int kGJ = kScal + 16;       // the elimination's row and column buffers,
+                                               // two of each (4 DYP doubles)
+  static constexpr int kFloats = kGJ + 8 * DYP;
+  static_assert(kGJ % 4 == 0, "FP64 buffers must be 16-byte aligned");
+  static constexpr size_t kBytes = (size_t)kFloats * 4;
 };
 
-// Loads one parameter block (A, b, Q, H, c, R, column-major FP64) into the zero-padded
-// row-major FP32 LDS matrices.
-__device__ __forceinline__ void kf_load_block(int dx, int dy, int dxp, int dyp, int LX, int LY,
-                                              const double* g_A, const double* g_b,
-                                              const double* g_Q, const double* g_H,
-                                              const double* g_c, const double* g_R, float* Am,
-                                              float* Qm, float* Hm, float* Rm, float* bv,
-                                              float* cv, int tid, int nthr) {
-  for (int e = tid; e < dxp * dxp; e += nthr) {
-    const int i = e / dxp, j = e % dxp;
-    const bool in = i < dx && j < dx;
-    Am[i * LX + j] = in ? (float)g_A[i + j * dx] : 0.f;
-    Qm[i * LX + j] = in ? (float)g_Q[i + j * dx] : 0.f;
+__device__ __forceinline__ f4v mfma4(f4v a, f4v b, f4v acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], acc, 0, 0, 0);
+  return acc;
+}
+
+// The 16 x 16 tile at (r0, c0) of X Y over k < K, X row-major (LDX columns) and Y given as
+// Y^T row-major (LDY columns).  MFMA u of the k-block k0 takes k = k0 + 4 lk + u in lane
+// (li, lk), so each operand is one 16-byte read per lane and block.
+// MFMA 16x16x4 f32 lane layout: A[l%16][l/16], B[l/16][l%16] in, C[4(l/16)+r][l%16] out.
+template <int LDX, int LDY>
+__device__ __forceinline__ f4v kf_mm(const float* X, const float* YT, int r0, int c0, int K, int li, int lk) {
+  constexpr int KB = (LDX < LDY ? LDX : LDY) / 16;
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    if (16 * kb < K) {
+      const f4v a = *reinterpret_cast<const f4v*>(X + swz<LDX>(r0 + li, 16 * kb + 4 * lk));
+      const f4v b = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c0 + li, 16 * kb + 4 * lk));
+      acc = mfma4(a, b, acc);
+    }
   }
-  for (int e = tid; e < dyp * dxp; e += nthr) {
-    const int i = e / dxp, j = e % dxp;
-    Hm[i * LX + j] = (i < dy && j < dx) ? (float)g_H[i + j * dy] : 0.f;
+  return acc;
+}
+
+// Stores an output tile at (r0, c0): as is (lane writes its 4 rows of one column), or
+// transposed into C^T (one 16-byte write per lane).
+template <int LD>
+__device__ __forceinline__ void kf_store(float* C, int r0, int c0, f4v v, int li, int lk) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) C[swz<LD>(r0 + 4 * lk + e, c0 + li)] = v[e];
+}
+template <int LD>
+__device__ __forceinline__ void kf_store_t(float* CT, int r0, int c0, f4v v, int li, int lk) {
+  *reinterpret_cast<f4v*>(CT + swz<LD>(c0 + li, r0 + 4 * lk)) = v;
+}
+// The lane's 4 entries of the tile at (r0, c0) of a matrix stored transposed.
+template <int LD>
+__device__ __forceinline__ f4v kf_load_t(const float* CT, int r0, int c0, int li, int lk) {
+  return *reinterpret_cast<const f4v*>(CT + swz<LD>(c0 + li, r0 + 4 * lk));
+}
+
+// One wave: S^-1 of the dy x dy symmetric positive definite matrix S (FP32, swizzled with
+// LD = DYP), written over S (the upper triangle, mirrored; zeros past dy up to dyp), by
+// Gauss-Jordan elimination without pivoting in FP64.  Lane l holds row i = l % DYP,
+// columns E h .. E h + E - 1 (h = l / DYP, E = DYP^2 / 64) in registers.  Step k scales
+// row k by 1 / a_kk, hands row k and column k to the wave through LDS (two buffers,
+// alternating) and updates every entry: a_ij -= a_ik a_kj / a_kk, a_ik = -a_ik / a_kk,
+// a_kk = 1 / a_kk.  The pivots are those of the LDL' factorisation, all positive iff S is
+// positive definite; log det S is the sum of their logarithms (a fixed-order DPP sum).
+// *quad = z' S^-1 z from the FP64 inverse.  Returns 0 if a pivot is not positive.
+template <int DYP>
+__device__ __forceinline__ int gj_inverse_wave(float* S, int dy, int dyp, const float* zv,
+                                               double* buf, double* logdet, double* quad) {
+  static_assert(DYP == 16 || DYP == 32, "DYP must be 16 or 32");
+  constexpr int G = 64 / DYP, E = DYP / G;
+  const int lane = threadIdx.x & 63, i = lane % DYP, h = lane / DYP;
+  double a[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = E * h + e;
+    a[e] = (i < dy && j < dy) ? (double)S[swz<DYP>(i, j)] : 0.0;
   }
-  for (int e = tid; e < dyp * dyp; e += nthr) {
-    const int i = e / dyp, j = e % dyp;
-    Rm[i * LY + j] = (i < dy && j < dy) ? (float)g_R[i + j * dy] : 0.f;
+  double piv = 1.0;
+  bool good = true;
+  static_for<0, DYP>([&](auto KK) {
+    constexpr int k = decltype(KK)::value, ek = k % E, hk = k / E;
+    if (k < dy) {  // uniform
+      double* rowb = buf + (k & 1) * 2 * DYP;
+      double* colb = rowb + DYP;
+      const double pk = readlane_d(a[ek], k + DYP * hk);
+      good = good && (pk > 0.0);
+      const double pp = pk > 0.0 ? pk : 1.0;
+      // 1 / pp: the hardware reciprocal and two Newton steps (error below 1 ulp)
+      double r = __builtin_amdgcn_rcp(pp);
+      r = fma(r, fma(-pp, r, 1.0), r);
+      r = fma(r, fma(-pp, r, 1.0), r);
+      piv = (lane == k) ? pk : piv;
+      if (i == k) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) rowb[E * h + e] = a[e] * r;
+      }
+      if (h == hk) colb[i] = a[ek];
+      wave_lds_sync();
+      const double cik = colb[i];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double rk = rowb[E * h + e];
+        a[e] = (i == k) ? rk : fma(-cik, rk, a[e]);
+      }
+      const double colk = (i == k) ? r : -cik * r;
+      a[ek] = (h == hk) ? colk : a[ek];
+    }
+  });
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = E * h + e;
+    if (i < dyp && j < dyp && j >= i) {
+      const float v = (float)a[e];
+      S[swz<DYP>(i, j)] = v;
+      S[swz<DYP>(j, i)] = v;
+    }
   }
-  for (int e = tid; e < dxp; e += nthr) bv[e] = e < dx ? (float)g_b[e] : 0.f;
-  for (int e = tid; e < dyp; e += nthr) cv[e] = e < dy ? (float)g_c[e] : 0.f;
+  double w = 0.0;  // (S^-1 z)_i, this lane's columns
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = E * h + e;
+    w = fma(a[e], j < dy ? (double)zv[j] : 0.0, w);
+  }
+  if constexpr (G == 4) w += __shfl_xor(w, 16, 64);
+  w += __shfl_xor(w, 32, 64);
+  const double zi = (h == 0 && i < dy) ? (double)zv[i] : 0.0;
+  *quad = wave_sum_dpp(zi * w);
+  const double lg = (lane < dy && good) ? log(piv) : 0.0;
+  *logdet = wave_sum_dpp(lg);
+  return good ? 1 : 0;
 }
 
 // TV: the kf_loglikelihood layout (separate arrays, one parameter block per step or one
 // for all steps); otherwise each filter's packed homogeneous parameters.
+// Register budget: the large class (74 KB of LDS, two workgroups a CU) at 2 waves a SIMD
+// (256 VGPRs: the FP64 elimination keeps 16 doubles a lane), the others at 4.
 template <int DXP, int DYP, int NW, bool TV>
-__global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(DXP == 64 ? 2 : 4)))
+k_kf_batch(KfBatchParams p) {
   using L = KfLds<DXP, DYP>;
-  constexpr int LX = L::LX, LY = L::LY;
+  constexpr int TX = DXP / 16;  // tiles per padded state dimension, at most
+  static_assert(NW >= TX, "a row block of A per wave");
+  constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) float lds[];
-  float* Sig = lds;
-  float* Am = Sig + DXP * LX;
-  float* Qm = Am + DXP * LX;
-  float* T1 = Qm + DXP * LX;
-  float* Mm = T1 + DXP * LX;
-  float* Hm = Mm + DXP * LX;
-  float* Pm = Hm + DYP * LX;
-  float* KT = Pm + DYP * LX;
-  float* Rm = KT + DYP * LX;
-  float* KR = Rm + DYP * LY;
-  float* mu = KR + DXP * LY;
-  float* muh = mu + DXP;
-  float* bv = muh + DXP;
-  float* cv = bv + DXP;
-  float* zv = cv + DYP;
-  float* yv = zv + DYP;
-  float* scal = yv + DYP;  // [0] quad, [1] logdet, [2] ok
-  int* iscal = reinterpret_cast<int*>(scal + 8);
-  double* Sd = reinterpret_cast<double*>(lds + ((L::kFloats + 1) & ~1));
-  double* colbuf = Sd + DYP * (DYP + 1);
-  double* invd = colbuf + 2 * DYP;
+  float* const B0 = lds + L::kB0;
+  float* const B1 = lds + L::kB1;
+  float* const Hm = lds + L::kH;
+  float* const Pm = lds + L::kP;
+  float* const PT = lds + L::kPT;
+  float* const Km = lds + L::kK;
+  float* const Rm = lds + L::kR;
+  float* const Sm = lds + L::kS;
+  float* const mu = lds + L::kMu;
+  float* const muh = lds + L::kMuh;
+  float* const bv = lds + L::kBv;
+  float* const cv = lds + L::kCv;
+  float* const zv = lds + L::kZ;
+  float* const yv = lds + L::kY;
+  int* const iscal = reinterpret_cast<int*>(lds + L::kScal);
+  double* const gjbuf = reinterpret_cast<double*>(lds + L::kGJ);
 
-  const int tid = threadIdx.x, nthr = 64 * NW;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li0 = lane & 15, lk0 = lane >> 4;
   while (true) {
+    const int li = li0, lk = lk0;
     if (tid == 0) iscal[0] = (int)atomicAdd(p.counter, 1u);
     __syncthreads();
     const int qi = iscal[0];
     if (qi >= p.count) break;  // every wave of the workgroup reads the same index
     const int f = p.order[qi];
-    const int dx = p.Dx[f], dy = p.Dy[f];
-    const int dxp = (dx + 15) & ~15, dyp = (dy + 15) & ~15;
-    // ---- load parameters (column-major FP64) into zero-padded row-major FP32 LDS ----
+    const int dx0 = p.Dx[f], dy0 = p.Dy[f];
+    const int dxp0 = (dx0 + 15) & ~15, dyp0 = (dy0 + 15) & ~15;
+    const int tn0 = dxp0 >> 4, tny0 = dyp0 >> 4;
+    const int dx = dx0, dy = dy0, dxp = dxp0, dyp = dyp0, tn = tn0;
+    // This wave's row block rb of A: it computes A Sigma's tiles (rb, c) and Sigma_p's
+    // tiles (r, rb) for c, r = jw, jw + nj, ... (nj waves share the row block).
+    const int rb = wave % tn, jw = wave / tn, nj = (NW - rb + tn - 1) / tn;
     const int64_t nxx = (int64_t)dx * dx, nyx = (int64_t)dy * dx, nyy = (int64_t)dy * dy;
-    const double *g_mu0, *g_S0;
     // block b of the TV arrays, or the packed block
     auto block = [&](int64_t b, const double** gA, const double** gb, const double** gQ,
                      const double** gH, const double** gc, const double** gR) {
@@ -135,6 +255,36 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
         *gR = g + 2 * nxx + dx + nyx + dy;
       }
     };
+    // areg[kb][u] = A[16 rb + li][16 kb + 4 lk + u] (the operand layout of kf_mm, which
+    // serves A Sigma as the left operand and T1 A' as the right one); Q is read from the
+    // parameter block (L2) where Sigma_p's tiles add it.
+    f4v areg[TX];
+    const double* gQ_cur = nullptr;
+    // Loads one parameter block (column-major FP64) into registers and zero-padded LDS.
+    auto load_params = [&](int64_t b) {
+      const double *gA, *gb, *gQ, *gH, *gc, *gR;
+      block(b, &gA, &gb, &gQ, &gH, &gc, &gR);
+      gQ_cur = gQ;
+      const int ar = 16 * rb + li;
+#pragma unroll
+      for (int kb = 0; kb < TX; ++kb)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int ac = 16 * kb + 4 * lk + u;
+          areg[kb][u] = (ar < dx && ac < dx) ? (float)gA[ar + (int64_t)ac * dx] : 0.f;
+        }
+      for (int e = tid; e < dyp * dxp; e += NT) {
+        const int i = e / dxp, j = e % dxp;
+        Hm[swz<DXP>(i, j)] = (i < dy && j < dx) ? (float)gH[i + (int64_t)j * dy] : 0.f;
+      }
+      for (int e = tid; e < dyp * dyp; e += NT) {
+        const int i = e / dyp, j = e % dyp;
+        Rm[swz<DYP>(i, j)] = (i < dy && j < dy) ? (float)gR[i + (int64_t)j * dy] : 0.f;
+      }
+      for (int e = tid; e < dxp; e += NT) bv[e] = e < dx ? (float)gb[e] : 0.f;
+      for (int e = tid; e < dyp; e += NT) cv[e] = e < dy ? (float)gc[e] : 0.f;
+    };
+    const double *g_mu0, *g_S0;
     if constexpr (TV) {
       g_mu0 = p.tv_mu0 + (int64_t)f * dx;
       g_S0 = p.tv_Sigma0 + (int64_t)f * nxx;
@@ -143,17 +293,14 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
       g_S0 = g_mu0 + dx;
     }
     const int64_t blk0 = TV ? (int64_t)f * p.tv_blocks : 0;
-    {
-      const double *gA, *gb, *gQ, *gH, *gc, *gR;
-      block(blk0, &gA, &gb, &gQ, &gH, &gc, &gR);
-      kf_load_block(dx, dy, dxp, dyp, LX, LY, gA, gb, gQ, gH, gc, gR, Am, Qm, Hm, Rm, bv, cv,
-                    tid, nthr);
-    }
-    for (int e = tid; e < dxp * dxp; e += nthr) {
+    load_params(blk0);
+    // Sigma0 from its upper triangle, as Symmetric(:U)
+    for (int e = tid; e < dxp * dxp; e += NT) {
       const int i = e / dxp, j = e % dxp;
-      Sig[i * LX + j] = (i < dx && j < dx) ? (float)g_S0[i + j * dx] : 0.f;
+      const int lo = i < j ? i : j, hi = i < j ? j : i;
+      B0[swz<DXP>(i, j)] = (hi < dx) ? (float)g_S0[lo + (int64_t)hi * dx] : 0.f;
     }
-    for (int e = tid; e < dxp; e += nthr) mu[e] = e < dx ? (float)g_mu0[e] : 0.f;
+    for (int e = tid; e < dxp; e += NT) mu[e] = e < dx ? (float)g_mu0[e] : 0.f;
     __syncthreads();
     const float jit = (float)p.jitter;
     double ll = 0.0;
@@ -161,58 +308,108 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
     const double* yrow = p.ys + p.y_off[f];
     const int Tn = p.Tf ? p.Tf[f] : p.T;
     for (int t = 0; t < Tn; ++t) {
+      // The lane and shape indices pass through empty asm each step, so the compiler
+      // recomputes the step's LDS addresses instead of keeping them all live across the
+      // loop (which took the kernel past its register budget).
+      int li = li0, lk = lk0;
+      asm volatile("" : "+v"(li), "+v"(lk));
+      int dx = dx0, dy = dy0, dxp = dxp0, dyp = dyp0, tn = tn0, tny = tny0;
+      asm volatile("" : "+s"(dx), "+s"(dy), "+s"(dxp), "+s"(dyp), "+s"(tn), "+s"(tny));
       if constexpr (TV) {
         if (p.tv_blocks > 1 && t > 0) {  // this step's parameters (the last step ended synced)
-          const double *gA, *gb, *gQ, *gH, *gc, *gR;
-          block(blk0 + t, &gA, &gb, &gQ, &gH, &gc, &gR);
-          kf_load_block(dx, dy, dxp, dyp, LX, LY, gA, gb, gQ, gH, gc, gR, Am, Qm, Hm, Rm, bv,
-                        cv, tid, nthr);
+          load_params(blk0 + t);
           __syncthreads();
         }
       }
-      // P1: mu_hat = A mu + b ; T1 = A Sigma
-      // matrix-vector products run over the zero padding up to dxp (exact zeros), 16 terms
-      // per unrolled block
-      for (int i = tid; i < dxp; i += nthr) {
-        float s = bv[i];
-        for (int k0 = 0; k0 < dxp; k0 += 16) {
+      // P1: T1 = A Sigma (Sigma symmetric, so its rows serve as Sigma^T's);
+      // mu_hat = A mu + b; y_t
+      for (int c = jw; c < tn; c += nj) {
+        f4v acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int u = 0; u < 16; ++u) s += Am[i * LX + k0 + u] * mu[k0 + u];
+        for (int kb = 0; kb < TX; ++kb) {
+          if (kb < tn) {
+            const f4v bb =
+                *reinterpret_cast<const f4v*>(B0 + swz<DXP>(16 * c + li, 16 * kb + 4 * lk));
+            acc = mfma4(areg[kb], bb, acc);
+          }
         }
-        muh[i] = (i < dx) ? s : 0.f;
+        kf_store<DXP>(B1, 16 * rb, 16 * c, acc, li, lk);
       }
-      for (int q = tid; q < dyp; q += nthr) yv[q] = q < dy ? (float)yrow[(size_t)t * dy + q] : 0.f;
-      gemm16<NW>(T1, LX, dxp, dxp, 1.f, Am, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
-                 nullptr, 0, false, 0, nullptr, 0, 0);
-      __syncthreads();
-      // P2: Sigma = T1 A' + Q ; z = y - H mu_hat - c
-      gemm16<NW>(Sig, LX, dxp, dxp, 1.f, T1, LX, false, Am, LX, true, dxp, nullptr, 0, false,
-                 nullptr, 0, false, 0, Qm, LX, 0);
-      for (int q = tid; q < dyp; q += nthr) {
-        float s = cv[q];
-        for (int k0 = 0; k0 < dxp; k0 += 16) {
+      if (jw == 0) {
+        float s = 0.f;
 #pragma unroll
-          for (int u = 0; u < 16; ++u) s += Hm[q * LX + k0 + u] * muh[k0 + u];
+        for (int kb = 0; kb < TX; ++kb) {
+          if (kb < tn) {
+            const f4v m4 = *reinterpret_cast<const f4v*>(mu + 16 * kb + 4 * lk);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) s += areg[kb][u] * m4[u];
+          }
+        }
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        const int i = 16 * rb + li;
+        if (lk == 0) muh[i] = (i < dx) ? s + bv[i] : 0.f;
+      }
+      for (int q = tid; q < dyp; q += NT) yv[q] = q < dy ? (float)yrow[(size_t)t * dy + q] : 0.f;
+      __syncthreads();
+      // P2: Sigma_p = T1 A' + Q, stored as Sigma_p^T over Sigma; z = y - H mu_hat - c
+#pragma unroll
+      for (int m = 0; m < TX; ++m) {
+        const int r = jw + m * nj;
+        if (r < tn) {
+          f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kb = 0; kb < TX; ++kb) {
+            if (kb < tn) {
+              const f4v aa =
+                  *reinterpret_cast<const f4v*>(B1 + swz<DXP>(16 * r + li, 16 * kb + 4 * lk));
+              acc = mfma4(aa, areg[kb], acc);
+            }
+          }
+          const int qc = 16 * rb + li;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int qr = 16 * r + 4 * lk + e;
+            acc[e] += (qr < dx && qc < dx) ? (float)gQ_cur[qr + (int64_t)qc * dx] : 0.f;
+          }
+          kf_store_t<DXP>(B0, 16 * r, 16 * rb, acc, li, lk);
+        }
+      }
+      for (int q = tid; q < dyp; q += NT) {
+        float s = cv[q];
+        for (int k = 0; k < dxp; k += 4) {
+          const f4v h4 = *reinterpret_cast<const f4v*>(Hm + swz<DXP>(q, k));
+          const f4v m4 = *reinterpret_cast<const f4v*>(muh + k);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) s += h4[u] * m4[u];
         }
         zv[q] = (q < dy) ? yv[q] - s : 0.f;
       }
       __syncthreads();
-      // P3: P = H Sigma (dyp x dxp)
-      gemm16<NW>(Pm, LX, dyp, dxp, 1.f, Hm, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
-                 nullptr, 0, false, 0, nullptr, 0, 0);
+      // P3: P^T = Sigma_p^T H' (dxp x dyp), kept as is and transposed (P = H Sigma_p)
+      for (int tile = wave; tile < tn * tny; tile += NW) {
+        const int r = tile / tny, c = tile % tny;
+        const f4v acc = kf_mm<DXP, DXP>(B0, Hm, 16 * r, 16 * c, dxp, li, lk);
+        kf_store<DYP>(PT, 16 * r, 16 * c, acc, li, lk);
+        kf_store_t<DXP>(Pm, 16 * r, 16 * c, acc, li, lk);
+      }
       __syncthreads();
-      // P4: S = P H' + R (into KR as scratch, dyp x dyp)
-      gemm16<NW>(KR, LY, dyp, dyp, 1.f, Pm, LX, false, Hm, LX, true, dxp, nullptr, 0, false,
-                 nullptr, 0, false, 0, Rm, LY, 0);
+      // P4: S = P H' + R
+      for (int tile = wave; tile < tny * tny; tile += NW) {
+        const int r = tile / tny, c = tile % tny;
+        f4v acc = kf_mm<DXP, DXP>(Pm, Hm, 16 * r, 16 * c, dxp, li, lk);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] += Rm[swz<DYP>(16 * r + 4 * lk + e, 16 * c + li)];
+        kf_store<DYP>(Sm, 16 * r, 16 * c, acc, li, lk);
+      }
       __syncthreads();
-      // P5: FP64 Cholesky of S and log-det by wave 0; the solves K' = S^-1 P and S^-1 z
-      // one column per thread over the whole workgroup.
+      // P5: S^-1, log det S and z' S^-1 z in FP64 on wave 0
       if (wave == 0) {
-        double ldet = 0.0;
-        const int good = chol_factor_wave<DYP>(KR, LY, dy, Sd, DYP + 1, colbuf, invd, &ldet);
+        double ldet, qd;
+        const int good = gj_inverse_wave<DYP>(Sm, dy, dyp, zv, gjbuf, &ldet, &qd);
         if (lane == 0) {
           iscal[1] = good;
-          reinterpret_cast<double*>(scal)[3] = ldet;  // scal[6..7]
+          ll += -0.5 * ((double)dy * kLog2Pi + ldet + qd);
         }
       }
       __syncthreads();
@@ -220,49 +417,62 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
         ok = 0;
         break;
       }
-      {
-        constexpr int LPC = DYP / 8;  // lanes per right-hand side
-        const int ncol = dx + 1 > dxp ? dx + 1 : dxp;
-        for (int c0 = 0; c0 < ncol; c0 += nthr / LPC) {  // whole groups in every pass
-          const int col = c0 + tid / LPC, r = tid % LPC;
-          const double qd = solve_group<DYP>(col < ncol ? col : dxp, r, Sd, DYP + 1, invd, Pm,
-                                             LX, zv, KT, LX, dx, dxp, dy, dyp);
-          if (col == dx && r == 0) reinterpret_cast<double*>(scal)[2] = qd;  // scal[4..5]
-        }
+      // P6: K = P^T S^-1 (dxp x dyp)
+      for (int tile = wave; tile < tn * tny; tile += NW) {
+        const int r = tile / tny, c = tile % tny;
+        kf_store<DYP>(Km, 16 * r, 16 * c, kf_mm<DYP, DYP>(PT, Sm, 16 * r, 16 * c, dyp, li, lk), li, lk);
       }
       __syncthreads();
-      if (tid == 0) {
-        const double qd = reinterpret_cast<double*>(scal)[2];
-        const double ld = reinterpret_cast<double*>(scal)[3];
-        ll += -0.5 * ((double)dy * kLog2Pi + ld + qd);
+      // P7: T2 = Sigma_p (I - K H)' = Sigma_p - P^T K', stored as T2^T over T1;
+      // mu = mu_hat + K z
+      for (int tile = wave; tile < tn * tn; tile += NW) {
+        const int r = tile / tn, c = tile % tn;
+        const f4v pk = kf_mm<DYP, DYP>(PT, Km, 16 * r, 16 * c, dyp, li, lk);
+        kf_store_t<DXP>(B1, 16 * r, 16 * c, kf_load_t<DXP>(B0, 16 * r, 16 * c, li, lk) - pk, li, lk);
       }
-      // P6: mu = mu_hat + K z ; M = I - K H ; KR = K R
-      for (int i = tid; i < dxp; i += nthr) {
-        float s = muh[i];
-        for (int q0 = 0; q0 < dyp; q0 += 16) {
+      for (int i = tid; i < dxp; i += NT) {
+        float s = 0.f;
+        for (int q = 0; q < dyp; q += 4) {
+          const f4v k4 = *reinterpret_cast<const f4v*>(Km + swz<DYP>(i, q));
+          const f4v z4 = *reinterpret_cast<const f4v*>(zv + q);
 #pragma unroll
-          for (int u = 0; u < 16; ++u) s += KT[(q0 + u) * LX + i] * zv[q0 + u];
+          for (int u = 0; u < 4; ++u) s += k4[u] * z4[u];
         }
-        mu[i] = (i < dx) ? s : 0.f;
+        mu[i] = (i < dx) ? muh[i] + s : 0.f;
       }
-      gemm16<NW>(Mm, LX, dxp, dxp, -1.f, KT, LX, true, Hm, LX, false, dyp, nullptr, 0, false,
-                 nullptr, 0, false, 0, nullptr, 0, dx);
-      gemm16<NW>(KR, LY, dxp, dyp, 1.f, KT, LX, true, Rm, LY, false, dyp, nullptr, 0, false,
-                 nullptr, 0, false, 0, nullptr, 0, 0);
       __syncthreads();
-      // P7: T1 = M Sigma
-      gemm16<NW>(T1, LX, dxp, dxp, 1.f, Mm, LX, false, Sig, LX, false, dxp, nullptr, 0, false,
-                 nullptr, 0, false, 0, nullptr, 0, 0);
+      // P8: U = R K' - H T2 (dyp x dxp), stored as U^T over P^T
+      for (int tile = wave; tile < tny * tn; tile += NW) {
+        const int r = tile / tn, c = tile % tn;
+        const f4v ht = kf_mm<DXP, DXP>(Hm, B1, 16 * r, 16 * c, dxp, li, lk);
+        const f4v rk = kf_mm<DYP, DYP>(Rm, Km, 16 * r, 16 * c, dyp, li, lk);
+        kf_store_t<DYP>(PT, 16 * r, 16 * c, rk - ht, li, lk);
+      }
       __syncthreads();
-      // P8: Sigma = T1 M' + (K R) K'
-      gemm16<NW>(Sig, LX, dxp, dxp, 1.f, T1, LX, false, Mm, LX, true, dxp, KR, LY, false, KT,
-                 LX, false, dyp, nullptr, 0, 0);
-      __syncthreads();
-      // P9: Symmetric(Sigma + jitter I) keeps the upper triangle (kalman.jl:61-66)
-      for (int e = tid; e < DXP * DXP; e += nthr) {
-        const int i = e / DXP, j = e % DXP;
-        if (i > j && i < dxp) Sig[i * LX + j] = Sig[j * LX + i];  // j < i < dxp
-        if (i == j && i < dx) Sig[i * LX + i] += jit;
+      // P9: Sigma = Symmetric(:U)(T2 + K U) + jitter I: the upper tiles, mirrored
+      for (int tile = wave; tile < tn * (tn + 1) / 2; tile += NW) {
+        int r = 0, c = tile;
+        while (c >= tn - r) {
+          c -= tn - r;
+          ++r;
+        }
+        c += r;
+        const f4v v = kf_load_t<DXP>(B1, 16 * r, 16 * c, li, lk) + kf_mm<DYP, DYP>(Km, PT, 16 * r, 16 * c, dyp, li, lk);
+        if (r < c) {
+          kf_store<DXP>(B0, 16 * r, 16 * c, v, li, lk);
+          kf_store_t<DXP>(B0, 16 * r, 16 * c, v, li, lk);
+        } else {
+          const int j = 16 * c + li;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int i = 16 * r + 4 * lk + e;
+            if (i <= j) {
+              const float w = v[e] + ((i == j && i < dx) ? jit : 0.f);
+              B0[swz<DXP>(i, j)] = w;
+              B0[swz<DXP>(j, i)] = w;
+            }
+          }
+        }
       }
       __syncthreads();
     }
@@ -272,10 +482,10 @@ __global__ void __launch_bounds__(64 * NW) k_kf_batch(KfBatchParams p) {
     }
     if (p.mu_out && ok) {
       const int64_t mo = p.mu_off[f], co = p.cov_off[f];
-      for (int i = tid; i < dx; i += nthr) p.mu_out[mo + i] = mu[i];
-      for (int e = tid; e < dx * dx; e += nthr) {
+      for (int i = tid; i < dx; i += NT) p.mu_out[mo + i] = mu[i];
+      for (int e = tid; e < dx * dx; e += NT) {
         const int i = e % dx, j = e / dx;  // column-major output
-        p.Sigma_out[co + e] = Sig[i * LX + j];
+        p.Sigma_out[co + e] = B0[swz<DXP>(i, j)];
       }
     }
     __syncthreads();
@@ -634,7 +844,6 @@ __global__ void __launch_bounds__(kTinyThreads) k_kf_tiny_grad(KfBatchParams p, 
   int ok = 1;
   // ---- forward (kalman_rrule.jl:27-50), caching each step ----
   for (int t = 0; t < Tn && ok; ++t) {
-    const int64_t k = blk(t);
     MN A, Q, H, R;
     double b[N], c[N];
     loadM(p.tv_A, t, nxx, dx, dx, A);
@@ -1210,7 +1419,8 @@ int launch_class(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int grid_c
 
 // Waves per workgroup of each size class: candidates the autotuner times, the default
 // before tuning, and the process-wide tuned choice (0 = not tuned).
-constexpr int kNwCand[kClasses][3] = {{1, 2, 4}, {2, 4, 8}, {4, 8, 16}};
+// (0: no candidate; 16 waves of the large class would leave 128 VGPRs a lane)
+constexpr int kNwCand[kClasses][3] = {{1, 2, 4}, {2, 4, 8}, {4, 8, 0}};
 constexpr int kNwDefault[kClasses] = {1, 2, 4};
 std::atomic<int> g_nw_tuned[kClasses] = {{0}, {0}, {0}};
 
@@ -1229,7 +1439,6 @@ int launch_class_nw(ssm_ctx* ctx, hipStream_t s, const KfBatchParams& p, int c, 
     case 108: return launch_class<32, 16, 8>(ctx, s, p, 0);
     case 204: return launch_class<64, 32, 4>(ctx, s, p, 0);
     case 208: return launch_class<64, 32, 8>(ctx, s, p, 0);
-    case 216: return launch_class<64, 32, 16>(ctx, s, p, 0);
     default: break;
   }
   return set_error(ctx, SSM_ERR_ARG, "no Kalman kernel for class %d with %d waves", c, nw);
@@ -1520,6 +1729,7 @@ extern "C" int32_t ssm_kf_batch_tune(ssm_kf_batch* kb, int32_t T_tune, int32_t* 
     int best_nw = kNwDefault[c];
     for (int k = 0; k < 3 && rc == SSM_OK; ++k) {
       const int nw = kNwCand[c][k];
+      if (nw == 0) continue;
       double t_min = 1e300;
       for (int rep = 0; rep < 3 && rc == SSM_OK; ++rep) {  // first launch warms up
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, kb->dcounter, 64);

@@ -114,7 +114,7 @@ class KFBatch:
         return ms.value
 
     CLASSES = ("Dx<=16", "Dx<=32", "Dx<=64")
-    CANDIDATES = ((1, 2, 4), (2, 4, 8), (4, 8, 16))
+    CANDIDATES = ((1, 2, 4), (2, 4, 8), (4, 8))
 
     def tune(self, T_tune=8):
         """Times each workgroup shape of each size class on this batch (T_tune steps) and
