@@ -61,10 +61,7 @@ struct KfLds {
   static constexpr int kMuh = kMu + DXP, kBv = kMuh + DXP, kCv = kBv + DXP, kZ = kCv + DYP,
                        kY = kZ + DYP;
   static constexpr int kScal = kY + DYP;       // 16 words: queue index, status
-  static constexpr int kGJ = kScal + 16;       // the elimination's row and column buffers,
-                                               // two of each (4 DYP doubles)
-  static constexpr int kFloats = kGJ + 8 * DYP;
-  static_assert(kGJ % 4 == 0, "FP64 buffers must be 16-byte aligned");
+  static constexpr int kFloats = kScal + 16;
   static constexpr size_t kBytes = (size_t)kFloats * 4;
 };
 
@@ -83,16 +80,77 @@ __device__ __forceinline__ f4v mfma4(f4v a, f4v b, f4v acc) {
 template <int LDX, int LDY>
 __device__ __forceinline__ f4v kf_mm(const float* X, const float* YT, int r0, int c0, int K, int li, int lk) {
   constexpr int KB = (LDX < LDY ? LDX : LDY) / 16;
-  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  f4v a[KB], b[KB];  // every block's operands are loaded before the first product
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     if (16 * kb < K) {
-      const f4v a = *reinterpret_cast<const f4v*>(X + swz<LDX>(r0 + li, 16 * kb + 4 * lk));
-      const f4v b = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c0 + li, 16 * kb + 4 * lk));
-      acc = mfma4(a, b, acc);
+      a[kb] = *reinterpret_cast<const f4v*>(X + swz<LDX>(r0 + li, 16 * kb + 4 * lk));
+      b[kb] = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c0 + li, 16 * kb + 4 * lk));
     }
   }
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb)
+    if (16 * kb < K) acc = mfma4(a[kb], b[kb], acc);
   return acc;
+}
+
+// Two tiles of X Y at (r0, c0) and (r1, c1), their matrix-core chains interleaved so each
+// hides the other's dependent-issue latency.
+template <int LDX, int LDY>
+__device__ __forceinline__ void kf_mm2(const float* X, const float* YT, int r0, int c0, int r1,
+                                       int c1, int K, int li, int lk, f4v& acc0, f4v& acc1) {
+  constexpr int KB = (LDX < LDY ? LDX : LDY) / 16;
+  // operands of block kb + 1 are loaded while block kb's products issue
+  f4v a0 = *reinterpret_cast<const f4v*>(X + swz<LDX>(r0 + li, 4 * lk));
+  f4v b0 = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c0 + li, 4 * lk));
+  f4v a1 = *reinterpret_cast<const f4v*>(X + swz<LDX>(r1 + li, 4 * lk));
+  f4v b1 = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c1 + li, 4 * lk));
+  acc0 = f4v{0.f, 0.f, 0.f, 0.f};
+  acc1 = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    if (16 * kb < K) {
+      f4v na0 = a0, nb0 = b0, na1 = a1, nb1 = b1;
+      if (kb + 1 < KB && 16 * (kb + 1) < K) {
+        na0 = *reinterpret_cast<const f4v*>(X + swz<LDX>(r0 + li, 16 * (kb + 1) + 4 * lk));
+        nb0 = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c0 + li, 16 * (kb + 1) + 4 * lk));
+        na1 = *reinterpret_cast<const f4v*>(X + swz<LDX>(r1 + li, 16 * (kb + 1) + 4 * lk));
+        nb1 = *reinterpret_cast<const f4v*>(YT + swz<LDY>(c1 + li, 16 * (kb + 1) + 4 * lk));
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[u], b0[u], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[u], b1[u], acc1, 0, 0, 0);
+      }
+      a0 = na0;
+      b0 = nb0;
+      a1 = na1;
+      b1 = nb1;
+    }
+  }
+}
+
+// The tiles t = first, first + stride, ... < count of X Y, two at a time; tile(t, r0, c0)
+// names tile t's origin and out(r0, c0, acc) consumes its product.
+template <int LDX, int LDY, typename TileF, typename OutF>
+__device__ __forceinline__ void kf_mm_tiles(const float* X, const float* YT, int K, int first,
+                                            int stride, int count, int li, int lk, TileF tile,
+                                            OutF out) {
+  for (int t = first; t < count; t += 2 * stride) {
+    int r0, c0;
+    tile(t, r0, c0);
+    if (t + stride < count) {
+      int r1, c1;
+      tile(t + stride, r1, c1);
+      f4v acc0, acc1;
+      kf_mm2<LDX, LDY>(X, YT, r0, c0, r1, c1, K, li, lk, acc0, acc1);
+      out(r0, c0, acc0);
+      out(r1, c1, acc1);
+    } else {
+      out(r0, c0, kf_mm<LDX, LDY>(X, YT, r0, c0, K, li, lk));
+    }
+  }
 }
 
 // Stores an output tile at (r0, c0): as is (lane writes its 4 rows of one column), or
@@ -112,92 +170,156 @@ __device__ __forceinline__ f4v kf_load_t(const float* CT, int r0, int c0, int li
   return *reinterpret_cast<const f4v*>(CT + swz<LD>(c0 + li, r0 + 4 * lk));
 }
 
-// One wave: S^-1 of the dy x dy symmetric positive definite matrix S (FP32, swizzled with
-// LD = DYP), written over S (the upper triangle, mirrored; zeros past dy up to dyp), by
-// Gauss-Jordan elimination without pivoting in FP64.  Lane l holds row i = l % DYP,
-// columns E h .. E h + E - 1 (h = l / DYP, E = DYP^2 / 64) in registers.  Step k scales
-// row k by 1 / a_kk, hands row k and column k to the wave through LDS (two buffers,
-// alternating) and updates every entry: a_ij -= a_ik a_kj / a_kk, a_ik = -a_ik / a_kk,
-// a_kk = 1 / a_kk.  The pivots are those of the LDL' factorisation, all positive iff S is
-// positive definite; log det S is the sum of their logarithms (a fixed-order DPP sum).
-// *quad = z' S^-1 z from the FP64 inverse.  Returns 0 if a pivot is not positive.
-template <int DYP>
-__device__ __forceinline__ int gj_inverse_wave(float* S, int dy, int dyp, const float* zv,
-                                               double* buf, double* logdet, double* quad) {
-  static_assert(DYP == 16 || DYP == 32, "DYP must be 16 or 32");
-  constexpr int G = 64 / DYP, E = DYP / G;
-  const int lane = threadIdx.x & 63, i = lane % DYP, h = lane / DYP;
-  double a[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int j = E * h + e;
-    a[e] = (i < dy && j < dy) ? (double)S[swz<DYP>(i, j)] : 0.0;
-  }
-  double piv = 1.0;
+// ---- FP64 inverse of the innovation covariance (one wave) -----------------------------
+// 16 x 16 blocks are held "column-strided": lane l has row i = l % 16, columns h + 4 e
+// (h = l / 16, e = 0..3), which is the output layout of v_mfma_f64_16x16x4f64.
+
+// Lane S of each row of 16 lanes, broadcast to that row (DPP row_newbcast).
+template <int S>
+__device__ __forceinline__ double row_bcast16(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + S, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + S, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// Row R of 16 lanes broadcast to all four rows, lane for lane: permlane16_swap of v with
+// itself gives rows (0, 0, 2, 2) and (1, 1, 3, 3), and permlane32_swap of the one holding
+// row R with itself gives (R, R, R, R).
+template <int R>
+__device__ __forceinline__ uint32_t rows_bcast(uint32_t v) {
+  const auto s16 = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  const uint32_t w = (R & 1) ? s16[1] : s16[0];
+  const auto s32 = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+  return (R & 2) ? s32[1] : s32[0];
+}
+template <int R>
+__device__ __forceinline__ double rows_bcast_d(double v) {
+  return __hiloint2double((int)rows_bcast<R>((uint32_t)__double2hiint(v)),
+                          (int)rows_bcast<R>((uint32_t)__double2loint(v)));
+}
+
+// One wave: in-place Gauss-Jordan inverse in FP64 of the leading n x n block of a 16 x 16
+// symmetric positive definite matrix M (column-strided), without pivoting.  Step k: the
+// pivot p = a_kk (readlane), row k to each row of 16 lanes (DPP), column k to the four rows
+// (permlane swaps), then a_ij -= a_ik a_kj / p, a_kj = a_kj / p, a_ik = -a_ik / p,
+// a_kk = 1 / p (row k's lanes start from zero, so one update form serves every row).  Rows
+// and columns >= n are zero and stay zero.  The pivots are those of the LDL' factorisation,
+// all positive iff M is positive definite; lane k keeps pivot k in *piv.  Returns false if
+// one is not.
+__device__ __forceinline__ bool gj16(double a[4], int n, double* piv) {
+  const int lane = threadIdx.x & 63, i = lane & 15, h = lane >> 4;
   bool good = true;
-  static_for<0, DYP>([&](auto KK) {
-    constexpr int k = decltype(KK)::value, ek = k % E, hk = k / E;
-    if (k < dy) {  // uniform
-      double* rowb = buf + (k & 1) * 2 * DYP;
-      double* colb = rowb + DYP;
-      const double pk = readlane_d(a[ek], k + DYP * hk);
+  static_for<0, 16>([&](auto KK) {
+    constexpr int k = decltype(KK)::value, ek = k >> 2, hk = k & 3;
+    if (k < n) {  // uniform
+      const double pk = readlane_d(a[ek], k + 16 * hk);
+      double rk[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rk[e] = row_bcast16<k>(a[e]);
+      const double cik = rows_bcast_d<hk>(a[ek]);
       good = good && (pk > 0.0);
       const double pp = pk > 0.0 ? pk : 1.0;
       // 1 / pp: the hardware reciprocal and two Newton steps (error below 1 ulp)
       double r = __builtin_amdgcn_rcp(pp);
       r = fma(r, fma(-pp, r, 1.0), r);
       r = fma(r, fma(-pp, r, 1.0), r);
-      piv = (lane == k) ? pk : piv;
-      if (i == k) {
+      *piv = (lane == k) ? pk : *piv;
+      const bool rowk = (i == k);
+      const double c = rowk ? -r : cik * r;
 #pragma unroll
-        for (int e = 0; e < E; ++e) rowb[E * h + e] = a[e] * r;
-      }
-      if (h == hk) colb[i] = a[ek];
-      wave_lds_sync();
-      const double cik = colb[i];
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const double rk = rowb[E * h + e];
-        a[e] = (i == k) ? rk : fma(-cik, rk, a[e]);
-      }
-      const double colk = (i == k) ? r : -cik * r;
+      for (int e = 0; e < 4; ++e) a[e] = fma(-c, rk[e], rowk ? 0.0 : a[e]);
+      const double colk = rowk ? r : -cik * r;
       a[ek] = (h == hk) ? colk : a[ek];
     }
   });
+  return good;
+}
+
+// Q P' for 16 x 16 FP64 matrices, all column-strided, on the matrix cores: MFMA u takes
+// k = h + 4 u in lane (i, h), so P's and Q's own registers are its two operands.
+__device__ __forceinline__ void mm16_qpt(const double p[4], const double q[4], double out[4]) {
+  typedef double d4v __attribute__((ext_vector_type(4)));
+  d4v acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int j = E * h + e;
-    if (i < dyp && j < dyp && j >= i) {
-      const float v = (float)a[e];
-      S[swz<DYP>(i, j)] = v;
-      S[swz<DYP>(j, i)] = v;
+  for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], q[u], acc, 0, 0, 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[e] = acc[e];
+}
+
+// One wave: S^-1 of the dy x dy symmetric positive definite matrix S (FP32, swizzled with
+// LD = DYP, exactly symmetric), written over S (zeros past dy up to dyp), in FP64.  Up to
+// 16: gj16.  Beyond, by blocks S = [A B; B' D]: X = A^-1 B, C = D - B' X (the Schur
+// complement), Y = X C^-1, S^-1 = [A^-1 + Y X', -Y; -Y', C^-1], log det S = log det A +
+// log det C, the products on the FP64 matrix cores.  *quad = z' S^-1 z and *logdet from
+// the pivots (fixed-order DPP sums).  Returns 0 if S is not positive definite.
+template <int DYP>
+__device__ __forceinline__ int spd_inverse_wave(float* S, int dy, const float* zv,
+                                                double* logdet, double* quad) {
+  static_assert(DYP == 16 || DYP == 32, "DYP must be 16 or 32");
+  const int lane = threadIdx.x & 63, i = lane & 15, h = lane >> 4;
+  auto sget = [&](int r, int c) -> double {
+    return (r < dy && c < dy) ? (double)S[swz<DYP>(r, c)] : 0.0;
+  };
+  auto zget = [&](int r) -> double { return r < dy ? (double)zv[r] : 0.0; };
+  auto sput = [&](int r, int c, double v) {  // entry and its mirror
+    S[swz<DYP>(r, c)] = (float)v;
+    S[swz<DYP>(c, r)] = (float)v;
+  };
+  double a[4], pa = 1.0, pc = 1.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) a[e] = sget(i, h + 4 * e);
+  bool good = gj16(a, dy < 16 ? dy : 16, &pa);
+  double q = 0.0;
+  if (DYP == 32 && dy > 16) {  // uniform
+    double bt[4], x[4], xt[4], cm[4], y[4], s11[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bt[e] = sget(16 + i, h + 4 * e);  // B' (S symmetric)
+    mm16_qpt(bt, a, x);    // X = A^-1 B
+    mm16_qpt(a, bt, xt);   // X' = B' A^-1
+    mm16_qpt(xt, bt, cm);  // B' X
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cm[e] = sget(16 + i, 16 + h + 4 * e) - cm[e];
+    good = gj16(cm, dy - 16, &pc) && good;  // C^-1
+    mm16_qpt(cm, x, y);    // Y = X C^-1
+    mm16_qpt(x, y, s11);   // Y X'
+    const double z1i = zget(i), z2i = zget(16 + i);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = h + 4 * e;
+      s11[e] += a[e];
+      q += z1i * s11[e] * zget(c) - 2.0 * z1i * y[e] * zget(16 + c) + z2i * cm[e] * zget(16 + c);
+      if (c >= i) {
+        sput(i, c, s11[e]);
+        sput(16 + i, 16 + c, cm[e]);
+      }
+      sput(i, 16 + c, -y[e]);
+    }
+  } else {
+    const double zi = zget(i);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = h + 4 * e;
+      q += zi * a[e] * zget(c);
+      if (c >= i) sput(i, c, a[e]);
     }
   }
-  double w = 0.0;  // (S^-1 z)_i, this lane's columns
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int j = E * h + e;
-    w = fma(a[e], j < dy ? (double)zv[j] : 0.0, w);
-  }
-  if constexpr (G == 4) w += __shfl_xor(w, 16, 64);
-  w += __shfl_xor(w, 32, 64);
-  const double zi = (h == 0 && i < dy) ? (double)zv[i] : 0.0;
-  *quad = wave_sum_dpp(zi * w);
-  const double lg = (lane < dy && good) ? log(piv) : 0.0;
-  *logdet = wave_sum_dpp(lg);
+  *quad = wave_sum_dpp(q);
+  // lane k < 16 holds pivot k of each block (1 past the block's size)
+  *logdet = wave_sum_dpp(good && lane < 16 ? log(pa * pc) : 0.0);
   return good ? 1 : 0;
 }
 
 // TV: the kf_loglikelihood layout (separate arrays, one parameter block per step or one
 // for all steps); otherwise each filter's packed homogeneous parameters.
 // Register budget: the large class (74 KB of LDS, two workgroups a CU) at 2 waves a SIMD
-// (256 VGPRs: the FP64 elimination keeps 16 doubles a lane), the others at 4.
+// (256 VGPRs), the mid class at 3 and the small one at 4, which keeps spills to a few
+// words.
 template <int DXP, int DYP, int NW, bool TV>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(DXP == 64 ? 2 : 4)))
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(DXP == 64 ? 2 : DXP == 32 ? 3 : 4)))
 k_kf_batch(KfBatchParams p) {
   using L = KfLds<DXP, DYP>;
   constexpr int TX = DXP / 16;  // tiles per padded state dimension, at most
   static_assert(NW >= TX, "a row block of A per wave");
+  static_assert(DYP <= 32, "S's upper tiles are enumerated for at most 2 x 2");
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) float lds[];
   float* const B0 = lds + L::kB0;
@@ -215,7 +337,6 @@ k_kf_batch(KfBatchParams p) {
   float* const zv = lds + L::kZ;
   float* const yv = lds + L::kY;
   int* const iscal = reinterpret_cast<int*>(lds + L::kScal);
-  double* const gjbuf = reinterpret_cast<double*>(lds + L::kGJ);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li0 = lane & 15, lk0 = lane >> 4;
@@ -273,12 +394,13 @@ k_kf_batch(KfBatchParams p) {
           const int ac = 16 * kb + 4 * lk + u;
           areg[kb][u] = (ar < dx && ac < dx) ? (float)gA[ar + (int64_t)ac * dx] : 0.f;
         }
+      // (consecutive threads read consecutive rows of the column-major inputs)
       for (int e = tid; e < dyp * dxp; e += NT) {
-        const int i = e / dxp, j = e % dxp;
+        const int i = e % dyp, j = e / dyp;
         Hm[swz<DXP>(i, j)] = (i < dy && j < dx) ? (float)gH[i + (int64_t)j * dy] : 0.f;
       }
       for (int e = tid; e < dyp * dyp; e += NT) {
-        const int i = e / dyp, j = e % dyp;
+        const int i = e % dyp, j = e / dyp;
         Rm[swz<DYP>(i, j)] = (i < dy && j < dy) ? (float)gR[i + (int64_t)j * dy] : 0.f;
       }
       for (int e = tid; e < dxp; e += NT) bv[e] = e < dx ? (float)gb[e] : 0.f;
@@ -294,9 +416,25 @@ k_kf_batch(KfBatchParams p) {
     }
     const int64_t blk0 = TV ? (int64_t)f * p.tv_blocks : 0;
     load_params(blk0);
+    // Q at this wave's tiles of Sigma_p, in registers (reloaded with a step's parameters).
+    // Q is symmetric (a covariance): reading Q[qc][qr] makes the 16 lanes of a row read one
+    // contiguous 128-byte run.
+    float qv[TX][4];
+    auto load_q = [&]() {
+#pragma unroll
+      for (int m = 0; m < TX; ++m) {
+        const int r = jw + m * nj, qc = 16 * rb + li;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int qr = 16 * r + 4 * lk + e;
+          qv[m][e] = (r < tn && qr < dx && qc < dx) ? (float)gQ_cur[qc + (int64_t)qr * dx] : 0.f;
+        }
+      }
+    };
+    load_q();
     // Sigma0 from its upper triangle, as Symmetric(:U)
     for (int e = tid; e < dxp * dxp; e += NT) {
-      const int i = e / dxp, j = e % dxp;
+      const int i = e % dxp, j = e / dxp;
       const int lo = i < j ? i : j, hi = i < j ? j : i;
       B0[swz<DXP>(i, j)] = (hi < dx) ? (float)g_S0[lo + (int64_t)hi * dx] : 0.f;
     }
@@ -307,6 +445,7 @@ k_kf_batch(KfBatchParams p) {
     int ok = 1;
     const double* yrow = p.ys + p.y_off[f];
     const int Tn = p.Tf ? p.Tf[f] : p.T;
+    float ynext = (tid < dy && Tn > 0) ? (float)yrow[tid] : 0.f;
     for (int t = 0; t < Tn; ++t) {
       // The lane and shape indices pass through empty asm each step, so the compiler
       // recomputes the step's LDS addresses instead of keeping them all live across the
@@ -318,95 +457,134 @@ k_kf_batch(KfBatchParams p) {
       if constexpr (TV) {
         if (p.tv_blocks > 1 && t > 0) {  // this step's parameters (the last step ended synced)
           load_params(blk0 + t);
+          load_q();
           __syncthreads();
         }
       }
       // P1: T1 = A Sigma (Sigma symmetric, so its rows serve as Sigma^T's);
-      // mu_hat = A mu + b; y_t
-      for (int c = jw; c < tn; c += nj) {
-        f4v acc = {0.f, 0.f, 0.f, 0.f};
+      // mu_hat = A mu + b; y_t (loaded one step ahead)
+      for (int c = jw; c < tn; c += 2 * nj) {  // two tiles at a time, chains interleaved
+        const int c1 = c + nj < tn ? c + nj : c;
+        f4v bb[TX], bb1[TX];
 #pragma unroll
-        for (int kb = 0; kb < TX; ++kb) {
+        for (int kb = 0; kb < TX; ++kb)
           if (kb < tn) {
-            const f4v bb =
-                *reinterpret_cast<const f4v*>(B0 + swz<DXP>(16 * c + li, 16 * kb + 4 * lk));
-            acc = mfma4(areg[kb], bb, acc);
+            bb[kb] = *reinterpret_cast<const f4v*>(B0 + swz<DXP>(16 * c + li, 16 * kb + 4 * lk));
+            bb1[kb] = *reinterpret_cast<const f4v*>(B0 + swz<DXP>(16 * c1 + li, 16 * kb + 4 * lk));
           }
-        }
+        f4v acc = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < TX; ++kb)
+          if (kb < tn) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[kb][u], bb[kb][u], acc, 0, 0, 0);
+              acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[kb][u], bb1[kb][u], acc1, 0, 0, 0);
+            }
+          }
         kf_store<DXP>(B1, 16 * rb, 16 * c, acc, li, lk);
+        if (c1 != c) kf_store<DXP>(B1, 16 * rb, 16 * c1, acc1, li, lk);
       }
       if (jw == 0) {
-        float s = 0.f;
+        f4v sp = {0.f, 0.f, 0.f, 0.f};  // four partial sums: short dependency chains
 #pragma unroll
         for (int kb = 0; kb < TX; ++kb) {
           if (kb < tn) {
             const f4v m4 = *reinterpret_cast<const f4v*>(mu + 16 * kb + 4 * lk);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) s += areg[kb][u] * m4[u];
+            sp += areg[kb] * m4;
           }
         }
+        float s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
         s += __shfl_xor(s, 16, 64);
         s += __shfl_xor(s, 32, 64);
         const int i = 16 * rb + li;
         if (lk == 0) muh[i] = (i < dx) ? s + bv[i] : 0.f;
       }
-      for (int q = tid; q < dyp; q += NT) yv[q] = q < dy ? (float)yrow[(size_t)t * dy + q] : 0.f;
+      if (tid < dyp) {
+        yv[tid] = ynext;
+        if (t + 1 < Tn && tid < dy) ynext = (float)yrow[(size_t)(t + 1) * dy + tid];
+      }
       __syncthreads();
       // P2: Sigma_p = T1 A' + Q, stored as Sigma_p^T over Sigma; z = y - H mu_hat - c
 #pragma unroll
-      for (int m = 0; m < TX; ++m) {
-        const int r = jw + m * nj;
+      for (int m = 0; m < TX; m += 2) {  // two tiles at a time, chains interleaved
+        const int r = jw + m * nj, r1 = jw + (m + 1) * nj;
         if (r < tn) {
-          f4v acc = {0.f, 0.f, 0.f, 0.f};
+          const bool two = (m + 1 < TX) && r1 < tn;
+          const int r1c = two ? r1 : r;
+          f4v aa[TX], aa1[TX];
 #pragma unroll
-          for (int kb = 0; kb < TX; ++kb) {
+          for (int kb = 0; kb < TX; ++kb)
             if (kb < tn) {
-              const f4v aa =
-                  *reinterpret_cast<const f4v*>(B1 + swz<DXP>(16 * r + li, 16 * kb + 4 * lk));
-              acc = mfma4(aa, areg[kb], acc);
+              aa[kb] = *reinterpret_cast<const f4v*>(B1 + swz<DXP>(16 * r + li, 16 * kb + 4 * lk));
+              aa1[kb] = *reinterpret_cast<const f4v*>(B1 + swz<DXP>(16 * r1c + li, 16 * kb + 4 * lk));
             }
-          }
-          const int qc = 16 * rb + li;
+          f4v acc = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int qr = 16 * r + 4 * lk + e;
-            acc[e] += (qr < dx && qc < dx) ? (float)gQ_cur[qr + (int64_t)qc * dx] : 0.f;
-          }
+          for (int kb = 0; kb < TX; ++kb)
+            if (kb < tn) {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[kb][u], areg[kb][u], acc, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(aa1[kb][u], areg[kb][u], acc1, 0, 0, 0);
+              }
+            }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[e] += qv[m][e];
           kf_store_t<DXP>(B0, 16 * r, 16 * rb, acc, li, lk);
+          if (m + 1 < TX && two) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc1[e] += qv[m + 1 < TX ? m + 1 : m][e];
+            kf_store_t<DXP>(B0, 16 * r1, 16 * rb, acc1, li, lk);
+          }
         }
       }
       for (int q = tid; q < dyp; q += NT) {
-        float s = cv[q];
+        f4v sp = {0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < dxp; k += 4) {
           const f4v h4 = *reinterpret_cast<const f4v*>(Hm + swz<DXP>(q, k));
           const f4v m4 = *reinterpret_cast<const f4v*>(muh + k);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) s += h4[u] * m4[u];
+          sp += h4 * m4;
         }
-        zv[q] = (q < dy) ? yv[q] - s : 0.f;
+        zv[q] = (q < dy) ? yv[q] - (cv[q] + ((sp[0] + sp[1]) + (sp[2] + sp[3]))) : 0.f;
       }
       __syncthreads();
       // P3: P^T = Sigma_p^T H' (dxp x dyp), kept as is and transposed (P = H Sigma_p)
-      for (int tile = wave; tile < tn * tny; tile += NW) {
-        const int r = tile / tny, c = tile % tny;
-        const f4v acc = kf_mm<DXP, DXP>(B0, Hm, 16 * r, 16 * c, dxp, li, lk);
-        kf_store<DYP>(PT, 16 * r, 16 * c, acc, li, lk);
-        kf_store_t<DXP>(Pm, 16 * r, 16 * c, acc, li, lk);
-      }
+      kf_mm_tiles<DXP, DXP>(
+          B0, Hm, dxp, wave, NW, tn * tny, li, lk,
+          [&](int t, int& r0, int& c0) { r0 = 16 * (t / tny); c0 = 16 * (t % tny); },
+          [&](int r0, int c0, f4v acc) {
+            kf_store<DYP>(PT, r0, c0, acc, li, lk);
+            kf_store_t<DXP>(Pm, r0, c0, acc, li, lk);
+          });
       __syncthreads();
-      // P4: S = P H' + R
-      for (int tile = wave; tile < tny * tny; tile += NW) {
-        const int r = tile / tny, c = tile % tny;
+      // P4: S = P H' + R from its upper tiles, mirrored (the upper triangle is what the
+      // factorisation of PDMat(S) reads)
+      for (int tile = wave; tile < tny * (tny + 1) / 2; tile += NW) {
+        const int r = tile == 2 ? 1 : 0, c = tile == 0 ? 0 : 1;  // (0,0), (0,1), (1,1)
         f4v acc = kf_mm<DXP, DXP>(Pm, Hm, 16 * r, 16 * c, dxp, li, lk);
 #pragma unroll
         for (int e = 0; e < 4; ++e) acc[e] += Rm[swz<DYP>(16 * r + 4 * lk + e, 16 * c + li)];
-        kf_store<DYP>(Sm, 16 * r, 16 * c, acc, li, lk);
+        if (r < c) {
+          kf_store<DYP>(Sm, 16 * r, 16 * c, acc, li, lk);
+          kf_store_t<DYP>(Sm, 16 * r, 16 * c, acc, li, lk);
+        } else {
+          const int j = 16 * c + li;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int ii = 16 * r + 4 * lk + e;
+            if (ii <= j) {
+              Sm[swz<DYP>(ii, j)] = acc[e];
+              Sm[swz<DYP>(j, ii)] = acc[e];
+            }
+          }
+        }
       }
       __syncthreads();
       // P5: S^-1, log det S and z' S^-1 z in FP64 on wave 0
       if (wave == 0) {
         double ldet, qd;
-        const int good = gj_inverse_wave<DYP>(Sm, dy, dyp, zv, gjbuf, &ldet, &qd);
+        const int good = spd_inverse_wave<DYP>(Sm, dy, zv, &ldet, &qd);
         if (lane == 0) {
           iscal[1] = good;
           ll += -0.5 * ((double)dy * kLog2Pi + ldet + qd);
@@ -418,27 +596,27 @@ k_kf_batch(KfBatchParams p) {
         break;
       }
       // P6: K = P^T S^-1 (dxp x dyp)
-      for (int tile = wave; tile < tn * tny; tile += NW) {
-        const int r = tile / tny, c = tile % tny;
-        kf_store<DYP>(Km, 16 * r, 16 * c, kf_mm<DYP, DYP>(PT, Sm, 16 * r, 16 * c, dyp, li, lk), li, lk);
-      }
+      kf_mm_tiles<DYP, DYP>(
+          PT, Sm, dyp, wave, NW, tn * tny, li, lk,
+          [&](int t, int& r0, int& c0) { r0 = 16 * (t / tny); c0 = 16 * (t % tny); },
+          [&](int r0, int c0, f4v acc) { kf_store<DYP>(Km, r0, c0, acc, li, lk); });
       __syncthreads();
       // P7: T2 = Sigma_p (I - K H)' = Sigma_p - P^T K', stored as T2^T over T1;
       // mu = mu_hat + K z
-      for (int tile = wave; tile < tn * tn; tile += NW) {
-        const int r = tile / tn, c = tile % tn;
-        const f4v pk = kf_mm<DYP, DYP>(PT, Km, 16 * r, 16 * c, dyp, li, lk);
-        kf_store_t<DXP>(B1, 16 * r, 16 * c, kf_load_t<DXP>(B0, 16 * r, 16 * c, li, lk) - pk, li, lk);
-      }
+      kf_mm_tiles<DYP, DYP>(
+          PT, Km, dyp, wave, NW, tn * tn, li, lk,
+          [&](int t, int& r0, int& c0) { r0 = 16 * (t / tn); c0 = 16 * (t % tn); },
+          [&](int r0, int c0, f4v pk) {
+            kf_store_t<DXP>(B1, r0, c0, kf_load_t<DXP>(B0, r0, c0, li, lk) - pk, li, lk);
+          });
       for (int i = tid; i < dxp; i += NT) {
-        float s = 0.f;
+        f4v sp = {0.f, 0.f, 0.f, 0.f};
         for (int q = 0; q < dyp; q += 4) {
           const f4v k4 = *reinterpret_cast<const f4v*>(Km + swz<DYP>(i, q));
           const f4v z4 = *reinterpret_cast<const f4v*>(zv + q);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) s += k4[u] * z4[u];
+          sp += k4 * z4;
         }
-        mu[i] = (i < dx) ? muh[i] + s : 0.f;
+        mu[i] = (i < dx) ? muh[i] + ((sp[0] + sp[1]) + (sp[2] + sp[3])) : 0.f;
       }
       __syncthreads();
       // P8: U = R K' - H T2 (dyp x dxp), stored as U^T over P^T
@@ -450,30 +628,35 @@ k_kf_batch(KfBatchParams p) {
       }
       __syncthreads();
       // P9: Sigma = Symmetric(:U)(T2 + K U) + jitter I: the upper tiles, mirrored
-      for (int tile = wave; tile < tn * (tn + 1) / 2; tile += NW) {
-        int r = 0, c = tile;
-        while (c >= tn - r) {
-          c -= tn - r;
-          ++r;
-        }
-        c += r;
-        const f4v v = kf_load_t<DXP>(B1, 16 * r, 16 * c, li, lk) + kf_mm<DYP, DYP>(Km, PT, 16 * r, 16 * c, dyp, li, lk);
-        if (r < c) {
-          kf_store<DXP>(B0, 16 * r, 16 * c, v, li, lk);
-          kf_store_t<DXP>(B0, 16 * r, 16 * c, v, li, lk);
-        } else {
-          const int j = 16 * c + li;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int i = 16 * r + 4 * lk + e;
-            if (i <= j) {
-              const float w = v[e] + ((i == j && i < dx) ? jit : 0.f);
-              B0[swz<DXP>(i, j)] = w;
-              B0[swz<DXP>(j, i)] = w;
+      kf_mm_tiles<DYP, DYP>(
+          Km, PT, dyp, wave, NW, tn * (tn + 1) / 2, li, lk,
+          [&](int t, int& r0, int& c0) {  // upper tile t, row by row
+            int r = 0, c = t;
+            while (c >= tn - r) {
+              c -= tn - r;
+              ++r;
             }
-          }
-        }
-      }
+            r0 = 16 * r;
+            c0 = 16 * (c + r);
+          },
+          [&](int r0, int c0, f4v ku) {
+            const f4v v = kf_load_t<DXP>(B1, r0, c0, li, lk) + ku;
+            if (r0 < c0) {
+              kf_store<DXP>(B0, r0, c0, v, li, lk);
+              kf_store_t<DXP>(B0, r0, c0, v, li, lk);
+            } else {
+              const int j = c0 + li;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const int i = r0 + 4 * lk + e;
+                if (i <= j) {
+                  const float w = v[e] + ((i == j && i < dx) ? jit : 0.f);
+                  B0[swz<DXP>(i, j)] = w;
+                  B0[swz<DXP>(j, i)] = w;
+                }
+              }
+            }
+          });
       __syncthreads();
     }
     if (tid == 0) {
