@@ -241,7 +241,7 @@ def bench_kf(args, T=200, budget_s=10.0):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (FP64 Cholesky of S)",
+        "dtype": "f32 (FP64 inverse of S)",
         "data": "synthetic (seeded LG models and simulated observations)",
         "config": {"workload": f"batched KalmanFilter, B={args.B}, Dx~U{{2..64}}, Dy=Dx//2, "
                                f"T={T}, Joseph update", "B": args.B, "T": T},
@@ -515,7 +515,7 @@ def bench_serve(args, budget=16 << 30, max_active=256, budget_s=10.0):
         "value": served_fps, "unit": "filters/s",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": el_all * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (FP64 weight sums, FP64 Cholesky of S)",
+        "dtype": "f32 (FP64 weight sums, FP64 inverse of S)",
         "data": "synthetic (seeded models and simulated observations)",
         "config": {"workload": f"{M} requests: 1-D BF / LG BF (Dx 1..16) / KF, N log-uniform "
                                f"[256, 262144], T ~ U[50, 2000]; budget {budget >> 30} GiB, "
@@ -892,7 +892,7 @@ def bench_kfll(args, T=100, budget_s=10.0):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_dev,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if Dx <= 4 and Dy <= 4 else "f32 (FP64 Cholesky of S)",
+        "dtype": "f64" if Dx <= 4 and Dy <= 4 else "f32 (FP64 inverse of S)",
         "data": "synthetic (seeded stable time-varying LG parameters and observations)",
         "config": {"workload": ("filter(model, SRKF(), ys)" if srkf else
                                 f"kf_loglikelihood{' and gradient' if grad else ''}") + f", B={B}, "
