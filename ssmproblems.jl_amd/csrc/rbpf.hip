@@ -25,6 +25,7 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "kf_team.h"
 #include "mfma_team.h"
 #include "pf_common.h"
 
@@ -32,9 +33,11 @@ namespace ssm {
 
 constexpr int kRbDX = 32;  // inner state dim (padded)
 constexpr int kRbDY = 16;  // observation dim (padded)
-constexpr int kRbWaves = 8;  // two waves per SIMD: the block's LDS is sized to fit 160 KB
+// Twelve waves (one particle each) a block and a CU: three per SIMD, so one wave's serial
+// phases (the FP64 inverse, LDS round trips) overlap the others' matrix-core work.  Each
+// wave's LDS is 12.3 KB, the block's parameters 7.4 KB.
+constexpr int kRbWaves = 12;
 constexpr int kRbThreads = 64 * kRbWaves;
-constexpr int kRbLX = kRbDX + 1, kRbLY = kRbDY + 1;
 
 struct RbModelDev {
   float A[kRbDX * kRbDX], Q[kRbDX * kRbDX], H[kRbDY * kRbDX], R0[kRbDY * kRbDY];
@@ -43,27 +46,19 @@ struct RbModelDev {
   int D, Dy;
 };
 
-// LDS: parameters shared by the block, then one scratch region per wave.
+// LDS: parameters shared by the block (swizzled row-major, kf_team.h), then one region per
+// wave.
 struct RbShared {
-  float A[kRbDX * kRbLX], Q[kRbDX * kRbLX], H[kRbDY * kRbLX], R0[kRbDY * kRbLY];
-  float b[kRbDX], C[kRbDX], c[kRbDY];
+  float Q[kRbDX * kRbDX], H[kRbDY * kRbDX], R0[kRbDY * kRbDY];
+  float b[kRbDX], C[kRbDX], c[kRbDY], y[kRbDY];
 };
-// Per wave (18 KB): T1 is dead while the innovation covariance is factorised, so the FP64
-// factor and its work space live in its place; the solve writes K' over P column by column
-// (each column is read into registers before it is written), and P is not used after it.
+// Per wave: B0 holds Sigma, then Sigma_p^T, then the new Sigma; B1 holds A Sigma, then P =
+// H Sigma_p (its first 16 rows) and S (behind it, 16 x 16), then T2^T; PT holds P^T, then
+// U^T.
 struct RbWave {
-  float Sig[kRbDX * kRbLX];
-  union {
-    float T1[kRbDX * kRbLX];
-    struct {
-      double Sd[kRbDY * (kRbDY + 1)];
-      double chol_work[3 * kRbDY];
-    } ch;
-  };
-  float M[kRbDX * kRbLX];
-  float P[kRbDY * kRbLX];  // H Sigma, then K' (KT)
-  float KR[kRbDX * kRbLY], Rt[kRbDY * kRbLY];
-  float mu[kRbDX], muh[kRbDX], z[kRbDY], y[kRbDY];
+  float B0[kRbDX * kRbDX], B1[kRbDX * kRbDX];
+  float PT[kRbDX * kRbDY], K[kRbDX * kRbDY];
+  float mu[kRbDX], muh[kRbDX], z[kRbDY];
 };
 
 struct RbStepParams {
@@ -86,46 +81,85 @@ struct RbStepParams {
   int32_t* __restrict__ err;  // set to 1 if an innovation covariance is not PD
 };
 
-// DYP: the compile-time bound of the innovation covariance's Cholesky and solves (8 when
-// Dy <= 8, else 16); the matrix-core operands stay padded to kRbDY rows.
-template <int DYP>
-__global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
+// One wave per particle (grid-stride): gather the ancestor's (x, mu, Sigma), then the inner
+// predict and Joseph update in the phases of the batched Kalman kernel (kf.hip), within the
+// wave: T1 = A Sigma and Sigma_p = T1 A' + Q with A's two row blocks in registers; P' and P;
+// S = P H' + R_t (exactly symmetric); S^-1, log det S and z' S^-1 z in FP64; K = P' S^-1;
+// T2 = Sigma_p - P' K'; U = R_t K' - H T2; Sigma = Symmetric(:U)(T2 + K U).  R_t =
+// exp(rs x_t) R0 is R0 scaled in the products.
+__global__ void __launch_bounds__(kRbThreads) __attribute__((amdgpu_waves_per_eu(3)))
+k_rbpf_step(RbStepParams p) {
   extern __shared__ __align__(16) unsigned char rb_lds[];
   RbShared& sh = *reinterpret_cast<RbShared*>(rb_lds);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane0 = threadIdx.x & 63;
   RbWave& w = *reinterpret_cast<RbWave*>(rb_lds + sizeof(RbShared) + wave * sizeof(RbWave));
+  float* const Sm = w.B1 + kRbDY * kRbDX;  // S, behind P
   const RbModelDev& m = *p.model;
-  const int D = m.D, Dy = m.Dy;
-  for (int e = threadIdx.x; e < kRbDX * kRbDX; e += kRbThreads) {
-    const int i = e / kRbDX, j = e % kRbDX;
-    sh.A[i * kRbLX + j] = m.A[e];
-    sh.Q[i * kRbLX + j] = m.Q[e];
-  }
+  const int D0 = m.D, Dy0 = m.Dy;
+  for (int e = threadIdx.x; e < kRbDX * kRbDX; e += kRbThreads)
+    sh.Q[swz<kRbDX>(e / kRbDX, e % kRbDX)] = m.Q[e];
   for (int e = threadIdx.x; e < kRbDY * kRbDX; e += kRbThreads)
-    sh.H[(e / kRbDX) * kRbLX + e % kRbDX] = m.H[e];
+    sh.H[swz<kRbDX>(e / kRbDX, e % kRbDX)] = m.H[e];
   for (int e = threadIdx.x; e < kRbDY * kRbDY; e += kRbThreads)
-    sh.R0[(e / kRbDY) * kRbLY + e % kRbDY] = m.R0[e];
+    sh.R0[swz<kRbDY>(e / kRbDY, e % kRbDY)] = m.R0[e];
   for (int e = threadIdx.x; e < kRbDX; e += kRbThreads) {
     sh.b[e] = m.b[e];
     sh.C[e] = m.C[e];
   }
-  for (int e = threadIdx.x; e < kRbDY; e += kRbThreads) sh.c[e] = m.c[e];
-  // this wave's padded Sigma stays zero outside the D x D block
-  for (int e = lane; e < kRbDX * kRbLX; e += 64) w.Sig[e] = 0.f;
   BfScalars* sc = p.sc;
   const bool rs = p.threshold_n >= sc->ess;
   const bool fresh = rs || sc->typeless;
   const float lse = (float)sc->lse;
   const int t = sc->step;
-  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
-  for (int q = lane; q < kRbDY; q += 64) w.y[q] = q < Dy ? (float)yrow[q] : 0.f;
+  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy0;
+  for (int q = threadIdx.x; q < kRbDY; q += kRbThreads) {
+    sh.c[q] = m.c[q];
+    sh.y[q] = q < Dy0 ? (float)yrow[q] : 0.f;
+  }
+  // this wave's Sigma is zero outside the D x D block (every later write keeps it so)
+  for (int e = lane0; e < kRbDX * kRbDX; e += 64) w.B0[e] = 0.f;
+  // A's row blocks in the matrix-core operand layout: areg[r][kb][u] = A[16 r + li][16 kb + 4 lk + u]
+  f4v areg[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+      areg[r][kb] = *reinterpret_cast<const f4v*>(m.A + (16 * r + (lane0 & 15)) * kRbDX +
+                                                  16 * kb + 4 * (lane0 >> 4));
   __syncthreads();
 
   const double log2pi = 1.8378770664093454835606594728112;
   const int G = gridDim.x * kRbWaves;
-  for (int i = blockIdx.x * kRbWaves + wave; i < p.N; i += G) {
-    const int a = rs ? p.anc[i] : i;
-    const float xprev = p.xo_in[a];
+  // The ancestor's (x, mu, Sigma) of the wave's next particle are loaded into registers
+  // while the current one computes (lane l holds Sigma entries l + 64 k, k < 16): the
+  // gather's HBM round trips leave the critical path.
+  constexpr int kSigRegs = kRbDX * kRbDX / 64;
+  float sig[kSigRegs], mu_pre, x_pre;
+  auto prefetch = [&](int ip) {
+    const int ap = rs ? p.anc[ip] : ip;
+    const float* sg = p.Sig_in + (size_t)ap * D0 * D0;
+#pragma unroll
+    for (int k = 0; k < kSigRegs; ++k) {
+      const int e = lane0 + 64 * k;
+      sig[k] = e < D0 * D0 ? sg[e] : 0.f;
+    }
+    mu_pre = lane0 < D0 ? p.mu_in[(size_t)ap * D0 + lane0] : 0.f;
+    x_pre = p.xo_in[ap];
+  };
+  const int i0 = blockIdx.x * kRbWaves + wave;
+  if (i0 < p.N) prefetch(i0);
+  for (int i = i0; i < p.N; i += G) {
+    // The lane and shape indices pass through empty asm each particle, so the compiler
+    // recomputes the LDS addresses instead of keeping them all live across the loop.
+    int lane = lane0, D = D0, Dy = Dy0;
+    asm volatile("" : "+v"(lane));
+    asm volatile("" : "+s"(D), "+s"(Dy));
+    const int li = lane & 15, lk = lane >> 4;
+    const int dxp = (D + 15) & ~15, tn = dxp >> 4;
+    // entry e = lane + 64 k of the D x D block is (r, c) = (e / D, e % D), advanced by
+    // 64 = q D + m each k
+    const int qd = 64 / D, md = 64 - qd * D;
+    const float xprev = x_pre;
     const float nl = fresh ? 0.f : p.logw[i] - lse;
     float zo;
     if (p.noise) {
@@ -137,78 +171,188 @@ __global__ void __launch_bounds__(kRbThreads) k_rbpf_step(RbStepParams p) {
       box_muller_f32(u.x, u.y, zo, z1);
     }
     const float xnew = m.oa * xprev + m.ob + m.sq * zo;
-    // gather the ancestor's inner distribution
-    const float* sg = p.Sig_in + (size_t)a * D * D;
-    for (int e = lane; e < D * D; e += 64) w.Sig[(e / D) * kRbLX + e % D] = sg[e];
-    for (int r = lane; r < kRbDX; r += 64) w.mu[r] = r < D ? p.mu_in[(size_t)a * D + r] : 0.f;
-    wave_lds_sync();
-    // predict: mu_hat = A mu + b + C x_prev ; Sigma = A Sigma A' + Q
-    for (int r = lane; r < kRbDX; r += 64) {
-      float s = sh.b[r] + sh.C[r] * xprev;
-      for (int k = 0; k < D; ++k) s += sh.A[r * kRbLX + k] * w.mu[k];
-      w.muh[r] = r < D ? s : 0.f;
+    const float sc_r = expf(m.rs * xnew);  // R_t = exp(rs x_t) R0
+    // the ancestor's inner distribution, from the prefetched registers
+    {
+      int r = lane / D, c = lane - (lane / D) * D;
+#pragma unroll
+      for (int k = 0; k < kSigRegs; ++k) {
+        if (lane + 64 * k < D * D) w.B0[swz<kRbDX>(r, c)] = sig[k];
+        r += qd;
+        c += md;
+        if (c >= D) {
+          c -= D;
+          ++r;
+        }
+      }
     }
-    gemm16<1>(w.T1, kRbLX, kRbDX, kRbDX, 1.f, sh.A, kRbLX, false, w.Sig, kRbLX, false, kRbDX,
-              nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
+    if (lane < kRbDX) w.mu[lane] = mu_pre;
     wave_lds_sync();
-    gemm16<1>(w.Sig, kRbLX, kRbDX, kRbDX, 1.f, w.T1, kRbLX, false, sh.A, kRbLX, true, kRbDX,
-              nullptr, 0, false, nullptr, 0, false, 0, sh.Q, kRbLX, 0);
-    // update: R_t = exp(rs x_new) R0 ; z = y - H mu_hat - c
-    const float sc_r = expf(m.rs * xnew);
-    for (int e = lane; e < kRbDY * kRbDY; e += 64) {
-      const int r = e / kRbDY, cc = e % kRbDY;
-      w.Rt[r * kRbLY + cc] = sc_r * sh.R0[r * kRbLY + cc];
+    if (i + G < p.N) prefetch(i + G);
+    // P1: T1 = A Sigma; mu_hat = A mu + b + C x_prev
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (r < tn) {
+        f4v acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+        f4v b00 = *reinterpret_cast<const f4v*>(w.B0 + swz<kRbDX>(li, 4 * lk));
+        f4v b01 = *reinterpret_cast<const f4v*>(w.B0 + swz<kRbDX>(li, 16 + 4 * lk));
+        f4v b10 = *reinterpret_cast<const f4v*>(w.B0 + swz<kRbDX>(16 + li, 4 * lk));
+        f4v b11 = *reinterpret_cast<const f4v*>(w.B0 + swz<kRbDX>(16 + li, 16 + 4 * lk));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[r][0][u], b00[u], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[r][0][u], b10[u], acc1, 0, 0, 0);
+        }
+        if (tn > 1) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[r][1][u], b01[u], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[r][1][u], b11[u], acc1, 0, 0, 0);
+          }
+        }
+        kf_store<kRbDX>(w.B1, 16 * r, 0, acc0, li, lk);
+        if (tn > 1) kf_store<kRbDX>(w.B1, 16 * r, 16, acc1, li, lk);
+        f4v sp = areg[r][0] * *reinterpret_cast<const f4v*>(w.mu + 4 * lk);
+        if (tn > 1) sp += areg[r][1] * *reinterpret_cast<const f4v*>(w.mu + 16 + 4 * lk);
+        float s = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        const int row = 16 * r + li;
+        if (lk == 0) w.muh[row] = (row < D) ? sh.b[row] + sh.C[row] * xprev + s : 0.f;
+      }
     }
     wave_lds_sync();
-    for (int q = lane; q < kRbDY; q += 64) {
-      float s = sh.c[q];
-      for (int k = 0; k < D; ++k) s += sh.H[q * kRbLX + k] * w.muh[k];
-      w.z[q] = q < Dy ? w.y[q] - s : 0.f;
+    // P2: Sigma_p = T1 A' + Q, stored transposed over Sigma; z = y - H mu_hat - c
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (r < tn) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          if (c < tn) {
+            f4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+              if (kb < tn)
+                acc = mfma4(*reinterpret_cast<const f4v*>(w.B1 + swz<kRbDX>(16 * r + li, 16 * kb + 4 * lk)),
+                            areg[c][kb], acc);
+            // Q symmetric: its tile (r, c) read through Q^T's rows
+            acc += kf_load_t<kRbDX>(sh.Q, 16 * r, 16 * c, li, lk);
+            kf_store_t<kRbDX>(w.B0, 16 * r, 16 * c, acc, li, lk);
+          }
+        }
+      }
     }
-    gemm16<1>(w.P, kRbLX, kRbDY, kRbDX, 1.f, sh.H, kRbLX, false, w.Sig, kRbLX, false, kRbDX,
-              nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
+    if (lane < kRbDY) {
+      f4v sp = {0.f, 0.f, 0.f, 0.f};
+      for (int k = 0; k < dxp; k += 4)
+        sp += *reinterpret_cast<const f4v*>(sh.H + swz<kRbDX>(lane, k)) *
+              *reinterpret_cast<const f4v*>(w.muh + k);
+      w.z[lane] = lane < Dy ? sh.y[lane] - (sh.c[lane] + ((sp[0] + sp[1]) + (sp[2] + sp[3]))) : 0.f;
+    }
     wave_lds_sync();
-    gemm16<1>(w.KR, kRbLY, kRbDY, kRbDY, 1.f, w.P, kRbLX, false, sh.H, kRbLX, true, kRbDX,
-              nullptr, 0, false, nullptr, 0, false, 0, w.Rt, kRbLY, 0);
+    // P3: P^T = Sigma_p^T H' (dxp x 16), kept as is and transposed into B1 (P = H Sigma_p)
+    kf_mm_tiles<kRbDX, kRbDX>(
+        w.B0, sh.H, dxp, 0, 1, tn, li, lk,
+        [&](int tt, int& r0, int& c0) { r0 = 16 * tt; c0 = 0; },
+        [&](int r0, int c0, f4v acc) {
+          kf_store<kRbDY>(w.PT, r0, c0, acc, li, lk);
+          kf_store_t<kRbDX>(w.B1, r0, c0, acc, li, lk);
+        });
     wave_lds_sync();
-    double quad = 0.0, logdet = 0.0;
-    float* const KT = w.P;
-    // rows Dy..kRbDY-1 of P (and so of K') are zero: the padded rows of H are
-    const int good = chol_solve_wave<DYP>(w.ch.Sd, kRbDY + 1, w.KR, kRbLY, w.P, kRbLX, w.z, KT,
-                                          kRbLX, D, kRbDX, Dy, DYP, &quad, &logdet,
-                                          w.ch.chol_work);
+    // P4: S = P H' + R_t, its upper triangle mirrored
+    {
+      f4v acc = kf_mm<kRbDX, kRbDX>(w.B1, sh.H, 0, 0, dxp, li, lk);
+      acc += sc_r * kf_load_t<kRbDY>(sh.R0, 0, 0, li, lk);  // R0 symmetric
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ii = 4 * lk + e;
+        if (ii <= li) {
+          Sm[swz<kRbDY>(ii, li)] = acc[e];
+          Sm[swz<kRbDY>(li, ii)] = acc[e];
+        }
+      }
+    }
+    wave_lds_sync();
+    // P5: S^-1 in FP64, log det S, z' S^-1 z and K = P^T S^-1
+    double ldet, quad;
+    SpdInv inv;
+    const int good = spd_inverse_wave<kRbDY>(Sm, Dy, w.z, inv, &ldet, &quad);
+    if (good)
+      for (int r = 0; r < tn; ++r) gain_rows<kRbDY>(inv, Dy, w.PT, w.K, r);
+    wave_lds_sync();
     float lnew;
     if (good) {
-      const double ll = -0.5 * ((double)Dy * log2pi + logdet + quad);
-      lnew = nl + (float)ll;
-      // mu = mu_hat + K z ; M = I - K H ; KR = K R_t
-      for (int r = lane; r < kRbDX; r += 64) {
-        float s = w.muh[r];
-        for (int q = 0; q < Dy; ++q) s += KT[q * kRbLX + r] * w.z[q];
-        w.mu[r] = r < D ? s : 0.f;
+      lnew = nl + (float)(-0.5 * ((double)Dy * log2pi + ldet + quad));
+      // P7: T2 = Sigma_p - P^T K', stored transposed over B1; mu = mu_hat + K z
+      kf_mm_tiles<kRbDY, kRbDY>(
+          w.PT, w.K, kRbDY, 0, 1, tn * tn, li, lk,
+          [&](int tt, int& r0, int& c0) { r0 = 16 * (tt / tn); c0 = 16 * (tt % tn); },
+          [&](int r0, int c0, f4v pk) {
+            kf_store_t<kRbDX>(w.B1, r0, c0, kf_load_t<kRbDX>(w.B0, r0, c0, li, lk) - pk, li, lk);
+          });
+      if (lane < kRbDX) {
+        f4v sp = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < kRbDY; q += 4)
+          sp += *reinterpret_cast<const f4v*>(w.K + swz<kRbDY>(lane, q)) *
+                *reinterpret_cast<const f4v*>(w.z + q);
+        w.mu[lane] = lane < D ? w.muh[lane] + ((sp[0] + sp[1]) + (sp[2] + sp[3])) : 0.f;
       }
-      gemm16<1>(w.M, kRbLX, kRbDX, kRbDX, -1.f, KT, kRbLX, true, sh.H, kRbLX, false, kRbDY,
-                nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, D);
-      gemm16<1>(w.KR, kRbLY, kRbDX, kRbDY, 1.f, KT, kRbLX, true, w.Rt, kRbLY, false, kRbDY,
-                nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
       wave_lds_sync();
-      gemm16<1>(w.T1, kRbLX, kRbDX, kRbDX, 1.f, w.M, kRbLX, false, w.Sig, kRbLX, false, kRbDX,
-                nullptr, 0, false, nullptr, 0, false, 0, nullptr, 0, 0);
+      // P8: U = R_t K' - H T2 (16 x dxp), stored transposed over P^T
+      kf_mm_tiles<kRbDX, kRbDX>(
+          sh.H, w.B1, dxp, 0, 1, tn, li, lk,
+          [&](int tt, int& r0, int& c0) { r0 = 0; c0 = 16 * tt; },
+          [&](int r0, int c0, f4v ht) {
+            const f4v rk = kf_mm<kRbDY, kRbDY>(sh.R0, w.K, r0, c0, kRbDY, li, lk);
+            kf_store_t<kRbDY>(w.PT, r0, c0, sc_r * rk - ht, li, lk);
+          });
       wave_lds_sync();
-      gemm16<1>(w.Sig, kRbLX, kRbDX, kRbDX, 1.f, w.T1, kRbLX, false, w.M, kRbLX, true, kRbDX,
-                w.KR, kRbLY, false, KT, kRbLX, false, kRbDY, nullptr, 0, 0);
+      // P9: Sigma = Symmetric(:U)(T2 + K U): the upper tiles, mirrored
+      kf_mm_tiles<kRbDY, kRbDY>(
+          w.K, w.PT, kRbDY, 0, 1, tn * (tn + 1) / 2, li, lk,
+          [&](int tt, int& r0, int& c0) {  // (0,0), (0,1), (1,1)
+            r0 = tt == 2 ? 16 : 0;
+            c0 = tt == 0 ? 0 : 16;
+          },
+          [&](int r0, int c0, f4v ku) {
+            const f4v v = kf_load_t<kRbDX>(w.B1, r0, c0, li, lk) + ku;
+            if (r0 < c0) {
+              kf_store<kRbDX>(w.B0, r0, c0, v, li, lk);
+              kf_store_t<kRbDX>(w.B0, r0, c0, v, li, lk);
+            } else {
+              const int j = c0 + li;
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const int ii = r0 + 4 * lk + e;
+                if (ii <= j) {
+                  w.B0[swz<kRbDX>(ii, j)] = v[e];
+                  w.B0[swz<kRbDX>(j, ii)] = v[e];
+                }
+              }
+            }
+          });
       wave_lds_sync();
     } else {
       lnew = -INFINITY;
       if (lane == 0) atomicOr(p.err, 1);
     }
-    // write the particle; Symmetric(Sigma) keeps the upper triangle
-    float* so = p.Sig_out + (size_t)i * D * D;
-    for (int e = lane; e < D * D; e += 64) {
-      const int r = e / D, cc = e % D;
-      so[e] = (r <= cc) ? w.Sig[r * kRbLX + cc] : w.Sig[cc * kRbLX + r];
+    // write the particle (Sigma is exactly symmetric)
+    {
+      float* so = p.Sig_out + (size_t)i * D * D;
+      int r = lane / D, c = lane - (lane / D) * D;
+#pragma unroll
+      for (int k = 0; k < kSigRegs; ++k) {
+        if (lane + 64 * k < D * D) so[lane + 64 * k] = w.B0[swz<kRbDX>(r, c)];
+        r += qd;
+        c += md;
+        if (c >= D) {
+          c -= D;
+          ++r;
+        }
+      }
     }
-    for (int r = lane; r < D; r += 64) p.mu_out[(size_t)i * D + r] = w.mu[r];
+    if (lane < D) p.mu_out[(size_t)i * D + lane] = w.mu[lane];
     if (lane == 0) {
       p.xo_out[i] = xnew;
       p.logw[i] = lnew;
@@ -386,12 +530,7 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   p.seed = rb->seed;
   p.noise = noise_dev;
   p.err = rb->err;
-  if (rb->Dy <= 8)
-    hipLaunchKernelGGL(k_rbpf_step<8>, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s,
-                       p);
-  else
-    hipLaunchKernelGGL(k_rbpf_step<16>, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s,
-                       p);
+  hipLaunchKernelGGL(k_rbpf_step, dim3(rb->nblocks), dim3(kRbThreads), rb_lds_bytes(), s, p);
   const int nb = (rb->N + kTile - 1) / kTile;
   hipLaunchKernelGGL(k_tile_partials<float>, dim3(nb), dim3(kTileThreads), 0, s, rb->N,
                      (const float*)rb->logw, (const float*)rb->nl_pre, (const BfScalars*)rb->sc,
@@ -444,7 +583,7 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
   rb->seed = seed;
   const int Npad = (int)(((int64_t)N + 2047) / 2048 * 2048);
   int per_cu = 0;
-  const void* kstep = (rb->Dy <= 8) ? (const void*)k_rbpf_step<8> : (const void*)k_rbpf_step<16>;
+  const void* kstep = (const void*)k_rbpf_step;
   SSM_HIP_CHECK(ctx, hipFuncSetAttribute(kstep, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)rb_lds_bytes()));
   SSM_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kstep, kRbThreads,
