@@ -163,11 +163,16 @@ __device__ __forceinline__ double rows_bcast_d(double v) {
 // all positive iff M is positive definite; lane k keeps pivot k in *piv.  Returns false if
 // one is not.
 __device__ __forceinline__ bool gj16(double a[4], int n, double* piv) {
-  const int lane = threadIdx.x & 63, i = lane & 15, h = lane >> 4;
+  const int lane0 = threadIdx.x & 63;
   bool good = true;
   static_for<0, 16>([&](auto KK) {
     constexpr int k = decltype(KK)::value, ek = k >> 2, hk = k & 3;
     if (k < n) {  // uniform
+      // the lane index passes through empty asm each step, so the step's lane masks are
+      // formed where they are used rather than all sixteen steps' kept live (and spilled)
+      int lane = lane0;
+      asm volatile("" : "+v"(lane));
+      const int i = lane & 15, h = lane >> 4;
       const double pk = readlane_d(a[ek], k + 16 * hk);
       double rk[4];
 #pragma unroll
