@@ -349,7 +349,7 @@ def bench_rbpf(args, N=16384, T=500, threshold=0.5, budget_s=10.0):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (FP64 Cholesky of S)",
+        "dtype": "f32 (FP64 inverse of S)",
         "data": "synthetic (seeded hierarchical model and simulated observations)",
         "config": {"workload": f"RBPF, outer AR(1), inner LG D={D} Dy={Dy}, R_t = exp(x_t) R0, "
                                f"N={N}, T={T}, systematic at threshold {threshold}",

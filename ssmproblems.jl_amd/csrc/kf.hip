@@ -34,7 +34,6 @@
 #include "internal.h"
 #include "kf_common.h"
 #include "kf_team.h"
-#include "mfma_team.h"
 
 namespace ssm {
 

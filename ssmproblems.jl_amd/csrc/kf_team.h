@@ -7,10 +7,28 @@
 
 #include <cstdint>
 
+#include <type_traits>
+
 #include "device_common.h"
-#include "mfma_team.h"
 
 namespace ssm {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = B, B+1, ..., E-1, so every
+// index into a register array is a constant.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 // LDS matrices are row-major with LD columns and no padding; the 16-byte column group
 // c / 4 of row r is XOR-ed with r mod min(LD / 4, 16).  A lane reading four consecutive

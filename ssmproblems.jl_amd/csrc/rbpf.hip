@@ -26,7 +26,6 @@
 #include "device_common.h"
 #include "internal.h"
 #include "kf_team.h"
-#include "mfma_team.h"
 #include "pf_common.h"
 
 namespace ssm {
