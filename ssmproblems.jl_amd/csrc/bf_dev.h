@@ -63,11 +63,12 @@ struct StepParams {
   const BfModelDev<T>* __restrict__ model;
   const double* __restrict__ ys;
   BfScalars* sc;
+  // the Philox key in device memory (graph replays survive a re-key); required.  Next to
+  // sc, so that the kernel's first argument load brings both.
+  const uint64_t* seed_dev = nullptr;
   double threshold_n;
   double logN;
-  uint64_t seed;
-  // when set, the Philox key is read from device memory (graph replays survive a re-key)
-  const uint64_t* seed_dev = nullptr;
+  uint64_t seed;  // the key as the host knows it (the step reads seed_dev)
   const T* __restrict__ noise;  // optional pre-drawn noise planes (parity mode)
   double* __restrict__ partials;
   // conditional SMC (the REF kernels): the reference trajectory, (T_ref + 1) x D row-major,
@@ -311,14 +312,17 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MDY = DYT ? DYT : kMaxDy;
   const int D = DT ? DT : p.D;
   const int Dy = DYT ? DYT : p.Dy;
-  BfScalars* sc = p.sc;
-  // the auxiliary filter (APF: p.aux set) decides on its look-ahead weights' ESS
-  const bool rs = p.threshold_n >= (APF ? p.aux->ess : sc->ess);
-  const bool fresh = rs || sc->typeless;  // pre-update log-weights 0 (or -log eta: apf)
-  const T lse = (T)sc->lse;
-  const int t = sc->step;
-  const uint64_t seed = p.seed_dev ? *p.seed_dev : p.seed;
-  const double* yrow = p.ys + (size_t)(t - 1 - sc->ybase) * Dy;
+  const int tile0 = tile * kTile;
+  // Every input that depends on nothing but the arguments is requested in one batch: the
+  // tile's ancestors (anc holds Npad entries, so every index is in bounds; a step that
+  // keeps its particles does not use them), the filter's scalars, the key and the model
+  // constants.  The empty asm statements then take the ancestors and the scalars, so the
+  // kernel waits once for the batch and the compiler cannot move a load behind the
+  // decision (a load behind a condition on a loaded value waits for the one before it).
+  // The gather thus starts one round trip after the kernel's arguments instead of four.
+  int a_in[kStepItems];
+#pragma unroll
+  for (int k = 0; k < kStepItems; ++k) a_in[k] = p.anc[tile0 + k * kStepThreads + threadIdx.x];
   // Model constants: registers for compile-time dimensions (read before any store, so the
   // loads are uniform scalar loads), an LDS copy otherwise.
   const BfModelDev<T>* msrc = p.model;
@@ -327,10 +331,29 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     msrc = &model_to_lds<T>(p.model, &lds_model);
   }
   const ModelView<T, MD, MDY, (DT > 0)> mv(*msrc, D, Dy);
-  T y[MDY];
+  BfScalars* sc = p.sc;
+  // the auxiliary filter (APF: p.aux set) decides on its look-ahead weights' ESS
+  const double ess_in = APF ? p.aux->ess : sc->ess;
+  const int32_t typeless = sc->typeless;
+  const double lse_in = sc->lse;
+  const int t = sc->step;
+  const int ybase = sc->ybase;
+  const uint64_t seed = *p.seed_dev;
 #pragma unroll
-  for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
-  // the reference state at t and particle 0's ancestor (uniform loads)
+  for (int k = 0; k < kStepItems; ++k) asm volatile("" : "+v"(a_in[k]));
+  asm volatile("" ::"s"(ess_in), "s"(lse_in), "s"(typeless), "s"(t), "s"(ybase), "s"(seed));
+  const T lse = (T)lse_in;
+  const bool rs = p.threshold_n >= ess_in;
+  const bool fresh = rs | (typeless != 0);  // pre-update log-weights 0 (or -log eta: apf)
+  const double* yrow = p.ys + (size_t)(t - 1 - ybase) * Dy;
+  // The observation: read before the body for the guided filter (its proposal depends on
+  // it), otherwise once the gathers are in flight.
+  T y[MDY];
+  auto load_y = [&]() {
+#pragma unroll
+    for (int q = 0; q < MDY; ++q) y[q] = (q < Dy) ? (T)yrow[q] : (T)0;
+  };
+  if constexpr (GUIDED) load_y();
   // the guided filter's proposal (uniform loads, before any store)
   struct NoProp {};
   using PV = std::conditional_t<GUIDED, PropView<T, MD, MDY, (DT > 0)>, NoProp>;
@@ -338,15 +361,25 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     if constexpr (GUIDED) return PV(*p.prop, D, Dy, y);
     else return PV{};
   }();
+  // the reference state at t and particle 0's ancestor (uniform loads)
   T xr[MD];
   int ref_a = 0;
   if constexpr (REF) {
 #pragma unroll
     for (int d = 0; d < MD; ++d) xr[d] = (d < D) ? p.xref[(size_t)t * D + d] : (T)0;
     ref_a = p.ref_anc ? *p.ref_anc : 0;
+    if (tile0 + threadIdx.x == 0) {  // item 0 of thread 0 of tile 0: particle 0
+      a_in[0] = ref_a;
+      if (rs) p.anc_w[0] = ref_a;
+    }
   }
-
-  const int tile0 = tile * kTile;
+  // each item's source particle: its ancestor when the step resampled, itself otherwise
+  int src[kStepItems];
+#pragma unroll
+  for (int k = 0; k < kStepItems; ++k) {
+    const int i = tile0 + k * kStepThreads + threadIdx.x;
+    src[k] = (rs && i < p.N) ? a_in[k] : i;
+  }
 
   // the normals of item k: pre-drawn planes, or Philox keyed by (seed, t, i)
   auto normals = [&](int k, T* z) {
@@ -378,7 +411,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
           for (int c = 0; c <= r; ++c) s += pv.LP(r, c) * z[c];
           xn[r] = s;
-          p.xout[(size_t)r * p.Npad + i] = s;
+          store_wt(&p.xout[(size_t)r * p.Npad + i], s);
           zz += z[r] * z[r];
         }
       }
@@ -417,7 +450,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
           for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
           if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
           xn[r] = s;
-          p.xout[(size_t)r * p.Npad + i] = s;
+          store_wt(&p.xout[(size_t)r * p.Npad + i], s);
         }
       }
     }
@@ -448,7 +481,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
       l = nlk + lpq + (mv.cst - (T)0.5 * quad);
     else
       l = nlk + (mv.cst - (T)0.5 * quad);
-    p.logw[i] = l;
+    store_wt(&p.logw[i], l);
     return l;
   };
 
@@ -456,32 +489,15 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     constexpr bool FRESH = decltype(fresh_c)::value;
     T nl[kStepItems], lnew[kStepItems];
     bool valid[kStepItems];
-    // loads that depend on nothing: ancestors (fresh) or log-weights (kept particles);
-    // anc and logw hold Npad entries, so every index below is in bounds
-    int a[kStepItems];
+    // the kept particles' log-weights (anc and logw hold Npad entries: in bounds)
     T lw[kStepItems];
 #pragma unroll
     for (int k = 0; k < kStepItems; ++k) {
       const int i = tile0 + k * kStepThreads + threadIdx.x;
       valid[k] = i < p.N;
-      if constexpr (FRESH)
-        a[k] = p.anc[i];
-      else
-        lw[k] = p.logw[i];
+      if constexpr (!FRESH) lw[k] = p.logw[i];
     }
-    if constexpr (FRESH && REF) {
-      if (tile0 + threadIdx.x == 0) {  // item 0 of thread 0 of tile 0: particle 0
-        a[0] = ref_a;
-        if (rs) p.anc_w[0] = ref_a;
-      }
-    }
-    auto src_of = [&](int k) {
-      const int i = tile0 + k * kStepThreads + threadIdx.x;
-      if constexpr (FRESH)
-        return (rs && valid[k]) ? a[k] : i;
-      else
-        return i;
-    };
+    auto src_of = [&](int k) { return src[k]; };  // src[k] = i unless the step resampled
 #pragma unroll
     for (int k = 0; k < kStepItems; ++k) {
       if constexpr (FRESH)
@@ -493,11 +509,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     // (resample_ancestor, GF/src/resamplers.jl:140-143); `apf && rs` is block-uniform
     [[maybe_unused]] const bool apf_rs = FRESH && APF && rs;
     if constexpr (DT > 0) {
-      // the normals while the ancestor (or log-weight) loads are in flight, then every
-      // gather, then the arithmetic
-      T z[kStepItems][MD];
-#pragma unroll
-      for (int k = 0; k < kStepItems; ++k) normals(k, z[k]);
+      // every gather, the normals while the gathers are in flight, then the arithmetic
       T x[kStepItems][MD];
 #pragma unroll
       for (int k = 0; k < kStepItems; ++k) {
@@ -505,6 +517,10 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
         for (int d = 0; d < MD; ++d) x[k][d] = p.xin[(size_t)d * p.Npad + src];
       }
+      if constexpr (!GUIDED) load_y();
+      T z[kStepItems][MD];
+#pragma unroll
+      for (int k = 0; k < kStepItems; ++k) normals(k, z[k]);
       if constexpr (FRESH && APF) {
         if (apf_rs) {
 #pragma unroll
@@ -515,6 +531,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
       for (int k = 0; k < kStepItems; ++k) lnew[k] = compute_item(k, x[k], z[k], nl[k]);
     } else {
+      if constexpr (!GUIDED) load_y();
       for (int k = 0; k < kStepItems; ++k) {
         const int src = src_of(k);
         T xk[MD], z[MD];

@@ -374,60 +374,64 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
 
 
 // ---- the fold of at most 1024 tiles by one 256-thread block ------------------------
-// Thread t owns the tiles t + 256 q (q < 4): row q of the tiles is one coalesced load per
-// field.  (Folding inside every block of the next step's resampler instead, which saves
-// this kernel's launch, was measured 4x slower at 1024 tiles: the fold's ~700 VALU
-// instructions per wave, repeated in every block, cost more than the launch.)
+// Thread t owns the consecutive tiles 4t..4t+3.  (Folding inside every block of the next
+// step's resampler instead, which saves this kernel's launch, was measured 4x slower at
+// 1024 tiles: the fold's ~700 VALU instructions per wave, repeated in every block, cost
+// more than the launch.)
 constexpr int kFoldThreads = 256;
 constexpr int kFoldTiles = 4;
 constexpr int kFoldMax = kFoldThreads * kFoldTiles;
 
+// A thread's tile partials.  They are requested before the step's scalars are waited for
+// (every index is clamped in bounds and the baseline fields are read even when unused), so
+// the fold starts one memory round trip after the kernel's arguments.
+struct Fold4In {
+  double m1[kFoldTiles], tau[kFoldTiles], s2[kFoldTiles], m0[kFoldTiles], s0[kFoldTiles];
+};
+__device__ __forceinline__ void fold_load4(int nb, const double* __restrict__ P, Fold4In& L) {
+  const int b0 = kFoldTiles * (int)threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kFoldTiles; ++q) {
+    const int bc = min(b0 + q, nb - 1);
+    L.m0[q] = P[bc];
+    L.s0[q] = P[nb + bc];
+    L.m1[q] = P[2 * nb + bc];
+    L.tau[q] = P[3 * nb + bc];
+    L.s2[q] = P[4 * nb + bc];
+  }
+}
+
 struct Fold4 {
-  double m1[kFoldTiles], c[kFoldTiles], pre[kFoldTiles];  // tile t + 256 q
+  double m1[kFoldTiles], c[kFoldTiles], pre[kFoldTiles];  // tiles 4t..4t+3
   double M1, S, S2, M0, S0, tot;
   bool dead;
 };
 
 // Folds the partials of nb <= kFoldMax tiles: maxima, the scaled sums (exp(m1_b - M1) in T)
-// and the exclusive prefixes of c_b tau_b, nondecreasing in tile order.  Row q's prefixes
-// are the row offset plus the in-row prefix; the in-row exclusive prefixes x are made
-// monotone across lanes by an exact running max, the waves of a row are chained by
-// off_{w+1} = max(off_w + tot_w, off_w + xmax_w), and the rows by the same rule on the row
-// totals, so no prefix exceeds the next tile's.  tot is the last offset, at least every
-// prefix.  The baseline partials are only read when the folded step did not start fresh.
-// Three barriers; the LDS of the maxima is reused for the prefix exchange.
+// and the exclusive prefixes of v_b = c_b tau_b, nondecreasing in tile order.  In a wave,
+// thread t's tiles take e_q = v_0 + ... + v_{q-1} in order; X_t, the exclusive tree scan of
+// the thread sums made monotone by an exact running max, is the thread's base, and tile q's
+// relative prefix is min(X_t + e_q, X_{t+1}) (X_64 = E, the wave's closing value max(total,
+// X_63)): nondecreasing in q, and never above the next thread's first.  The waves are
+// chained by off_{w+1} = off_w + E_w, so no prefix exceeds the next tile's; tot is the last
+// offset, at least every prefix.  The baseline partials only count when the folded step
+// did not start fresh.  Three barriers.
 template <typename T>
-__device__ __forceinline__ void fold_step4(int nb, const double* __restrict__ P, bool fresh,
-                                           Fold4& F) {
+__device__ __forceinline__ void fold_step4(int nb, const Fold4In& L, bool fresh, Fold4& F) {
   constexpr int NW = kFoldThreads / 64, R = kFoldTiles;
   __shared__ double red[5][NW];
-  __shared__ double rows[2][R][NW];
+  __shared__ double wend[NW];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double m0[R], s0[R], tau[R], s2[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int bc = min(q * kFoldThreads + t, nb - 1);
-    F.m1[q] = P[2 * nb + bc];
-    tau[q] = P[3 * nb + bc];
-    s2[q] = P[4 * nb + bc];
-  }
-  if (!fresh) {
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int bc = min(q * kFoldThreads + t, nb - 1);
-      m0[q] = P[bc];
-      s0[q] = P[nb + bc];
-    }
-  }
   double M1 = -INFINITY, M0 = -INFINITY;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
-    const bool in = q * kFoldThreads + t < nb;
-    m0[q] = (in && !fresh) ? m0[q] : -INFINITY;
-    s0[q] = (in && !fresh) ? s0[q] : 0.0;
-    F.m1[q] = in ? F.m1[q] : -INFINITY;
-    tau[q] = in ? tau[q] : 0.0;
-    s2[q] = in ? s2[q] : 0.0;
+    const bool in = R * t + q < nb;
+    m0[q] = (in && !fresh) ? L.m0[q] : -INFINITY;
+    s0[q] = (in && !fresh) ? L.s0[q] : 0.0;
+    F.m1[q] = in ? L.m1[q] : -INFINITY;
+    tau[q] = in ? L.tau[q] : 0.0;
+    s2[q] = in ? L.s2[q] : 0.0;
     M1 = fmax(M1, F.m1[q]);
     M0 = fmax(M0, m0[q]);
   }
@@ -462,32 +466,27 @@ __device__ __forceinline__ void fold_step4(int nb, const double* __restrict__ P,
   S0 = ((red[4][0] + red[4][1]) + red[4][2]) + red[4][3];
   const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
   const double invS = dead ? 0.0 : 1.0 / S;
-  double x[R];
+  double e[R], run = 0.0;
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     F.c[q] *= invS;
-    const double v = F.c[q] * tau[q];
-    const double incl = wave_scan_dpp(v);
-    double xe = wave_shfl_up(incl, 1);
-    if (lane == 0) xe = 0.0;
-    x[q] = wave_scan_max_dpp(xe);
-    if (lane == 63) {
-      rows[0][q][wv] = incl;
-      rows[1][q][wv] = x[q];
-    }
+    e[q] = run;
+    run += F.c[q] * tau[q];
   }
+  const double incl = wave_scan_dpp(run);
+  const double X = wave_scan_max_dpp(dpp_d<kDppWaveShr1>(0.0, incl));  // lane 0: 0
+  const double E = fmax(lane63(incl), lane63(X));
+  const double Xn = dpp_d<kDppWaveShl1>(E, X);  // the next lane's base; lane 63: E
+  if (lane == 0) wend[wv] = E;
   __syncthreads();
-  double off = 0.0;
+  double off = 0.0, mine = 0.0;
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    double mine = 0.0;
-#pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      if (k == wv) mine = off;
-      off = fmax(off + rows[0][q][k], off + rows[1][q][k]);
-    }
-    F.pre[q] = mine + x[q];
+  for (int k = 0; k < NW; ++k) {
+    if (k == wv) mine = off;
+    off += wend[k];
   }
+#pragma unroll
+  for (int q = 0; q < R; ++q) F.pre[q] = mine + fmin(X + e[q], Xn);
   F.M1 = M1;
   F.S = S;
   F.S2 = S2;
@@ -537,14 +536,16 @@ __global__ void __launch_bounds__(kFoldThreads)
     k_pf_finish_fold(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
                      double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
                      int record, const BfScalars* aux) {
+  Fold4In L;
+  fold_load4(nb, ts.partials, L);
   const BfScalars in = *sc_in;
   const StepDecision dec = step_decision(in, aux, threshold_n, logN);
   const bool rs = dec.rs, fresh = dec.fresh;
   Fold4 F;
-  fold_step4<T>(nb, ts.partials, fresh, F);
+  fold_step4<T>(nb, L, fresh, F);
 #pragma unroll
   for (int q = 0; q < kFoldTiles; ++q) {
-    const int b = q * kFoldThreads + threadIdx.x;
+    const int b = kFoldTiles * threadIdx.x + q;
     if (b < nb) {
       ts.prefix[b] = F.pre[q];
       ts.blk_c[b] = F.c[q];

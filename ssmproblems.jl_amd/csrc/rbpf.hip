@@ -392,6 +392,7 @@ struct ssm_rbpf {
   ssm_ctx* ctx = nullptr;
   int N = 0, D = 0, Dy = 0;
   uint64_t seed = 0;
+  uint64_t* seed_dev = nullptr;  // device copy of the key (the resampler reads it there)
   float* xo[2] = {nullptr, nullptr};
   float* mu[2] = {nullptr, nullptr};
   float* Sig[2] = {nullptr, nullptr};
@@ -501,6 +502,7 @@ int rb_enqueue_step(ssm_rbpf* rb, hipStream_t s, double threshold, int kind,
   a.w = rb->logw;
   a.u = u_dev;
   a.seed = rb->seed;
+  a.seed_dev = rb->seed_dev;
   a.sc = rb->sc;
   a.threshold_n = threshold * (double)rb->N;
   a.anc = rb->anc;
@@ -604,6 +606,7 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
        hipMalloc(&rb->rs_scratch, resample_scratch_bytes(N)) == hipSuccess &&
        hipMalloc(&rb->w_dbg, sizeof(double) * Npad) == hipSuccess &&
        hipMalloc(&rb->err, sizeof(int32_t)) == hipSuccess &&
+       hipMalloc(&rb->seed_dev, sizeof(uint64_t)) == hipSuccess &&
        hipEventCreate(&rb->ev0) == hipSuccess && hipEventCreate(&rb->ev1) == hipSuccess;
   if (!ok) {
     ssm_rbpf_destroy(rb);
@@ -614,6 +617,7 @@ extern "C" int32_t ssm_rbpf_create(ssm_ctx* ctx, const ssm_rb_model* model, int3
   (void)hipMemset(rb->sc, 0, sizeof(BfScalars));
   (void)hipMemset(rb->rs_scratch, 0, resample_scratch_bytes(N));
   (void)hipMemset(rb->err, 0, sizeof(int32_t));
+  (void)hipMemcpy(rb->seed_dev, &rb->seed, sizeof(uint64_t), hipMemcpyHostToDevice);
   int rc = rb_upload(rb, model);
   if (!rc) rc = rb_steps_cap(rb, 1);
   if (rc) {
@@ -635,7 +639,7 @@ extern "C" int32_t ssm_rbpf_destroy(ssm_rbpf* rb) {
   void* ptrs[] = {rb->xo[0],  rb->xo[1], rb->mu[0],      rb->mu[1],   rb->Sig[0],
                   rb->Sig[1], rb->logw,  rb->nl_pre,     rb->anc,     rb->model,
                   rb->sc,     rb->rs_scratch, rb->w_dbg, rb->ys,      rb->ll_steps,
-                  rb->rs_steps, rb->err, rb->noise,      rb->u};
+                  rb->rs_steps, rb->err, rb->noise,      rb->u,       rb->seed_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (rb->ev0) (void)hipEventDestroy(rb->ev0);

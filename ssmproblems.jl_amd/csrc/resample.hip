@@ -117,7 +117,9 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
 }
 
 // Main kernel: one block per tile (see scan_offspring_tile).  KIND/PLAIN: the particle
-// filter's specialisations (a fixed resampler, Philox uniforms, no debug outputs).
+// filter's specialisations (a fixed resampler, Philox uniforms keyed from device memory,
+// the filter's scalars, weights of whole tiles in a 16-byte aligned buffer, no debug
+// outputs).
 template <int MODE, int KIND = -1, bool PLAIN = false>
 __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
   scan_offspring_tile<MODE, KIND, PLAIN>(p, blockIdx.x);
@@ -126,7 +128,10 @@ __global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
 template <int MODE>
 void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
   const dim3 g(nb), blk(kRsThreads);
-  const bool plain = rp.u == nullptr && rp.w_debug64 == nullptr && rp.counts_out == nullptr;
+  // a filter's scalars and key; its weights are Npad (whole tiles) entries
+  const bool plain = rp.u == nullptr && rp.w_debug64 == nullptr && rp.counts_out == nullptr &&
+                     rp.sc != nullptr && rp.seed_dev != nullptr &&
+                     ((uintptr_t)rp.w & 15) == 0;
   if (plain && rp.kind == SSM_SYSTEMATIC)
     hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC, true>), g, blk, 0, stream, rp);
   else if (plain && rp.kind == SSM_SYSTEMATIC_MURRAY)
@@ -134,13 +139,16 @@ void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
                        rp);
   else if (plain && rp.kind == SSM_STRATIFIED)
     hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_STRATIFIED, true>), g, blk, 0, stream, rp);
+  else if (plain && rp.kind == SSM_MULTINOMIAL)
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_MULTINOMIAL, true>), g, blk, 0, stream, rp);
   else
     hipLaunchKernelGGL((k_scan_offspring<MODE>), g, blk, 0, stream, rp);
 }
 
-// Multinomial queries: one block per 1024 queries.
+// Multinomial queries: one block per 1024 queries (PLAIN as for k_scan_offspring).
+template <bool PLAIN>
 __global__ void __launch_bounds__(kRsThreads) k_multinomial_search(RsParams p) {
-  multinomial_search_block(p, blockIdx.x);
+  multinomial_search_block<PLAIN>(p, blockIdx.x);
 }
 
 // The kernels' parameter block for one resample call.
@@ -218,8 +226,12 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
       break;
     default: return set_error(ctx, SSM_ERR_ARG, "resample: bad weight mode");
   }
-  if (a.kind == SSM_MULTINOMIAL)
-    hipLaunchKernelGGL(k_multinomial_search, g, blk, 0, stream, rp);
+  if (a.kind == SSM_MULTINOMIAL) {
+    if (rp.u == nullptr && rp.sc != nullptr && rp.seed_dev != nullptr)
+      hipLaunchKernelGGL(k_multinomial_search<true>, g, blk, 0, stream, rp);
+    else
+      hipLaunchKernelGGL(k_multinomial_search<false>, g, blk, 0, stream, rp);
+  }
   SSM_HIP_CHECK(ctx, hipGetLastError());
   return SSM_OK;
 }

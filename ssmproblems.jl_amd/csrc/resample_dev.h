@@ -17,30 +17,28 @@ namespace ssm {
 
 constexpr int kRsThreads = kTileThreads;
 
+// The pointers a tile reads first lead the block, next to each other, so that the kernel's
+// first argument load brings them together.
 struct RsParams {
   int kind, n, mode;
   const void* w;
+  const BfScalars* sc;
+  const uint64_t* seed_dev = nullptr;  // when set, the Philox key is read from here
+  double* prefix;
+  const double* blk_m;
+  const double* blk_c;
   const double* u;
   uint64_t seed;
-  const uint64_t* seed_dev = nullptr;  // when set, the Philox key is read from here
   uint32_t step_host;
-  const BfScalars* sc;
   double threshold_n;
   int32_t* anc;
   double* w_debug64;
-  const double* blk_m;
-  const double* blk_c;
   double* tile_sum;  // explicit modes: tile sums (the partials area is reused)
-  double* prefix;
   double* Wbuf;   // multinomial: cumulative weights, tile-padded (nb * kTile)
   double* Wmid;   // multinomial: every 4th of them (nb * kTile / 4)
   double* Wroot;  // multinomial: every 64th of them (nb * kTile / 64)
   int32_t* counts_out;  // optional offspring count per particle (systematic, stratified)
 };
-
-__device__ __forceinline__ bool rs_active(const RsParams& p) {
-  return p.sc == nullptr || p.threshold_n >= p.sc->ess;
-}
 
 __device__ __forceinline__ uint64_t rs_seed(const RsParams& p) {
   return p.seed_dev ? *p.seed_dev : p.seed;
@@ -48,6 +46,43 @@ __device__ __forceinline__ uint64_t rs_seed(const RsParams& p) {
 
 __device__ __forceinline__ uint32_t rs_step(const RsParams& p) {
   return p.sc ? (uint32_t)p.sc->step : p.step_host;
+}
+
+// The scalars a resampling block decides with: the decision, the lazy-uniform flag (log-
+// weight modes), the Philox step and key.  PLAIN (a particle filter's resample: the filter
+// scalars and the device key are always there) reads them with no branch between the
+// loads; its callers then wait for them together with the block's other inputs (an empty
+// asm takes them all): a load behind a condition on a loaded value waits for the one
+// before it, and the compiler otherwise moves loads behind the block's early return on the
+// decision.
+struct RsScalars {
+  bool rs, typeless;
+  uint32_t step;
+  uint64_t seed;
+  double ess;  // PLAIN: the loaded values, for the caller's wait
+  int32_t typeless_in, step_in;
+};
+template <bool PLAIN>
+__device__ __forceinline__ RsScalars rs_scalars(const RsParams& p, bool log_weights) {
+  RsScalars r;
+  if constexpr (PLAIN) {
+    const double ess = p.sc->ess;
+    const int32_t typeless = p.sc->typeless, step = p.sc->step;
+    const uint64_t key = *p.seed_dev;
+    r.ess = ess;
+    r.typeless_in = typeless;
+    r.step_in = step;
+    r.rs = p.threshold_n >= ess;
+    r.typeless = log_weights && typeless != 0;
+    r.step = (uint32_t)step;
+    r.seed = key;
+  } else {
+    r.rs = p.sc == nullptr || p.threshold_n >= p.sc->ess;
+    r.typeless = log_weights && p.sc != nullptr && p.sc->typeless != 0;
+    r.step = rs_step(p);
+    r.seed = rs_seed(p);
+  }
+  return r;
 }
 
 // Weight of particle j in tile b as the resampler consumes it (FP64).  Particle-filter
@@ -148,7 +183,7 @@ __device__ __forceinline__ int clamp_floor(double v, int n) {
 // conditions.
 template <int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double Wn, double u0,
-                                            uint32_t step) {
+                                            uint32_t step, uint64_t seed) {
   const int n = p.n;
   const int kind = KIND >= 0 ? KIND : p.kind;
   if (kind == SSM_SYSTEMATIC) return offspring_systematic(W, u0, n);
@@ -158,7 +193,7 @@ __device__ __forceinline__ int offspring_of(const RsParams& p, double W, double 
   int k = clamp_floor(r + 1.0, n);
   if (k < 1) k = 1;
   const double* u = PLAIN ? nullptr : p.u;
-  const double uk = u ? u[k - 1] : resample_uniform(rs_seed(p), step, (uint32_t)(k - 1));
+  const double uk = u ? u[k - 1] : resample_uniform(seed, step, (uint32_t)(k - 1));
   return clamp_floor(r + uk, n);
 }
 
@@ -178,18 +213,22 @@ struct TileIn {
   bool rs, typeless;
   double P, Pnext, Wn, m_b, c_b;
   uint32_t step;
+  uint64_t seed;
 };
 
 // Loads the raw weights of tile b for thread t's positions 4t..4t+3 (clamped in-bounds
-// indices; a full tile is read with one vector load per thread).
-template <int MODE>
+// indices; a full tile is read with one vector load per thread).  WHOLE: the weights hold
+// whole tiles (a particle filter's Npad entries) in a 16-byte aligned buffer, so every
+// tile is read with the vector loads and no branch (positions past n are masked by the
+// caller).
+template <int MODE, bool WHOLE = false>
 __device__ __forceinline__ void load_tile_raw(const RsParams& p, int b,
                                               RawW<MODE> raw[kTileItems]) {
   using V = typename RawW<MODE>::type;
   const int n = p.n, tile0 = b * kTile;
   const int j0 = tile0 + kTileItems * threadIdx.x;
   const V* w = (const V*)p.w;
-  if (tile0 + kTile <= n && ((uintptr_t)w & 15) == 0) {
+  if (WHOLE || (tile0 + kTile <= n && ((uintptr_t)w & 15) == 0)) {
     if constexpr (sizeof(V) == 4) {
       const float4 v = *reinterpret_cast<const float4*>(w + j0);
       raw[0].v = v.x;
@@ -258,14 +297,12 @@ __device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
     acc[k] = run;
   }
   const double incl = wave_scan_dpp(run);
-  double x = wave_shfl_up(incl, 1);
-  if (lane == 0) x = 0.0;
+  const double x = dpp_d<kDppWaveShr1>(0.0, incl);  // the previous lane's; lane 0: 0
   double y[kTileItems];
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) y[k] = x + acc[k];  // nondecreasing in k
   const double ym_incl = wave_scan_max_dpp(y[kTileItems - 1]);
-  double ym_prev = wave_shfl_up(ym_incl, 1);
-  if (lane == 0) ym_prev = 0.0;
+  const double ym_prev = dpp_d<kDppWaveShr1>(0.0, ym_incl);
   if (lane == kWave - 1) {
     lds_tot[w] = incl;
     lds_ymax[w] = ym_incl;
@@ -306,7 +343,7 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
       const int e = __builtin_amdgcn_readfirstlane(min(ow[carry], wend));
       if (e - base >= kFillWin) {
         const int v = anc0 + carry;
-        for (int s = base + lane; s < e; s += kWave) anc[s] = v;
+        for (int s = base + lane; s < e; s += kWave) store_wt(&anc[s], v);
         base = e;
         continue;
       }
@@ -337,7 +374,7 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
     for (int i = 0; i < kTileItems; ++i) {
       const int s = base + i * kWave + lane;
       const int a = marks[i * kWave + lane];
-      if (s < wend) anc[s] = anc0 + a;
+      if (s < wend) store_wt(&anc[s], anc0 + a);
     }
     wave_lds_sync();
     base += kFillWin;
@@ -390,13 +427,15 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   }
   const double u0 = (kind == SSM_STRATIFIED)
                         ? 0.0
-                        : (uvec ? uvec[0] : resample_uniform(rs_seed(p), step, 0u));
-  const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step);
-  const int o_end = (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step);
+                        : (uvec ? uvec[0] : resample_uniform(in.seed, step, 0u));
+  const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step, in.seed);
+  const int o_end =
+      (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step, in.seed);
   // the tile's last particle closes its range; positions past n own none
   auto count_at = [&](int pos, double Wv) {
     return (pos >= cnt - 1) ? o_end
-                            : min(max(offspring_of<KIND, PLAIN>(p, Wv, Wn, u0, step), o_prev),
+                            : min(max(offspring_of<KIND, PLAIN>(p, Wv, Wn, u0, step, in.seed),
+                                      o_prev),
                                   o_end);
   };
   int O[kTileItems];
@@ -404,8 +443,7 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   for (int k = 0; k < kTileItems; ++k) O[k] = count_at(kTileItems * t + k, W[k]);
   const int wstart = __builtin_amdgcn_readfirstlane(
       (w == 0) ? o_prev : count_at(kFillWin * w - 1, Wedge));
-  int s0 = wave_shfl_up(O[kTileItems - 1], 1);
-  if (lane == 0) s0 = wstart;
+  const int s0 = dpp_i<kDppWaveShr1>(wstart, O[kTileItems - 1]);  // lane 0: wstart
   const int start[kTileItems] = {s0, O[0], O[1], O[2]};
   if (counts_out) {  // particle k owns the output slots [start[k], O[k])
 #pragma unroll
@@ -429,7 +467,7 @@ template <int MODE, int KIND = -1, bool PLAIN = false>
 __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   const int nb = (p.n + kTile - 1) / kTile;
   RawW<MODE> raw[kTileItems];
-  load_tile_raw<MODE>(p, b, raw);
+  load_tile_raw<MODE, PLAIN>(p, b, raw);
   constexpr bool kLog = (MODE == kWeightsLogF32 || MODE == kWeightsLogF64);
   TileIn in;
   in.P = p.prefix[b];
@@ -437,9 +475,18 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   in.Wn = p.prefix[nb];
   in.m_b = kLog ? p.blk_m[b] : 0.0;
   in.c_b = kLog ? p.blk_c[b] : 0.0;
-  in.rs = rs_active(p);
-  in.typeless = kLog && p.sc->typeless;
-  in.step = rs_step(p);
+  const RsScalars rsc = rs_scalars<PLAIN>(p, kLog);
+  if constexpr (PLAIN) {
+    // one wait for the scalars, the tile's prefixes and scales and its weights
+    asm volatile("" ::"s"(rsc.ess), "s"(rsc.typeless_in), "s"(rsc.step_in), "s"(rsc.seed),
+                 "s"(in.P), "s"(in.Pnext), "s"(in.Wn), "s"(in.m_b), "s"(in.c_b));
+#pragma unroll
+    for (int k = 0; k < kTileItems; ++k) asm volatile("" : "+v"(raw[k].v));
+  }
+  in.rs = rsc.rs;
+  in.typeless = rsc.typeless;
+  in.step = rsc.step;
+  in.seed = rsc.seed;
   scan_offspring_tile_in<MODE, KIND, PLAIN>(p, b, raw, in);
 }
 
@@ -506,15 +553,21 @@ __device__ __forceinline__ int team_node_rank(const double* __restrict__ node, d
 // rounds of kTeamBatch per team, their loads in flight together.  All threads of the
 // block call it.
 constexpr int kTeamBatch = 8;
+template <bool PLAIN>
 __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int blk) {
   __shared__ double coarse[kCoarse];
   __shared__ double qs[kTile];
   __shared__ int tiles[kTile];
-  if (!rs_active(p)) return;
   const int n = p.n, nb = (n + kTile - 1) / kTile, t = threadIdx.x;
-  const uint32_t step = rs_step(p);
-  const bool typeless = (p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64) && p.sc->typeless;
-  const double Wn = typeless ? 1.0 : p.prefix[nb];
+  const RsScalars rsc = rs_scalars<PLAIN>(p, p.mode == kWeightsLogF32 || p.mode == kWeightsLogF64);
+  const double Wtot = p.prefix[nb];  // read before the decision (prefix holds nb + 1 entries)
+  if constexpr (PLAIN)
+    asm volatile("" ::"s"(rsc.ess), "s"(rsc.typeless_in), "s"(rsc.step_in), "s"(rsc.seed),
+                 "s"(Wtot));
+  if (!rsc.rs) return;
+  const uint32_t step = rsc.step;
+  const bool typeless = rsc.typeless;
+  const double Wn = typeless ? 1.0 : Wtot;
   int sh = 0;
   while (((nb + (1 << sh) - 1) >> sh) > kCoarse) ++sh;
   const int ns = (nb + (1 << sh) - 1) >> sh;
@@ -530,7 +583,7 @@ __device__ __forceinline__ void multinomial_search_block(const RsParams& p, int 
     for (int i = i0; i < i0 + 2; ++i) {
       const int j = w * kPerWave + i * kWave + lane;
       const int kc = min(blk * kTile + j, n - 1);
-      const double uk = p.u ? p.u[kc] : resample_uniform(rs_seed(p), step, (uint32_t)kc);
+      const double uk = p.u ? p.u[kc] : resample_uniform(rsc.seed, step, (uint32_t)kc);
       const double q = uk * Wn;
       int c = 0;
 #pragma unroll
@@ -626,6 +679,7 @@ __device__ __forceinline__ void categorical_one_block(const RsParams& p, double 
   in.m_b = kLog ? p.blk_m[b] : 0.0;
   in.c_b = kLog ? p.blk_c[b] : 0.0;
   in.step = step;
+  in.seed = 0;  // the tile's weights only
   double W[kTileItems], P, Pnext, Wt, Wedge;
   tile_cum_weights<MODE>(p, b, raw, in, nullptr, W, &P, &Pnext, &Wt, &Wedge);
   const int tile0 = b * kTile, cnt = min(kTile, n - tile0);

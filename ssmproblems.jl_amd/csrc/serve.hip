@@ -77,6 +77,7 @@ __device__ __forceinline__ RsParams bfb_rs_params(const BfbFilter& F) {
   p.w = F.logw;
   p.u = nullptr;
   p.seed = F.seed;
+  p.seed_dev = &F.seed;
   p.step_host = 0;
   p.sc = F.sc;
   p.threshold_n = F.threshold_n;
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(kTileThreads)
   const BfbFilter& F = fs[o];
   if (F.kind != SSM_MULTINOMIAL || F.sc->step > F.t_last) return;
   const RsParams p = bfb_rs_params(F);
-  multinomial_search_block(p, blockIdx.x - F.tile_base);
+  multinomial_search_block<true>(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
@@ -138,6 +139,7 @@ __global__ void __launch_bounds__(kTileThreads)
   p.threshold_n = F.threshold_n;
   p.logN = F.logN;
   p.seed = F.seed;
+  p.seed_dev = &F.seed;
   p.noise = nullptr;
   p.partials = F.ts.partials;
   const int tile = blockIdx.x - F.tile_base;
