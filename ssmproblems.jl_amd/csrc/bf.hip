@@ -34,11 +34,23 @@
 
 namespace ssm {
 
-// The fused predict + update + partial-reduction kernel: one block per tile.
+// The fused predict + update + partial-reduction kernel: one block per tile.  The pointers
+// of the step's first loads lead the arguments as scalars, which the hardware preloads into
+// registers (the build asks for it), so those loads need no argument load before them.
 template <typename T, int DT, int DYT, bool NOISE, bool REF, bool GUIDED = false, bool APF = false>
-__global__ void __launch_bounds__(kStepThreads) k_bf_step(StepParams<T> p) {
-  bf_step_tile<T, DT, DYT, NOISE, REF, GUIDED, APF>(p, blockIdx.x);
+__global__ void __launch_bounds__(kStepThreads)
+    k_bf_step(const int32_t* anc, BfScalars* sc, const uint64_t* seed_dev,
+              const BfModelDev<T>* model, StepParams<T> p) {
+  StepParams<T> q = p;
+  q.anc = anc;
+  q.sc = sc;
+  q.seed_dev = seed_dev;
+  q.model = model;
+  bf_step_tile<T, DT, DYT, NOISE, REF, GUIDED, APF>(q, blockIdx.x);
 }
+
+// k_bf_step's arguments for a parameter block
+#define STEP_ARGS(p) p.anc, p.sc, p.seed_dev, p.model, p
 
 // Conditional SMC: ancestor-sampling log-weights for the reference state x_ref[t] at the
 // filter's current step t (GF/src/ancestor_sampling.jl:64-130 with the generic
@@ -339,67 +351,67 @@ void launch_step_t(ssm_bf* bf, hipStream_t s, double threshold_n, const T* noise
   if (!bf->prop_F.empty()) {  // guided filter (no reference pinning)
     if (noise) {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, true>), g, b, 0, s, STEP_ARGS(p));
       else if (bf->D == 1 && bf->Dy == 1)
-        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, true>), g, b, 0, s, STEP_ARGS(p));
     } else {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else if (bf->D == 1 && bf->Dy == 1)
-        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, true>), g, b, 0, s, STEP_ARGS(p));
     }
     return;
   }
   if (bf->apf) {  // auxiliary particle filter (no reference, no proposal)
     if (noise) {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else if (bf->D == 1 && bf->Dy == 1)
-        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false, false, true>), g, b, 0, s, STEP_ARGS(p));
     } else {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else if (bf->D == 1 && bf->Dy == 1)
-        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false, false, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false, false, true>), g, b, 0, s, STEP_ARGS(p));
     }
     return;
   }
   if (bf->ref_T >= 0) {  // conditional SMC: particle 0 follows the reference
     if (noise) {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, true>), g, b, 0, s, STEP_ARGS(p));
     } else {
       if (bf->D == 4 && bf->Dy == 2)
-        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, true>), g, b, 0, s, STEP_ARGS(p));
       else
-        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, true>), g, b, 0, s, p);
+        hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, true>), g, b, 0, s, STEP_ARGS(p));
     }
     return;
   }
   if (noise) {  // parity mode: pre-drawn normals
     if (bf->D == 4 && bf->Dy == 2)
-      hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, true, false>), g, b, 0, s, STEP_ARGS(p));
     else if (bf->D == 1 && bf->Dy == 1)
-      hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, true, false>), g, b, 0, s, STEP_ARGS(p));
     else
-      hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, true, false>), g, b, 0, s, STEP_ARGS(p));
   } else {
     if (bf->D == 4 && bf->Dy == 2)
-      hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 4, 2, false, false>), g, b, 0, s, STEP_ARGS(p));
     else if (bf->D == 1 && bf->Dy == 1)
-      hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 1, 1, false, false>), g, b, 0, s, STEP_ARGS(p));
     else
-      hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false>), g, b, 0, s, p);
+      hipLaunchKernelGGL((k_bf_step<T, 0, 0, false, false>), g, b, 0, s, STEP_ARGS(p));
   }
 }
 

@@ -531,13 +531,14 @@ __device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
 
 // The finish of nb <= 1024 tiles (one block): the fold, every tile's scale, maximum and
 // prefix, the total and the new scalars (sc_in may equal sc: it is read before the write).
+// The first loads' pointers and count lead the arguments (preloaded into registers).
 template <typename T>
 __global__ void __launch_bounds__(kFoldThreads)
-    k_pf_finish_fold(int N, int nb, const BfScalars* sc_in, BfScalars* sc, double threshold_n,
-                     double logN, TileScratch ts, double* ll_steps, int32_t* rs_steps,
-                     int record, const BfScalars* aux) {
+    k_pf_finish_fold(const double* partials, const BfScalars* sc_in, int nb, int N,
+                     BfScalars* sc, double threshold_n, double logN, TileScratch ts,
+                     double* ll_steps, int32_t* rs_steps, int record, const BfScalars* aux) {
   Fold4In L;
-  fold_load4(nb, ts.partials, L);
+  fold_load4(nb, partials, L);
   const BfScalars in = *sc_in;
   const StepDecision dec = step_decision(in, aux, threshold_n, logN);
   const bool rs = dec.rs, fresh = dec.fresh;
@@ -728,8 +729,9 @@ inline void launch_pf_finish(hipStream_t s, int N, int nb, BfScalars* sc, double
                              double logN, const TileScratch& ts, double* ll_steps,
                              int32_t* rs_steps, int record, const BfScalars* aux = nullptr) {
   if (nb <= kFoldMax) {
-    hipLaunchKernelGGL((k_pf_finish_fold<T>), dim3(1), dim3(kFoldThreads), 0, s, N, nb, sc, sc,
-                       threshold_n, logN, ts, ll_steps, rs_steps, record, aux);
+    hipLaunchKernelGGL((k_pf_finish_fold<T>), dim3(1), dim3(kFoldThreads), 0, s,
+                       (const double*)ts.partials, (const BfScalars*)sc, nb, N, sc, threshold_n,
+                       logN, ts, ll_steps, rs_steps, record, aux);
   } else {
     const int G = (nb + kFinishGroup - 1) / kFinishGroup;
     hipLaunchKernelGGL((k_pf_finish_groups<T>), dim3(G), dim3(kFinishGroup), 0, s, nb, sc,

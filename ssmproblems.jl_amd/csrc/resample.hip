@@ -120,10 +120,26 @@ __global__ void __launch_bounds__(256) k_tile_prefix(int nb, const double* tile_
 // filter's specialisations (a fixed resampler, Philox uniforms keyed from device memory,
 // the filter's scalars, weights of whole tiles in a 16-byte aligned buffer, no debug
 // outputs).
+// The pointers and count of a tile's first loads lead the arguments as scalars, which the
+// hardware preloads into registers (the build asks for it).
 template <int MODE, int KIND = -1, bool PLAIN = false>
-__global__ void __launch_bounds__(kRsThreads) k_scan_offspring(RsParams p) {
-  scan_offspring_tile<MODE, KIND, PLAIN>(p, blockIdx.x);
+__global__ void __launch_bounds__(kRsThreads)
+    k_scan_offspring(const void* w, const BfScalars* sc, const uint64_t* seed_dev,
+                     double* prefix, const double* blk_m, const double* blk_c, int n,
+                     RsParams p) {
+  RsParams q = p;
+  q.w = w;
+  q.sc = sc;
+  q.seed_dev = seed_dev;
+  q.prefix = prefix;
+  q.blk_m = blk_m;
+  q.blk_c = blk_c;
+  q.n = n;
+  scan_offspring_tile<MODE, KIND, PLAIN>(q, blockIdx.x);
 }
+
+// k_scan_offspring's arguments for a parameter block
+#define RS_ARGS(rp) rp.w, rp.sc, rp.seed_dev, rp.prefix, rp.blk_m, rp.blk_c, rp.n, rp
 
 template <int MODE>
 void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
@@ -133,16 +149,19 @@ void launch_scan_offspring(hipStream_t stream, int nb, const RsParams& rp) {
                      rp.sc != nullptr && rp.seed_dev != nullptr &&
                      ((uintptr_t)rp.w & 15) == 0;
   if (plain && rp.kind == SSM_SYSTEMATIC)
-    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC, true>), g, blk, 0, stream, rp);
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC, true>), g, blk, 0, stream,
+                       RS_ARGS(rp));
   else if (plain && rp.kind == SSM_SYSTEMATIC_MURRAY)
     hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_SYSTEMATIC_MURRAY, true>), g, blk, 0, stream,
-                       rp);
+                       RS_ARGS(rp));
   else if (plain && rp.kind == SSM_STRATIFIED)
-    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_STRATIFIED, true>), g, blk, 0, stream, rp);
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_STRATIFIED, true>), g, blk, 0, stream,
+                       RS_ARGS(rp));
   else if (plain && rp.kind == SSM_MULTINOMIAL)
-    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_MULTINOMIAL, true>), g, blk, 0, stream, rp);
+    hipLaunchKernelGGL((k_scan_offspring<MODE, SSM_MULTINOMIAL, true>), g, blk, 0, stream,
+                       RS_ARGS(rp));
   else
-    hipLaunchKernelGGL((k_scan_offspring<MODE>), g, blk, 0, stream, rp);
+    hipLaunchKernelGGL((k_scan_offspring<MODE>), g, blk, 0, stream, RS_ARGS(rp));
 }
 
 // Multinomial queries: one block per 1024 queries (PLAIN as for k_scan_offspring).
@@ -210,13 +229,13 @@ int launch_resample(ssm_ctx* ctx, hipStream_t stream, const ResampleArgs& a) {
       hipLaunchKernelGGL(k_tile_sums<kWeightsF64>, g, blk, 0, stream, rp);
       hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(256), 0, stream, nb, rp.tile_sum,
                          rp.prefix);
-      hipLaunchKernelGGL(k_scan_offspring<kWeightsF64>, g, blk, 0, stream, rp);
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsF64>, g, blk, 0, stream, RS_ARGS(rp));
       break;
     case kWeightsF32:
       hipLaunchKernelGGL(k_tile_sums<kWeightsF32>, g, blk, 0, stream, rp);
       hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(256), 0, stream, nb, rp.tile_sum,
                          rp.prefix);
-      hipLaunchKernelGGL(k_scan_offspring<kWeightsF32>, g, blk, 0, stream, rp);
+      hipLaunchKernelGGL(k_scan_offspring<kWeightsF32>, g, blk, 0, stream, RS_ARGS(rp));
       break;
     case kWeightsLogF32:
       launch_scan_offspring<kWeightsLogF32>(stream, nb, rp);
