@@ -24,10 +24,32 @@ constexpr int kStepItems = kTileItems;
 constexpr int kStepBlock = kTile;
 constexpr int kPadTo = kTile;
 
+// Offsets of the packed model constants of dimensions (D, Dy): row-major A, b, the lower
+// triangle of L_Q by rows, row-major H, c, the lower triangle of L_R^-1 by rows, cst.
+constexpr int kModelPacked = 64;  // entries of the packed copy
+struct PackedLayout {
+  int A, b, LQ, H, c, LRi, cst, total;
+};
+__host__ __device__ constexpr PackedLayout packed_layout(int D, int Dy) {
+  PackedLayout L{};
+  L.A = 0;
+  L.b = D * D;
+  L.LQ = L.b + D;
+  L.H = L.LQ + D * (D + 1) / 2;
+  L.c = L.H + Dy * D;
+  L.LRi = L.c + Dy;
+  L.cst = L.LRi + Dy * (Dy + 1) / 2;
+  L.total = L.cst + 1;
+  return L;
+}
+
 // Per-filter model constants, precomputed on the host in FP64 and stored in `T`.
-// Column-major with leading dimension D (or Dy).
+// Column-major with leading dimension D (or Dy).  `packed` repeats the ones the step kernels
+// hold in registers, contiguously (packed_layout; filled when they fit), so that a wave's
+// scalar loads of them touch 3 cache lines at D = 4 instead of 10.
 template <typename T>
 struct BfModelDev {
+  T packed[kModelPacked];
   T A[kMaxD * kMaxD];
   T b[kMaxD];
   T LQ[kMaxD * kMaxD];     // lower Cholesky factor of Q
@@ -142,25 +164,30 @@ template <typename T, int MD, int MDY>
 struct ModelView<T, MD, MDY, true> {
   T a[MD * MD], lq[MD * MD], bb[MD], h[MDY * MD], cc[MDY], lri[MDY * MDY];
   T cst;
+  // from the packed copy (MD, MDY are the model's dimensions here)
   __device__ __forceinline__ ModelView(const BfModelDev<T>& m, int D, int Dy) {
+    constexpr PackedLayout L = packed_layout(MD, MDY);
+    static_assert(L.total <= kModelPacked, "packed model constants do not fit");
+    const T* pk = m.packed;
 #pragma unroll
     for (int r = 0; r < MD; ++r) {
-      bb[r] = m.b[r];
+      bb[r] = pk[L.b + r];
 #pragma unroll
       for (int c = 0; c < MD; ++c) {
-        a[r * MD + c] = m.A[r + c * MD];
-        lq[r * MD + c] = (c <= r) ? m.LQ[r + c * MD] : (T)0;
+        a[r * MD + c] = pk[L.A + r * MD + c];
+        lq[r * MD + c] = (c <= r) ? pk[L.LQ + r * (r + 1) / 2 + c] : (T)0;
       }
     }
 #pragma unroll
     for (int q = 0; q < MDY; ++q) {
-      cc[q] = m.c[q];
+      cc[q] = pk[L.c + q];
 #pragma unroll
-      for (int c = 0; c < MD; ++c) h[q * MD + c] = m.H[q + c * MDY];
+      for (int c = 0; c < MD; ++c) h[q * MD + c] = pk[L.H + q * MD + c];
 #pragma unroll
-      for (int c = 0; c < MDY; ++c) lri[q * MDY + c] = (c <= q) ? m.LRi[q + c * MDY] : (T)0;
+      for (int c = 0; c < MDY; ++c)
+        lri[q * MDY + c] = (c <= q) ? pk[L.LRi + q * (q + 1) / 2 + c] : (T)0;
     }
-    cst = m.cst;
+    cst = pk[L.cst];
   }
   __device__ __forceinline__ T A(int r, int c) const { return a[r * MD + c]; }
   __device__ __forceinline__ T LQ(int r, int c) const { return lq[r * MD + c]; }
@@ -685,6 +712,20 @@ int bf_build_model(ssm_ctx* ctx, const ssm_lg_model* mdl, BfModelDev<T>* out) {
   for (int e = 0; e < Dy; ++e) h.c[e] = (T)mdl->c[e];
   for (int e = 0; e < Dy * Dy; ++e) h.LRi[e] = (T)LRi[e];
   h.cst = (T)(-0.5 * ((double)Dy * 1.8378770664093454835606594728112 + logdet));
+  const PackedLayout L = packed_layout(D, Dy);
+  if (L.total <= kModelPacked) {  // the same values, packed (ModelView's register copy)
+    for (int r = 0; r < D; ++r) {
+      h.packed[L.b + r] = h.b[r];
+      for (int c = 0; c < D; ++c) h.packed[L.A + r * D + c] = h.A[r + c * D];
+      for (int c = 0; c <= r; ++c) h.packed[L.LQ + r * (r + 1) / 2 + c] = h.LQ[r + c * D];
+    }
+    for (int q = 0; q < Dy; ++q) {
+      h.packed[L.c + q] = h.c[q];
+      for (int c = 0; c < D; ++c) h.packed[L.H + q * D + c] = h.H[q + c * Dy];
+      for (int c = 0; c <= q; ++c) h.packed[L.LRi + q * (q + 1) / 2 + c] = h.LRi[q + c * Dy];
+    }
+    h.packed[L.cst] = h.cst;
+  }
   return SSM_OK;
 }
 
