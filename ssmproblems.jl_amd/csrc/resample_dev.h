@@ -428,9 +428,13 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   const double u0 = (kind == SSM_STRATIFIED)
                         ? 0.0
                         : (uvec ? uvec[0] : resample_uniform(in.seed, step, 0u));
-  const int o_prev = (b == 0) ? 0 : offspring_of<KIND, PLAIN>(p, P, Wn, u0, step, in.seed);
-  const int o_end =
-      (b == nb - 1) ? n : offspring_of<KIND, PLAIN>(p, Pnext, Wn, u0, step, in.seed);
+  // The wave-uniform counts (the tile's start and end, the wave's first slot) are one
+  // lane's work each and then read from it: four waves share a SIMD's FP64 issue.
+  const double Wq = (lane == 0) ? P : (lane == 1) ? Pnext : Wedge;
+  const int oq = offspring_of<KIND, PLAIN>(p, Wq, Wn, u0, step, in.seed);
+  const int o_prev = (b == 0) ? 0 : __builtin_amdgcn_readlane(oq, 0);
+  const int o_end = (b == nb - 1) ? n : __builtin_amdgcn_readlane(oq, 1);
+  const int o_edge = __builtin_amdgcn_readlane(oq, 2);
   // the tile's last particle closes its range; positions past n own none
   auto count_at = [&](int pos, double Wv) {
     return (pos >= cnt - 1) ? o_end
@@ -441,8 +445,9 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   int O[kTileItems];
 #pragma unroll
   for (int k = 0; k < kTileItems; ++k) O[k] = count_at(kTileItems * t + k, W[k]);
-  const int wstart = __builtin_amdgcn_readfirstlane(
-      (w == 0) ? o_prev : count_at(kFillWin * w - 1, Wedge));
+  const int wstart = (w == 0)                       ? o_prev
+                     : (kFillWin * w - 1 >= cnt - 1) ? o_end
+                                                     : min(max(o_edge, o_prev), o_end);
   const int s0 = dpp_i<kDppWaveShr1>(wstart, O[kTileItems - 1]);  // lane 0: wstart
   const int start[kTileItems] = {s0, O[0], O[1], O[2]};
   if (counts_out) {  // particle k owns the output slots [start[k], O[k])
