@@ -544,13 +544,23 @@ __global__ void __launch_bounds__(kFoldThreads)
   const bool rs = dec.rs, fresh = dec.fresh;
   Fold4 F;
   fold_step4<T>(nb, L, fresh, F);
+  const int b0 = kFoldTiles * threadIdx.x;
+  if (b0 + kFoldTiles <= nb) {  // 16-byte stores (the scratch arrays are 256-byte aligned)
 #pragma unroll
-  for (int q = 0; q < kFoldTiles; ++q) {
-    const int b = kFoldTiles * threadIdx.x + q;
-    if (b < nb) {
-      ts.prefix[b] = F.pre[q];
-      ts.blk_c[b] = F.c[q];
-      ts.blk_m[b] = F.m1[q];
+    for (int q = 0; q < kFoldTiles; q += 2) {
+      *reinterpret_cast<double2*>(ts.prefix + b0 + q) = make_double2(F.pre[q], F.pre[q + 1]);
+      *reinterpret_cast<double2*>(ts.blk_c + b0 + q) = make_double2(F.c[q], F.c[q + 1]);
+      *reinterpret_cast<double2*>(ts.blk_m + b0 + q) = make_double2(F.m1[q], F.m1[q + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kFoldTiles; ++q) {
+      const int b = b0 + q;
+      if (b < nb) {
+        ts.prefix[b] = F.pre[q];
+        ts.blk_c[b] = F.c[q];
+        ts.blk_m[b] = F.m1[q];
+      }
     }
   }
   if (threadIdx.x == 0) {

@@ -360,8 +360,8 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
     m.y = max(m.y, m.x);
     m.z = max(m.z, m.y);
     m.w = max(m.w, m.z);
-    int before = wave_shfl_up(wave_scan_max_i32(m.w), 1);
-    before = max(lane == 0 ? -1 : before, carry);
+    // the previous lane's running max (lane 0: none), then the earlier windows' carry
+    const int before = max(dpp_i<kDppWaveShr1>(-1, wave_scan_max_i32(m.w)), carry);
     m.x = max(m.x, before);
     m.y = max(m.y, before);
     m.z = max(m.z, before);
