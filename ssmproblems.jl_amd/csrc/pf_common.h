@@ -382,9 +382,12 @@ constexpr int kFoldThreads = 256;
 constexpr int kFoldTiles = 4;
 constexpr int kFoldMax = kFoldThreads * kFoldTiles;
 
-// A thread's tile partials.  They are requested before the step's scalars are waited for
-// (every index is clamped in bounds and the baseline fields are read even when unused), so
-// the fold starts one memory round trip after the kernel's arguments.
+// A thread's tile partials.  The updated weights' fields are requested before the step's
+// scalars are waited for (every index is clamped in bounds), so the fold starts one memory
+// round trip after the kernel's arguments; the baseline fields, which only a step that
+// kept its particles uses, are requested once the scalars say so.  One CU issues every
+// load of the fold and the lines it can have in flight bound this phase, so a fresh step
+// (resampled, or the lazy uniform weights) reads 24 of the 40 bytes a tile.
 struct Fold4In {
   double m1[kFoldTiles], tau[kFoldTiles], s2[kFoldTiles], m0[kFoldTiles], s0[kFoldTiles];
 };
@@ -393,11 +396,27 @@ __device__ __forceinline__ void fold_load4(int nb, const double* __restrict__ P,
 #pragma unroll
   for (int q = 0; q < kFoldTiles; ++q) {
     const int bc = min(b0 + q, nb - 1);
-    L.m0[q] = P[bc];
-    L.s0[q] = P[nb + bc];
     L.m1[q] = P[2 * nb + bc];
     L.tau[q] = P[3 * nb + bc];
     L.s2[q] = P[4 * nb + bc];
+  }
+}
+__device__ __forceinline__ void fold_load4_base(int nb, const double* __restrict__ P,
+                                                bool fresh, Fold4In& L) {
+  const int b0 = kFoldTiles * (int)threadIdx.x;
+  if (fresh) {
+#pragma unroll
+    for (int q = 0; q < kFoldTiles; ++q) {
+      L.m0[q] = -INFINITY;
+      L.s0[q] = 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kFoldTiles; ++q) {
+      const int bc = min(b0 + q, nb - 1);
+      L.m0[q] = P[bc];
+      L.s0[q] = P[nb + bc];
+    }
   }
 }
 
@@ -542,6 +561,7 @@ __global__ void __launch_bounds__(kFoldThreads)
   const BfScalars in = *sc_in;
   const StepDecision dec = step_decision(in, aux, threshold_n, logN);
   const bool rs = dec.rs, fresh = dec.fresh;
+  fold_load4_base(nb, partials, fresh, L);
   Fold4 F;
   fold_step4<T>(nb, L, fresh, F);
   const int b0 = kFoldTiles * threadIdx.x;
