@@ -32,7 +32,7 @@ constexpr int kPartial = 5;  // doubles per tile partial (field-major)
 // Per-filter device scratch for tiles (all sized by the tile count nb).
 struct TileScratch {
   double* partials;  // nb * kPartial
-  double* blk_m;     // nb: m1_b (exactly representable in T)
+  double* blk_m;     // nb: m1_b (exactly representable in T): the partials' m1 row
   double* blk_c;     // nb: c_b
   double* prefix;    // nb + 1: exclusive tile prefixes of the weights, prefix[nb] = total
   double* groups;    // kGroupRec per group of kFinishGroup tiles (finishes of > 1024 tiles)
@@ -363,7 +363,6 @@ __device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* 
     if (b < F.hi) {
       ts.prefix[b] = F.pre[q];
       ts.blk_c[b] = F.c[q];
-      ts.blk_m[b] = F.m1[q];
     }
   }
   if (threadIdx.x == 0) {
@@ -548,8 +547,9 @@ __device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
   return G;
 }
 
-// The finish of nb <= 1024 tiles (one block): the fold, every tile's scale, maximum and
-// prefix, the total and the new scalars (sc_in may equal sc: it is read before the write).
+// The finish of nb <= 1024 tiles (one block): the fold, every tile's scale and prefix (its
+// maximum stays in the partials' m1 row, which blk_m points at), the total and the new
+// scalars (sc_in may equal sc: it is read before the write).
 // The first loads' pointers and count lead the arguments (preloaded into registers).
 template <typename T>
 __global__ void __launch_bounds__(kFoldThreads)
@@ -570,7 +570,6 @@ __global__ void __launch_bounds__(kFoldThreads)
     for (int q = 0; q < kFoldTiles; q += 2) {
       *reinterpret_cast<double2*>(ts.prefix + b0 + q) = make_double2(F.pre[q], F.pre[q + 1]);
       *reinterpret_cast<double2*>(ts.blk_c + b0 + q) = make_double2(F.c[q], F.c[q + 1]);
-      *reinterpret_cast<double2*>(ts.blk_m + b0 + q) = make_double2(F.m1[q], F.m1[q + 1]);
     }
   } else {
 #pragma unroll
@@ -579,7 +578,6 @@ __global__ void __launch_bounds__(kFoldThreads)
       if (b < nb) {
         ts.prefix[b] = F.pre[q];
         ts.blk_c[b] = F.c[q];
-        ts.blk_m[b] = F.m1[q];
       }
     }
   }
@@ -734,7 +732,6 @@ __global__ void __launch_bounds__(kFinishGroup)
   if (b < nb) {
     ts.blk_c[b] = ts.blk_c[b] * f_mine;
     ts.prefix[b] = off_mine + f_mine * ts.prefix[b];
-    ts.blk_m[b] = ts.partials[2 * nb + b];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ts.prefix[nb] = off;

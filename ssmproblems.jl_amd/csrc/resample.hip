@@ -34,7 +34,7 @@ constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 constexpr size_t scratch_bytes_for(size_t n) {
   const size_t nb = (n + kTile - 1) / kTile;
   const size_t ng = (nb + kFinishGroup - 1) / kFinishGroup;
-  return round256(nb * kPartial * 8) + round256(nb * 8) + round256(nb * 8) +
+  return round256(nb * kPartial * 8) + round256(nb * 8) +
          round256((nb + 1) * 8) + round256(ng * kGroupRec * 8) + round256(nb * kTile * 8) +
          round256(nb * kMidPerTile * 8) + round256(nb * kRootPerTile * 8);
 }
@@ -47,8 +47,7 @@ void resample_bind_scratch(ResampleArgs& a, void* scratch, int n) {
   char* p = (char*)scratch;
   a.partials = (double*)p;
   p += r(nb * kPartial * 8);
-  a.blk_m = (double*)p;
-  p += r(nb * 8);
+  a.blk_m = a.partials + 2 * nb;  // the partials' m1 row: the step kernel wrote m1_b there
   a.blk_c = (double*)p;
   p += r(nb * 8);
   a.prefix = (double*)p;
