@@ -154,13 +154,9 @@ __global__ void __launch_bounds__(kTileThreads)
   tile_partial<T>(nl, lv, valid, fresh, partials, gridDim.x, blockIdx.x);
 }
 
-// The fold of one step's tile partials by one block (every thread ends with the same
-// block-wide values).  T is the filter's type: the per-tile scale factors exp(m1_b - M) are
-// evaluated in T.  Each thread owns a contiguous chunk [lo, hi) of at most CH tiles held in
-// registers (one load phase), so nb <= NT * CH.  For its chunk a thread keeps the tile
-// maxima m1, the scales c_b = exp(m1_b - M) / S and the exclusive prefixes of c_b tau_b,
-// made nondecreasing by an exact running max (the resampler's tiles meet at these values,
-// so they must never step backwards).
+// A folded step as the scalars need it (fold_scalars): the maxima M1 (updated) and M0
+// (baseline), the scaled sums S, S2 and S0, the total of the prefixes, and whether every
+// weight is zero; per-thread tile fields for the folds that keep them in this form.
 template <int CH>
 struct TileFold {
   int lo, hi;
@@ -168,130 +164,6 @@ struct TileFold {
   double M1, S, S2, M0, S0, tot;
   bool dead;
 };
-
-template <typename T, int CH, int NT>
-__device__ __forceinline__ void fold_tiles(int nb, const double* __restrict__ partials,
-                                           bool fresh, TileFold<CH>& F) {
-  constexpr int NW = NT / 64;
-  __shared__ double red[3][NW];
-  __shared__ double wtot[NW], wmax[NW];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int chunk = (nb + NT - 1) / NT;
-  const int lo = min(nb, tid * chunk), hi = min(nb, lo + chunk);
-  F.lo = lo;
-  F.hi = hi;
-  double m0[CH], s0[CH], tau[CH], s2[CH];
-  // every load is issued unconditionally from a clamped, in-bounds index and masked
-  // afterwards, so all 5 CH loads are in flight together (a load under a condition
-  // becomes a branch, one round trip per tile)
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const int bc = min(lo + q, nb - 1);
-    m0[q] = partials[bc];
-    s0[q] = partials[nb + bc];
-    F.m1[q] = partials[2 * nb + bc];
-    tau[q] = partials[3 * nb + bc];
-    s2[q] = partials[4 * nb + bc];
-  }
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const bool in = lo + q < hi;
-    // the baseline partials are only used when the step was not a fresh start
-    m0[q] = (in && !fresh) ? m0[q] : -INFINITY;
-    s0[q] = (in && !fresh) ? s0[q] : 0.0;
-    F.m1[q] = in ? F.m1[q] : -INFINITY;
-    tau[q] = in ? tau[q] : 0.0;
-    s2[q] = in ? s2[q] : 0.0;
-  }
-  // maxima of the tile maxima (exact, any order)
-  double M1 = -INFINITY, M0 = -INFINITY;
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    M1 = fmax(M1, F.m1[q]);
-    M0 = fmax(M0, m0[q]);
-  }
-  M1 = wave_max_dpp(M1);
-  M0 = wave_max_dpp(M0);
-  if (lane == 0) {
-    red[0][wv] = M1;
-    red[1][wv] = M0;
-  }
-  __syncthreads();
-  M1 = red[0][0];
-  M0 = red[1][0];
-  for (int k = 1; k < NW; ++k) {
-    M1 = fmax(M1, red[0][k]);
-    M0 = fmax(M0, red[1][k]);
-  }
-  __syncthreads();
-  // scaled sums: one exponential per tile, sums in FP64, fixed order
-  double S = 0.0, S2 = 0.0, S0 = 0.0;
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    F.c[q] = (F.m1[q] != -INFINITY) ? (double)exp_w((T)(F.m1[q] - M1)) : 0.0;
-    S += tau[q] * F.c[q];
-    S2 += s2[q] * F.c[q] * F.c[q];
-    if (!fresh && m0[q] != -INFINITY) S0 += s0[q] * (double)exp_w((T)(m0[q] - M0));
-  }
-  S = wave_sum_dpp(S);
-  S2 = wave_sum_dpp(S2);
-  S0 = wave_sum_dpp(S0);
-  if (lane == 0) {
-    red[0][wv] = S;
-    red[1][wv] = S2;
-    red[2][wv] = S0;
-  }
-  __syncthreads();
-  S = red[0][0];
-  S2 = red[1][0];
-  S0 = red[2][0];
-  for (int k = 1; k < NW; ++k) {
-    S += red[0][k];
-    S2 += red[1][k];
-    S0 += red[2][k];
-  }
-  const bool dead = !(M1 > -INFINITY) || !(S > 0.0);
-  const double invS = dead ? 0.0 : 1.0 / S;
-  // c_b = f_b / S and the exclusive scan of c_b tau_b over the chunks
-  double cs = 0.0;
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    F.c[q] *= invS;
-    cs += F.c[q] * tau[q];
-  }
-  const double incl = wave_scan_dpp(cs);
-  double run = wave_shfl_up(incl, 1);
-  if (lane == 0) run = 0.0;
-  if (lane == 63) wtot[wv] = incl;
-  __syncthreads();
-  for (int k = 0; k < wv; ++k) run += wtot[k];
-  double pmax = -INFINITY;
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    pmax = fmax(pmax, run);
-    F.pre[q] = pmax;
-    run += F.c[q] * tau[q];
-  }
-  double cm = wave_scan_max_dpp((lo < hi) ? pmax : -INFINITY);
-  if (lane == 63) wmax[wv] = cm;
-  double carry = wave_shfl_up(cm, 1);
-  if (lane == 0) carry = -INFINITY;
-  __syncthreads();
-  for (int k = 0; k < wv; ++k) carry = fmax(carry, wmax[k]);
-#pragma unroll
-  for (int q = 0; q < CH; ++q) F.pre[q] = fmax(F.pre[q], carry);
-  double tot = wtot[0];
-  for (int k = 1; k < NW; ++k) tot += wtot[k];
-  for (int k = 0; k < NW; ++k) tot = fmax(tot, wmax[k]);
-  __syncthreads();  // the LDS of this fold may be reused by the caller
-  F.M1 = M1;
-  F.S = S;
-  F.S2 = S2;
-  F.M0 = M0;
-  F.S0 = S0;
-  F.tot = tot;
-  F.dead = dead;
-}
 
 // ESS of a folded step (resamplers.jl:179-183): N for dead weights, clamped to <= N.
 template <int CH>
@@ -345,32 +217,6 @@ __device__ __forceinline__ BfScalars fold_scalars(const TileFold<CH>& F, const B
   }
   return s;
 }
-
-// One block: fold the tile partials, finish the step, prepare the next resampling (the
-// per-tile scales and prefixes go to ts for the resampler).
-template <typename T, int CH, int NT>
-__device__ __forceinline__ void pf_finish_block(int N, int nb, const BfScalars* sc_in,
-                                                BfScalars* sc, double threshold_n, double logN,
-                                                const TileScratch& ts, double* ll_steps,
-                                                int32_t* rs_steps, int record) {
-  const bool rs = threshold_n >= sc_in->ess;
-  const bool fresh = rs || sc_in->typeless;
-  TileFold<CH> F;
-  fold_tiles<T, CH, NT>(nb, ts.partials, fresh, F);
-#pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const int b = F.lo + q;
-    if (b < F.hi) {
-      ts.prefix[b] = F.pre[q];
-      ts.blk_c[b] = F.c[q];
-    }
-  }
-  if (threadIdx.x == 0) {
-    ts.prefix[nb] = F.tot;
-    *sc = fold_scalars<T, CH>(F, *sc_in, N, logN, rs, fresh, ll_steps, rs_steps, record);
-  }
-}
-
 
 // ---- the fold of at most 1024 tiles by one 256-thread block ------------------------
 // Thread t owns the consecutive tiles 4t..4t+3.  (Folding inside every block of the next
@@ -547,15 +393,18 @@ __device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
   return G;
 }
 
-// The finish of nb <= 1024 tiles (one block): the fold, every tile's scale and prefix (its
-// maximum stays in the partials' m1 row, which blk_m points at), the total and the new
-// scalars (sc_in may equal sc: it is read before the write).
-// The first loads' pointers and count lead the arguments (preloaded into registers).
+// The finish of nb <= 1024 tiles by one block of kFoldThreads threads: the fold, every
+// tile's scale and prefix (its maximum stays in the partials' m1 row, which blk_m points
+// at), the total and the new scalars (sc_in may equal sc: it is read before the write).
+// The single-filter finish kernel and the server's batched finish both run this function,
+// so a served filter's results equal the plain path's bit for bit.
 template <typename T>
-__global__ void __launch_bounds__(kFoldThreads)
-    k_pf_finish_fold(const double* partials, const BfScalars* sc_in, int nb, int N,
-                     BfScalars* sc, double threshold_n, double logN, TileScratch ts,
-                     double* ll_steps, int32_t* rs_steps, int record, const BfScalars* aux) {
+__device__ __forceinline__ void pf_finish_fold4(const double* partials, const BfScalars* sc_in,
+                                                int nb, int N, BfScalars* sc,
+                                                double threshold_n, double logN,
+                                                const TileScratch& ts, double* ll_steps,
+                                                int32_t* rs_steps, int record,
+                                                const BfScalars* aux) {
   Fold4In L;
   fold_load4(nb, partials, L);
   const BfScalars in = *sc_in;
@@ -586,6 +435,17 @@ __global__ void __launch_bounds__(kFoldThreads)
     *sc = fold_scalars<T, 1>(fold_summary(F), in, N, logN, rs, fresh, ll_steps, rs_steps,
                              record, dec.base);
   }
+}
+
+// The finish kernel of nb <= 1024 tiles.  The first loads' pointers and count lead the
+// arguments (preloaded into registers).
+template <typename T>
+__global__ void __launch_bounds__(kFoldThreads)
+    k_pf_finish_fold(const double* partials, const BfScalars* sc_in, int nb, int N,
+                     BfScalars* sc, double threshold_n, double logN, TileScratch ts,
+                     double* ll_steps, int32_t* rs_steps, int record, const BfScalars* aux) {
+  pf_finish_fold4<T>(partials, sc_in, nb, N, sc, threshold_n, logN, ts, ll_steps, rs_steps,
+                     record, aux);
 }
 
 // Exclusive prefix of one value per thread across NT threads, nondecreasing in thread order

@@ -47,6 +47,7 @@
 namespace ssm {
 
 constexpr int kServeMaxN = 1 << 18;  // one finish block per filter (<= 256 tiles)
+static_assert(kTileThreads == kFoldThreads, "the batched finish runs the fold with tile blocks");
 constexpr int kMaxChunk = 64;        // batched steps between admission points
 constexpr int kMinChunk = 16;        // ... at least
 constexpr int kMaxGraphs = 64;       // captured chunks kept by the lane
@@ -158,8 +159,9 @@ __global__ void __launch_bounds__(kTileThreads)
   const BfbFilter& F = fs[blockIdx.x];
   // every thread reads the step before the block's barriers, thread 0 writes it after
   if (F.sc->step > F.t_last) return;
-  pf_finish_block<float, 1, kTileThreads>(F.N, F.nb, F.sc, F.sc, F.threshold_n, F.logN, F.ts,
-                                          nullptr, nullptr, 1);
+  // the single-filter finish's fold (a filter has at most 256 tiles here)
+  pf_finish_fold4<float>(F.ts.partials, F.sc, F.nb, F.N, F.sc, F.threshold_n, F.logN, F.ts,
+                         nullptr, nullptr, 1, nullptr);
 }
 
 template <int DT>

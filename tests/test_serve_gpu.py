@@ -59,6 +59,10 @@ CASES = [  # (Dx, Dy, N, T, threshold, kind)
     (3, 2, 5000, 45, 1.0, 3),
     (7, 5, 1024, 35, 0.5, 0),
     (16, 8, 2049, 25, 1.0, 2),
+    # many tiles a fold thread (196 and 256 tiles): the served and plain finishes must
+    # fold in the same order
+    (4, 2, 200000, 30, 1.0, 0),
+    (2, 1, 262144, 20, 0.5, 3),
 ]
 
 
