@@ -332,8 +332,12 @@ __device__ __forceinline__ const BfModelDev<T>& model_to_lds(const BfModelDev<T>
 // REF (conditional SMC, GF/src/algorithms/particles.jl:59-66, resamplers.jl:51-65):
 // particle 0 takes the reference state x_ref[t] instead of a propagated one, and when the
 // step resamples its ancestor is *ref_anc (ancestor sampling) or particle 0.
+//
+// SINGLE: the form for one filter's latency-bound launch (inputs' wait pinned, bulk outputs
+// stored at device scope); the server's batched step, whose grid covers many filters and
+// whose speed occupancy sets, takes plain loads and stores.
 template <typename T, int DT, int DYT, bool NOISE = false, bool REF = false, bool GUIDED = false,
-          bool APF = false>
+          bool APF = false, bool SINGLE = true>
 __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   constexpr int MD = DT ? DT : kMaxD;
   constexpr int MDY = DYT ? DYT : kMaxDy;
@@ -347,9 +351,13 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   // kernel waits once for the batch and the compiler cannot move a load behind the
   // decision (a load behind a condition on a loaded value waits for the one before it).
   // The gather thus starts one round trip after the kernel's arguments instead of four.
+  // The server's batched step (!SINGLE) reads the ancestors inside the fresh body instead.
+  static_assert(SINGLE || !REF, "the batched step has no reference particle");
   int a_in[kStepItems];
+  if constexpr (SINGLE) {
 #pragma unroll
-  for (int k = 0; k < kStepItems; ++k) a_in[k] = p.anc[tile0 + k * kStepThreads + threadIdx.x];
+    for (int k = 0; k < kStepItems; ++k) a_in[k] = p.anc[tile0 + k * kStepThreads + threadIdx.x];
+  }
   // Model constants: registers for compile-time dimensions (read before any store, so the
   // loads are uniform scalar loads), an LDS copy otherwise.
   const BfModelDev<T>* msrc = p.model;
@@ -366,9 +374,15 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   const int t = sc->step;
   const int ybase = sc->ybase;
   const uint64_t seed = *p.seed_dev;
+  // The wait is pinned for a single filter's FP32 steps without a proposal.  With FP64
+  // constants, a guided filter's proposal on top of the model, or the server's kernel that
+  // holds three dimension variants, holding the whole batch in registers at once costs
+  // more occupancy than the early gather gains (112 to 167 VGPRs at FP64).
+  if constexpr (sizeof(T) == 4 && !GUIDED && SINGLE) {
 #pragma unroll
-  for (int k = 0; k < kStepItems; ++k) asm volatile("" : "+v"(a_in[k]));
-  asm volatile("" ::"s"(ess_in), "s"(lse_in), "s"(typeless), "s"(t), "s"(ybase), "s"(seed));
+    for (int k = 0; k < kStepItems; ++k) asm volatile("" : "+v"(a_in[k]));
+    asm volatile("" ::"s"(ess_in), "s"(lse_in), "s"(typeless), "s"(t), "s"(ybase), "s"(seed));
+  }
   const T lse = (T)lse_in;
   const bool rs = p.threshold_n >= ess_in;
   const bool fresh = rs | (typeless != 0);  // pre-update log-weights 0 (or -log eta: apf)
@@ -402,11 +416,14 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   }
   // each item's source particle: its ancestor when the step resampled, itself otherwise
   int src[kStepItems];
+  auto set_src = [&]() {
 #pragma unroll
-  for (int k = 0; k < kStepItems; ++k) {
-    const int i = tile0 + k * kStepThreads + threadIdx.x;
-    src[k] = (rs && i < p.N) ? a_in[k] : i;
-  }
+    for (int k = 0; k < kStepItems; ++k) {
+      const int i = tile0 + k * kStepThreads + threadIdx.x;
+      src[k] = (rs && i < p.N) ? a_in[k] : i;
+    }
+  };
+  if constexpr (SINGLE) set_src();
 
   // the normals of item k: pre-drawn planes, or Philox keyed by (seed, t, i)
   auto normals = [&](int k, T* z) {
@@ -438,7 +455,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
           for (int c = 0; c <= r; ++c) s += pv.LP(r, c) * z[c];
           xn[r] = s;
-          store_wt(&p.xout[(size_t)r * p.Npad + i], s);
+          store_out<SINGLE>(&p.xout[(size_t)r * p.Npad + i], s);
           zz += z[r] * z[r];
         }
       }
@@ -477,7 +494,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
           for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
           if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
           xn[r] = s;
-          store_wt(&p.xout[(size_t)r * p.Npad + i], s);
+          store_out<SINGLE>(&p.xout[(size_t)r * p.Npad + i], s);
         }
       }
     }
@@ -508,7 +525,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
       l = nlk + lpq + (mv.cst - (T)0.5 * quad);
     else
       l = nlk + (mv.cst - (T)0.5 * quad);
-    store_wt(&p.logw[i], l);
+    store_out<SINGLE>(&p.logw[i], l);
     return l;
   };
 
@@ -522,9 +539,18 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
     for (int k = 0; k < kStepItems; ++k) {
       const int i = tile0 + k * kStepThreads + threadIdx.x;
       valid[k] = i < p.N;
+      if constexpr (!SINGLE && FRESH) a_in[k] = p.anc[i];
       if constexpr (!FRESH) lw[k] = p.logw[i];
     }
-    auto src_of = [&](int k) { return src[k]; };  // src[k] = i unless the step resampled
+    // an item's source particle: its ancestor when the step resampled, itself otherwise
+    auto src_of = [&](int k) {
+      if constexpr (SINGLE) return src[k];
+      const int i = tile0 + k * kStepThreads + threadIdx.x;
+      if constexpr (FRESH)
+        return (rs && valid[k]) ? a_in[k] : i;
+      else
+        return i;
+    };
 #pragma unroll
     for (int k = 0; k < kStepItems; ++k) {
       if constexpr (FRESH)

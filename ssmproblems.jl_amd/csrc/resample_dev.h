@@ -331,6 +331,7 @@ __device__ __forceinline__ void tile_cum_weights(const RsParams& p, int b,
 // consecutive slots.  A particle whose range covers a whole window is written directly
 // (ow holds every position's range end), which keeps degenerate weights balanced.  anc0 is
 // the particle index of the wave's position 0.
+template <bool WT>
 __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
                                           const int start[kTileItems],
                                           const int O[kTileItems], int wstart, int wend,
@@ -343,7 +344,7 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
       const int e = __builtin_amdgcn_readfirstlane(min(ow[carry], wend));
       if (e - base >= kFillWin) {
         const int v = anc0 + carry;
-        for (int s = base + lane; s < e; s += kWave) store_wt(&anc[s], v);
+        for (int s = base + lane; s < e; s += kWave) store_out<WT>(&anc[s], v);
         base = e;
         continue;
       }
@@ -374,7 +375,7 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
     for (int i = 0; i < kTileItems; ++i) {
       const int s = base + i * kWave + lane;
       const int a = marks[i * kWave + lane];
-      if (s < wend) store_wt(&anc[s], anc0 + a);
+      if (s < wend) store_out<WT>(&anc[s], anc0 + a);
     }
     wave_lds_sync();
     base += kFillWin;
@@ -391,7 +392,7 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
 // (positions past n hold the tile's closing prefix), every 4th of them to Wmid and every
 // 64th to Wroot.
 // All threads of the block call it; it returns at once when !in.rs.
-template <int MODE, int KIND = -1, bool PLAIN = false>
+template <int MODE, int KIND = -1, bool PLAIN = false, bool WT = true>
 __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
                                                        const RawW<MODE> raw[kTileItems],
                                                        const TileIn& in) {
@@ -461,14 +462,14 @@ __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
   *reinterpret_cast<int4*>(ow + kTileItems * lane) = make_int4(O[0], O[1], O[2], O[3]);
   wave_lds_sync();
   const int wend = __builtin_amdgcn_readlane(O[kTileItems - 1], kWave - 1);
-  wave_fill(marks[w], ow, start, O, wstart, wend, tile0 + w * kFillWin, p.anc);
+  wave_fill<WT>(marks[w], ow, start, O, wstart, wend, tile0 + w * kFillWin, p.anc);
 }
 
 // The resample of tile b with its inputs from memory: the decision and flags from the
 // filter's scalars (or always, for explicit weights), the prefixes and scales from the
 // finish kernel.  Every global input is requested before the decision is known (all the
 // addresses are in bounds), so the kernel pays one round trip before its arithmetic.
-template <int MODE, int KIND = -1, bool PLAIN = false>
+template <int MODE, int KIND = -1, bool PLAIN = false, bool WT = true>
 __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   const int nb = (p.n + kTile - 1) / kTile;
   RawW<MODE> raw[kTileItems];
@@ -492,7 +493,7 @@ __device__ __forceinline__ void scan_offspring_tile(const RsParams& p, int b) {
   in.typeless = rsc.typeless;
   in.step = rsc.step;
   in.seed = rsc.seed;
-  scan_offspring_tile_in<MODE, KIND, PLAIN>(p, b, raw, in);
+  scan_offspring_tile_in<MODE, KIND, PLAIN, WT>(p, b, raw, in);
 }
 
 // The cumulative weight that closes tile b: the last entry of its Wbuf positions.

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(kTileThreads)
   const BfbFilter& F = fs[o];
   if (F.sc->step > F.t_last) return;  // finished inside this chunk
   const RsParams p = bfb_rs_params(F);
-  scan_offspring_tile<kWeightsLogF32, -1, true>(p, blockIdx.x - F.tile_base);
+  scan_offspring_tile<kWeightsLogF32, -1, true, false>(p, blockIdx.x - F.tile_base);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
@@ -146,11 +146,11 @@ __global__ void __launch_bounds__(kTileThreads)
   const int tile = blockIdx.x - F.tile_base;
   // the same specialisations the single-filter path launches
   if (F.D == 4 && F.Dy == 2)
-    bf_step_tile<float, 4, 2>(p, tile);
+    bf_step_tile<float, 4, 2, false, false, false, false, false>(p, tile);
   else if (F.D == 1 && F.Dy == 1)
-    bf_step_tile<float, 1, 1>(p, tile);
+    bf_step_tile<float, 1, 1, false, false, false, false, false>(p, tile);
   else
-    bf_step_tile<float, 0, 0>(p, tile);
+    bf_step_tile<float, 0, 0, false, false, false, false, false>(p, tile);
 }
 
 __global__ void __launch_bounds__(kTileThreads)
