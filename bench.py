@@ -447,6 +447,22 @@ def bench_serve(args, budget=16 << 30, max_active=256, budget_s=10.0):
         return sv.submit_params(S.REQ_KF if q["kind"] == 1 else S.REQ_BF, q["params"], q["ys"],
                                 q["N"], q["threshold"], 0, q["seed"])
 
+    if args.serve_saturation_only:
+        runs = []
+        for _ in range(2):
+            with S.Server(budget, max_active, ctx=ctx) as sv:
+                t0 = time.perf_counter()
+                ids = [submit(sv, q) for q in reqs]
+                res = [sv.wait(i) for i in ids]
+                el = time.perf_counter() - t0
+                st = sv.stats()
+            assert all(r.state == S.DONE for r in res), "served requests failed"
+            runs.append({"filters_per_s": M / el, "bf_chunk_ms": st["bf_chunk_ms"]})
+            log(f"saturation: {M} in {el:.2f}s = {M / el:.1f} filters/s, "
+                f"chunks {st['bf_chunk_ms']:.1f} ms")
+        print(json.dumps({"workload": "serve saturation runs", "requests": M, "runs": runs}),
+              flush=True)
+        return
     # warm-up: every code path once (kernels, graphs, server threads)
     plain_run(ctx, reqs[0])
     with S.Server(budget, max_active, ctx=ctx) as sv:
@@ -974,6 +990,9 @@ def main():
                          "storage after every filter step; csmc: CSMC-AS sweeps; kfll: "
                          "batched kf_loglikelihood with per-step parameters")
     ap.add_argument("--requests", type=int, default=256, help="serve: requests per phase")
+    ap.add_argument("--serve-saturation-only", action="store_true",
+                    help="serve: only two saturation runs of the server (no plain path, no "
+                         "arrival phases), for kernel profiles of the batched lane")
     ap.add_argument("--B", type=int, default=None,
                     help="kf, kfll: number of filters (default 65536)")
     ap.add_argument("--K", type=int, default=100, help="csmc: steps per sweep")
