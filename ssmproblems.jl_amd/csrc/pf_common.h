@@ -280,12 +280,27 @@ struct Fold4 {
 // chained by off_{w+1} = off_w + E_w, so no prefix exceeds the next tile's; tot is the last
 // offset, at least every prefix.  The baseline partials only count when the folded step
 // did not start fresh.  Three barriers.
-template <typename T>
+//
+// NT = 64 (one wave: nb <= 4 x 64) combines as the four-wave block does, with the three
+// missing waves' values standing in as constants (maxima -inf, sums and closing values 0),
+// so both forms give the same bits: with nb <= 256 those are exactly what waves 1-3 hold.
+template <typename T, int NT = kFoldThreads>
 __device__ __forceinline__ void fold_step4(int nb, const Fold4In& L, bool fresh, Fold4& F) {
-  constexpr int NW = kFoldThreads / 64, R = kFoldTiles;
-  __shared__ double red[5][NW];
-  __shared__ double wend[NW];
+  constexpr int NW = NT / 64, R = kFoldTiles, NV = kFoldThreads / 64;
+  static_assert(NV == 4 && (NW == NV || NW == 1), "one wave or the four-wave block");
+  __shared__ double red[5][NV];
+  __shared__ double wend[NV];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if constexpr (NW == 1) {
+    if (lane >= 1 && lane < NV) {
+      red[0][lane] = -INFINITY;
+      red[1][lane] = -INFINITY;
+      red[2][lane] = 0.0;
+      red[3][lane] = 0.0;
+      red[4][lane] = 0.0;
+      wend[lane] = 0.0;
+    }
+  }
   double m0[R], s0[R], tau[R], s2[R];
   double M1 = -INFINITY, M0 = -INFINITY;
 #pragma unroll
@@ -345,7 +360,7 @@ __device__ __forceinline__ void fold_step4(int nb, const Fold4In& L, bool fresh,
   __syncthreads();
   double off = 0.0, mine = 0.0;
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
+  for (int k = 0; k < NV; ++k) {
     if (k == wv) mine = off;
     off += wend[k];
   }
@@ -398,7 +413,7 @@ __device__ __forceinline__ TileFold<1> fold_summary(const Fold4& F) {
 // at), the total and the new scalars (sc_in may equal sc: it is read before the write).
 // The single-filter finish kernel and the server's batched finish both run this function,
 // so a served filter's results equal the plain path's bit for bit.
-template <typename T>
+template <typename T, int NT = kFoldThreads>
 __device__ __forceinline__ void pf_finish_fold4(const double* partials, const BfScalars* sc_in,
                                                 int nb, int N, BfScalars* sc,
                                                 double threshold_n, double logN,
@@ -412,7 +427,7 @@ __device__ __forceinline__ void pf_finish_fold4(const double* partials, const Bf
   const bool rs = dec.rs, fresh = dec.fresh;
   fold_load4_base(nb, partials, fresh, L);
   Fold4 F;
-  fold_step4<T>(nb, L, fresh, F);
+  fold_step4<T, NT>(nb, L, fresh, F);
   const int b0 = kFoldTiles * threadIdx.x;
   if (b0 + kFoldTiles <= nb) {  // 16-byte stores (the scratch arrays are 256-byte aligned)
 #pragma unroll

@@ -46,8 +46,8 @@
 
 namespace ssm {
 
-constexpr int kServeMaxN = 1 << 18;  // one finish block per filter (<= 256 tiles)
-static_assert(kTileThreads == kFoldThreads, "the batched finish runs the fold with tile blocks");
+constexpr int kServeMaxN = 1 << 18;  // one finish wave per filter (<= 256 tiles)
+static_assert(kServeMaxN / kTile <= kWave * kFoldTiles, "a filter's tiles fit one fold wave");
 constexpr int kMaxChunk = 64;        // batched steps between admission points
 constexpr int kMinChunk = 16;        // ... at least
 constexpr int kMaxGraphs = 64;       // captured chunks kept by the lane
@@ -153,15 +153,16 @@ __global__ void __launch_bounds__(kTileThreads)
     bf_step_tile<float, 0, 0, false, false, false, false, false>(p, tile);
 }
 
-__global__ void __launch_bounds__(kTileThreads)
+// One wave per filter: the single filter's fold in its one-wave form, which gives the same
+// bits as the four-wave block for the at most 256 tiles a served filter has.
+__global__ void __launch_bounds__(kWave)
     k_bfb_finish(const BfbFilter* __restrict__ fs, const int32_t* __restrict__ nf) {
   if ((int)blockIdx.x >= *nf) return;
   const BfbFilter& F = fs[blockIdx.x];
   // every thread reads the step before the block's barriers, thread 0 writes it after
   if (F.sc->step > F.t_last) return;
-  // the single-filter finish's fold (a filter has at most 256 tiles here)
-  pf_finish_fold4<float>(F.ts.partials, F.sc, F.nb, F.N, F.sc, F.threshold_n, F.logN, F.ts,
-                         nullptr, nullptr, 1, nullptr);
+  pf_finish_fold4<float, kWave>(F.ts.partials, F.sc, F.nb, F.N, F.sc, F.threshold_n, F.logN,
+                                F.ts, nullptr, nullptr, 1, nullptr);
 }
 
 template <int DT>
@@ -538,7 +539,7 @@ void bf_lane(ssm_server* sv) {
                                sv->d_desc, sv->d_owner);
           hipLaunchKernelGGL(k_bfb_step, dim3(tiles_pad), dim3(kTileThreads), 0, s,
                              sv->d_desc, sv->d_owner);
-          hipLaunchKernelGGL(k_bfb_finish, dim3(nf_pad), dim3(kTileThreads), 0, s, sv->d_desc,
+          hipLaunchKernelGGL(k_bfb_finish, dim3(nf_pad), dim3(kWave), 0, s, sv->d_desc,
                              sv->d_nf);
         }
         ok = (hipStreamEndCapture(s, &g) == hipSuccess) && ok;
