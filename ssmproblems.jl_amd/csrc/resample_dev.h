@@ -391,7 +391,9 @@ __device__ __forceinline__ void wave_fill(int* marks, const int* ow,
 // writes the nondecreasing cumulative weights of every position of the tile to Wbuf
 // (positions past n hold the tile's closing prefix), every 4th of them to Wmid and every
 // 64th to Wroot.
-// All threads of the block call it; it returns at once when !in.rs.
+// All threads of the block call it; it returns at once when !in.rs.  WT: the ancestors
+// are stored at device scope (a single filter's launch); the server's batched resampler
+// stores them plainly.
 template <int MODE, int KIND = -1, bool PLAIN = false, bool WT = true>
 __device__ __forceinline__ void scan_offspring_tile_in(const RsParams& p, int b,
                                                        const RawW<MODE> raw[kTileItems],
