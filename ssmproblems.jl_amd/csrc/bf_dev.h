@@ -425,6 +425,16 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
   };
   if constexpr (SINGLE) set_src();
 
+  // Entry e of state plane `plane` (Npad entries a plane).  The server's batched step
+  // (!SINGLE) indexes with 32-bit offsets: its filters hold at most kMaxD x 2^18 entries,
+  // and a 32-bit offset with the uniform base takes one VGPR instead of a 64-bit address's
+  // two (the batched kernel: 166 to 118 VGPRs, three to four waves a SIMD).
+  auto plane_at = [&](int plane, int e) -> size_t {
+    if constexpr (SINGLE)
+      return (size_t)plane * p.Npad + e;
+    else
+      return (size_t)(uint32_t)(plane * p.Npad + e);
+  };
   // the normals of item k: pre-drawn planes, or Philox keyed by (seed, t, i)
   auto normals = [&](int k, T* z) {
     const int i = tile0 + k * kStepThreads + threadIdx.x;
@@ -455,7 +465,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
 #pragma unroll
           for (int c = 0; c <= r; ++c) s += pv.LP(r, c) * z[c];
           xn[r] = s;
-          store_out<SINGLE>(&p.xout[(size_t)r * p.Npad + i], s);
+          store_out<SINGLE>(&p.xout[plane_at(r, i)], s);
           zz += z[r] * z[r];
         }
       }
@@ -494,7 +504,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
           for (int c = 0; c <= r; ++c) s += mv.LQ(r, c) * z[c];
           if constexpr (REF) s = (i == 0) ? xr[r] : s;  // the pinned reference particle
           xn[r] = s;
-          store_out<SINGLE>(&p.xout[(size_t)r * p.Npad + i], s);
+          store_out<SINGLE>(&p.xout[plane_at(r, i)], s);
         }
       }
     }
@@ -590,7 +600,7 @@ __device__ __forceinline__ void bf_step_tile(const StepParams<T>& p, int tile) {
         T xk[MD], z[MD];
 #pragma unroll
         for (int d = 0; d < MD; ++d)
-          if (d < D) xk[d] = p.xin[(size_t)d * p.Npad + src];
+          if (d < D) xk[d] = p.xin[plane_at(d, src)];
         normals(k, z);
         if constexpr (FRESH && APF) {
           if (apf_rs) nl[k] = -apf_log_eta<T, MD, MDY>(mv, D, Dy, xk, y);

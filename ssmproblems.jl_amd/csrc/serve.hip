@@ -48,6 +48,8 @@ namespace ssm {
 
 constexpr int kServeMaxN = 1 << 18;  // one finish wave per filter (<= 256 tiles)
 static_assert(kServeMaxN / kTile <= kWave * kFoldTiles, "a filter's tiles fit one fold wave");
+static_assert((long long)kServeMaxN * (kMaxD + 1) < (1LL << 31),
+              "the batched step's 32-bit state offsets cover every served filter");
 constexpr int kMaxChunk = 64;        // batched steps between admission points
 constexpr int kMinChunk = 16;        // ... at least
 constexpr int kMaxGraphs = 64;       // captured chunks kept by the lane
